@@ -1,0 +1,208 @@
+"""Throughput of the acceleration path on one MI355X.
+
+Workload (direct sum at scale): 1,048,576 particles, 65,536 massive (mu = 1/65,536, Plummer
+sphere, scale 1) and 983,040 massless uniform in a cube of side 4, softening 1e-3,
+AccelerationSoftened::checked, kick-drift-kick leapfrog at dt = 1e-3 with FP64 positions. One step
+is one full acceleration evaluation (1,048,576 x 65,536 = 6.87e10 pair interactions) plus the
+FP64 kick/drift. Inputs are seeded synthetic data, resident in HBM before the timed region.
+
+Prints ONE JSON line. `value` is G pair interactions per second over the timed steps. The roofline
+entry uses the brute-force kernel's average launch time from HIP events recorded on its stream in
+the timed region, 20 flop per interaction, against the 157.3 TFLOP/s FP32 peak. The CPU baseline
+times the oracle's restatement of parallel::BruteForce (same per-particle fold, all host threads
+up to 16) on a bounded sample of affected particles against the full massive set.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP32_PEAK_TFLOPS = 157.3
+HBM_PEAK_GBS = 8000.0
+FLOP_PER_INTERACTION = 20
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_threads() -> int:
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def load_traffic(name: str):
+    path = os.path.join(ROOT, "profiles", f"{name}_traffic.json")
+    if os.path.exists(path):
+        with open(path) as f:
+            return json.load(f)
+    return None
+
+
+def bench_bf(args):
+    import particular_amd as pa
+    from particular_amd import _lib, workloads
+
+    L = _lib.lib()
+    log("generating workload")
+    s = workloads.massive_massless_case()
+    n = s.n
+    mu = np.ascontiguousarray(s.particles[:, 3])
+    pos = np.ascontiguousarray(s.pos)
+    vel = np.ascontiguousarray(s.vel)
+    dt = 1e-3
+    res = pa.HipResources(0)
+    log("init")
+    res.check(L.pnb_leapfrog_init(res.handle, pos.ctypes.data, vel.ctypes.data, mu.ctypes.data, n,
+                                  s.softening, 1))
+    views = [ctypes.c_void_p() for _ in range(3)]
+    nn, mm = ctypes.c_size_t(), ctypes.c_size_t()
+    res.check(L.pnb_leapfrog_views(res.handle, *[ctypes.byref(v) for v in views], ctypes.byref(nn),
+                                   ctypes.byref(mm)))
+    m = mm.value
+    log(f"n={n} m={m}; warmup {args.warmup}")
+    res.check(L.pnb_leapfrog_run(res.handle, args.warmup, dt, 0))
+    res.synchronize()
+
+    res.profile(True)
+    t0 = time.perf_counter()
+    res.check(L.pnb_leapfrog_run(res.handle, args.steps, dt, 0))
+    res.synchronize()
+    t1 = time.perf_counter()
+    bf_ms, bf_launches = res.profile_read(_lib.PROF_BF)
+    drift_ms, drift_launches = res.profile_read(_lib.PROF_DRIFT)
+    res.profile(False)
+    elapsed = t1 - t0
+    interactions = n * m * args.steps
+    log(f"timed {args.steps} steps in {elapsed:.4f} s; bf kernel {bf_ms / max(bf_launches, 1):.3f} ms avg")
+
+    graph_ms = None
+    if args.graph:
+        res.check(L.pnb_leapfrog_run(res.handle, args.steps, dt, 1))  # capture + first replay
+        res.synchronize()
+        g0 = time.perf_counter()
+        res.check(L.pnb_leapfrog_run(res.handle, args.steps, dt, 1))
+        res.synchronize()
+        graph_ms = (time.perf_counter() - g0) * 1e3 / args.steps
+
+    # accuracy of the last accelerations against the FP64 judge on a sample of affected particles
+    import torch
+
+    sample = 16384
+    shadow, afg, acc = (v.value for v in views)
+    out64 = torch.empty((sample, 3), dtype=torch.float64, device="cuda")
+    res.check(L.pnb_direct_f64_dev(res.handle, shadow, sample, afg, m, float(s.softening),
+                                   out64.data_ptr()))
+    acc_h = np.zeros((n, 4), np.float32)
+    res.check(L.pnb_leapfrog_read(res.handle, None, None, acc_h.ctypes.data))
+    from particular_amd import metrics
+
+    acc_err = metrics.summary(acc_h[:sample], out64.cpu().numpy())
+    log(f"accuracy vs FP64 on {sample} particles: {acc_err}")
+
+    avg_kernel_s = bf_ms / 1e3 / max(bf_launches, 1)
+    achieved = FLOP_PER_INTERACTION * n * m / avg_kernel_s / 1e12
+    traffic = load_traffic("bf")
+    result = {
+        "metric": "G interactions/s (BF); Mparticles/s BH+NN vs plain BH @1e-4 rms err, 1 GPU",
+        "value": interactions / elapsed / 1e9,
+        "unit": "G interactions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded Plummer + uniform cube)",
+        "config": {
+            "workload": "direct sum at scale: 1,048,576 particles (65,536 massive Plummer, "
+                        "983,040 massless cube side 4), eps=1e-3, KDK leapfrog dt=1e-3, FP64 x/v",
+            "n_affected": n,
+            "n_affecting": m,
+            "interactions_per_step": n * m,
+        },
+        "roofline": {
+            "bound": "valu",
+            "kernel": "bf_kernel",
+            "achieved": achieved,
+            "peak": FP32_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / FP32_PEAK_TFLOPS,
+            "traffic": traffic["bytes_per_launch"] if traffic else None,
+            "flop_per_interaction": FLOP_PER_INTERACTION,
+            "avg_kernel_ms": avg_kernel_s * 1e3,
+            "launches": bf_launches,
+        },
+        "accuracy_vs_fp64": acc_err,
+        "drift_kernel_avg_ms": drift_ms / max(drift_launches, 1),
+        "graph_ms_per_step": graph_ms,
+    }
+    res.close()
+    return result, s
+
+
+def cpu_baseline_bf(s, sample: int):
+    from oracle import oracle
+
+    threads = cpu_threads()
+    massive = s.particles[s.particles[:, 3] != 0]
+    aff = s.particles[:sample]
+    oracle.brute_force(aff[:256], massive, s.softening, True, threads)  # warm up
+    t0 = time.perf_counter()
+    oracle.brute_force(aff, massive, s.softening, True, threads)
+    dt = time.perf_counter() - t0
+    return {
+        "value": aff.shape[0] * massive.shape[0] / dt / 1e9,
+        "unit": "G interactions/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{aff.shape[0]} affected x {massive.shape[0]} massive, one evaluation "
+                  f"({dt:.2f} s), C restatement of parallel::BruteForce; CPU: {cpu_model()}",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--cpu-sample", type=int, default=131072)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--graph", action="store_true", default=True)
+    ap.add_argument("--no-graph", dest="graph", action="store_false")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(the acceleration path is built and measured on a single MI355X)", file=sys.stderr)
+        sys.exit(2)
+    result, s = bench_bf(args)
+    result["cpu_baseline"] = None if args.no_cpu else cpu_baseline_bf(s, args.cpu_sample)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,142 @@
+/*
+ * particular_amd -- MI355X (gfx950) backend for particular's gravitational acceleration path.
+ *
+ * C ABI. Plain pointers and sizes only. Every entry point returns an int status (PNB_OK = 0);
+ * pnb_last_error() gives the message of the last failure on a context.
+ *
+ * Particle records are 16 bytes, {x, y, z, w} as 4 floats:
+ *   affecting particles: w = mu (standard gravitational parameter), the
+ *     GravitationalField<Vec3, f32> record of particular/src/gravity/mod.rs:12-18 and the
+ *     `Affecting` vec4 of the reference GPU layout (particular/src/gravity/impls/mod.rs:430-434);
+ *   affected particles: w ignored (the padded `Affected` vec3 of the same layout);
+ *   outputs: {ax, ay, az, 0} (the padded `Interaction` vec3).
+ *
+ * The `_dev` entry points take device pointers and are asynchronous on the context's stream;
+ * the others take host pointers and are synchronous on return, like the reference's
+ * gpu::BruteForce::compute (particular/src/gpu/mod.rs:186-207, which blocks on the result).
+ *
+ * Interaction parameters mirror particular's interaction types:
+ *   Acceleration::checked()                softening = 0, checked = 1
+ *     (particular/src/gravity/newtonian/acceleration.rs:27-33)
+ *   Acceleration::unchecked()              softening = 0, checked = 0
+ *   AccelerationSoftened::checked(eps)     softening = eps, checked = 1
+ *     (particular/src/gravity/newtonian/acceleration_softened.rs:30-37)
+ *   AccelerationSoftened::unchecked(eps)   softening = eps, checked = 0
+ * With checked = 1 a pair whose unsoftened squared distance is 0 contributes nothing
+ * (particular/src/gravity/impls/mod.rs:160-161).
+ *
+ * A context is used by one host thread at a time (the reference takes &mut GpuResources,
+ * particular/src/gpu/mod.rs:152).
+ */
+#ifndef PARTICULAR_AMD_H
+#define PARTICULAR_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PNB_OK 0
+#define PNB_E_ARG 1   /* invalid argument: null pointer with a non-zero length, bad theta, ... */
+#define PNB_E_NODEV 2 /* no usable GPU */
+#define PNB_E_OOM 3   /* device allocation failed */
+#define PNB_E_HIP 4   /* a HIP runtime call failed */
+#define PNB_E_DEPTH 5 /* octree deeper than the build supports (pathological input) */
+#define PNB_E_STATE 6 /* call out of order, e.g. a step loop that was never initialised */
+
+/* Kernel ids for pnb_profile_read. */
+#define PNB_PROF_BF 0          /* brute-force acceleration kernel */
+#define PNB_PROF_BH_BUILD 1    /* whole octree build (keys, sort, nodes, centres of mass) */
+#define PNB_PROF_BH_WALK 2     /* Barnes-Hut traversal kernel */
+#define PNB_PROF_DRIFT 3       /* leapfrog kick + drift kernel */
+#define PNB_PROF_BATCH 4       /* batched-ensemble force kernel */
+#define PNB_PROF_MLP 5         /* MLP training / inference kernels */
+#define PNB_PROF_SORT 6        /* radix sort inside the octree build */
+#define PNB_PROF_NUM_KERNELS 8
+
+typedef struct pnb_ctx pnb_ctx;
+
+/* ---- context: the GpuResources analogue (particular/src/gpu/mod.rs:84-143) ---------------- */
+
+/* Number of visible GPUs; returns PNB_E_NODEV (and *count = 0) when there are none. */
+int pnb_device_count(int *count);
+/* Creates a context on `device` with its own stream (default priority). Replaces the lazily
+ * initialised GpuResources::new / get_or_init (particular/src/gpu/mod.rs:96-142). */
+int pnb_create(int device, pnb_ctx **out);
+void pnb_destroy(pnb_ctx *ctx);
+const char *pnb_last_error(const pnb_ctx *ctx);
+const char *pnb_status_string(int status);
+/* Makes later calls launch on `hip_stream` (a hipStream_t; NULL is HIP's null stream, which is
+ * what a framework's default stream usually is). pnb_reset_stream returns to the context's own
+ * stream. Graph capture (use_graph) needs a non-null stream. */
+int pnb_set_stream(pnb_ctx *ctx, void *hip_stream);
+int pnb_reset_stream(pnb_ctx *ctx);
+/* The stream the context currently launches on, as a hipStream_t. */
+void *pnb_get_stream(pnb_ctx *ctx);
+int pnb_synchronize(pnb_ctx *ctx);
+
+/* Per-kernel device time, measured with HIP events on the context stream around each launch of
+ * the kernels named by PNB_PROF_*. Enabling resets the counters. */
+int pnb_profile_enable(pnb_ctx *ctx, int enable);
+int pnb_profile_read(pnb_ctx *ctx, int kernel_id, double *total_ms, uint64_t *launches);
+
+/* ---- brute force ----------------------------------------------------------------------- */
+
+/* Between(affected[0..n), affecting[0..m)) brute force. Replaces gpu::BruteForce::compute
+ * (particular/src/gpu/mod.rs:179-208) and matches sequential::BruteForce
+ * (particular/src/sequential.rs:181-194): out_i = sum_j pair(affected_i, affecting_j).
+ * m = 0 gives zeros, n = 0 is a no-op (particular/src/gpu/resources.rs:272-280). */
+int pnb_bf_accel_f32(pnb_ctx *ctx, const float *affected, size_t n, const float *affecting,
+                     size_t m, float softening, int checked, float *out);
+int pnb_bf_accel_f32_dev(pnb_ctx *ctx, const float *affected, size_t n, const float *affecting,
+                         size_t m, float softening, int checked, float *out);
+
+/* Reordered lowering (particular/src/storage.rs:219-229): the affecting set is the particles
+ * with mu != 0, in their original relative order (Ordered::new, storage.rs:85-95); every particle
+ * is affected and out follows the original order. */
+int pnb_reordered_bf_accel_f32(pnb_ctx *ctx, const float *particles, size_t n, float softening,
+                               int checked, float *out);
+int pnb_reordered_bf_accel_f32_dev(pnb_ctx *ctx, const float *particles, size_t n,
+                                   float softening, int checked, float *out);
+
+/* Brute-force kernel shape. variant: 0 = affecting tile staged in LDS, 1 = affecting particles
+ * read as wave-uniform scalar loads. per_lane: affected particles per lane (1, 2, 4, 8).
+ * j_split: affecting ranges per affected block (1..64, partial sums reduced in fixed order).
+ * Applies to problems with n >= n_min until replaced; pnb_bf_clear_tuning drops all entries. */
+int pnb_bf_set_tuning(pnb_ctx *ctx, size_t n_min, int variant, int per_lane, int j_split);
+int pnb_bf_clear_tuning(pnb_ctx *ctx);
+
+/* ---- accuracy judge -------------------------------------------------------------------- */
+
+/* FP64 direct sum on the GPU (device pointers): out3[3 i + k] = sum_j mu_j d / (|d|^2+eps^2)^1.5
+ * with pairs at zero distance skipped. */
+int pnb_direct_f64_dev(pnb_ctx *ctx, const float *affected, size_t n, const float *affecting,
+                       size_t m, double softening, double *out3);
+
+
+/* ---- leapfrog step loop ---------------------------------------------------------------------
+ * particular has no integrator (particular/README.md:22-25); its tests step with symplectic Euler
+ * around compute (particular/src/gravity/newtonian/mod.rs:318-330). This loop keeps the state
+ * on the GPU: kick-drift-kick leapfrog, FP64 positions and velocities, brute-force accelerations
+ * from an f32 shadow, affecting set = particles with mu != 0 (compacted once, original order). */
+
+/* Uploads pos3/vel3 (n x 3 doubles) and mu (n floats) and computes the initial accelerations. */
+int pnb_leapfrog_init(pnb_ctx *ctx, const double *pos3, const double *vel3, const float *mu,
+                      size_t n, float softening, int checked);
+/* Enqueues `steps` KDK steps (v += a dt/2; x += v dt; a = F(x); v += a dt/2) on the context
+ * stream; use_graph captures them once into a hipGraph and replays it. Asynchronous. */
+int pnb_leapfrog_run(pnb_ctx *ctx, size_t steps, double dt, int use_graph);
+/* Copies the state back (any pointer may be NULL): pos3/vel3 n x 3 doubles, acc4 n x 4 floats. */
+int pnb_leapfrog_read(pnb_ctx *ctx, double *pos3, double *vel3, float *acc4);
+/* Device views of the f32 particle shadow (n x 4), the compacted affecting set (m x 4) and the
+ * accelerations (n x 4); valid until the next init. */
+int pnb_leapfrog_views(pnb_ctx *ctx, const float **particles_dev, const float **affecting_dev,
+                       const float **acc_dev, size_t *n, size_t *m);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PARTICULAR_AMD_H */

@@ -1,0 +1,344 @@
+"""particular_amd: MI355X (gfx950) backend for particular's gravitational acceleration path.
+
+Host-side mirror of the reference interface, over the C ABI of ``libparticular_amd.so``:
+
+==============================  ==============================================================
+reference (particular 0.8)       here
+==============================  ==============================================================
+``Between(affected, affecting)``  :class:`Between`            (particular/src/lib.rs:299-300)
+``Interaction::compute``          ``.compute(storage)``       (particular/src/lib.rs:364-370)
+``Acceleration::checked()``       :meth:`Acceleration.checked` (newtonian/acceleration.rs:27-44)
+``AccelerationSoftened``          :class:`AccelerationSoftened` (acceleration_softened.rs:17-49)
+``Ordered`` / ``Reordered``       :class:`Ordered` / :class:`Reordered` (storage.rs:51-205)
+``GpuResources``                  :class:`HipResources`       (particular/src/gpu/mod.rs:84-143)
+``gpu::BruteForce``               :class:`BruteForce`         (particular/src/gpu/mod.rs:149-208)
+``parallel::BarnesHut``           :class:`BarnesHut`          (particular/src/parallel.rs:298-367)
+``GpuCompute`` sugar              :func:`hip_brute_force`, :func:`hip_barnes_hut` (gpu/mod.rs:13-37)
+==============================  ==============================================================
+
+Storages lower to ``Between`` exactly as the reference's blanket impls do (storage.rs:207-241):
+a particle array ``p`` -> ``Between(p, p)``; ``Ordered`` -> ``Between(particles, affecting)``;
+``Reordered`` -> ``Between(unordered, affecting)`` with outputs in the original order.
+
+Particles are ``(n, 4)`` float32 arrays ``{x, y, z, mu}`` (numpy on the host, or torch tensors on
+the GPU, which stay resident in HBM). Results are ``(n, 3)`` float32 numpy arrays for host input and
+``(n, 4)`` float32 torch tensors (w = 0) for device input.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+from typing import Any, Callable, NamedTuple
+
+import numpy as np
+
+from . import _lib
+from ._lib import PnbError, check
+
+__all__ = [
+    "Between", "Acceleration", "AccelerationSoftened", "Ordered", "Reordered", "HipResources",
+    "BruteForce", "BarnesHut", "hip_brute_force", "hip_barnes_hut", "PnbError", "device_count",
+]
+
+
+class Between(NamedTuple):
+    """A pair of storages: the first is affected by the second (particular/src/lib.rs:299-300)."""
+
+    affected: Any
+    affecting: Any
+
+
+@dataclass(frozen=True)
+class Acceleration:
+    """Newtonian acceleration between point masses (newtonian/acceleration.rs:17-44)."""
+
+    is_checked: bool = True
+    softening: float = 0.0
+
+    @staticmethod
+    def checked() -> "Acceleration":
+        return Acceleration(True)
+
+    @staticmethod
+    def unchecked() -> "Acceleration":
+        return Acceleration(False)
+
+
+@dataclass(frozen=True)
+class AccelerationSoftened:
+    """Newtonian acceleration with Plummer softening eps (acceleration_softened.rs:17-49)."""
+
+    softening: float
+    is_checked: bool = True
+
+    @staticmethod
+    def checked(softening: float) -> "AccelerationSoftened":
+        return AccelerationSoftened(float(softening), True)
+
+    @staticmethod
+    def unchecked(softening: float) -> "AccelerationSoftened":
+        return AccelerationSoftened(float(softening), False)
+
+
+def _is_affecting(particles) -> np.ndarray:
+    """GravitationalField::is_affecting: mu != 0 (particular/src/gravity/mod.rs:27-34)."""
+    return np.asarray(particles)[:, 3] != 0
+
+
+class Ordered:
+    """Affecting particles first, each group in its original order (storage.rs:51-138)."""
+
+    def __init__(self, particles: np.ndarray, affecting_len: int):
+        self._particles = particles
+        self._affecting_len = int(affecting_len)
+
+    @staticmethod
+    def new(unordered, is_affecting_fn: Callable | None = None) -> "Ordered":
+        unordered = _as_host(unordered)
+        mask = _is_affecting(unordered) if is_affecting_fn is None else np.asarray(
+            [bool(is_affecting_fn(p)) for p in unordered])
+        parts = np.concatenate([unordered[mask], unordered[~mask]])
+        return Ordered(np.ascontiguousarray(parts), int(mask.sum()))
+
+    def affecting_len(self) -> int:
+        return self._affecting_len
+
+    def affecting(self) -> np.ndarray:
+        return self._particles[: self._affecting_len]
+
+    def non_affecting(self) -> np.ndarray:
+        return self._particles[self._affecting_len:]
+
+    def particles(self) -> np.ndarray:
+        return self._particles
+
+
+class Reordered:
+    """The original particles plus an :class:`Ordered` copy (storage.rs:140-205).
+
+    With the default affecting test (mu != 0) the split of a device tensor is done on the GPU by
+    stream compaction instead of a host copy.
+    """
+
+    def __init__(self, unordered, is_affecting_fn: Callable | None = None):
+        self.unordered = unordered
+        self._fn = is_affecting_fn
+        self._ordered = None
+
+    @staticmethod
+    def new(unordered, is_affecting_fn: Callable | None = None) -> "Reordered":
+        return Reordered(unordered, is_affecting_fn)
+
+    def ordered(self) -> Ordered:
+        if self._ordered is None:
+            self._ordered = Ordered.new(self.unordered, self._fn)
+        return self._ordered
+
+    def affecting_len(self) -> int:
+        return self.ordered().affecting_len()
+
+    def affecting(self) -> np.ndarray:
+        return self.ordered().affecting()
+
+    def non_affecting(self) -> np.ndarray:
+        return self.ordered().non_affecting()
+
+    def reordered(self) -> np.ndarray:
+        return self.ordered().particles()
+
+
+def device_count() -> int:
+    n = ctypes.c_int(0)
+    rc = _lib.lib().pnb_device_count(ctypes.byref(n))
+    return n.value if rc == _lib.PNB_OK else 0
+
+
+class HipResources:
+    """Device resources of one GPU: a context with its stream and grow-only buffers.
+
+    The GpuResources analogue (particular/src/gpu/mod.rs:84-143): create it once and reuse it;
+    one host thread at a time.
+    """
+
+    def __init__(self, device: int = 0):
+        L = _lib.lib()
+        h = ctypes.c_void_p()
+        check(L.pnb_create(int(device), ctypes.byref(h)))
+        self.handle = h
+        self.device = int(device)
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            _lib.lib().pnb_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def check(self, status: int) -> None:
+        check(status, self.handle)
+
+    def synchronize(self) -> None:
+        self.check(_lib.lib().pnb_synchronize(self.handle))
+
+    def set_stream(self, stream_handle: int | None) -> None:
+        """Launch on the given hipStream_t (None or 0: HIP's null stream)."""
+        self.check(_lib.lib().pnb_set_stream(self.handle, stream_handle or None))
+
+    def reset_stream(self) -> None:
+        """Launch on the context's own stream again."""
+        self.check(_lib.lib().pnb_reset_stream(self.handle))
+
+    def use_torch_stream(self) -> None:
+        """Launch on torch's current stream, so device tensors need no extra synchronisation."""
+        import torch
+
+        self.set_stream(torch.cuda.current_stream(self.device).cuda_stream)
+
+    def profile(self, enable: bool = True) -> None:
+        self.check(_lib.lib().pnb_profile_enable(self.handle, int(enable)))
+
+    def profile_read(self, kernel_id: int) -> tuple[float, int]:
+        ms = ctypes.c_double()
+        cnt = ctypes.c_uint64()
+        self.check(_lib.lib().pnb_profile_read(self.handle, kernel_id, ctypes.byref(ms),
+                                               ctypes.byref(cnt)))
+        return ms.value, cnt.value
+
+    def set_bf_tuning(self, n_min: int, variant: int, per_lane: int, j_split: int) -> None:
+        self.check(_lib.lib().pnb_bf_set_tuning(self.handle, n_min, variant, per_lane, j_split))
+
+    def clear_bf_tuning(self) -> None:
+        self.check(_lib.lib().pnb_bf_clear_tuning(self.handle))
+
+
+# ---- storage handling --------------------------------------------------------------------------
+
+def _is_torch(x) -> bool:
+    return type(x).__module__.startswith("torch")
+
+
+def _as_host(x) -> np.ndarray:
+    if _is_torch(x):
+        x = x.detach().cpu().numpy()
+    a = np.ascontiguousarray(x, dtype=np.float32)
+    if a.ndim != 2 or a.shape[1] != 4:
+        raise ValueError("particles must be an (n, 4) float32 array {x, y, z, mu}")
+    return a
+
+
+def _check_dev(t):
+    import torch
+
+    if t.dtype != torch.float32 or t.dim() != 2 or t.shape[1] != 4 or not t.is_cuda:
+        raise ValueError("device particles must be a CUDA float32 tensor of shape (n, 4)")
+    return t.contiguous()
+
+
+def _lower(storage):
+    """Lower a storage to (kind, affected, affecting) following storage.rs:207-241."""
+    if isinstance(storage, Between):
+        return "between", storage.affected, storage.affecting
+    if isinstance(storage, Reordered):
+        if storage._fn is None:
+            return "reordered", storage.unordered, None
+        return "between", storage.unordered, storage.affecting()
+    if isinstance(storage, Ordered):
+        return "between", storage.particles(), storage.affecting()
+    return "between", storage, storage
+
+
+class _Algorithm:
+    def __init__(self, resources: HipResources, interaction):
+        if not isinstance(interaction, (Acceleration, AccelerationSoftened)):
+            raise TypeError("interaction must be Acceleration or AccelerationSoftened")
+        self.resources = resources
+        self.interaction = interaction
+
+    def _params(self):
+        return float(self.interaction.softening), int(self.interaction.is_checked)
+
+    def compute(self, storage):
+        kind, affected, affecting = _lower(storage)
+        dev = _is_torch(affected) and affected.is_cuda
+        if dev:
+            import torch
+
+            affected = _check_dev(affected)
+            out = torch.empty((affected.shape[0], 4), dtype=torch.float32, device=affected.device)
+            self.resources.use_torch_stream()
+            if kind == "reordered":
+                self._reordered_dev(affected, out)
+            else:
+                affecting = _check_dev(affecting)
+                self._between_dev(affected, affecting, out)
+            return out
+        affected = _as_host(affected)
+        out = np.zeros((affected.shape[0], 4), np.float32)
+        if kind == "reordered":
+            self._reordered_host(affected, out)
+        else:
+            self._between_host(affected, _as_host(affecting), out)
+        return out[:, :3]
+
+
+class BruteForce(_Algorithm):
+    """Brute force on the GPU: replaces gpu::BruteForce (particular/src/gpu/mod.rs:149-208).
+
+    Per affected particle, the sum over every affecting particle of the pair interaction, as
+    sequential::BruteForce computes it (particular/src/sequential.rs:181-194).
+    """
+
+    def _between_host(self, a, b, out):
+        eps, chk = self._params()
+        self.resources.check(_lib.lib().pnb_bf_accel_f32(
+            self.resources.handle, a.ctypes.data, a.shape[0], b.ctypes.data, b.shape[0], eps,
+            chk, out.ctypes.data))
+
+    def _reordered_host(self, a, out):
+        eps, chk = self._params()
+        self.resources.check(_lib.lib().pnb_reordered_bf_accel_f32(
+            self.resources.handle, a.ctypes.data, a.shape[0], eps, chk, out.ctypes.data))
+
+    def _between_dev(self, a, b, out):
+        eps, chk = self._params()
+        self.resources.check(_lib.lib().pnb_bf_accel_f32_dev(
+            self.resources.handle, a.data_ptr(), a.shape[0], b.data_ptr(), b.shape[0], eps, chk,
+            out.data_ptr()))
+
+    def _reordered_dev(self, a, out):
+        eps, chk = self._params()
+        self.resources.check(_lib.lib().pnb_reordered_bf_accel_f32_dev(
+            self.resources.handle, a.data_ptr(), a.shape[0], eps, chk, out.data_ptr()))
+
+
+class BarnesHut(_Algorithm):
+    """Barnes-Hut on the GPU: replaces parallel::BarnesHut (particular/src/parallel.rs:298-367)."""
+
+    def __init__(self, resources: HipResources, theta: float, interaction, mlp: bool = False):
+        super().__init__(resources, interaction)
+        self.theta = float(theta)
+        self.mlp = bool(mlp)
+
+    def _between_host(self, a, b, out):
+        raise NotImplementedError("Barnes-Hut host path not built yet")
+
+    _reordered_host = _between_dev = _reordered_dev = _between_host
+
+
+def hip_brute_force(storage, resources: HipResources, interaction):
+    """GpuCompute::gpu_brute_force sugar (particular/src/gpu/mod.rs:19-30)."""
+    return BruteForce(resources, interaction).compute(storage)
+
+
+def hip_barnes_hut(storage, resources: HipResources, theta: float, interaction):
+    return BarnesHut(resources, theta, interaction).compute(storage)
