@@ -108,6 +108,52 @@ int pnb_reordered_bf_accel_f32_dev(pnb_ctx *ctx, const float *particles, size_t 
 int pnb_bf_set_tuning(pnb_ctx *ctx, size_t n_min, int variant, int per_lane, int j_split);
 int pnb_bf_clear_tuning(pnb_ctx *ctx);
 
+/* ---- Barnes-Hut -----------------------------------------------------------------------------
+ * Replaces parallel::BarnesHut (particular/src/parallel.rs:342-367): the octree is built from
+ * the affecting particles exactly as RootedOrthtree::new builds it (particular/src/storage.rs:20-33,
+ * particular/src/tree/mod.rs:73-139, same f32 box arithmetic, same cells, same pre-order), with
+ * the centre of mass of every node reduced in FP64; every affected particle then applies the
+ * traversal rule of sequential::BarnesHut (particular/src/sequential.rs:465-505): skip a node at
+ * zero distance, open an internal node when (theta*theta)*d^2 < width^2, else add its pair term.
+ * theta = 0 degenerates to brute force over the leaves (sequential.rs:441-442). `checked` is
+ * accepted for symmetry with the reference signature; the traversal's zero-distance skip makes
+ * both forms identical. flags: 0 (reserved for PNB_BH_* options). */
+#define PNB_BH_MLP 1u /* add the learned far-field correction of each accepted node */
+
+int pnb_bh_accel_f32(pnb_ctx *ctx, const float *affected, size_t n, const float *affecting,
+                     size_t m, float theta, float softening, int checked, unsigned flags,
+                     float *out);
+int pnb_bh_accel_f32_dev(pnb_ctx *ctx, const float *affected, size_t n, const float *affecting,
+                         size_t m, float theta, float softening, int checked, unsigned flags,
+                         float *out);
+/* Reordered lowering (particular/src/storage.rs:219-229): tree from the particles with mu != 0. */
+int pnb_reordered_bh_accel_f32(pnb_ctx *ctx, const float *particles, size_t n, float theta,
+                               float softening, int checked, unsigned flags, float *out);
+int pnb_reordered_bh_accel_f32_dev(pnb_ctx *ctx, const float *particles, size_t n, float theta,
+                                   float softening, int checked, unsigned flags, float *out);
+
+/* The last octree built on a context, for inspection (the reference exposes its tree through
+ * RootedOrthtree::get, particular/src/storage.rs:41-45). Nodes are in the reference's order
+ * (depth-first pre-order, children in orthant order). */
+typedef struct {
+    float com[3];     /* centre of mass, f32 (rounded from the FP64 reduction) */
+    float mass;       /* total mu, f32 */
+    double mass64;    /* total mu, FP64 */
+    float width;      /* x extent of the node's box (partition.rs:183-188) */
+    uint32_t skip;    /* index of the first node after this node's subtree */
+    uint32_t begin;   /* first particle of the node, in tree order */
+    uint32_t count;   /* particles in the node */
+    uint32_t level;   /* depth below the root */
+    uint32_t internal;/* 1 for Node::Internal, 0 for Node::External */
+} pnb_tree_node;
+
+int pnb_bh_tree_size(pnb_ctx *ctx, size_t *nodes, int *max_level);
+/* Copies the nodes (cap >= nodes) and, if `order` is not NULL, the original index of every
+ * affecting particle in tree order (m >= affecting count). */
+int pnb_bh_tree_export(pnb_ctx *ctx, pnb_tree_node *nodes, size_t cap, uint32_t *order, size_t m);
+/* Counters of the walks since the last build: wavefront node steps and accepted pair terms. */
+int pnb_bh_stats(pnb_ctx *ctx, uint64_t *wave_steps, uint64_t *accepted);
+
 /* ---- accuracy judge -------------------------------------------------------------------- */
 
 /* FP64 direct sum on the GPU (device pointers): out3[3 i + k] = sum_j mu_j d / (|d|^2+eps^2)^1.5
@@ -134,6 +180,26 @@ int pnb_leapfrog_read(pnb_ctx *ctx, double *pos3, double *vel3, float *acc4);
  * accelerations (n x 4); valid until the next init. */
 int pnb_leapfrog_views(pnb_ctx *ctx, const float **particles_dev, const float **affecting_dev,
                        const float **acc_dev, size_t *n, size_t *m);
+
+/* ---- batched ensemble -----------------------------------------------------------------------
+ * Many independent systems of uneven sizes stepped together; no reference counterpart (the
+ * reference computes one storage per call). System k owns particles [offsets[k], offsets[k+1])
+ * and is its own Between(system, system) brute force; all systems advance with the KDK leapfrog
+ * of the step loop above. Each step's force pass is one persistent launch over a queue of
+ * (system, tile) items; tile shapes per system-size bucket come from pnb_batch_set_tuning. */
+
+/* Affected bodies per lane (1, 2, 4, 8) for systems with at least n_min bodies. */
+int pnb_batch_set_tuning(pnb_ctx *ctx, size_t n_min, int per_lane);
+/* Uploads the ensemble (pos3/vel3: total x 3 doubles, mu: total floats, offsets: num_systems+1
+ * int64 starting at 0) and computes the initial accelerations. */
+int pnb_batch_init(pnb_ctx *ctx, const double *pos3, const double *vel3, const float *mu,
+                   const int64_t *offsets, size_t num_systems, float softening, int checked);
+/* Enqueues `steps` KDK steps for every system; use_graph captures them once into a hipGraph. */
+int pnb_batch_run(pnb_ctx *ctx, size_t steps, double dt, int use_graph);
+int pnb_batch_read(pnb_ctx *ctx, double *pos3, double *vel3, float *acc4);
+/* Particle count, work items per force pass and pair interactions per step (sum of N_k^2). */
+int pnb_batch_info(pnb_ctx *ctx, size_t *particles, size_t *items,
+                   unsigned long long *interactions_per_step);
 
 #ifdef __cplusplus
 }

@@ -322,17 +322,71 @@ class BruteForce(_Algorithm):
 
 
 class BarnesHut(_Algorithm):
-    """Barnes-Hut on the GPU: replaces parallel::BarnesHut (particular/src/parallel.rs:298-367)."""
+    """Barnes-Hut on the GPU: replaces parallel::BarnesHut (particular/src/parallel.rs:298-367).
+
+    The octree is built from the affecting particles node for node as the reference builds it
+    (FP64 centres of mass); each affected particle applies the reference's traversal rule
+    (particular/src/sequential.rs:465-505). ``theta = 0`` degenerates to brute force.
+    """
 
     def __init__(self, resources: HipResources, theta: float, interaction, mlp: bool = False):
         super().__init__(resources, interaction)
         self.theta = float(theta)
         self.mlp = bool(mlp)
 
-    def _between_host(self, a, b, out):
-        raise NotImplementedError("Barnes-Hut host path not built yet")
+    def _flags(self) -> int:
+        return 1 if self.mlp else 0
 
-    _reordered_host = _between_dev = _reordered_dev = _between_host
+    def _between_host(self, a, b, out):
+        # Between(p, p) over one array passes one pointer twice; the library then uploads one
+        # copy and reuses the tree order for the affected particles.
+        eps, chk = self._params()
+        self.resources.check(_lib.lib().pnb_bh_accel_f32(
+            self.resources.handle, a.ctypes.data, a.shape[0], b.ctypes.data, b.shape[0],
+            self.theta, eps, chk, self._flags(), out.ctypes.data))
+
+    def _reordered_host(self, a, out):
+        eps, chk = self._params()
+        self.resources.check(_lib.lib().pnb_reordered_bh_accel_f32(
+            self.resources.handle, a.ctypes.data, a.shape[0], self.theta, eps, chk, self._flags(),
+            out.ctypes.data))
+
+    def _between_dev(self, a, b, out):
+        eps, chk = self._params()
+        self.resources.check(_lib.lib().pnb_bh_accel_f32_dev(
+            self.resources.handle, a.data_ptr(), a.shape[0], b.data_ptr(), b.shape[0],
+            self.theta, eps, chk, self._flags(), out.data_ptr()))
+
+    def _reordered_dev(self, a, out):
+        eps, chk = self._params()
+        self.resources.check(_lib.lib().pnb_reordered_bh_accel_f32_dev(
+            self.resources.handle, a.data_ptr(), a.shape[0], self.theta, eps, chk,
+            self._flags(), out.data_ptr()))
+
+
+def tree_export(resources: HipResources, m: int | None = None):
+    """Nodes of the last octree built on `resources` (pre-order) and the tree particle order."""
+    L = _lib.lib()
+    nodes = ctypes.c_size_t()
+    lvl = ctypes.c_int()
+    resources.check(L.pnb_bh_tree_size(resources.handle, ctypes.byref(nodes), ctypes.byref(lvl)))
+    arr = np.zeros(nodes.value, _lib.tree_node_dtype())
+    order = np.zeros(m if m is not None else 0, np.uint32)
+    resources.check(L.pnb_bh_tree_export(resources.handle, arr.ctypes.data, arr.shape[0],
+                                         order.ctypes.data if order.size else None, order.size))
+    return arr, order
+
+
+def bh_stats(resources: HipResources) -> tuple[int, int]:
+    """(wavefront node steps, accepted pair terms) of the walks since the last build."""
+    a = ctypes.c_uint64()
+    b = ctypes.c_uint64()
+    resources.check(_lib.lib().pnb_bh_stats(resources.handle, ctypes.byref(a), ctypes.byref(b)))
+    return a.value, b.value
+
+
+def _bh_available() -> bool:
+    return hasattr(_lib.lib(), "pnb_bh_accel_f32")
 
 
 def hip_brute_force(storage, resources: HipResources, interaction):

@@ -59,12 +59,35 @@ SIGNATURES = {
     "pnb_bf_set_tuning": (_i, [_vp, _sz, _i, _i, _i]),
     "pnb_bf_clear_tuning": (_i, [_vp]),
     "pnb_direct_f64_dev": (_i, [_vp, _vp, _sz, _vp, _sz, _d, _vp]),
+    "pnb_bh_accel_f32": (_i, [_vp, _vp, _sz, _vp, _sz, _f, _f, _i, _u32, _vp]),
+    "pnb_bh_accel_f32_dev": (_i, [_vp, _vp, _sz, _vp, _sz, _f, _f, _i, _u32, _vp]),
+    "pnb_reordered_bh_accel_f32": (_i, [_vp, _vp, _sz, _f, _f, _i, _u32, _vp]),
+    "pnb_reordered_bh_accel_f32_dev": (_i, [_vp, _vp, _sz, _f, _f, _i, _u32, _vp]),
+    "pnb_bh_tree_size": (_i, [_vp, _c.POINTER(_sz), _pi]),
+    "pnb_bh_tree_export": (_i, [_vp, _vp, _sz, _vp, _sz]),
+    "pnb_bh_stats": (_i, [_vp, _pu64, _pu64]),
+    "pnb_batch_set_tuning": (_i, [_vp, _sz, _i]),
+    "pnb_batch_init": (_i, [_vp, _vp, _vp, _vp, _vp, _sz, _f, _i]),
+    "pnb_batch_run": (_i, [_vp, _sz, _d, _i]),
+    "pnb_batch_read": (_i, [_vp, _vp, _vp, _vp]),
+    "pnb_batch_info": (_i, [_vp, _c.POINTER(_sz), _c.POINTER(_sz), _c.POINTER(_c.c_ulonglong)]),
     "pnb_leapfrog_init": (_i, [_vp, _vp, _vp, _vp, _sz, _f, _i]),
     "pnb_leapfrog_run": (_i, [_vp, _sz, _d, _i]),
     "pnb_leapfrog_read": (_i, [_vp, _vp, _vp, _vp]),
     "pnb_leapfrog_views": (_i, [_vp, _c.POINTER(_vp), _c.POINTER(_vp), _c.POINTER(_vp),
                                 _c.POINTER(_sz), _c.POINTER(_sz)]),
 }
+
+TREE_NODE_DTYPE = None  # numpy dtype of pnb_tree_node, see tree_node_dtype()
+
+
+def tree_node_dtype():
+    import numpy as np
+
+    return np.dtype([("com", "<f4", (3,)), ("mass", "<f4"), ("mass64", "<f8"), ("width", "<f4"),
+                     ("skip", "<u4"), ("begin", "<u4"), ("count", "<u4"), ("level", "<u4"),
+                     ("internal", "<u4")], align=True)
+
 
 _lib = None
 
@@ -83,6 +106,13 @@ def lib() -> ctypes.CDLL:
             raise ImportError(
                 f"{LIB_PATH} is missing: the HIP extension has not been built "
                 "(run `python -m particular_amd.build`); there is no CPU fallback")
+        # torch ships its own copy of the HIP runtime under the same soname
+        # (libamdhip64.so.7). Importing torch first lets the loader bind this library to that
+        # copy, so the process holds one HIP runtime and both can use the GPU.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name)

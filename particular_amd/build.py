@@ -26,6 +26,7 @@ HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 # Packed FP32 VALU ops are disabled on purpose (they are not generated from the float code).
 FLAGS = [
     f"--offload-arch={ARCH}",
+    "-mcode-object-version=5",
     "-O3",
     "-std=c++17",
     "-fPIC",
@@ -60,7 +61,8 @@ def build(force: bool = False, verbose: bool = False) -> str:
     with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
         objs = list(ex.map(lambda s: _compile(s, force, verbose), srcs))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-mcode-object-version=5", "-shared", "-fPIC", "-o",
+           tmp, *objs]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)

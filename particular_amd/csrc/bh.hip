@@ -1,7 +1,798 @@
-// Placeholder translation unit; filled in by the subsystem that owns BhState.
+// Barnes-Hut on gfx950: the reference's octree rebuilt in parallel, node for node, plus a
+// wavefront-coherent stackless traversal with the reference's per-particle acceptance rule.
+//
+// Reference behaviour (particular/src/tree/mod.rs:73-139, partition.rs:136-220,
+// sequential.rs:456-506):
+//  * root box = square_with(min/max of the coordinates), in f32;
+//  * a node subdivides when its particles do not all share one position; child index bit a is
+//    (pos[a] < centre[a]) with centre = (min + max) / 2 of the node box, in f32; bit 0 takes
+//    [centre, max], bit 1 takes [min, centre]; children are built depth-first in orthant order,
+//    so node ids are a pre-order and the particles of a node are contiguous in "tree order";
+//  * node data = centre of mass of its particles; traversal skips a node at zero distance,
+//    opens an internal node when (theta^2 d^2) < width^2 (width = x extent of its box), and
+//    otherwise adds the node's pair term.
+//
+// Build (all on the GPU, no recursion):
+//  1. bbox reduction and the root box (same f32 operations as square_with);
+//  2. per particle, its path through the tree as a 63-bit key: 21 levels of the same f32 midpoint
+//     descent, 3 bits per level (the child index). Sorting by key (stable) gives tree order;
+//  3. for adjacent particles p-1, p: lcp[p] = number of leading levels they share (or "same
+//     position"). Runs that share all 21 levels but hold distinct positions are refined with
+//     further 21-level keys until separated;
+//  4. with runs of identical positions [r0, r1): the deepest internal level around r0 is
+//     I = max(lcp[r0], lcp[r1]); the nodes starting at r0 are levels lcp[r0]+1 .. I+1 (the last a
+//     leaf). Counting them per particle and scanning gives every node its pre-order index; a
+//     node's subtree ends at the first particle with lcp < level, and skip = index of the first
+//     node starting there;
+//  5. FP64 mass / mass-weighted position sums: leaves directly, internal nodes from their
+//     children, one launch per level from the deepest up; node records get the f32 centre of mass.
+// Traversal: one wavefront walks the pre-order node array with one shared cursor. A lane that
+// opens node k lets the wave step to k+1; a lane that accepts or skips k waits until the cursor
+// reaches skip[k]. Each lane therefore applies exactly the reference's decisions; node records
+// are wave-uniform, so they arrive through scalar loads.
 #include "pnb_internal.hpp"
 
+#include <algorithm>
+#include <cmath>
+
 namespace pnb {
-struct BhState {};
-void bh_free(BhState* s) { delete s; }
+
+int radix_sort_pairs(pnb_ctx* c, uint64_t* keys, uint32_t* vals, uint64_t* keys_alt,
+                     uint32_t* vals_alt, size_t n, int key_bits, DevBuf& hist_buf,
+                     DevBuf& bits_buf, uint64_t** keys_out, uint32_t** vals_out);
+int split_affecting(pnb_ctx* c, const float4* particles, size_t n, float4* dst, uint32_t* m_host);
+
+constexpr int KEY_LEVELS = 21;
+constexpr int LCP_SAME = 0x3fffffff;  // adjacent particles at the same position
+constexpr int MAX_LEVEL = 252;        // 12 rounds of 21 levels
+constexpr int DEEP_STACK = 24;
+
+constexpr uint32_t FLAG_NONFINITE = 1u;
+constexpr uint32_t FLAG_DEPTH = 2u;
+
+struct __align__(32) NodeRec {
+    float cx, cy, cz, mass;  // centre of mass (f32) and total mu
+    float w2;                // width^2 of an internal node; -inf for a leaf (never opened)
+    uint32_t skip;           // pre-order index of the first node after this subtree
+    uint32_t begin, end;     // particle range in tree order
+};
+static_assert(sizeof(NodeRec) == 32, "node record is 32 bytes");
+
+struct BhInfo {
+    float mn[3], mx[3];  // root box
+    uint32_t flags;
+    int max_level;
+    uint32_t total;  // node count (S[M])
+    uint32_t pad;
+    unsigned long long steps, accepted;  // traversal counters of the last walk
+};
+
+struct BhState {
+    size_t m = 0;
+    size_t nodes = 0;
+    int max_level = -1;
+    bool built = false;
+    const uint64_t* skeys = nullptr;  // sorted keys (points into keys or keys_alt)
+    const uint32_t* sidx = nullptr;   // tree order -> original index
+    DevBuf keys, keys_alt, idx, idx_alt, sp, lcp, cnt, ip, key2, rec, sums, sums2, level, width;
+    DevBuf info, bbox_part, hist, bits, afg_tmp;
+    DevBuf akeys, akeys_alt, aidx, aidx_alt, apts, h_aff, h_afg, h_out;
+    float last_theta = 0.f;
+    unsigned long long last_steps = 0, last_accepted = 0;
+};
+
+void bh_free(BhState* s) {
+    if (!s) return;
+    for (DevBuf* b : {&s->keys, &s->keys_alt, &s->idx, &s->idx_alt, &s->sp, &s->lcp, &s->cnt,
+                      &s->ip, &s->key2, &s->rec, &s->sums, &s->sums2, &s->level, &s->width,
+                      &s->info, &s->bbox_part, &s->hist, &s->bits, &s->afg_tmp, &s->akeys,
+                      &s->akeys_alt, &s->aidx, &s->aidx_alt, &s->apts, &s->h_aff, &s->h_afg,
+                      &s->h_out})
+        b->release();
+    delete s;
+}
+
+// ---- f32 helpers with the reference's rounding (no contraction) ------------------------------
+
+__device__ __forceinline__ float midpoint(float a, float b) { return __fmul_rn(__fadd_rn(a, b), 0.5f); }
+
+// glam Vec3::length_squared: (x*x + y*y) + z*z
+__device__ __forceinline__ float norm2_ref(float x, float y, float z) {
+    return __fadd_rn(__fadd_rn(__fmul_rn(x, x), __fmul_rn(y, y)), __fmul_rn(z, z));
+}
+
+__device__ __forceinline__ uint64_t descend_key(float x, float y, float z, float* mn, float* mx,
+                                                int levels) {
+    uint64_t key = 0;
+    for (int l = 0; l < levels; ++l) {
+        const float cx = midpoint(mn[0], mx[0]), cy = midpoint(mn[1], mx[1]),
+                    cz = midpoint(mn[2], mx[2]);
+        const uint32_t bx = x < cx, by = y < cy, bz = z < cz;
+        if (bx) mx[0] = cx; else mn[0] = cx;
+        if (by) mx[1] = cy; else mn[1] = cy;
+        if (bz) mx[2] = cz; else mn[2] = cz;
+        key = (key << 3) | (bx | (by << 1) | (bz << 2));
+    }
+    return key;
+}
+
+__device__ __forceinline__ bool same_pos(const float4& a, const float4& b) {
+    return a.x == b.x && a.y == b.y && a.z == b.z;
+}
+
+// ---- 1. bounding box -------------------------------------------------------------------------
+
+constexpr int BB_BLOCKS = 512;
+
+__global__ __launch_bounds__(256) void bh_bbox_kernel(const float4* __restrict__ p, size_t m,
+                                                      float* __restrict__ part,
+                                                      BhInfo* __restrict__ info) {
+    float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    bool bad = false;
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < m;
+         i += size_t(gridDim.x) * blockDim.x) {
+        const float4 q = p[i];
+        bad |= !(isfinite(q.x) && isfinite(q.y) && isfinite(q.z));
+        mn[0] = fminf(mn[0], q.x); mx[0] = fmaxf(mx[0], q.x);
+        mn[1] = fminf(mn[1], q.y); mx[1] = fmaxf(mx[1], q.y);
+        mn[2] = fminf(mn[2], q.z); mx[2] = fmaxf(mx[2], q.z);
+    }
+    for (int off = 32; off > 0; off >>= 1)
+        for (int a = 0; a < 3; ++a) {
+            mn[a] = fminf(mn[a], __shfl_xor(mn[a], off, 64));
+            mx[a] = fmaxf(mx[a], __shfl_xor(mx[a], off, 64));
+        }
+    __shared__ float red[4][6];
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+        for (int a = 0; a < 3; ++a) { red[w][a] = mn[a]; red[w][3 + a] = mx[a]; }
+    if (bad) atomicOr(&info->flags, FLAG_NONFINITE);
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        float v = red[0][threadIdx.x];
+        for (int ww = 1; ww < 4; ++ww)
+            v = threadIdx.x < 3 ? fminf(v, red[ww][threadIdx.x]) : fmaxf(v, red[ww][threadIdx.x]);
+        part[blockIdx.x * 6 + threadIdx.x] = v;
+    }
+}
+
+// square_with (partition.rs:136-153): centre = (min+max)/2, half = max(0, sizes...)/2,
+// box = centre -+ half, all in f32.
+__global__ void bh_root_kernel(const float* __restrict__ part, int blocks, BhInfo* __restrict__ info) {
+    if (threadIdx.x != 0) return;
+    float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int b = 0; b < blocks; ++b)
+        for (int a = 0; a < 3; ++a) {
+            mn[a] = fminf(mn[a], part[b * 6 + a]);
+            mx[a] = fmaxf(mx[a], part[b * 6 + 3 + a]);
+        }
+    float centre[3], big = 0.0f;
+    for (int a = 0; a < 3; ++a) {
+        centre[a] = midpoint(mn[a], mx[a]);
+        big = fmaxf(big, __fsub_rn(mx[a], mn[a]));
+    }
+    const float half = midpoint(big, 0.0f);
+    for (int a = 0; a < 3; ++a) {
+        info->mn[a] = __fsub_rn(centre[a], half);
+        info->mx[a] = __fadd_rn(centre[a], half);
+    }
+}
+
+__global__ void bh_info_init_kernel(BhInfo* info) {
+    info->flags = 0;
+    info->max_level = -1;
+    info->total = 0;
+    info->steps = 0;
+    info->accepted = 0;
+}
+
+// ---- 2. keys ------------------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void bh_keys_kernel(const float4* __restrict__ p, size_t m,
+                                                      const BhInfo* __restrict__ info,
+                                                      uint64_t* __restrict__ keys,
+                                                      uint32_t* __restrict__ idx) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const float4 q = p[i];
+    float mn[3] = {info->mn[0], info->mn[1], info->mn[2]};
+    float mx[3] = {info->mx[0], info->mx[1], info->mx[2]};
+    keys[i] = descend_key(q.x, q.y, q.z, mn, mx, KEY_LEVELS);
+    idx[i] = uint32_t(i);
+}
+
+__global__ __launch_bounds__(256) void bh_gather_kernel(const float4* __restrict__ src, size_t m,
+                                                        const uint32_t* __restrict__ idx,
+                                                        float4* __restrict__ dst) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < m) dst[i] = src[idx[i]];
+}
+
+// ---- 3. common prefixes and deep runs -----------------------------------------------------------
+
+__global__ __launch_bounds__(256) void bh_lcp_kernel(const uint64_t* __restrict__ keys,
+                                                     const float4* __restrict__ sp, size_t m,
+                                                     int* __restrict__ lcp) {
+    const size_t p = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p >= m) return;
+    if (p == 0) {
+        lcp[0] = -1;
+        return;
+    }
+    const uint64_t a = keys[p - 1], b = keys[p];
+    int v;
+    if (a != b) v = (__clzll(a ^ b) - 1) / 3;
+    else v = same_pos(sp[p - 1], sp[p]) ? LCP_SAME : KEY_LEVELS;
+    lcp[p] = v;
+}
+
+// One thread per run of equal 21-level keys that holds distinct positions: extend the keys by
+// further 21-level rounds from the run's box and insertion-sort the run (stable), until every
+// adjacent pair is either separated or at the same position.
+__global__ __launch_bounds__(256) void bh_deep_kernel(const uint64_t* __restrict__ keys, size_t m,
+                                                      float4* __restrict__ sp,
+                                                      uint32_t* __restrict__ idx,
+                                                      int* __restrict__ lcp,
+                                                      uint64_t* __restrict__ key2,
+                                                      BhInfo* __restrict__ info) {
+    const size_t p = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p + 1 >= m) return;
+    const uint64_t k = keys[p];
+    if (keys[p + 1] != k || (p > 0 && keys[p - 1] == k)) return;
+    size_t r1 = p + 1;
+    bool deep = false;
+    while (r1 < m && keys[r1] == k) {
+        deep |= lcp[r1] == KEY_LEVELS;
+        ++r1;
+    }
+    if (!deep) return;
+
+    struct Work {
+        uint32_t b, e;
+        int base;
+        float mn[3], mx[3];
+    };
+    Work st[DEEP_STACK];
+    int top = 0;
+    {
+        Work w;
+        w.b = uint32_t(p);
+        w.e = uint32_t(r1);
+        w.base = KEY_LEVELS;
+        for (int a = 0; a < 3; ++a) { w.mn[a] = info->mn[a]; w.mx[a] = info->mx[a]; }
+        const float4 q = sp[p];
+        descend_key(q.x, q.y, q.z, w.mn, w.mx, KEY_LEVELS);
+        st[top++] = w;
+    }
+    while (top > 0) {
+        Work w = st[--top];
+        if (w.base + KEY_LEVELS > MAX_LEVEL) {
+            atomicOr(&info->flags, FLAG_DEPTH);
+            return;
+        }
+        for (uint32_t q = w.b; q < w.e; ++q) {
+            const float4 v = sp[q];
+            float mn[3] = {w.mn[0], w.mn[1], w.mn[2]}, mx[3] = {w.mx[0], w.mx[1], w.mx[2]};
+            key2[q] = descend_key(v.x, v.y, v.z, mn, mx, KEY_LEVELS);
+        }
+        for (uint32_t q = w.b + 1; q < w.e; ++q) {  // stable insertion sort by key2
+            const uint64_t kk = key2[q];
+            const float4 pv = sp[q];
+            const uint32_t iv = idx[q];
+            uint32_t r = q;
+            while (r > w.b && key2[r - 1] > kk) {
+                key2[r] = key2[r - 1];
+                sp[r] = sp[r - 1];
+                idx[r] = idx[r - 1];
+                --r;
+            }
+            key2[r] = kk;
+            sp[r] = pv;
+            idx[r] = iv;
+        }
+        uint32_t q = w.b;
+        while (q < w.e) {
+            const uint32_t s = q++;
+            bool sub_deep = false;
+            while (q < w.e && key2[q] == key2[s]) {
+                const bool same = same_pos(sp[q - 1], sp[q]);
+                lcp[q] = same ? LCP_SAME : w.base + KEY_LEVELS;
+                sub_deep |= !same;
+                ++q;
+            }
+            if (q < w.e) lcp[q] = w.base + (__clzll(key2[q - 1] ^ key2[q]) - 1) / 3;
+            if (sub_deep) {
+                if (top == DEEP_STACK) {
+                    atomicOr(&info->flags, FLAG_DEPTH);
+                    return;
+                }
+                Work c;
+                c.b = s;
+                c.e = q;
+                c.base = w.base + KEY_LEVELS;
+                for (int a = 0; a < 3; ++a) { c.mn[a] = w.mn[a]; c.mx[a] = w.mx[a]; }
+                const float4 v = sp[s];
+                descend_key(v.x, v.y, v.z, c.mn, c.mx, KEY_LEVELS);
+                st[top++] = c;
+            }
+        }
+    }
+}
+
+// ---- 4. node counts, pre-order indices, node records --------------------------------------------
+
+__global__ __launch_bounds__(256) void bh_count_kernel(const int* __restrict__ lcp, size_t m,
+                                                       uint32_t* __restrict__ cnt,
+                                                       int* __restrict__ ip,
+                                                       BhInfo* __restrict__ info) {
+    const size_t p = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p >= m) return;
+    const int lp = lcp[p];  // -1 at p = 0
+    if (lp == LCP_SAME) {
+        cnt[p] = 0;
+        return;
+    }
+    size_t r1 = p + 1;
+    while (r1 < m && lcp[r1] == LCP_SAME) ++r1;
+    const int nxt = r1 < m ? lcp[r1] : -1;
+    const int I = max(lp, nxt);
+    cnt[p] = uint32_t(I + 1 - lp);
+    ip[p] = I;
+    atomicMax(&info->max_level, I + 1);
+    if (I + 1 > MAX_LEVEL) atomicOr(&info->flags, FLAG_DEPTH);
+}
+
+__device__ __forceinline__ bool same_prefix(uint64_t a, uint64_t b, int level) {
+    return level == 0 || (a >> (63 - 3 * level)) == (b >> (63 - 3 * level));
+}
+
+// One thread per particle; run starts (lcp != LCP_SAME) write the nodes that start there.
+// S is the exclusive scan of the per-particle node counts (m + 1 entries).
+__global__ __launch_bounds__(256) void bh_node_kernel(
+    const uint64_t* __restrict__ keys, const float4* __restrict__ sp, const int* __restrict__ lcp,
+    const int* __restrict__ ip, const uint32_t* __restrict__ S, size_t m,
+    const BhInfo* __restrict__ info, NodeRec* __restrict__ rec, double4* __restrict__ sums,
+    double4* __restrict__ sums2, uint8_t* __restrict__ level, float* __restrict__ width) {
+    const size_t p = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p >= m) return;
+    const int lp = lcp[p];
+    if (lp == LCP_SAME) return;
+    const int I = ip[p];
+    const uint32_t base = S[p];
+    const float4 q = sp[p];
+
+    // widths along the x-axis descent (levels 0 .. I+1), written for the levels starting here
+    float mn = info->mn[0], mx = info->mx[0];
+    for (int l = 0; l <= I + 1; ++l) {
+        if (l > 0) {
+            const float c = midpoint(mn, mx);
+            if (q.x < c) mx = c; else mn = c;
+        }
+        if (l > lp) {
+            const uint32_t k = base + uint32_t(l - lp - 1);
+            const float wd = __fsub_rn(mx, mn);
+            width[k] = wd;
+            level[k] = uint8_t(l);
+            rec[k].w2 = (l <= I) ? __fmul_rn(wd, wd) : -INFINITY;
+            rec[k].begin = uint32_t(p);
+        }
+    }
+    // subtree ends, deepest level first (ranges only grow towards the root)
+    const uint64_t kp = keys[p];
+    size_t e = p + 1;
+    for (int l = I + 1; l > lp; --l) {
+        if (l > KEY_LEVELS) {
+            while (e < m && lcp[e] >= l) ++e;
+        } else if (e < m && same_prefix(keys[e], kp, l)) {
+            // gallop to a bracket [lo, hi) with prefix(lo) equal and prefix(hi) different or m
+            size_t lo = e, step = 1, hi = e + 1;
+            while (hi < m && same_prefix(keys[hi], kp, l)) {
+                lo = hi;
+                step <<= 1;
+                hi = lo + step;
+            }
+            if (hi > m) hi = m;
+            while (hi - lo > 1) {
+                const size_t mid = lo + (hi - lo) / 2;
+                if (same_prefix(keys[mid], kp, l)) lo = mid; else hi = mid;
+            }
+            e = hi;
+        }
+        const uint32_t k = base + uint32_t(l - lp - 1);
+        rec[k].end = uint32_t(e);
+        rec[k].skip = S[e];
+        if (l == I + 1) {  // the leaf: a run of identical positions
+            double tot = 0.0;
+            for (size_t r = p; r < e; ++r) tot += double(sp[r].w);
+            const double cnt_d = double(e - p);
+            sums[k] = make_double4(tot, tot * q.x, tot * q.y, tot * q.z);
+            sums2[k] = make_double4(cnt_d * q.x, cnt_d * q.y, cnt_d * q.z, cnt_d);
+            rec[k].cx = q.x;
+            rec[k].cy = q.y;
+            rec[k].cz = q.z;
+            rec[k].mass = float(tot);
+        }
+    }
+}
+
+// FP64 sums of the internal nodes of one level from their children (pre-order: first child at
+// k+1, next sibling at the previous child's skip), then the f32 centre of mass
+// (impls/mod.rs:126-131: mass-weighted mean, or the plain mean when the total mass is 0).
+__global__ __launch_bounds__(256) void bh_up_kernel(NodeRec* __restrict__ rec,
+                                                    double4* __restrict__ sums,
+                                                    double4* __restrict__ sums2,
+                                                    const uint8_t* __restrict__ level, size_t total,
+                                                    int L) {
+    for (size_t k = size_t(blockIdx.x) * blockDim.x + threadIdx.x; k < total;
+         k += size_t(gridDim.x) * blockDim.x) {
+        if (level[k] != L) continue;
+        const NodeRec r = rec[k];
+        if (!(r.w2 >= 0.0f)) continue;  // leaf
+        double4 a = make_double4(0, 0, 0, 0), b = make_double4(0, 0, 0, 0);
+        size_t c = k + 1;
+        while (c < r.skip) {
+            const double4 s1 = sums[c], s2 = sums2[c];
+            a.x += s1.x; a.y += s1.y; a.z += s1.z; a.w += s1.w;
+            b.x += s2.x; b.y += s2.y; b.z += s2.z; b.w += s2.w;
+            const uint32_t nx = rec[c].skip;
+            c = nx > c ? nx : c + 1;
+        }
+        sums[k] = a;
+        sums2[k] = b;
+        float cx, cy, cz;
+        if (a.x != 0.0) {
+            cx = float(a.y / a.x); cy = float(a.z / a.x); cz = float(a.w / a.x);
+        } else {
+            cx = float(b.x / b.w); cy = float(b.y / b.w); cz = float(b.z / b.w);
+        }
+        rec[k].cx = cx;
+        rec[k].cy = cy;
+        rec[k].cz = cz;
+        rec[k].mass = float(a.x);
+    }
+}
+
+// ---- traversal -----------------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void bh_walk_kernel(const NodeRec* __restrict__ nodes,
+                                                      uint32_t total,
+                                                      const float4* __restrict__ pts,
+                                                      const uint32_t* __restrict__ perm,
+                                                      uint32_t n, float theta2, float eps2,
+                                                      float4* __restrict__ out,
+                                                      BhInfo* __restrict__ info) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = i < n;
+    const float4 p = pts[valid ? i : n - 1];
+    float ax = 0.f, ay = 0.f, az = 0.f, cx = 0.f, cy = 0.f, cz = 0.f;
+    uint32_t resume = valid ? 0u : 0xffffffffu;
+    uint32_t accepted = 0, steps = 0;
+    uint32_t k = 0;
+    while (k < total) {
+        const NodeRec r = nodes[k];
+        const bool active = k >= resume;
+        const float dx = __fsub_rn(r.cx, p.x), dy = __fsub_rn(r.cy, p.y), dz = __fsub_rn(r.cz, p.z);
+        const float d2 = norm2_ref(dx, dy, dz);
+        const bool nz = d2 != 0.0f;
+        const bool crit = __fmul_rn(theta2, d2) < r.w2;
+        const bool open = active && nz && crit;
+        if (active && nz && !crit) {
+            const float r2 = __fadd_rn(d2, eps2);
+            const float inv = __builtin_amdgcn_rsqf(r2);
+            const float s = (r.mass * inv) * (inv * inv);
+            float y = fmaf(dx, s, -cx), t = ax + y;
+            cx = (t - ax) - y; ax = t;
+            y = fmaf(dy, s, -cy); t = ay + y;
+            cy = (t - ay) - y; ay = t;
+            y = fmaf(dz, s, -cz); t = az + y;
+            cz = (t - az) - y; az = t;
+            ++accepted;
+        }
+        if (active && !open) resume = r.skip;
+        ++steps;
+        const uint32_t nxt = __any(open) ? k + 1 : max(r.skip, k + 1);
+        k = __builtin_amdgcn_readfirstlane(nxt);
+    }
+    if (valid) out[perm ? perm[i] : i] = make_float4(ax, ay, az, 0.f);
+    // counters: node steps per wave (lane 0) and accepted pair terms
+    unsigned long long acc64 = accepted;
+    for (int off = 32; off > 0; off >>= 1) acc64 += __shfl_xor(acc64, off, 64);
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&info->steps, (unsigned long long)steps);
+        atomicAdd(&info->accepted, acc64);
+    }
+}
+
+// ---- host side --------------------------------------------------------------------------------
+
+static int build_tree(pnb_ctx* c, const float4* afg, size_t m) {
+    if (!c->bh) c->bh = new BhState();
+    BhState* s = c->bh;
+    s->built = false;
+    s->m = m;
+    s->nodes = 0;
+    s->max_level = -1;
+    PNB_HIP(c, s->info.ensure(sizeof(BhInfo)));
+    BhInfo* info = s->info.as<BhInfo>();
+    bh_info_init_kernel<<<1, 1, 0, c->stream>>>(info);
+    PNB_CHECK_LAUNCH(c);
+    if (m == 0) {
+        s->built = true;
+        return PNB_OK;
+    }
+    if (m > 0x7fffffffu) return fail(c, PNB_E_ARG, "Barnes-Hut supports at most 2^31-1 bodies");
+    const unsigned g = div_up(m, 256);
+    PNB_HIP(c, s->bbox_part.ensure(BB_BLOCKS * 6 * sizeof(float)));
+    PNB_HIP(c, s->keys.ensure(m * 8));
+    PNB_HIP(c, s->keys_alt.ensure(m * 8));
+    PNB_HIP(c, s->idx.ensure(m * 4));
+    PNB_HIP(c, s->idx_alt.ensure(m * 4));
+    PNB_HIP(c, s->sp.ensure(m * sizeof(float4)));
+    PNB_HIP(c, s->lcp.ensure(m * sizeof(int)));
+    PNB_HIP(c, s->cnt.ensure((m + 1) * 4));
+    PNB_HIP(c, s->ip.ensure(m * sizeof(int)));
+    PNB_HIP(c, s->key2.ensure(m * 8));
+
+    return timed(c, PNB_PROF_BH_BUILD, [&]() -> int {
+        const int bb = int(std::min<size_t>(BB_BLOCKS, div_up(m, 256)));
+        bh_bbox_kernel<<<bb, 256, 0, c->stream>>>(afg, m, s->bbox_part.as<float>(), info);
+        PNB_CHECK_LAUNCH(c);
+        bh_root_kernel<<<1, 64, 0, c->stream>>>(s->bbox_part.as<float>(), bb, info);
+        PNB_CHECK_LAUNCH(c);
+        bh_keys_kernel<<<g, 256, 0, c->stream>>>(afg, m, info, s->keys.as<uint64_t>(), s->idx.as<uint32_t>());
+        PNB_CHECK_LAUNCH(c);
+        uint64_t* sk;
+        uint32_t* si;
+        int rc = timed(c, PNB_PROF_SORT, [&]() -> int {
+            return radix_sort_pairs(c, s->keys.as<uint64_t>(), s->idx.as<uint32_t>(),
+                                    s->keys_alt.as<uint64_t>(), s->idx_alt.as<uint32_t>(), m, 63,
+                                    s->hist, s->bits, &sk, &si);
+        });
+        if (rc) return rc;
+        s->skeys = sk;
+        s->sidx = si;
+        bh_gather_kernel<<<g, 256, 0, c->stream>>>(afg, m, si, s->sp.as<float4>());
+        PNB_CHECK_LAUNCH(c);
+        int* lcp = s->lcp.as<int>();
+        bh_lcp_kernel<<<g, 256, 0, c->stream>>>(sk, s->sp.as<float4>(), m, lcp);
+        PNB_CHECK_LAUNCH(c);
+        bh_deep_kernel<<<g, 256, 0, c->stream>>>(sk, m, s->sp.as<float4>(), si, lcp,
+                                                 s->key2.as<uint64_t>(), info);
+        PNB_CHECK_LAUNCH(c);
+        uint32_t* cnt = s->cnt.as<uint32_t>();
+        bh_count_kernel<<<g, 256, 0, c->stream>>>(lcp, m, cnt, s->ip.as<int>(), info);
+        PNB_CHECK_LAUNCH(c);
+        if ((rc = scan_u32(c, cnt, cnt, m, true))) return rc;
+        BhInfo h;
+        PNB_HIP(c, hipMemcpyAsync(&h, info, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+        uint32_t total = 0;
+        PNB_HIP(c, hipMemcpyAsync(&total, cnt + m, 4, hipMemcpyDeviceToHost, c->stream));
+        PNB_HIP(c, hipStreamSynchronize(c->stream));
+        if (h.flags & FLAG_NONFINITE) return fail(c, PNB_E_ARG, "non-finite affecting position");
+        if (h.flags & FLAG_DEPTH)
+            return fail(c, PNB_E_DEPTH, "octree deeper than %d levels", MAX_LEVEL);
+        s->nodes = total;
+        s->max_level = h.max_level;
+        PNB_HIP(c, s->rec.ensure(size_t(total) * sizeof(NodeRec)));
+        PNB_HIP(c, s->sums.ensure(size_t(total) * sizeof(double4)));
+        PNB_HIP(c, s->sums2.ensure(size_t(total) * sizeof(double4)));
+        PNB_HIP(c, s->level.ensure(total));
+        PNB_HIP(c, s->width.ensure(size_t(total) * sizeof(float)));
+        NodeRec* rec = s->rec.as<NodeRec>();
+        bh_node_kernel<<<g, 256, 0, c->stream>>>(sk, s->sp.as<float4>(), lcp, s->ip.as<int>(),
+                                                 cnt, m, info, rec, s->sums.as<double4>(),
+                                                 s->sums2.as<double4>(), s->level.as<uint8_t>(),
+                                                 s->width.as<float>());
+        PNB_CHECK_LAUNCH(c);
+        const unsigned ug = std::min<unsigned>(4 * std::max(c->cu_count, 1), div_up(total, 256));
+        for (int L = h.max_level - 1; L >= 0; --L) {
+            bh_up_kernel<<<ug, 256, 0, c->stream>>>(rec, s->sums.as<double4>(),
+                                                    s->sums2.as<double4>(), s->level.as<uint8_t>(),
+                                                    total, L);
+            PNB_CHECK_LAUNCH(c);
+        }
+        s->built = true;
+        return PNB_OK;
+    });
+}
+
+static int walk(pnb_ctx* c, const float4* pts, const uint32_t* perm, size_t n, float theta,
+                float softening, float4* out) {
+    BhState* s = c->bh;
+    const float theta2 = theta * theta;  // (self.theta * self.theta), sequential.rs:491
+    const float eps2 = softening * softening;
+    return timed(c, PNB_PROF_BH_WALK, [&]() -> int {
+        bh_walk_kernel<<<div_up(n, 256), 256, 0, c->stream>>>(
+            s->rec.as<NodeRec>(), uint32_t(s->nodes), pts, perm, uint32_t(n), theta2, eps2, out,
+            s->info.as<BhInfo>());
+        PNB_CHECK_LAUNCH(c);
+        return PNB_OK;
+    });
+}
+
+// Orders the affected particles along the tree's key order so that each wavefront holds
+// neighbours, walks, and scatters results back to the caller's order.
+static int accel_dev(pnb_ctx* c, const float4* affected, size_t n, const float4* afg, size_t m,
+                     float theta, float softening, float4* out) {
+    if (n == 0) return PNB_OK;
+    if (n > 0x7fffffffu) return fail(c, PNB_E_ARG, "Barnes-Hut supports at most 2^31-1 particles");
+    if (int rc = build_tree(c, afg, m)) return rc;
+    BhState* s = c->bh;
+    if (affected == afg && n == m) return walk(c, s->sp.as<float4>(), s->sidx, n, theta, softening, out);
+    const unsigned g = div_up(n, 256);
+    PNB_HIP(c, s->akeys.ensure(n * 8));
+    PNB_HIP(c, s->akeys_alt.ensure(n * 8));
+    PNB_HIP(c, s->aidx.ensure(n * 4));
+    PNB_HIP(c, s->aidx_alt.ensure(n * 4));
+    PNB_HIP(c, s->apts.ensure(n * sizeof(float4)));
+    if (m == 0) {
+        // no tree: every particle gets zero (the reference's root is None)
+        return walk(c, affected, nullptr, n, theta, softening, out);
+    }
+    bh_keys_kernel<<<g, 256, 0, c->stream>>>(affected, n, s->info.as<BhInfo>(),
+                                             s->akeys.as<uint64_t>(), s->aidx.as<uint32_t>());
+    PNB_CHECK_LAUNCH(c);
+    uint64_t* sk;
+    uint32_t* si;
+    if (int rc = radix_sort_pairs(c, s->akeys.as<uint64_t>(), s->aidx.as<uint32_t>(),
+                                  s->akeys_alt.as<uint64_t>(), s->aidx_alt.as<uint32_t>(), n, 63,
+                                  s->hist, s->bits, &sk, &si))
+        return rc;
+    bh_gather_kernel<<<g, 256, 0, c->stream>>>(affected, n, si, s->apts.as<float4>());
+    PNB_CHECK_LAUNCH(c);
+    return walk(c, s->apts.as<float4>(), si, n, theta, softening, out);
+}
+
+static int copy_h2d(pnb_ctx* c, DevBuf& b, const float* src, size_t count) {
+    PNB_HIP(c, b.ensure(std::max<size_t>(count, 1) * sizeof(float4)));
+    if (count)
+        PNB_HIP(c, hipMemcpyAsync(b.ptr, src, count * sizeof(float4), hipMemcpyHostToDevice, c->stream));
+    return PNB_OK;
+}
+
 }  // namespace pnb
+
+using namespace pnb;
+
+#define PNB_BH_ARGS_CHECK()                                                                 \
+    if (!c) return PNB_E_ARG;                                                               \
+    if ((n && (!affected || !out)) || (m && !affecting))                                    \
+        return fail(c, PNB_E_ARG, "null pointer with non-zero length");                     \
+    if (!(softening >= 0.0f)) return fail(c, PNB_E_ARG, "softening must be >= 0");
+
+extern "C" {
+
+int pnb_bh_accel_f32_dev(pnb_ctx* c, const float* affected, size_t n, const float* affecting,
+                         size_t m, float theta, float softening, int checked, unsigned flags,
+                         float* out) {
+    PNB_BH_ARGS_CHECK();
+    (void)checked;  // the traversal skips zero-distance nodes for both (sequential.rs:485-487)
+    (void)flags;
+    if (int rc = activate(c)) return rc;
+    return accel_dev(c, reinterpret_cast<const float4*>(affected), n,
+                     reinterpret_cast<const float4*>(affecting), m, theta, softening,
+                     reinterpret_cast<float4*>(out));
+}
+
+int pnb_bh_accel_f32(pnb_ctx* c, const float* affected, size_t n, const float* affecting,
+                     size_t m, float theta, float softening, int checked, unsigned flags,
+                     float* out) {
+    PNB_BH_ARGS_CHECK();
+    if (n == 0) return PNB_OK;
+    if (int rc = activate(c)) return rc;
+    if (!c->bh) c->bh = new BhState();
+    BhState* s = c->bh;
+    const bool same = affected == affecting && n == m;
+    if (int rc = copy_h2d(c, s->h_afg, affecting, m)) return rc;
+    if (!same)
+        if (int rc = copy_h2d(c, s->h_aff, affected, n)) return rc;
+    PNB_HIP(c, s->h_out.ensure(n * sizeof(float4)));
+    const float* daff = same ? s->h_afg.as<float>() : s->h_aff.as<float>();
+    if (int rc = pnb_bh_accel_f32_dev(c, daff, n, s->h_afg.as<float>(), m, theta, softening,
+                                      checked, flags, s->h_out.as<float>()))
+        return rc;
+    PNB_HIP(c, hipMemcpyAsync(out, s->h_out.ptr, n * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
+    PNB_HIP(c, hipStreamSynchronize(c->stream));
+    return PNB_OK;
+}
+
+int pnb_reordered_bh_accel_f32_dev(pnb_ctx* c, const float* particles, size_t n, float theta,
+                                   float softening, int checked, unsigned flags, float* out) {
+    if (!c) return PNB_E_ARG;
+    if (n && (!particles || !out)) return fail(c, PNB_E_ARG, "null pointer with non-zero length");
+    if (!(softening >= 0.0f)) return fail(c, PNB_E_ARG, "softening must be >= 0");
+    if (n == 0) return PNB_OK;
+    if (int rc = activate(c)) return rc;
+    if (!c->bh) c->bh = new BhState();
+    BhState* s = c->bh;
+    PNB_HIP(c, s->afg_tmp.ensure(n * sizeof(float4)));
+    uint32_t m = 0;
+    const float4* p = reinterpret_cast<const float4*>(particles);
+    if (int rc = split_affecting(c, p, n, s->afg_tmp.as<float4>(), &m)) return rc;
+    const float4* afg = s->afg_tmp.as<float4>();
+    if (m == n) afg = p;  // every particle is affecting: same storage, the tree order is reused
+    (void)checked;
+    (void)flags;
+    return accel_dev(c, p, n, afg, m, theta, softening, reinterpret_cast<float4*>(out));
+}
+
+int pnb_reordered_bh_accel_f32(pnb_ctx* c, const float* particles, size_t n, float theta,
+                               float softening, int checked, unsigned flags, float* out) {
+    if (!c) return PNB_E_ARG;
+    if (n && (!particles || !out)) return fail(c, PNB_E_ARG, "null pointer with non-zero length");
+    if (n == 0) return PNB_OK;
+    if (int rc = activate(c)) return rc;
+    if (!c->bh) c->bh = new BhState();
+    BhState* s = c->bh;
+    if (int rc = copy_h2d(c, s->h_aff, particles, n)) return rc;
+    PNB_HIP(c, s->h_out.ensure(n * sizeof(float4)));
+    if (int rc = pnb_reordered_bh_accel_f32_dev(c, s->h_aff.as<float>(), n, theta, softening,
+                                                checked, flags, s->h_out.as<float>()))
+        return rc;
+    PNB_HIP(c, hipMemcpyAsync(out, s->h_out.ptr, n * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
+    PNB_HIP(c, hipStreamSynchronize(c->stream));
+    return PNB_OK;
+}
+
+int pnb_bh_tree_size(pnb_ctx* c, size_t* nodes, int* max_level) {
+    if (!c) return PNB_E_ARG;
+    if (!c->bh || !c->bh->built) return fail(c, PNB_E_STATE, "no octree has been built");
+    if (nodes) *nodes = c->bh->nodes;
+    if (max_level) *max_level = c->bh->max_level;
+    return PNB_OK;
+}
+
+int pnb_bh_tree_export(pnb_ctx* c, pnb_tree_node* out, size_t cap, uint32_t* order, size_t m) {
+    if (!c) return PNB_E_ARG;
+    BhState* s = c->bh;
+    if (!s || !s->built) return fail(c, PNB_E_STATE, "no octree has been built");
+    if (cap < s->nodes || (order && m < s->m)) return fail(c, PNB_E_ARG, "output too small");
+    if (int rc = activate(c)) return rc;
+    const size_t T = s->nodes;
+    if (T) {
+        NodeRec* rec = new NodeRec[T];
+        uint8_t* lev = new uint8_t[T];
+        float* wid = new float[T];
+        double4* sm = new double4[T];
+        hipError_t e = hipMemcpyAsync(rec, s->rec.ptr, T * sizeof(NodeRec), hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(lev, s->level.ptr, T, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(wid, s->width.ptr, T * 4, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(sm, s->sums.ptr, T * sizeof(double4), hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e == hipSuccess)
+            for (size_t k = 0; k < T; ++k) {
+                pnb_tree_node& o = out[k];
+                o.com[0] = rec[k].cx; o.com[1] = rec[k].cy; o.com[2] = rec[k].cz;
+                o.mass = rec[k].mass;
+                o.mass64 = sm[k].x;
+                o.width = wid[k];
+                o.skip = rec[k].skip;
+                o.begin = rec[k].begin;
+                o.count = rec[k].end - rec[k].begin;
+                o.level = lev[k];
+                o.internal = rec[k].w2 >= 0.0f ? 1u : 0u;
+            }
+        delete[] rec;
+        delete[] lev;
+        delete[] wid;
+        delete[] sm;
+        if (e != hipSuccess) return hip_fail(c, e, "tree export copy", __FILE__, __LINE__);
+    }
+    if (order && s->m)
+        PNB_HIP(c, hipMemcpy(order, s->sidx, s->m * 4, hipMemcpyDeviceToHost));
+    return PNB_OK;
+}
+
+int pnb_bh_stats(pnb_ctx* c, uint64_t* wave_steps, uint64_t* accepted) {
+    if (!c) return PNB_E_ARG;
+    if (!c->bh || !c->bh->built) return fail(c, PNB_E_STATE, "no octree has been built");
+    if (int rc = activate(c)) return rc;
+    BhInfo h;
+    PNB_HIP(c, hipMemcpyAsync(&h, c->bh->info.ptr, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    PNB_HIP(c, hipStreamSynchronize(c->stream));
+    if (wave_steps) *wave_steps = h.steps;
+    if (accepted) *accepted = h.accepted;
+    return PNB_OK;
+}
+
+}  // extern "C"

@@ -51,10 +51,11 @@ struct pnb_ctx {
     bool capturing = false;  // inside hipStreamBeginCapture: no events, no allocation
     double prof_ms[PNB_PROF_NUM_KERNELS] = {};
     uint64_t prof_count[PNB_PROF_NUM_KERNELS] = {};
-    static constexpr int EV_POOL = 512;
+    static constexpr int EV_POOL = 1024;
     hipEvent_t ev_start[EV_POOL] = {}, ev_stop[EV_POOL] = {};
     int ev_kernel[EV_POOL] = {};
     int ev_used = 0;
+    int ev_open = 0;  // slots whose stop event is not recorded yet (nested timed launches)
 
     pnb::BfTune* bf_tune = nullptr;
     pnb::BhState* bh = nullptr;
@@ -88,18 +89,21 @@ int prof_resolve(pnb_ctx* c);
 template <class F>
 int timed(pnb_ctx* c, int kernel_id, F&& launch) {
     if (!c->profile || c->capturing) return launch();
-    if (c->ev_used == pnb_ctx::EV_POOL)
+    if (c->ev_used == pnb_ctx::EV_POOL && c->ev_open == 0)
         if (int rc = prof_resolve(c)) return rc;
-    const int k = c->ev_used;
-    hipError_t e = hipEventRecord(c->ev_start[k], c->stream);
-    if (e != hipSuccess) return hip_fail(c, e, "hipEventRecord", __FILE__, __LINE__);
-    int rc = launch();
-    if (rc) return rc;
-    e = hipEventRecord(c->ev_stop[k], c->stream);
-    if (e != hipSuccess) return hip_fail(c, e, "hipEventRecord", __FILE__, __LINE__);
+    if (c->ev_used == pnb_ctx::EV_POOL) return launch();  // pool full inside a nested launch
+    const int k = c->ev_used++;
     c->ev_kernel[k] = kernel_id;
-    c->ev_used = k + 1;
-    return PNB_OK;
+    c->ev_open++;
+    hipError_t e = hipEventRecord(c->ev_start[k], c->stream);
+    int rc = (e == hipSuccess) ? launch() : hip_fail(c, e, "hipEventRecord", __FILE__, __LINE__);
+    if (rc == PNB_OK) {
+        e = hipEventRecord(c->ev_stop[k], c->stream);
+        if (e != hipSuccess) rc = hip_fail(c, e, "hipEventRecord", __FILE__, __LINE__);
+    }
+    c->ev_open--;
+    if (rc != PNB_OK) c->ev_used = k;  // drop this slot and any nested ones
+    return rc;
 }
 
 void bf_tune_free(BfTune*);

@@ -1,0 +1,158 @@
+// Stable LSD radix sort of (uint64 key, uint32 value) pairs, 8-bit digits, for gfx950.
+//
+// One pass per digit, three launches: per-tile digit histograms (digit-major, so one device-wide
+// exclusive scan turns them into scatter offsets), the scan, and a scatter in which every
+// wavefront ranks its keys with ballots (for each lane, the set of lanes holding the same digit is
+// the AND of eight ballot masks; the rank is the popcount of that set below the lane). A tile of
+// 4096 keys is split into four contiguous wave segments processed in order, so the ranking is
+// stable. Digits on which all keys agree are skipped (found from one OR/AND reduction).
+#include "pnb_internal.hpp"
+
+namespace pnb {
+
+constexpr int RS_BLOCK = 256;
+constexpr int RS_WAVES = RS_BLOCK / 64;
+constexpr int RS_ROUNDS = 16;                      // 64-key rounds per wave
+constexpr int RS_TILE = RS_BLOCK * RS_ROUNDS;      // 4096 keys per workgroup
+constexpr int RS_WAVE_SPAN = 64 * RS_ROUNDS;       // 1024 keys per wave segment
+
+__global__ __launch_bounds__(RS_BLOCK) void rs_hist_kernel(const uint64_t* __restrict__ keys,
+                                                           size_t n, int shift,
+                                                           uint32_t* __restrict__ hist,
+                                                           uint32_t tiles) {
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const size_t base = size_t(blockIdx.x) * RS_TILE;
+#pragma unroll 4
+    for (int k = 0; k < RS_ROUNDS; ++k) {
+        const size_t i = base + size_t(k) * RS_BLOCK + threadIdx.x;
+        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 0xffu], 1u);
+    }
+    __syncthreads();
+    hist[size_t(threadIdx.x) * tiles + blockIdx.x] = h[threadIdx.x];
+}
+
+__global__ __launch_bounds__(RS_BLOCK) void rs_scatter_kernel(
+    const uint64_t* __restrict__ keys, const uint32_t* __restrict__ vals, size_t n, int shift,
+    const uint32_t* __restrict__ offsets, uint32_t tiles, uint64_t* __restrict__ keys_out,
+    uint32_t* __restrict__ vals_out) {
+    __shared__ uint32_t wcnt[RS_WAVES][256];
+    __shared__ uint32_t goff[256];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int d = lane; d < 256; d += 64) wcnt[w][d] = 0;
+    goff[threadIdx.x] = offsets[size_t(threadIdx.x) * tiles + blockIdx.x];
+    __syncthreads();
+
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const size_t seg = size_t(blockIdx.x) * RS_TILE + size_t(w) * RS_WAVE_SPAN;
+    uint64_t key[RS_ROUNDS];
+    uint32_t val[RS_ROUNDS];
+    uint32_t rank[RS_ROUNDS];
+#pragma unroll
+    for (int r = 0; r < RS_ROUNDS; ++r) {
+        const size_t i = seg + size_t(r) * 64 + lane;
+        const bool valid = i < n;
+        key[r] = valid ? keys[i] : 0ull;
+        val[r] = valid ? vals[i] : 0u;
+        const uint32_t d = uint32_t(key[r] >> shift) & 0xffu;
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const uint64_t bal = __ballot((d >> b) & 1u);
+            peers &= ((d >> b) & 1u) ? bal : ~bal;
+        }
+        const uint32_t before = wcnt[w][d];
+        rank[r] = before + uint32_t(__popcll(peers & lt_mask));
+        // the lowest lane of each digit group advances the wave's running count for that digit
+        if (valid && (peers & lt_mask) == 0) wcnt[w][d] = before + uint32_t(__popcll(peers));
+        __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+    // per digit: exclusive prefix over the wave segments, plus the tile's global offset
+    {
+        const int d = threadIdx.x;
+        uint32_t run = goff[d];
+#pragma unroll
+        for (int ww = 0; ww < RS_WAVES; ++ww) {
+            const uint32_t c = wcnt[ww][d];
+            wcnt[ww][d] = run;
+            run += c;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < RS_ROUNDS; ++r) {
+        const size_t i = seg + size_t(r) * 64 + lane;
+        if (i < n) {
+            const uint32_t d = uint32_t(key[r] >> shift) & 0xffu;
+            const uint32_t pos = wcnt[w][d] + rank[r];
+            keys_out[pos] = key[r];
+            vals_out[pos] = val[r];
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void rs_bits_kernel(const uint64_t* __restrict__ keys, size_t n,
+                                                      unsigned long long* __restrict__ or_and) {
+    unsigned long long o = 0ull, a = ~0ull;
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += size_t(gridDim.x) * blockDim.x) {
+        o |= keys[i];
+        a &= keys[i];
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        o |= __shfl_xor(o, off, 64);
+        a &= __shfl_xor(a, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicOr(&or_and[0], o);
+        atomicAnd(&or_and[1], a);
+    }
+}
+
+__global__ void rs_bits_init_kernel(unsigned long long* or_and) {
+    or_and[0] = 0ull;
+    or_and[1] = ~0ull;
+}
+
+// Sorts keys[0..n) with their values; the result ends in (*keys_out, *vals_out), which is either
+// the input pair or the alternate pair. Only bits [0, key_bits) are considered.
+int radix_sort_pairs(pnb_ctx* c, uint64_t* keys, uint32_t* vals, uint64_t* keys_alt,
+                     uint32_t* vals_alt, size_t n, int key_bits, DevBuf& hist_buf,
+                     DevBuf& bits_buf, uint64_t** keys_out, uint32_t** vals_out) {
+    *keys_out = keys;
+    *vals_out = vals;
+    if (n <= 1) return PNB_OK;
+    if (n > 0xffffffffu) return fail(c, PNB_E_ARG, "radix sort supports at most 2^32-1 keys");
+    const uint32_t tiles = div_up(n, RS_TILE);
+    PNB_HIP(c, hist_buf.ensure((size_t(256) * tiles + 1) * sizeof(uint32_t)));
+    PNB_HIP(c, bits_buf.ensure(2 * sizeof(unsigned long long)));
+    unsigned long long* oa = bits_buf.as<unsigned long long>();
+    rs_bits_init_kernel<<<1, 1, 0, c->stream>>>(oa);
+    PNB_CHECK_LAUNCH(c);
+    rs_bits_kernel<<<std::min<unsigned>(1024, div_up(n, 256)), 256, 0, c->stream>>>(keys, n, oa);
+    PNB_CHECK_LAUNCH(c);
+    unsigned long long h_oa[2];
+    PNB_HIP(c, hipMemcpyAsync(h_oa, oa, sizeof(h_oa), hipMemcpyDeviceToHost, c->stream));
+    PNB_HIP(c, hipStreamSynchronize(c->stream));
+    const uint64_t varying = h_oa[0] ^ h_oa[1];
+    uint32_t* hist = hist_buf.as<uint32_t>();
+    uint64_t *kin = keys, *kout = keys_alt;
+    uint32_t *vin = vals, *vout = vals_alt;
+    for (int shift = 0; shift < key_bits; shift += 8) {
+        if (((varying >> shift) & 0xffull) == 0) continue;
+        rs_hist_kernel<<<tiles, RS_BLOCK, 0, c->stream>>>(kin, n, shift, hist, tiles);
+        PNB_CHECK_LAUNCH(c);
+        if (int rc = scan_u32(c, hist, hist, size_t(256) * tiles, false)) return rc;
+        rs_scatter_kernel<<<tiles, RS_BLOCK, 0, c->stream>>>(kin, vin, n, shift, hist, tiles, kout, vout);
+        PNB_CHECK_LAUNCH(c);
+        std::swap(kin, kout);
+        std::swap(vin, vout);
+    }
+    *keys_out = kin;
+    *vals_out = vin;
+    return PNB_OK;
+}
+
+}  // namespace pnb

@@ -1,0 +1,170 @@
+"""Barnes-Hut on the GPU through the C ABI, against the pinned CPU oracle.
+
+Structure: the GPU octree must equal the restated reference octree node for node (same count,
+order, levels, particle ranges, internal/leaf split and bit-identical f32 widths).
+Accelerations: against the oracle traversal with FP64 centres of mass (the same node data) the
+bar is rms <= 1e-6; against the reference-exact f32 centres of mass rms <= 1e-5.
+"""
+import numpy as np
+import pytest
+
+import particular_amd as pa
+from particular_amd import metrics, workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def bh(res, storage, theta, interaction=None):
+    return pa.BarnesHut(res, theta, interaction or pa.Acceleration.checked()).compute(storage)
+
+
+def assert_same_tree(res, orc, particles):
+    t = orc.Tree(particles)
+    on = t.nodes()
+    gn, gorder = pa.tree_export(res, particles.shape[0])
+    assert gn.shape[0] == on.shape[0], (gn.shape[0], on.shape[0])
+    np.testing.assert_array_equal(gorder, t.order())
+    np.testing.assert_array_equal(gn["level"], on["level"])
+    np.testing.assert_array_equal(gn["begin"], on["begin"])
+    np.testing.assert_array_equal(gn["count"], on["count"])
+    np.testing.assert_array_equal(gn["internal"], on["internal"])
+    ow = on["bmax"][:, 0] - on["bmin"][:, 0]
+    internal = on["internal"] == 1
+    np.testing.assert_array_equal(gn["width"][internal], ow[internal])
+    np.testing.assert_allclose(gn["mass64"], on["mass64"], rtol=1e-12)
+    np.testing.assert_allclose(gn["com"], on["com64"].astype(np.float32), rtol=3e-7, atol=1e-30)
+    # skip pointers: pre-order subtree ends
+    ends = np.arange(len(on)) + 1
+    for k in range(len(on) - 1, -1, -1):
+        kids = [c for c in on["child"][k] if c >= 0]
+        if kids:
+            ends[k] = ends[max(kids)]
+    np.testing.assert_array_equal(gn["skip"], ends)
+    return t
+
+
+@pytest.mark.parametrize("n,seed", [(1, 1), (2, 2), (3, 3), (100, 4), (5000, 5)])
+def test_tree_matches_oracle_plummer(res, orc, n, seed):
+    p = workloads.direct_sum_case(n, seed=seed).particles
+    bh(res, p, 0.5)
+    assert_same_tree(res, orc, p)
+
+
+def test_tree_hand_worked_chain(res, orc):
+    p = np.array([[0, 0, 0, 1], [0.1, 0, 0, 1], [1, 1, 1, 1]], np.float32)
+    bh(res, p, 0.5)
+    gn, order = pa.tree_export(res, 3)
+    assert gn.shape[0] == 7 and order.tolist() == [2, 1, 0]
+    assert gn["level"].tolist() == [0, 1, 1, 2, 3, 4, 4]
+    assert_same_tree(res, orc, p)
+
+
+def test_tree_with_duplicates(res, orc):
+    p = workloads.direct_sum_case(800, seed=6).particles
+    p = np.concatenate([p, p[:40], p[10:12], p[10:11]])
+    bh(res, p, 0.5)
+    assert_same_tree(res, orc, p)
+    same = np.repeat(p[:1], 9, axis=0)  # all particles at one position: a single leaf root
+    bh(res, same, 0.5)
+    t = assert_same_tree(res, orc, same)
+    assert t.num_nodes == 1
+
+
+@pytest.mark.parametrize("scale", [1e-7, 1e-16, 1e-30])
+def test_tree_deep_runs(res, orc, scale):
+    """Distinct positions closer than 21 (and 42, 84) levels of subdivision resolve."""
+    rng = np.random.default_rng(7)
+    base = workloads.direct_sum_case(300, seed=8).particles
+    cluster = np.zeros((12, 4), np.float32)
+    cluster[:, :3] = np.float32(0.25) + (rng.integers(0, 4, (12, 3)) * scale).astype(np.float32)
+    cluster[:, 3] = 1.0 / 300
+    tiny = np.zeros((6, 4), np.float32)
+    tiny[:, 0] = (np.arange(6) * scale).astype(np.float32)
+    tiny[:, 3] = 1.0 / 300
+    p = np.concatenate([base, cluster, tiny, cluster[:3]])
+    bh(res, p, 0.5)
+    t = assert_same_tree(res, orc, p)
+    print("max level", t.max_level)
+
+
+@pytest.mark.parametrize("theta", [0.0, 0.3, 0.5, 0.8])
+def test_accel_matches_oracle(res, orc, theta):
+    s = workloads.direct_sum_case(20000, seed=9)
+    got = bh(res, s.particles, theta)
+    t = orc.Tree(s.particles)
+    ref64 = t.barnes_hut(s.particles, theta, com64=True)
+    ref32 = t.barnes_hut(s.particles, theta, com64=False)
+    e64, e32 = metrics.summary(got, ref64), metrics.summary(got, ref32)
+    print(theta, "vs oracle (FP64 COM)", e64, "vs reference-exact", e32)
+    assert e64["rms_rel"] < 1e-6 and e64["max_rel"] < 1e-4
+    assert e32["rms_rel"] < 1e-5
+
+
+def test_theta_zero_is_brute_force(res):
+    s = workloads.direct_sum_case(3000, seed=10)
+    got = bh(res, s.particles, 0.0)
+    ref = pa.BruteForce(res, pa.Acceleration.checked()).compute(s.particles)
+    assert metrics.rms_rel(got, ref) < 1e-6
+
+
+def test_between_distinct_sets_and_reordered(res, orc):
+    s = workloads.massive_massless_case(n_massive=5000, n_total=30000, softening=1e-3)
+    inter = pa.AccelerationSoftened.checked(s.softening)
+    r = pa.Reordered.new(s.particles)
+    massive = r.affecting()
+    got = bh(res, r, 0.5, inter)
+    t = orc.Tree(massive)
+    ref = t.barnes_hut(s.particles, 0.5, softening=s.softening, com64=True)
+    assert metrics.rms_rel(got, ref) < 1e-6
+    got2 = bh(res, pa.Between(s.particles, massive), 0.5, inter)
+    np.testing.assert_array_equal(got, got2)
+    import torch
+
+    dev = bh(res, pa.Reordered.new(torch.from_numpy(s.particles).cuda()), 0.5, inter)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(dev.cpu().numpy()[:, :3], got)
+
+
+def test_edge_cases(res, orc):
+    a = workloads.direct_sum_case(100, seed=11).particles
+    empty = np.zeros((0, 4), np.float32)
+    np.testing.assert_array_equal(bh(res, pa.Between(a, empty), 0.5), np.zeros((100, 3)))
+    assert bh(res, pa.Between(empty, a), 0.5).shape == (0, 3)
+    one = a[:1].copy()
+    np.testing.assert_allclose(bh(res, pa.Between(a, one), 0.7),
+                               orc.barnes_hut(a, one, 0.7)[:, :3], rtol=1e-6)
+    massless = a.copy()
+    massless[:, 3] = 0
+    np.testing.assert_array_equal(bh(res, pa.Reordered.new(massless), 0.5), np.zeros((100, 3)))
+    # the tree of a massless slice uses plain means (impls/mod.rs:127-128)
+    got = bh(res, massless, 0.5)
+    np.testing.assert_array_equal(got, np.zeros((100, 3)))
+    bad = a.copy()
+    bad[5, 0] = np.nan
+    with pytest.raises(pa.PnbError) as e:
+        bh(res, bad, 0.5)
+    assert e.value.status == pa._lib.PNB_E_ARG
+
+
+def test_two_plummer_reduced(res, orc):
+    s = workloads.two_plummer_case(131072)
+    t = orc.Tree(s.particles)
+    idx = np.random.default_rng(1).choice(s.n, 4096, replace=False)
+    for theta in (0.3, 0.5, 0.8):
+        got = bh(res, s.particles, theta)
+        ref64 = t.barnes_hut(s.particles[idx], theta, com64=True)
+        ref32 = t.barnes_hut(s.particles[idx], theta, com64=False)
+        e64, e32 = metrics.summary(got[idx], ref64), metrics.summary(got[idx], ref32)
+        print(theta, e64, e32)
+        assert e64["rms_rel"] < 1e-6 and e32["rms_rel"] < 1e-5
+    gn, _ = pa.tree_export(res)
+    assert gn.shape[0] == t.num_nodes
+
+
+def test_stats_count_accepted_terms(res, orc):
+    s = workloads.direct_sum_case(4000, seed=12)
+    bh(res, s.particles, 0.6)
+    steps, accepted = pa.bh_stats(res)
+    _, visits = orc.Tree(s.particles).barnes_hut(s.particles, 0.6, com64=True, return_visits=True)
+    assert accepted == int(visits.sum())
+    assert steps >= accepted / 64
