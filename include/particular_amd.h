@@ -117,8 +117,13 @@ int pnb_bf_clear_tuning(pnb_ctx *ctx);
  * zero distance, open an internal node when (theta*theta)*d^2 < width^2, else add its pair term.
  * theta = 0 degenerates to brute force over the leaves (sequential.rs:441-442). `checked` is
  * accepted for symmetry with the reference signature; the traversal's zero-distance skip makes
- * both forms identical. flags: 0 (reserved for PNB_BH_* options). */
-#define PNB_BH_MLP 1u /* add the learned far-field correction of each accepted node */
+ * both forms identical. flags: a combination of PNB_BH_MLP and PNB_BH_REF_COM. */
+#define PNB_BH_MLP 1u /* add the learned correction of each accepted internal node (see below) */
+/* Node data exactly as the reference computes it: f32 sequential folds of mass and mass-weighted
+ * position per node (particular/src/gravity/impls/mod.rs:126-131), instead of the FP64 reduction.
+ * The folds are sequential, so a node of k bodies costs k dependent steps (the root: all of
+ * them); meant for reference parity checks, not for speed. */
+#define PNB_BH_REF_COM 2u
 
 int pnb_bh_accel_f32(pnb_ctx *ctx, const float *affected, size_t n, const float *affecting,
                      size_t m, float theta, float softening, int checked, unsigned flags,
@@ -200,6 +205,44 @@ int pnb_batch_read(pnb_ctx *ctx, double *pos3, double *vel3, float *acc4);
 /* Particle count, work items per force pass and pair interactions per step (sum of N_k^2). */
 int pnb_batch_info(pnb_ctx *ctx, size_t *particles, size_t *items,
                    unsigned long long *interactions_per_step);
+
+/* ---- learned far-field correction (Barnes-Hut + MLP) -----------------------------------------
+ * No reference counterpart. A 7 -> 64 -> 64 -> 3 ReLU MLP (BF16 weights on MFMA, FP32
+ * accumulate) corrects the monopole term of every internal node a wavefront accepts in the
+ * Barnes-Hut walk (pnb_bh_accel_f32* with PNB_BH_MLP). One row = one (wavefront, accepted internal
+ * node); its features are seen from the wavefront centroid (unit direction to the node's centre
+ * of mass, width / distance, log mass fraction, wavefront radius / distance, accepting-lane
+ * fraction) and its output y adds (mass / d^2) * y to every accepting lane. Training data are
+ * generated by the method itself: per-lane targets are brute force minus Barnes-Hut. */
+
+int pnb_mlp_num_params(void);
+/* He-normal initialisation from a seed (SplitMix64 + Box-Muller); resets the Adam state. */
+int pnb_mlp_init(pnb_ctx *ctx, uint64_t seed);
+/* Parameters as one FP32 vector: W1[7][64], b1[64], W2[64][64], b2[64], W3[64][3], b3[3]. */
+int pnb_mlp_set_params(pnb_ctx *ctx, const float *params, size_t n);
+int pnb_mlp_get_params(pnb_ctx *ctx, float *params, size_t n);
+/* Builds the octree of `particles` (device, n x 4, all affecting), walks `waves` seeded
+ * wavefronts of 64 consecutive particles in tree order at `theta` recording their rows (at most
+ * row_cap per wavefront; the rest are counted as dropped), computes their brute-force
+ * accelerations, and appends rows and targets to the training set. */
+int pnb_mlp_collect(pnb_ctx *ctx, const float *particles, size_t n, float theta, float softening,
+                    size_t waves, uint64_t seed, size_t row_cap);
+int pnb_mlp_clear_dataset(pnb_ctx *ctx);
+int pnb_mlp_dataset_info(pnb_ctx *ctx, size_t *rows, size_t *waves, uint64_t *dropped);
+/* Training set on the host: feats rows x 8, scale rows, mask rows (64-bit lane masks), row_wave
+ * rows, wave_rows waves + 1 (row offsets), target waves x 64 x 4, wgt waves x 64. */
+int pnb_mlp_dataset_read(pnb_ctx *ctx, float *feats, float *scale, uint64_t *mask,
+                         uint32_t *row_wave, uint32_t *wave_rows, float *target, float *wgt);
+int pnb_mlp_dataset_load(pnb_ctx *ctx, const float *feats, const float *scale,
+                         const uint64_t *mask, const uint32_t *row_wave,
+                         const uint32_t *wave_rows, size_t rows, size_t waves,
+                         const float *target, const float *wgt);
+/* Loss and gradients over the whole training set, without an update. */
+int pnb_mlp_grad(pnb_ctx *ctx, float *loss, float *grads, size_t n);
+/* Full-batch Adam steps; loss_history (optional, `iterations` floats) gets the loss before each. */
+int pnb_mlp_train(pnb_ctx *ctx, size_t iterations, float lr, float *loss_history);
+/* Inference on device rows: y (rows x 4) = MLP(feats (rows x 8)). */
+int pnb_mlp_forward_dev(pnb_ctx *ctx, const float *feats, size_t rows, float *y);
 
 #ifdef __cplusplus
 }

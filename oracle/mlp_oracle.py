@@ -1,0 +1,104 @@
+"""FP64 CPU oracle of the far-field correction MLP -- TEST INFRASTRUCTURE ONLY.
+
+No reference counterpart (particular has no learned component). The network and its loss are
+defined here in plain numpy and pinned by finite differences (tests/test_mlp_oracle.py); the GPU
+forward/backward kernels are checked against it.
+
+Network: 7 -> 64 -> 64 -> 3, ReLU, y = relu(relu(x W1 + b1) W2 + b2) W3 + b3. When
+``bf16=True`` the inputs of every matrix product (x, h1, h2, g_y, g_z2, g_z1 and W1..W3) are
+rounded to bfloat16 (round to nearest even) as the MFMA kernels round them; sums are FP64.
+
+Rows are grouped by wavefront: row r belongs to wave ``wave[r]``, applies to the lanes set in
+``mask[r]`` (64-bit) with scale ``scale[r]``; lane l of wave w predicts
+``pred[w, l] = sum_{r in w, l in mask[r]} scale[r] * y[r]`` for target ``t[w, l]`` with weight
+``wgt[w, l]``; loss = sum wgt * |pred - t|^2 / (number of lanes with wgt > 0).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+IN, HID, OUT = 7, 64, 3
+
+
+def bf16_round(a) -> np.ndarray:
+    """Round float values to bfloat16 (nearest even), returned as float64."""
+    f = np.asarray(a, np.float64).astype(np.float32)
+    u = f.view(np.uint32).astype(np.uint64)
+    lsb = (u >> 16) & 1
+    u = ((u + 0x7FFF + lsb) >> 16) << 16
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def init_params(seed: int = 0) -> dict:
+    rng = np.random.default_rng(seed)
+
+    def he(fan_in, shape):
+        return rng.normal(0.0, np.sqrt(2.0 / fan_in), shape)
+
+    return {
+        "W1": he(IN, (IN, HID)), "b1": np.zeros(HID),
+        "W2": he(HID, (HID, HID)), "b2": np.zeros(HID),
+        "W3": he(HID, (HID, OUT)) * 0.1, "b3": np.zeros(OUT),
+    }
+
+
+def forward(p: dict, x: np.ndarray, bf16: bool = False):
+    r = bf16_round if bf16 else (lambda a: np.asarray(a, np.float64))
+    z1 = r(x) @ r(p["W1"]) + p["b1"]
+    h1 = np.maximum(z1, 0.0)
+    z2 = r(h1) @ r(p["W2"]) + p["b2"]
+    h2 = np.maximum(z2, 0.0)
+    y = r(h2) @ r(p["W3"]) + p["b3"]
+    return y, (z1, h1, z2, h2)
+
+
+def lane_bits(mask: np.ndarray) -> np.ndarray:
+    """(rows, 64) bool matrix of the lanes set in each 64-bit mask."""
+    m = np.asarray(mask, np.uint64)
+    return ((m[:, None] >> np.arange(64, dtype=np.uint64)[None, :]) & np.uint64(1)).astype(bool)
+
+
+def predict(y, wave, mask, scale, num_waves):
+    bits = lane_bits(mask)
+    pred = np.zeros((num_waves, 64, OUT))
+    contrib = (scale[:, None] * y)[:, None, :] * bits[:, :, None]
+    np.add.at(pred, wave, contrib)
+    return pred
+
+
+def loss_and_grads(p, x, wave, mask, scale, target, wgt, bf16=False):
+    num_waves = target.shape[0]
+    y, (z1, h1, z2, h2) = forward(p, x, bf16)
+    pred = predict(y, wave, mask, scale, num_waves)
+    count = max(int((wgt > 0).sum()), 1)
+    diff = pred - target
+    loss = float((wgt[..., None] * diff * diff).sum() / count)
+    g_pred = 2.0 * wgt[..., None] * diff / count
+    bits = lane_bits(mask)
+    g_y = scale[:, None] * (bits[:, :, None] * g_pred[wave]).sum(1)
+    r = bf16_round if bf16 else (lambda a: np.asarray(a, np.float64))
+    g = {}
+    g["W3"] = r(h2).T @ r(g_y)
+    g["b3"] = g_y.sum(0)
+    g_z2 = (r(g_y) @ r(p["W3"]).T) * (z2 > 0)
+    g["W2"] = r(h1).T @ r(g_z2)
+    g["b2"] = g_z2.sum(0)
+    g_z1 = (r(g_z2) @ r(p["W2"]).T) * (z1 > 0)
+    g["W1"] = r(x).T @ r(g_z1)
+    g["b1"] = g_z1.sum(0)
+    return loss, g, pred
+
+
+def adam_step(p, g, state, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):
+    """One Adam step in FP64 (state: dict with 'm', 'v', 't')."""
+    state["t"] = state.get("t", 0) + 1
+    t = state["t"]
+    for k in p:
+        m = state.setdefault("m", {}).setdefault(k, np.zeros_like(p[k]))
+        v = state.setdefault("v", {}).setdefault(k, np.zeros_like(p[k]))
+        m[...] = b1 * m + (1 - b1) * g[k]
+        v[...] = b2 * v + (1 - b2) * g[k] * g[k]
+        mh = m / (1 - b1**t)
+        vh = v / (1 - b2**t)
+        p[k] = p[k] - lr * mh / (np.sqrt(vh) + eps)
+    return p

@@ -293,7 +293,7 @@ void orc_tree_root_box(const orc_tree *t, float *bmin3, float *bmax3) {
  * (theta*theta)*d2 < width*width with width = bbox.max[0] - bbox.min[0] (partition.rs:183-188);
  * otherwise add the pair kernel with the node's {com, mass}. */
 int orc_bh_f32(const orc_tree *t, const float *affected, size_t n, float theta, float softening,
-               int checked, int com64, float *out, uint32_t *visits, int threads) {
+               int checked, int com64, int accum64, float *out, uint32_t *visits, int threads) {
     if (!t || (n && (!affected || !out))) return 1;
     set_threads(threads);
     const orc_node *nodes = t->nodes;
@@ -313,6 +313,7 @@ int orc_bh_f32(const orc_tree *t, const float *affected, size_t n, float theta, 
             if (!stack) continue;
             const float *a = affected + 4 * i;
             float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+            double dx64 = 0.0, dy64 = 0.0, dz64 = 0.0;
             uint32_t acc = 0;
             size_t sp = 0;
             if (t->len) stack[sp++] = 0;
@@ -330,12 +331,21 @@ int orc_bh_f32(const orc_tree *t, const float *affected, size_t n, float theta, 
                 if (nd->internal && (th2 * d2) < (w * w)) {
                     for (int c = 0; c < 8; ++c) stack[sp++] = nd->child[c];
                 } else {
-                    float ox, oy, oz;
-                    pair(cx, cy, cz, ms, a[0], a[1], a[2], softening, checked, &ox, &oy, &oz);
-                    sx = sx + ox; sy = sy + oy; sz = sz + oz;
+                    if (accum64) {
+                        /* the same decisions, the pair term and the sum in FP64 */
+                        double ex = (double)cx - a[0], ey = (double)cy - a[1], ez = (double)cz - a[2];
+                        double r2 = ex * ex + ey * ey + ez * ez + (double)softening * softening;
+                        double s = (double)ms / (r2 * sqrt(r2));
+                        dx64 += ex * s; dy64 += ey * s; dz64 += ez * s;
+                    } else {
+                        float ox, oy, oz;
+                        pair(cx, cy, cz, ms, a[0], a[1], a[2], softening, checked, &ox, &oy, &oz);
+                        sx = sx + ox; sy = sy + oy; sz = sz + oz;
+                    }
                     acc++;
                 }
             }
+            if (accum64) { sx = (float)dx64; sy = (float)dy64; sz = (float)dz64; }
             out[4 * i] = sx; out[4 * i + 1] = sy; out[4 * i + 2] = sz; out[4 * i + 3] = 0.0f;
             if (visits) visits[i] = acc;
         }

@@ -67,9 +67,11 @@ void orc_tree_root_box(const orc_tree *t, float *bmin3, float *bmax3);
 
 /* sequential::BarnesHut traversal (particular/src/sequential.rs:456-506) for every affected
  * particle. com64 = 0 uses the reference's f32 node data, 1 uses the FP64 reduction rounded to
- * f32. `visits` (optional, n entries) receives the number of accepted node interactions. */
+ * f32. accum64 = 1 keeps every decision but evaluates and sums the accepted pair terms in FP64
+ * (isolates the summation error of an implementation). `visits` (optional, n entries) receives
+ * the number of accepted node interactions. */
 int orc_bh_f32(const orc_tree *t, const float *affected, size_t n, float theta, float softening,
-               int checked, int com64, float *out, uint32_t *visits, int threads);
+               int checked, int com64, int accum64, float *out, uint32_t *visits, int threads);
 
 #ifdef __cplusplus
 }

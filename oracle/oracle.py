@@ -86,7 +86,7 @@ def lib() -> ctypes.CDLL:
         L.orc_tree_root_box.argtypes = [ctypes.c_void_p, fp, fp]
         L.orc_bh_f32.argtypes = [
             ctypes.c_void_p, fp, sz, ctypes.c_float, ctypes.c_float, ctypes.c_int, ctypes.c_int,
-            fp, ctypes.POINTER(ctypes.c_uint32), ctypes.c_int,
+            ctypes.c_int, fp, ctypes.POINTER(ctypes.c_uint32), ctypes.c_int,
         ]
         _lib = L
     return _lib
@@ -181,12 +181,14 @@ class Tree:
         return mn, mx
 
     def barnes_hut(self, affected, theta, softening=0.0, checked=True, com64=False, threads=0,
-                   return_visits=False):
+                   return_visits=False, accum64=False):
+        """Traversal of sequential::BarnesHut. com64: node data from the FP64 reduction;
+        accum64: same decisions, pair terms summed in FP64."""
         a = _f32x4(affected)
         out = np.zeros((a.shape[0], 4), np.float32)
         visits = np.zeros(a.shape[0], np.uint32)
         rc = lib().orc_bh_f32(self._h, _fp(a), a.shape[0], theta, softening, int(checked),
-                              int(com64), _fp(out),
+                              int(com64), int(accum64), _fp(out),
                               visits.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), threads)
         if rc:
             raise RuntimeError(f"orc_bh_f32 failed ({rc})")

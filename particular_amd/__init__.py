@@ -329,13 +329,15 @@ class BarnesHut(_Algorithm):
     (particular/src/sequential.rs:465-505). ``theta = 0`` degenerates to brute force.
     """
 
-    def __init__(self, resources: HipResources, theta: float, interaction, mlp: bool = False):
+    def __init__(self, resources: HipResources, theta: float, interaction, mlp: bool = False,
+                 reference_com: bool = False):
         super().__init__(resources, interaction)
         self.theta = float(theta)
         self.mlp = bool(mlp)
+        self.reference_com = bool(reference_com)
 
     def _flags(self) -> int:
-        return 1 if self.mlp else 0
+        return (1 if self.mlp else 0) | (2 if self.reference_com else 0)
 
     def _between_host(self, a, b, out):
         # Between(p, p) over one array passes one pointer twice; the library then uploads one

@@ -71,6 +71,18 @@ SIGNATURES = {
     "pnb_batch_run": (_i, [_vp, _sz, _d, _i]),
     "pnb_batch_read": (_i, [_vp, _vp, _vp, _vp]),
     "pnb_batch_info": (_i, [_vp, _c.POINTER(_sz), _c.POINTER(_sz), _c.POINTER(_c.c_ulonglong)]),
+    "pnb_mlp_num_params": (_i, []),
+    "pnb_mlp_init": (_i, [_vp, _u64]),
+    "pnb_mlp_set_params": (_i, [_vp, _vp, _sz]),
+    "pnb_mlp_get_params": (_i, [_vp, _vp, _sz]),
+    "pnb_mlp_collect": (_i, [_vp, _vp, _sz, _f, _f, _sz, _u64, _sz]),
+    "pnb_mlp_clear_dataset": (_i, [_vp]),
+    "pnb_mlp_dataset_info": (_i, [_vp, _c.POINTER(_sz), _c.POINTER(_sz), _pu64]),
+    "pnb_mlp_dataset_read": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "pnb_mlp_dataset_load": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _sz, _sz, _vp, _vp]),
+    "pnb_mlp_grad": (_i, [_vp, _vp, _vp, _sz]),
+    "pnb_mlp_train": (_i, [_vp, _sz, _f, _vp]),
+    "pnb_mlp_forward_dev": (_i, [_vp, _vp, _sz, _vp]),
     "pnb_leapfrog_init": (_i, [_vp, _vp, _vp, _vp, _sz, _f, _i]),
     "pnb_leapfrog_run": (_i, [_vp, _sz, _d, _i]),
     "pnb_leapfrog_read": (_i, [_vp, _vp, _vp, _vp]),

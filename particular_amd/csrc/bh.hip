@@ -31,9 +31,11 @@
 // reaches skip[k]. Each lane therefore applies exactly the reference's decisions; node records
 // are wave-uniform, so they arrive through scalar loads.
 #include "pnb_internal.hpp"
+#include "mlp_common.hpp"
 
 #include <algorithm>
 #include <cmath>
+#include <vector>
 
 namespace pnb {
 
@@ -77,6 +79,8 @@ struct BhState {
     DevBuf keys, keys_alt, idx, idx_alt, sp, lcp, cnt, ip, key2, rec, sums, sums2, level, width;
     DevBuf info, bbox_part, hist, bits, afg_tmp;
     DevBuf akeys, akeys_alt, aidx, aidx_alt, apts, h_aff, h_afg, h_out;
+    DevBuf c_starts, c_pts, c_bh, c_bf, c_feats, c_scale, c_mask, c_count, c_drop, c_target,
+        c_wgt, c_rows, c_rwave, c_cfeats, c_cscale, c_cmask;
     float last_theta = 0.f;
     unsigned long long last_steps = 0, last_accepted = 0;
 };
@@ -87,7 +91,9 @@ void bh_free(BhState* s) {
                       &s->ip, &s->key2, &s->rec, &s->sums, &s->sums2, &s->level, &s->width,
                       &s->info, &s->bbox_part, &s->hist, &s->bits, &s->afg_tmp, &s->akeys,
                       &s->akeys_alt, &s->aidx, &s->aidx_alt, &s->apts, &s->h_aff, &s->h_afg,
-                      &s->h_out})
+                      &s->h_out, &s->c_starts, &s->c_pts, &s->c_bh, &s->c_bf, &s->c_feats,
+                      &s->c_scale, &s->c_mask, &s->c_count, &s->c_drop, &s->c_target, &s->c_wgt,
+                      &s->c_rows, &s->c_rwave, &s->c_cfeats, &s->c_cscale, &s->c_cmask})
         b->release();
     delete s;
 }
@@ -452,21 +458,141 @@ __global__ __launch_bounds__(256) void bh_up_kernel(NodeRec* __restrict__ rec,
     }
 }
 
+// Reference-exact node data (PNB_BH_REF_COM): the f32 folds of impls/mod.rs:126-131 in the
+// node's particle order (tree order restricted to the node), one thread per node. The folds are
+// sequential by definition, so a node of k bodies costs k dependent steps on one lane.
+__global__ __launch_bounds__(256) void bh_ref_com_kernel(NodeRec* __restrict__ rec,
+                                                         const float4* __restrict__ sp,
+                                                         size_t total) {
+    for (size_t k = size_t(blockIdx.x) * blockDim.x + threadIdx.x; k < total;
+         k += size_t(gridDim.x) * blockDim.x) {
+        const uint32_t b = rec[k].begin, e = rec[k].end;
+        float tot = 0.0f;
+        for (uint32_t q = b; q < e; ++q) tot = __fadd_rn(tot, sp[q].w);
+        float cx = 0.0f, cy = 0.0f, cz = 0.0f;
+        if (tot == 0.0f) {
+            for (uint32_t q = b; q < e; ++q) {
+                const float4 v = sp[q];
+                cx = __fadd_rn(cx, v.x); cy = __fadd_rn(cy, v.y); cz = __fadd_rn(cz, v.z);
+            }
+            const float len = float(e - b);
+            cx = __fdiv_rn(cx, len); cy = __fdiv_rn(cy, len); cz = __fdiv_rn(cz, len);
+        } else {
+            for (uint32_t q = b; q < e; ++q) {
+                const float4 v = sp[q];
+                const float f = __fdiv_rn(v.w, tot);
+                cx = __fadd_rn(cx, __fmul_rn(v.x, f));
+                cy = __fadd_rn(cy, __fmul_rn(v.y, f));
+                cz = __fadd_rn(cz, __fmul_rn(v.z, f));
+            }
+        }
+        rec[k].cx = cx;
+        rec[k].cy = cy;
+        rec[k].cz = cz;
+        rec[k].mass = tot;
+    }
+}
+
 // ---- traversal -----------------------------------------------------------------------------------
 
+constexpr int WALK_PLAIN = 0;   // plain Barnes-Hut
+constexpr int WALK_RECORD = 1;  // also write one row per (wavefront, accepted internal node)
+constexpr int WALK_MLP = 2;     // add the learned correction of each accepted internal node
+constexpr int MLP_ROWS = 64;    // rows buffered per wavefront before one MLP batch
+
+struct WalkRecord {
+    float* feats;                // [wave][cap][MLP_FEAT]
+    float* scale;                // [wave][cap]
+    unsigned long long* mask;    // [wave][cap]
+    uint32_t* count;             // [wave] rows written
+    uint32_t cap;
+    unsigned long long* dropped;
+};
+
+struct WalkRows {  // per-wavefront LDS batch of rows for the MLP
+    uint16_t x[MLP_ROWS][MLP_KIN];
+    float y[MLP_ROWS][4];
+    float scale[MLP_ROWS];
+    unsigned long long mask[MLP_ROWS];
+};
+
+template <int MODE>
 __global__ __launch_bounds__(256) void bh_walk_kernel(const NodeRec* __restrict__ nodes,
                                                       uint32_t total,
+                                                      const float* __restrict__ widths,
                                                       const float4* __restrict__ pts,
                                                       const uint32_t* __restrict__ perm,
                                                       uint32_t n, float theta2, float eps2,
                                                       float4* __restrict__ out,
-                                                      BhInfo* __restrict__ info) {
+                                                      BhInfo* __restrict__ info,
+                                                      const MlpWeights* __restrict__ wg,
+                                                      WalkRecord rec) {
+    const MlpForward* mwp = nullptr;
+    WalkRows* rows_base = nullptr;
+    if constexpr (MODE == WALK_MLP) {  // LDS only in the MLP instantiation
+        __shared__ MlpForward mw;
+        __shared__ WalkRows rows_lds[4];
+        const uint32_t* src = reinterpret_cast<const uint32_t*>(wg);
+        uint32_t* dst = reinterpret_cast<uint32_t*>(&mw);
+        for (int t = threadIdx.x; t < int(sizeof(MlpForward) / 4); t += blockDim.x) dst[t] = src[t];
+        __syncthreads();
+        mwp = &mw;
+        rows_base = rows_lds;
+    }
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
     const bool valid = i < n;
     const float4 p = pts[valid ? i : n - 1];
     float ax = 0.f, ay = 0.f, az = 0.f, cx = 0.f, cy = 0.f, cz = 0.f;
+    float kx = 0.f, ky = 0.f, kz = 0.f;  // learned correction
     uint32_t resume = valid ? 0u : 0xffffffffu;
     uint32_t accepted = 0, steps = 0;
+
+    // wavefront centroid and rms radius (row features are relative to the wavefront)
+    float wx = 0.f, wy = 0.f, wz = 0.f, wrad = 0.f, total_mass = 1.f;
+    uint32_t nrows = 0;
+    const uint32_t wave_slot = i / 64;
+    if (MODE != WALK_PLAIN) {
+        float sx = valid ? p.x : 0.f, sy = valid ? p.y : 0.f, sz = valid ? p.z : 0.f;
+        float cnt = valid ? 1.f : 0.f;
+        for (int off = 32; off > 0; off >>= 1) {
+            sx += __shfl_xor(sx, off, 64);
+            sy += __shfl_xor(sy, off, 64);
+            sz += __shfl_xor(sz, off, 64);
+            cnt += __shfl_xor(cnt, off, 64);
+        }
+        wx = sx / cnt; wy = sy / cnt; wz = sz / cnt;
+        float r2 = valid ? (p.x - wx) * (p.x - wx) + (p.y - wy) * (p.y - wy) + (p.z - wz) * (p.z - wz) : 0.f;
+        for (int off = 32; off > 0; off >>= 1) r2 += __shfl_xor(r2, off, 64);
+        wrad = sqrtf(r2 / cnt);
+        if (total > 0) total_mass = nodes[0].mass;
+    }
+    WalkRows* wr = MODE == WALK_MLP ? rows_base + (threadIdx.x >> 6) : nullptr;
+
+    auto flush = [&]() {
+        // y for the buffered rows (16-row tiles, MFMA), then each row's correction to its lanes
+        wave_lds_sync();
+        const int r16 = lane & 15, g = lane >> 4;
+        for (uint32_t t0 = 0; t0 < nrows; t0 += 16) {
+            const f32x4_t yv = mlp_forward16_regs(*mwp, frag(&wr->x[t0 + r16][4 * g]));
+            if (g == 0) {
+                wr->y[t0 + r16][0] = yv[0];
+                wr->y[t0 + r16][1] = yv[1];
+                wr->y[t0 + r16][2] = yv[2];
+            }
+        }
+        wave_lds_sync();
+        for (uint32_t r = 0; r < nrows; ++r) {
+            const float s = wr->scale[r];
+            const bool mine = (wr->mask[r] >> lane) & 1ull;
+            kx += mine ? s * wr->y[r][0] : 0.f;
+            ky += mine ? s * wr->y[r][1] : 0.f;
+            kz += mine ? s * wr->y[r][2] : 0.f;
+        }
+        wave_lds_sync();
+        nrows = 0;
+    };
+
     uint32_t k = 0;
     while (k < total) {
         const NodeRec r = nodes[k];
@@ -476,7 +602,8 @@ __global__ __launch_bounds__(256) void bh_walk_kernel(const NodeRec* __restrict_
         const bool nz = d2 != 0.0f;
         const bool crit = __fmul_rn(theta2, d2) < r.w2;
         const bool open = active && nz && crit;
-        if (active && nz && !crit) {
+        const bool acc = active && nz && !crit;
+        if (acc) {
             const float r2 = __fadd_rn(d2, eps2);
             const float inv = __builtin_amdgcn_rsqf(r2);
             const float s = (r.mass * inv) * (inv * inv);
@@ -488,16 +615,47 @@ __global__ __launch_bounds__(256) void bh_walk_kernel(const NodeRec* __restrict_
             cz = (t - az) - y; az = t;
             ++accepted;
         }
+        if (MODE != WALK_PLAIN && r.w2 >= 0.0f) {  // internal node: candidate row
+            const unsigned long long m = __ballot(acc);
+            if (m) {
+                float f[MLP_FEAT];
+                const float sc = mlp_features(r.cx, r.cy, r.cz, r.mass, widths[k], wx, wy, wz, wrad,
+                                              total_mass, __popcll(m), f);
+                if (MODE == WALK_RECORD) {
+                    if (lane == 0) {
+                        if (nrows < rec.cap) {
+                            const size_t row = size_t(wave_slot) * rec.cap + nrows;
+                            for (int q = 0; q < MLP_FEAT; ++q) rec.feats[row * MLP_FEAT + q] = f[q];
+                            rec.scale[row] = sc;
+                            rec.mask[row] = m;
+                        } else {
+                            atomicAdd(rec.dropped, 1ull);
+                        }
+                    }
+                    ++nrows;
+                } else {
+                    if (lane == 0) {
+#pragma unroll
+                        for (int q = 0; q < MLP_KIN; ++q)
+                            wr->x[nrows][q] = q < MLP_IN ? f2bf(f[q]) : uint16_t(0);
+                        wr->scale[nrows] = sc;
+                        wr->mask[nrows] = m;
+                    }
+                    if (++nrows == MLP_ROWS) flush();
+                }
+            }
+        }
         if (active && !open) resume = r.skip;
         ++steps;
         const uint32_t nxt = __any(open) ? k + 1 : max(r.skip, k + 1);
         k = __builtin_amdgcn_readfirstlane(nxt);
     }
-    if (valid) out[perm ? perm[i] : i] = make_float4(ax, ay, az, 0.f);
-    // counters: node steps per wave (lane 0) and accepted pair terms
+    if (MODE == WALK_MLP && nrows) flush();
+    if (MODE == WALK_RECORD && lane == 0) rec.count[wave_slot] = min(nrows, rec.cap);
+    if (valid) out[perm ? perm[i] : i] = make_float4(ax + kx, ay + ky, az + kz, 0.f);
     unsigned long long acc64 = accepted;
     for (int off = 32; off > 0; off >>= 1) acc64 += __shfl_xor(acc64, off, 64);
-    if ((threadIdx.x & 63) == 0) {
+    if (lane == 0) {
         atomicAdd(&info->steps, (unsigned long long)steps);
         atomicAdd(&info->accepted, acc64);
     }
@@ -505,7 +663,7 @@ __global__ __launch_bounds__(256) void bh_walk_kernel(const NodeRec* __restrict_
 
 // ---- host side --------------------------------------------------------------------------------
 
-static int build_tree(pnb_ctx* c, const float4* afg, size_t m) {
+static int build_tree(pnb_ctx* c, const float4* afg, size_t m, bool ref_com = false) {
     if (!c->bh) c->bh = new BhState();
     BhState* s = c->bh;
     s->built = false;
@@ -591,20 +749,36 @@ static int build_tree(pnb_ctx* c, const float4* afg, size_t m) {
                                                     total, L);
             PNB_CHECK_LAUNCH(c);
         }
+        if (ref_com) {
+            bh_ref_com_kernel<<<div_up(total, 256), 256, 0, c->stream>>>(rec, s->sp.as<float4>(), total);
+            PNB_CHECK_LAUNCH(c);
+        }
         s->built = true;
         return PNB_OK;
     });
 }
 
+const MlpWeights* mlp_weights(pnb_ctx* c);  // mlp.hip: nullptr until initialised
+
 static int walk(pnb_ctx* c, const float4* pts, const uint32_t* perm, size_t n, float theta,
-                float softening, float4* out) {
+                float softening, float4* out, int mode = WALK_PLAIN,
+                const WalkRecord* record = nullptr) {
     BhState* s = c->bh;
     const float theta2 = theta * theta;  // (self.theta * self.theta), sequential.rs:491
     const float eps2 = softening * softening;
+    const MlpWeights* wg = nullptr;
+    if (mode == WALK_MLP) {
+        wg = mlp_weights(c);
+        if (!wg) return fail(c, PNB_E_STATE, "PNB_BH_MLP needs an initialised MLP (pnb_mlp_init)");
+    }
+    WalkRecord rec = record ? *record : WalkRecord{};
     return timed(c, PNB_PROF_BH_WALK, [&]() -> int {
-        bh_walk_kernel<<<div_up(n, 256), 256, 0, c->stream>>>(
-            s->rec.as<NodeRec>(), uint32_t(s->nodes), pts, perm, uint32_t(n), theta2, eps2, out,
-            s->info.as<BhInfo>());
+        const unsigned grid = div_up(n, 256);
+        auto* kern = mode == WALK_MLP ? bh_walk_kernel<WALK_MLP>
+                     : mode == WALK_RECORD ? bh_walk_kernel<WALK_RECORD> : bh_walk_kernel<WALK_PLAIN>;
+        kern<<<grid, 256, 0, c->stream>>>(s->rec.as<NodeRec>(), uint32_t(s->nodes),
+                                          s->width.as<float>(), pts, perm, uint32_t(n), theta2,
+                                          eps2, out, s->info.as<BhInfo>(), wg, rec);
         PNB_CHECK_LAUNCH(c);
         return PNB_OK;
     });
@@ -613,12 +787,14 @@ static int walk(pnb_ctx* c, const float4* pts, const uint32_t* perm, size_t n, f
 // Orders the affected particles along the tree's key order so that each wavefront holds
 // neighbours, walks, and scatters results back to the caller's order.
 static int accel_dev(pnb_ctx* c, const float4* affected, size_t n, const float4* afg, size_t m,
-                     float theta, float softening, float4* out) {
+                     float theta, float softening, float4* out, int mode = WALK_PLAIN,
+                     bool ref_com = false) {
     if (n == 0) return PNB_OK;
     if (n > 0x7fffffffu) return fail(c, PNB_E_ARG, "Barnes-Hut supports at most 2^31-1 particles");
-    if (int rc = build_tree(c, afg, m)) return rc;
+    if (int rc = build_tree(c, afg, m, ref_com)) return rc;
     BhState* s = c->bh;
-    if (affected == afg && n == m) return walk(c, s->sp.as<float4>(), s->sidx, n, theta, softening, out);
+    if (affected == afg && n == m)
+        return walk(c, s->sp.as<float4>(), s->sidx, n, theta, softening, out, mode);
     const unsigned g = div_up(n, 256);
     PNB_HIP(c, s->akeys.ensure(n * 8));
     PNB_HIP(c, s->akeys_alt.ensure(n * 8));
@@ -627,7 +803,7 @@ static int accel_dev(pnb_ctx* c, const float4* affected, size_t n, const float4*
     PNB_HIP(c, s->apts.ensure(n * sizeof(float4)));
     if (m == 0) {
         // no tree: every particle gets zero (the reference's root is None)
-        return walk(c, affected, nullptr, n, theta, softening, out);
+        return walk(c, affected, nullptr, n, theta, softening, out, mode);
     }
     bh_keys_kernel<<<g, 256, 0, c->stream>>>(affected, n, s->info.as<BhInfo>(),
                                              s->akeys.as<uint64_t>(), s->aidx.as<uint32_t>());
@@ -640,7 +816,7 @@ static int accel_dev(pnb_ctx* c, const float4* affected, size_t n, const float4*
         return rc;
     bh_gather_kernel<<<g, 256, 0, c->stream>>>(affected, n, si, s->apts.as<float4>());
     PNB_CHECK_LAUNCH(c);
-    return walk(c, s->apts.as<float4>(), si, n, theta, softening, out);
+    return walk(c, s->apts.as<float4>(), si, n, theta, softening, out, mode);
 }
 
 static int copy_h2d(pnb_ctx* c, DevBuf& b, const float* src, size_t count) {
@@ -648,6 +824,117 @@ static int copy_h2d(pnb_ctx* c, DevBuf& b, const float* src, size_t count) {
     if (count)
         PNB_HIP(c, hipMemcpyAsync(b.ptr, src, count * sizeof(float4), hipMemcpyHostToDevice, c->stream));
     return PNB_OK;
+}
+
+// ---- training data for the learned correction -------------------------------------------------
+
+__global__ void mlp_gather_waves_kernel(const float4* __restrict__ sp, const uint32_t* __restrict__ starts,
+                                        uint32_t waves, float4* __restrict__ dst) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < waves * 64) dst[i] = sp[starts[i / 64] + (i % 64)];
+}
+
+__global__ void mlp_zero_u64_kernel(unsigned long long* p) { *p = 0ull; }
+
+// target = brute force - Barnes-Hut, weight = 1 / |brute force|^2 (0 when it vanishes)
+__global__ void mlp_target_kernel(const float4* __restrict__ bf, const float4* __restrict__ bh,
+                                  uint32_t n, float* __restrict__ target, float* __restrict__ wgt) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 a = bf[i], b = bh[i];
+    target[4 * i] = a.x - b.x;
+    target[4 * i + 1] = a.y - b.y;
+    target[4 * i + 2] = a.z - b.z;
+    target[4 * i + 3] = 0.f;
+    const float n2 = a.x * a.x + a.y * a.y + a.z * a.z;
+    wgt[i] = n2 > 0.f ? 1.f / n2 : 0.f;
+}
+
+// Moves each wave's recorded rows [w * cap, w * cap + count) to [wave_rows[w], wave_rows[w+1]).
+__global__ void mlp_compact_rows_kernel(const float* __restrict__ feats, const float* __restrict__ scale,
+                                        const unsigned long long* __restrict__ mask,
+                                        const uint32_t* __restrict__ wave_rows, uint32_t cap,
+                                        float* __restrict__ cfeats, float* __restrict__ cscale,
+                                        unsigned long long* __restrict__ cmask,
+                                        uint32_t* __restrict__ rwave) {
+    const uint32_t w = blockIdx.x;
+    const uint32_t b = wave_rows[w], cnt = wave_rows[w + 1] - b;
+    for (uint32_t r = threadIdx.x; r < cnt; r += blockDim.x) {
+        const size_t src = size_t(w) * cap + r, dst = size_t(b) + r;
+        for (int q = 0; q < MLP_FEAT; ++q) cfeats[dst * MLP_FEAT + q] = feats[src * MLP_FEAT + q];
+        cscale[dst] = scale[src];
+        cmask[dst] = mask[src];
+        rwave[dst] = w;
+    }
+}
+
+int mlp_append(pnb_ctx* c, const float* feats, const float* scale, const unsigned long long* mask,
+               const uint32_t* row_wave_local, const uint32_t* wave_rows_local, size_t rows,
+               size_t waves, const float* target, const float* wgt, unsigned long long dropped);
+
+static int collect(pnb_ctx* c, const float4* p, size_t n, float theta, float softening,
+                   size_t waves, uint64_t seed, uint32_t cap) {
+    if (int rc = build_tree(c, p, n)) return rc;
+    BhState* s = c->bh;
+    const size_t full = n / 64;
+    const size_t W = std::min(waves, full);
+    if (W == 0) return fail(c, PNB_E_ARG, "need at least 64 particles");
+    std::vector<uint32_t> starts(W);
+    const size_t stride = full / W;
+    const size_t offset = size_t(seed % stride);
+    for (size_t w = 0; w < W; ++w) starts[w] = uint32_t((offset + w * stride) * 64);
+    const size_t nr = W * 64;
+    PNB_HIP(c, s->c_starts.ensure(W * 4));
+    PNB_HIP(c, s->c_pts.ensure(nr * sizeof(float4)));
+    PNB_HIP(c, s->c_bh.ensure(nr * sizeof(float4)));
+    PNB_HIP(c, s->c_bf.ensure(nr * sizeof(float4)));
+    PNB_HIP(c, s->c_feats.ensure(W * cap * MLP_FEAT * 4));
+    PNB_HIP(c, s->c_scale.ensure(W * cap * 4));
+    PNB_HIP(c, s->c_mask.ensure(W * cap * 8));
+    PNB_HIP(c, s->c_count.ensure(W * 4));
+    PNB_HIP(c, s->c_drop.ensure(8));
+    PNB_HIP(c, s->c_target.ensure(nr * 16));
+    PNB_HIP(c, s->c_wgt.ensure(nr * 4));
+    PNB_HIP(c, s->c_rows.ensure((W + 1) * 4));
+    PNB_HIP(c, hipMemcpyAsync(s->c_starts.ptr, starts.data(), W * 4, hipMemcpyHostToDevice, c->stream));
+    mlp_gather_waves_kernel<<<div_up(nr, 256), 256, 0, c->stream>>>(
+        s->sp.as<float4>(), s->c_starts.as<uint32_t>(), uint32_t(W), s->c_pts.as<float4>());
+    PNB_CHECK_LAUNCH(c);
+    mlp_zero_u64_kernel<<<1, 1, 0, c->stream>>>(s->c_drop.as<unsigned long long>());
+    PNB_CHECK_LAUNCH(c);
+    WalkRecord rec{s->c_feats.as<float>(), s->c_scale.as<float>(), s->c_mask.as<unsigned long long>(),
+                   s->c_count.as<uint32_t>(), cap, s->c_drop.as<unsigned long long>()};
+    if (int rc = walk(c, s->c_pts.as<float4>(), nullptr, nr, theta, softening, s->c_bh.as<float4>(),
+                      WALK_RECORD, &rec))
+        return rc;
+    if (int rc = bf_launch(c, s->c_pts.as<float4>(), nr, s->sp.as<float4>(), n, softening, 1,
+                           s->c_bf.as<float4>()))
+        return rc;
+    mlp_target_kernel<<<div_up(nr, 256), 256, 0, c->stream>>>(s->c_bf.as<float4>(), s->c_bh.as<float4>(),
+                                                            uint32_t(nr), s->c_target.as<float>(),
+                                                            s->c_wgt.as<float>());
+    PNB_CHECK_LAUNCH(c);
+    std::vector<uint32_t> counts(W), rows(W + 1, 0);
+    unsigned long long dropped = 0;
+    PNB_HIP(c, hipMemcpyAsync(counts.data(), s->c_count.ptr, W * 4, hipMemcpyDeviceToHost, c->stream));
+    PNB_HIP(c, hipMemcpyAsync(&dropped, s->c_drop.ptr, 8, hipMemcpyDeviceToHost, c->stream));
+    PNB_HIP(c, hipStreamSynchronize(c->stream));
+    for (size_t w = 0; w < W; ++w) rows[w + 1] = rows[w] + counts[w];
+    const size_t R = rows[W];
+    PNB_HIP(c, s->c_cfeats.ensure(std::max<size_t>(R, 1) * MLP_FEAT * 4));
+    PNB_HIP(c, s->c_cscale.ensure(std::max<size_t>(R, 1) * 4));
+    PNB_HIP(c, s->c_cmask.ensure(std::max<size_t>(R, 1) * 8));
+    PNB_HIP(c, s->c_rwave.ensure(std::max<size_t>(R, 1) * 4));
+    PNB_HIP(c, hipMemcpyAsync(s->c_rows.ptr, rows.data(), (W + 1) * 4, hipMemcpyHostToDevice, c->stream));
+    mlp_compact_rows_kernel<<<unsigned(W), 256, 0, c->stream>>>(
+        s->c_feats.as<float>(), s->c_scale.as<float>(), s->c_mask.as<unsigned long long>(),
+        s->c_rows.as<uint32_t>(), cap, s->c_cfeats.as<float>(), s->c_cscale.as<float>(),
+        s->c_cmask.as<unsigned long long>(), s->c_rwave.as<uint32_t>());
+    PNB_CHECK_LAUNCH(c);
+    return mlp_append(c, s->c_cfeats.as<float>(), s->c_cscale.as<float>(),
+                      s->c_cmask.as<unsigned long long>(), s->c_rwave.as<uint32_t>(),
+                      s->c_rows.as<uint32_t>(), R, W, s->c_target.as<float>(), s->c_wgt.as<float>(),
+                      dropped);
 }
 
 }  // namespace pnb
@@ -667,11 +954,12 @@ int pnb_bh_accel_f32_dev(pnb_ctx* c, const float* affected, size_t n, const floa
                          float* out) {
     PNB_BH_ARGS_CHECK();
     (void)checked;  // the traversal skips zero-distance nodes for both (sequential.rs:485-487)
-    (void)flags;
+    if (flags & ~(PNB_BH_MLP | PNB_BH_REF_COM)) return fail(c, PNB_E_ARG, "unknown Barnes-Hut flags");
     if (int rc = activate(c)) return rc;
     return accel_dev(c, reinterpret_cast<const float4*>(affected), n,
                      reinterpret_cast<const float4*>(affecting), m, theta, softening,
-                     reinterpret_cast<float4*>(out));
+                     reinterpret_cast<float4*>(out), (flags & PNB_BH_MLP) ? WALK_MLP : WALK_PLAIN,
+                     (flags & PNB_BH_REF_COM) != 0);
 }
 
 int pnb_bh_accel_f32(pnb_ctx* c, const float* affected, size_t n, const float* affecting,
@@ -712,8 +1000,9 @@ int pnb_reordered_bh_accel_f32_dev(pnb_ctx* c, const float* particles, size_t n,
     const float4* afg = s->afg_tmp.as<float4>();
     if (m == n) afg = p;  // every particle is affecting: same storage, the tree order is reused
     (void)checked;
-    (void)flags;
-    return accel_dev(c, p, n, afg, m, theta, softening, reinterpret_cast<float4*>(out));
+    if (flags & ~(PNB_BH_MLP | PNB_BH_REF_COM)) return fail(c, PNB_E_ARG, "unknown Barnes-Hut flags");
+    return accel_dev(c, p, n, afg, m, theta, softening, reinterpret_cast<float4*>(out),
+                     (flags & PNB_BH_MLP) ? WALK_MLP : WALK_PLAIN, (flags & PNB_BH_REF_COM) != 0);
 }
 
 int pnb_reordered_bh_accel_f32(pnb_ctx* c, const float* particles, size_t n, float theta,
@@ -793,6 +1082,17 @@ int pnb_bh_stats(pnb_ctx* c, uint64_t* wave_steps, uint64_t* accepted) {
     if (wave_steps) *wave_steps = h.steps;
     if (accepted) *accepted = h.accepted;
     return PNB_OK;
+}
+
+int pnb_mlp_collect(pnb_ctx* c, const float* particles, size_t n, float theta, float softening,
+                    size_t waves, uint64_t seed, size_t row_cap) {
+    if (!c) return PNB_E_ARG;
+    if (!particles || n == 0 || waves == 0) return fail(c, PNB_E_ARG, "no particles or waves");
+    if (row_cap == 0 || row_cap > 65536) return fail(c, PNB_E_ARG, "row_cap must be in 1..65536");
+    if (!(softening >= 0.0f)) return fail(c, PNB_E_ARG, "softening must be >= 0");
+    if (int rc = activate(c)) return rc;
+    return collect(c, reinterpret_cast<const float4*>(particles), n, theta, softening, waves, seed,
+                   uint32_t(row_cap));
 }
 
 }  // extern "C"

@@ -1,7 +1,666 @@
-// Placeholder translation unit; filled in by the subsystem that owns MlpState.
+// Training and standalone inference of the far-field correction MLP (no reference counterpart).
+//
+// Parameters live as an FP32 master vector (Adam state beside it) and are packed into BF16 MFMA
+// operand copies (MlpWeights) after every update. A training set holds rows (features, monopole
+// scale, accepting-lane mask, wavefront id) and per-lane targets (brute force minus Barnes-Hut
+// acceleration) with weights 1 / |a_bf|^2. One full-batch step:
+//   mlp_fwd_kernel    y = MLP(x) for every row (MFMA, 16-row tiles per wavefront)
+//   mlp_lane_kernel   per wavefront: pred_l = sum over its rows of scale * y for lanes in the
+//                     mask; loss; g_pred; g_y(row) = scale * sum over its lanes of g_pred
+//   mlp_bwd_kernel    recomputes the forward per 16-row tile and back-propagates with MFMA;
+//                     weight gradients accumulate in registers, one partial per wavefront
+//   mlp_reduce_kernel fixed-order sum of the partials (deterministic)
+//   mlp_adam_kernel   Adam on the FP32 master weights, then mlp_pack_kernel
+// The loss and its gradients are those of oracle/mlp_oracle.py with the same BF16 rounding points.
 #include "pnb_internal.hpp"
+#include "mlp_common.hpp"
+
+#include <cmath>
+#include <vector>
 
 namespace pnb {
-struct MlpState {};
-void mlp_free(MlpState* s) { delete s; }
+
+constexpr int MLP_WAVES_PER_BLOCK = 4;
+constexpr int MLP_BWD_BLOCKS = 256;
+
+struct MlpState {
+    bool ready = false;
+    int adam_t = 0;
+    DevBuf params, m, v, grads, weights, partial, loss_hist;
+    // training set
+    size_t rows = 0, waves = 0, cap_rows = 0, cap_waves = 0;
+    unsigned long long dropped = 0;
+    DevBuf feats, scale, mask, rwave, target, wgt, wave_rows;
+    DevBuf y, gy, loss_part;
+};
+
+void mlp_free(MlpState* s) {
+    if (!s) return;
+    for (DevBuf* b : {&s->params, &s->m, &s->v, &s->grads, &s->weights, &s->partial, &s->loss_hist,
+                      &s->feats, &s->scale, &s->mask, &s->rwave, &s->target, &s->wgt,
+                      &s->wave_rows, &s->y, &s->gy, &s->loss_part})
+        b->release();
+    delete s;
+}
+
+// ---- parameter packing -------------------------------------------------------------------------
+
+__global__ void mlp_pack_kernel(const float* __restrict__ p, MlpWeights* __restrict__ w) {
+    for (int t = threadIdx.x; t < MLP_H * MLP_H; t += blockDim.x) {
+        const int n = t / MLP_H, k = t % MLP_H;
+        w->w2t[n][k] = f2bf(p[MLP_OFF_W2 + k * MLP_H + n]);
+        w->w2[n][k] = f2bf(p[MLP_OFF_W2 + n * MLP_H + k]);
+    }
+    for (int t = threadIdx.x; t < MLP_H * MLP_KIN; t += blockDim.x) {
+        const int n = t / MLP_KIN, k = t % MLP_KIN;
+        w->w1t[n][k] = k < MLP_IN ? f2bf(p[MLP_OFF_W1 + k * MLP_H + n]) : uint16_t(0);
+        w->w3[n][k] = k < MLP_OUT ? f2bf(p[MLP_OFF_W3 + n * MLP_OUT + k]) : uint16_t(0);
+    }
+    for (int t = threadIdx.x; t < MLP_NOUT * MLP_H; t += blockDim.x) {
+        const int n = t / MLP_H, k = t % MLP_H;
+        w->w3t[n][k] = n < MLP_OUT ? f2bf(p[MLP_OFF_W3 + k * MLP_OUT + n]) : uint16_t(0);
+    }
+    for (int t = threadIdx.x; t < MLP_H; t += blockDim.x) {
+        w->b1[t] = p[MLP_OFF_B1 + t];
+        w->b2[t] = p[MLP_OFF_B2 + t];
+    }
+    for (int t = threadIdx.x; t < MLP_NOUT; t += blockDim.x) w->b3[t] = t < MLP_OUT ? p[MLP_OFF_B3 + t] : 0.f;
+}
+
+template <class W>
+__device__ void load_weights(W& dst, const MlpWeights* __restrict__ src) {
+    const uint32_t* s = reinterpret_cast<const uint32_t*>(src);
+    uint32_t* d = reinterpret_cast<uint32_t*>(&dst);
+    for (int t = threadIdx.x; t < int(sizeof(W) / 4); t += blockDim.x) d[t] = s[t];
+    __syncthreads();
+}
+
+// ---- forward over rows --------------------------------------------------------------------------
+
+__device__ __forceinline__ void load_x16(MlpTile& t, const float* __restrict__ feats, size_t row0,
+                                         size_t rows, uint16_t (*xt)[16] = nullptr) {
+    const int lane = threadIdx.x & 63;
+    if (lane < 16) {
+        const size_t r = row0 + lane;
+        const bool valid = r < rows;
+#pragma unroll
+        for (int k = 0; k < MLP_KIN; ++k) {
+            const uint16_t v = (valid && k < MLP_IN) ? f2bf(feats[r * MLP_FEAT + k]) : uint16_t(0);
+            t.x[lane][k] = v;
+            if (xt) xt[k][lane] = v;
+        }
+    }
+    wave_lds_sync();
+}
+
+// y (rows x 4, w = 0) = MLP(feats); the register-resident forward used by the Barnes-Hut walk.
+__global__ __launch_bounds__(256) void mlp_fwd_kernel(const float* __restrict__ feats, size_t rows,
+                                                      const MlpWeights* __restrict__ wg,
+                                                      float* __restrict__ y) {
+    __shared__ MlpForward w;
+    load_weights(w, wg);
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int r16 = lane & 15, g = lane >> 4;
+    const size_t gw = size_t(blockIdx.x) * MLP_WAVES_PER_BLOCK + wv;
+    const size_t nw = size_t(gridDim.x) * MLP_WAVES_PER_BLOCK;
+    for (size_t tile = gw; tile * 16 < rows; tile += nw) {
+        const size_t r = tile * 16 + r16;
+        bf16x4_t bx;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int k = 4 * g + i;
+            bx[i] = short((r < rows && k < MLP_IN) ? f2bf(feats[r * MLP_FEAT + k]) : uint16_t(0));
+        }
+        const f32x4_t yv = mlp_forward16_regs(w, bx);
+        if (g == 0 && r < rows) {
+            y[r * 4] = yv[0];
+            y[r * 4 + 1] = yv[1];
+            y[r * 4 + 2] = yv[2];
+            y[r * 4 + 3] = 0.f;
+        }
+    }
+}
+
+// ---- per-lane aggregation, loss, gradient of y ---------------------------------------------------
+
+__global__ __launch_bounds__(64) void mlp_lane_kernel(const float* __restrict__ y,
+                                                      const uint32_t* __restrict__ wave_rows,
+                                                      const unsigned long long* __restrict__ mask,
+                                                      const float* __restrict__ scale,
+                                                      const float* __restrict__ target,
+                                                      const float* __restrict__ wgt, float inv_count,
+                                                      float* __restrict__ gy,
+                                                      float* __restrict__ loss_part) {
+    const size_t w = blockIdx.x;
+    const int lane = threadIdx.x;
+    const uint32_t rb = wave_rows[w], re = wave_rows[w + 1];
+    float px = 0.f, py = 0.f, pz = 0.f;
+    for (uint32_t r = rb; r < re; ++r) {
+        if ((mask[r] >> lane) & 1ull) {
+            const float s = scale[r];
+            px = fmaf(s, y[4 * size_t(r)], px);
+            py = fmaf(s, y[4 * size_t(r) + 1], py);
+            pz = fmaf(s, y[4 * size_t(r) + 2], pz);
+        }
+    }
+    const size_t tl = w * 64 + lane;
+    const float dx = px - target[4 * tl], dy = py - target[4 * tl + 1], dz = pz - target[4 * tl + 2];
+    const float wl = wgt[tl];
+    float loss = wl * (dx * dx + dy * dy + dz * dz) * inv_count;
+    const float k = 2.f * wl * inv_count;
+    const float gx = k * dx, gyv = k * dy, gz = k * dz;
+    for (int off = 32; off > 0; off >>= 1) loss += __shfl_xor(loss, off, 64);
+    if (lane == 0) loss_part[w] = loss;
+    for (uint32_t r = rb; r < re; ++r) {
+        const bool in = (mask[r] >> lane) & 1ull;
+        float sx = in ? gx : 0.f, sy = in ? gyv : 0.f, sz = in ? gz : 0.f;
+        for (int off = 32; off > 0; off >>= 1) {
+            sx += __shfl_xor(sx, off, 64);
+            sy += __shfl_xor(sy, off, 64);
+            sz += __shfl_xor(sz, off, 64);
+        }
+        if (lane == 0) {
+            const float s = scale[r];
+            gy[4 * size_t(r)] = s * sx;
+            gy[4 * size_t(r) + 1] = s * sy;
+            gy[4 * size_t(r) + 2] = s * sz;
+            gy[4 * size_t(r) + 3] = 0.f;
+        }
+    }
+}
+
+// ---- backward ------------------------------------------------------------------------------------
+
+struct BwdTile {
+    MlpTile f;
+    uint16_t xt[MLP_KIN][16];   // x transposed [k][row]
+    uint16_t h1t[MLP_H][16];    // h1 transposed
+    uint16_t h2t[MLP_H][16];    // h2 transposed
+    uint16_t gyr[16][MLP_NOUT]; // g_y row-major
+    uint16_t gyt[MLP_NOUT][16]; // g_y transposed
+    uint16_t g2r[16][MLP_H];    // g_z2 row-major
+    uint16_t g2t[MLP_H][16];    // g_z2 transposed
+    uint16_t g1t[MLP_H][16];    // g_z1 transposed
+};
+
+__global__ __launch_bounds__(256) void mlp_bwd_kernel(const float* __restrict__ feats,
+                                                      const float* __restrict__ gyin, size_t rows,
+                                                      const MlpWeights* __restrict__ wg,
+                                                      float* __restrict__ partial) {
+    __shared__ MlpWeights w;
+    __shared__ BwdTile tiles[MLP_WAVES_PER_BLOCK];
+    load_weights(w, wg);
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int r16 = lane & 15, g = lane >> 4;
+    BwdTile& t = tiles[wv];
+    const f32x4_t zero = {0.f, 0.f, 0.f, 0.f};
+    f32x4_t gw1[4], gw2[4][4], gw3[4];
+    float gb1[4] = {0, 0, 0, 0}, gb2[4] = {0, 0, 0, 0}, gb3[3] = {0, 0, 0};
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        gw1[a] = zero;
+        gw3[a] = zero;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) gw2[a][b] = zero;
+    }
+    const size_t gwave = size_t(blockIdx.x) * MLP_WAVES_PER_BLOCK + wv;
+    const size_t nw = size_t(gridDim.x) * MLP_WAVES_PER_BLOCK;
+    for (size_t tile = gwave; tile * 16 < rows; tile += nw) {
+        const size_t row0 = tile * 16;
+        load_x16(t.f, feats, row0, rows, t.xt);
+        if (lane < 16) {
+            const size_t r = row0 + lane;
+            const bool valid = r < rows;
+#pragma unroll
+            for (int n = 0; n < MLP_NOUT; ++n) {
+                const float gv = (valid && n < MLP_OUT) ? gyin[4 * r + n] : 0.f;
+                if (n < MLP_OUT) gb3[n] += gv;
+                const uint16_t b = f2bf(gv);
+                t.gyr[lane][n] = b;
+                t.gyt[n][lane] = b;
+            }
+        }
+        f32x4_t z1[4], z2[4];
+        mlp_forward16(w, t.f, nullptr, z1, z2, t.h1t, t.h2t);  // syncs the tiles written above
+        // gW3 += h2^T g_y
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt)
+            gw3[mt] = mfma16(frag(&t.h2t[16 * mt + r16][4 * g]), frag(&t.gyt[r16][4 * g]), gw3[mt]);
+        // g_z2 = (g_y W3^T) * relu'(z2)
+        const bf16x4_t agy = frag(&t.gyr[r16][4 * g]);
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt) {
+            const f32x4_t gh = mfma16(agy, frag(&w.w3[16 * nt + r16][4 * g]), zero);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const float gz = z2[nt][i] > 0.f ? gh[i] : 0.f;
+                gb2[nt] += gz;
+                const uint16_t b = f2bf(gz);
+                t.g2r[4 * g + i][16 * nt + r16] = b;
+                t.g2t[16 * nt + r16][4 * g + i] = b;
+            }
+        }
+        wave_lds_sync();
+        // gW2 += h1^T g_z2
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt)
+                gw2[mt][nt] = mfma16(frag(&t.h1t[16 * mt + r16][4 * g]), frag(&t.g2t[16 * nt + r16][4 * g]),
+                                     gw2[mt][nt]);
+        // g_z1 = (g_z2 W2^T) * relu'(z1)
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt) {
+            f32x4_t gh = zero;
+#pragma unroll
+            for (int ks = 0; ks < 4; ++ks)
+                gh = mfma16(frag(&t.g2r[r16][16 * ks + 4 * g]), frag(&w.w2[16 * nt + r16][16 * ks + 4 * g]), gh);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const float gz = z1[nt][i] > 0.f ? gh[i] : 0.f;
+                gb1[nt] += gz;
+                t.g1t[16 * nt + r16][4 * g + i] = f2bf(gz);
+            }
+        }
+        wave_lds_sync();
+        // gW1 += x^T g_z1
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt)
+            gw1[nt] = mfma16(frag(&t.xt[r16][4 * g]), frag(&t.g1t[16 * nt + r16][4 * g]), gw1[nt]);
+        wave_lds_sync();
+    }
+    float* out = partial + gwave * MLP_NPARAM;
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int k = 4 * g + i, n = 16 * nt + r16;
+            if (k < MLP_IN) out[MLP_OFF_W1 + k * MLP_H + n] = gw1[nt][i];
+        }
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                out[MLP_OFF_W2 + (16 * mt + 4 * g + i) * MLP_H + 16 * nt + r16] = gw2[mt][nt][i];
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            if (r16 < MLP_OUT) out[MLP_OFF_W3 + (16 * mt + 4 * g + i) * MLP_OUT + r16] = gw3[mt][i];
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) {
+        float a = gb1[nt], b = gb2[nt];
+        a += __shfl_xor(a, 16, 64);
+        a += __shfl_xor(a, 32, 64);
+        b += __shfl_xor(b, 16, 64);
+        b += __shfl_xor(b, 32, 64);
+        if (g == 0) {
+            out[MLP_OFF_B1 + 16 * nt + r16] = a;
+            out[MLP_OFF_B2 + 16 * nt + r16] = b;
+        }
+    }
+#pragma unroll
+    for (int n = 0; n < MLP_OUT; ++n) {
+        float a = gb3[n];
+        for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
+        if (lane == 0) out[MLP_OFF_B3 + n] = a;
+    }
+}
+
+__global__ void mlp_reduce_kernel(const float* __restrict__ partial, int parts,
+                                  float* __restrict__ grads) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= MLP_NPARAM) return;
+    float s = 0.f;
+    for (int p = 0; p < parts; ++p) s += partial[size_t(p) * MLP_NPARAM + k];
+    grads[k] = s;
+}
+
+__global__ void mlp_adam_kernel(float* __restrict__ p, const float* __restrict__ g,
+                                float* __restrict__ m, float* __restrict__ v, float lr, float c1,
+                                float c2) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= MLP_NPARAM) return;
+    const float gk = g[k];
+    const float mk = 0.9f * m[k] + 0.1f * gk;
+    const float vk = 0.999f * v[k] + 0.001f * gk * gk;
+    m[k] = mk;
+    v[k] = vk;
+    p[k] -= lr * (mk / c1) / (sqrtf(vk / c2) + 1e-8f);
+}
+
+__global__ void mlp_loss_sum_kernel(const float* __restrict__ part, size_t n, float* __restrict__ out) {
+    __shared__ float red[256];
+    float s = 0.f;
+    for (size_t i = threadIdx.x; i < n; i += 256) s += part[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = red[0];
+}
+
+// ---- host helpers ------------------------------------------------------------------------------
+
+static uint64_t splitmix(uint64_t& s) {
+    uint64_t z = (s += 0x9e3779b97f4a7c15ull);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+int mlp_ensure(pnb_ctx* c) {
+    if (!c->mlp) c->mlp = new MlpState();
+    MlpState* s = c->mlp;
+    PNB_HIP(c, s->params.ensure(MLP_NPARAM * 4));
+    PNB_HIP(c, s->m.ensure(MLP_NPARAM * 4));
+    PNB_HIP(c, s->v.ensure(MLP_NPARAM * 4));
+    PNB_HIP(c, s->grads.ensure(MLP_NPARAM * 4));
+    PNB_HIP(c, s->weights.ensure(sizeof(MlpWeights)));
+    return PNB_OK;
+}
+
+static int pack(pnb_ctx* c) {
+    mlp_pack_kernel<<<1, 256, 0, c->stream>>>(c->mlp->params.as<float>(), c->mlp->weights.as<MlpWeights>());
+    PNB_CHECK_LAUNCH(c);
+    return PNB_OK;
+}
+
+const MlpWeights* mlp_weights(pnb_ctx* c) {
+    return (c->mlp && c->mlp->ready) ? c->mlp->weights.as<MlpWeights>() : nullptr;
+}
+
+// Appends `rows` rows and `waves` wavefronts (already on the device) to the training set.
+int mlp_append(pnb_ctx* c, const float* feats, const float* scale, const unsigned long long* mask,
+               const uint32_t* row_wave_local, const uint32_t* wave_rows_local, size_t rows,
+               size_t waves, const float* target, const float* wgt, unsigned long long dropped);
+
+// Grows a device array, keeping its first `used` bytes.
+static int grow(pnb_ctx* c, DevBuf& b, size_t used, size_t want) {
+    if (want <= b.cap) return PNB_OK;
+    DevBuf nb;
+    PNB_HIP(c, nb.ensure(want));
+    if (used) PNB_HIP(c, hipMemcpyAsync(nb.ptr, b.ptr, used, hipMemcpyDeviceToDevice, c->stream));
+    PNB_HIP(c, hipStreamSynchronize(c->stream));
+    b.release();
+    b = nb;
+    return PNB_OK;
+}
+
+__global__ void mlp_offset_kernel(const uint32_t* __restrict__ src, size_t n, uint32_t add,
+                                  uint32_t* __restrict__ dst) {
+    size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i] + add;
+}
+
+int mlp_append(pnb_ctx* c, const float* feats, const float* scale, const unsigned long long* mask,
+               const uint32_t* row_wave_local, const uint32_t* wave_rows_local, size_t rows,
+               size_t waves, const float* target, const float* wgt, unsigned long long dropped) {
+    if (int rc = mlp_ensure(c)) return rc;
+    MlpState* s = c->mlp;
+    const size_t nr = s->rows + rows, nwv = s->waves + waves;
+    if (nr > 0xffffffffu) return fail(c, PNB_E_ARG, "training set too large");
+    auto cap = [](size_t need, size_t have) { return need <= have ? have : std::max(need, 2 * have); };
+    const size_t cr = cap(nr, s->cap_rows), cw = cap(nwv, s->cap_waves);
+    if (int rc = grow(c, s->feats, s->rows * MLP_FEAT * 4, cr * MLP_FEAT * 4)) return rc;
+    if (int rc = grow(c, s->scale, s->rows * 4, cr * 4)) return rc;
+    if (int rc = grow(c, s->mask, s->rows * 8, cr * 8)) return rc;
+    if (int rc = grow(c, s->rwave, s->rows * 4, cr * 4)) return rc;
+    if (int rc = grow(c, s->target, s->waves * 64 * 16, cw * 64 * 16)) return rc;
+    if (int rc = grow(c, s->wgt, s->waves * 64 * 4, cw * 64 * 4)) return rc;
+    if (int rc = grow(c, s->wave_rows, (s->waves + 1) * 4, (cw + 1) * 4)) return rc;
+    s->cap_rows = cr;
+    s->cap_waves = cw;
+    hipStream_t st = c->stream;
+    if (rows) {
+        PNB_HIP(c, hipMemcpyAsync(s->feats.as<float>() + s->rows * MLP_FEAT, feats, rows * MLP_FEAT * 4, hipMemcpyDeviceToDevice, st));
+        PNB_HIP(c, hipMemcpyAsync(s->scale.as<float>() + s->rows, scale, rows * 4, hipMemcpyDeviceToDevice, st));
+        PNB_HIP(c, hipMemcpyAsync(s->mask.as<unsigned long long>() + s->rows, mask, rows * 8, hipMemcpyDeviceToDevice, st));
+        mlp_offset_kernel<<<div_up(rows, 256), 256, 0, st>>>(row_wave_local, rows, uint32_t(s->waves),
+                                                             s->rwave.as<uint32_t>() + s->rows);
+        PNB_CHECK_LAUNCH(c);
+    }
+    PNB_HIP(c, hipMemcpyAsync(s->target.as<float>() + s->waves * 64 * 4, target, waves * 64 * 16, hipMemcpyDeviceToDevice, st));
+    PNB_HIP(c, hipMemcpyAsync(s->wgt.as<float>() + s->waves * 64, wgt, waves * 64 * 4, hipMemcpyDeviceToDevice, st));
+    mlp_offset_kernel<<<div_up(waves + 1, 256), 256, 0, st>>>(wave_rows_local, waves + 1, uint32_t(s->rows),
+                                                               s->wave_rows.as<uint32_t>() + s->waves);
+    PNB_CHECK_LAUNCH(c);
+    PNB_HIP(c, hipStreamSynchronize(st));
+    s->rows = nr;
+    s->waves = nwv;
+    s->dropped += dropped;
+    return PNB_OK;
+}
+
+static int step_grads(pnb_ctx* c, float* loss_dev) {
+    MlpState* s = c->mlp;
+    if (s->waves == 0) return fail(c, PNB_E_STATE, "the training set is empty");
+    PNB_HIP(c, s->y.ensure(std::max<size_t>(1, s->rows) * 16));
+    PNB_HIP(c, s->gy.ensure(std::max<size_t>(1, s->rows) * 16));
+    PNB_HIP(c, s->loss_part.ensure(s->waves * 4));
+    PNB_HIP(c, s->partial.ensure(size_t(MLP_BWD_BLOCKS) * MLP_WAVES_PER_BLOCK * MLP_NPARAM * 4));
+    // lanes with a positive weight, counted on the host once per call
+    std::vector<float> hw(s->waves * 64);
+    PNB_HIP(c, hipMemcpyAsync(hw.data(), s->wgt.ptr, hw.size() * 4, hipMemcpyDeviceToHost, c->stream));
+    PNB_HIP(c, hipStreamSynchronize(c->stream));
+    size_t cnt = 0;
+    for (float x : hw) cnt += x > 0.f;
+    const float inv_count = 1.f / float(std::max<size_t>(cnt, 1));
+    const MlpWeights* wg = s->weights.as<MlpWeights>();
+    const unsigned tiles = div_up(std::max<size_t>(s->rows, 1), 16);
+    return timed(c, PNB_PROF_MLP, [&]() -> int {
+        if (s->rows) {
+            mlp_fwd_kernel<<<std::min<unsigned>(1024, div_up(tiles, MLP_WAVES_PER_BLOCK)), 256, 0, c->stream>>>(
+                s->feats.as<float>(), s->rows, wg, s->y.as<float>());
+            PNB_CHECK_LAUNCH(c);
+        }
+        mlp_lane_kernel<<<unsigned(s->waves), 64, 0, c->stream>>>(
+            s->y.as<float>(), s->wave_rows.as<uint32_t>(), s->mask.as<unsigned long long>(),
+            s->scale.as<float>(), s->target.as<float>(), s->wgt.as<float>(), inv_count,
+            s->gy.as<float>(), s->loss_part.as<float>());
+        PNB_CHECK_LAUNCH(c);
+        mlp_loss_sum_kernel<<<1, 256, 0, c->stream>>>(s->loss_part.as<float>(), s->waves, loss_dev);
+        PNB_CHECK_LAUNCH(c);
+        mlp_bwd_kernel<<<MLP_BWD_BLOCKS, 256, 0, c->stream>>>(s->feats.as<float>(), s->gy.as<float>(),
+                                                             s->rows, wg, s->partial.as<float>());
+        PNB_CHECK_LAUNCH(c);
+        mlp_reduce_kernel<<<div_up(MLP_NPARAM, 256), 256, 0, c->stream>>>(
+            s->partial.as<float>(), MLP_BWD_BLOCKS * MLP_WAVES_PER_BLOCK, s->grads.as<float>());
+        PNB_CHECK_LAUNCH(c);
+        return PNB_OK;
+    });
+}
+
 }  // namespace pnb
+
+using namespace pnb;
+
+extern "C" {
+
+int pnb_mlp_init(pnb_ctx* c, uint64_t seed) {
+    if (!c) return PNB_E_ARG;
+    if (int rc = activate(c)) return rc;
+    if (int rc = mlp_ensure(c)) return rc;
+    MlpState* s = c->mlp;
+    // He-normal initialisation (Box-Muller over SplitMix64), last layer scaled by 0.1, zero biases
+    std::vector<float> p(MLP_NPARAM, 0.f);
+    uint64_t st = seed;
+    auto normal = [&]() {
+        const double u1 = (double(splitmix(st) >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+        const double u2 = double(splitmix(st) >> 11) * (1.0 / 9007199254740992.0);
+        return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+    };
+    for (int k = 0; k < MLP_IN * MLP_H; ++k) p[MLP_OFF_W1 + k] = float(normal() * std::sqrt(2.0 / MLP_IN));
+    for (int k = 0; k < MLP_H * MLP_H; ++k) p[MLP_OFF_W2 + k] = float(normal() * std::sqrt(2.0 / MLP_H));
+    for (int k = 0; k < MLP_H * MLP_OUT; ++k) p[MLP_OFF_W3 + k] = float(0.1 * normal() * std::sqrt(2.0 / MLP_H));
+    std::vector<float> zeros(MLP_NPARAM, 0.f);
+    PNB_HIP(c, hipMemcpyAsync(s->params.ptr, p.data(), MLP_NPARAM * 4, hipMemcpyHostToDevice, c->stream));
+    PNB_HIP(c, hipMemcpyAsync(s->m.ptr, zeros.data(), MLP_NPARAM * 4, hipMemcpyHostToDevice, c->stream));
+    PNB_HIP(c, hipMemcpyAsync(s->v.ptr, zeros.data(), MLP_NPARAM * 4, hipMemcpyHostToDevice, c->stream));
+    if (int rc = pack(c)) return rc;
+    PNB_HIP(c, hipStreamSynchronize(c->stream));
+    s->adam_t = 0;
+    s->ready = true;
+    return PNB_OK;
+}
+
+int pnb_mlp_set_params(pnb_ctx* c, const float* params, size_t n) {
+    if (!c || !params || n != MLP_NPARAM) return c ? fail(c, PNB_E_ARG, "expected %d parameters", MLP_NPARAM) : PNB_E_ARG;
+    if (int rc = activate(c)) return rc;
+    if (int rc = mlp_ensure(c)) return rc;
+    MlpState* s = c->mlp;
+    std::vector<float> zeros(MLP_NPARAM, 0.f);
+    PNB_HIP(c, hipMemcpyAsync(s->params.ptr, params, n * 4, hipMemcpyHostToDevice, c->stream));
+    PNB_HIP(c, hipMemcpyAsync(s->m.ptr, zeros.data(), MLP_NPARAM * 4, hipMemcpyHostToDevice, c->stream));
+    PNB_HIP(c, hipMemcpyAsync(s->v.ptr, zeros.data(), MLP_NPARAM * 4, hipMemcpyHostToDevice, c->stream));
+    if (int rc = pack(c)) return rc;
+    PNB_HIP(c, hipStreamSynchronize(c->stream));
+    s->adam_t = 0;
+    s->ready = true;
+    return PNB_OK;
+}
+
+int pnb_mlp_get_params(pnb_ctx* c, float* params, size_t n) {
+    if (!c || !params || n != MLP_NPARAM) return c ? fail(c, PNB_E_ARG, "expected %d parameters", MLP_NPARAM) : PNB_E_ARG;
+    if (!c->mlp || !c->mlp->ready) return fail(c, PNB_E_STATE, "the MLP has not been initialised");
+    if (int rc = activate(c)) return rc;
+    PNB_HIP(c, hipMemcpyAsync(params, c->mlp->params.ptr, n * 4, hipMemcpyDeviceToHost, c->stream));
+    PNB_HIP(c, hipStreamSynchronize(c->stream));
+    return PNB_OK;
+}
+
+int pnb_mlp_num_params(void) { return MLP_NPARAM; }
+
+int pnb_mlp_clear_dataset(pnb_ctx* c) {
+    if (!c) return PNB_E_ARG;
+    if (c->mlp) {
+        c->mlp->rows = 0;
+        c->mlp->waves = 0;
+        c->mlp->dropped = 0;
+    }
+    return PNB_OK;
+}
+
+int pnb_mlp_dataset_info(pnb_ctx* c, size_t* rows, size_t* waves, uint64_t* dropped) {
+    if (!c) return PNB_E_ARG;
+    MlpState* s = c->mlp;
+    if (rows) *rows = s ? s->rows : 0;
+    if (waves) *waves = s ? s->waves : 0;
+    if (dropped) *dropped = s ? s->dropped : 0;
+    return PNB_OK;
+}
+
+int pnb_mlp_dataset_read(pnb_ctx* c, float* feats, float* scale, uint64_t* mask, uint32_t* row_wave,
+                         uint32_t* wave_rows, float* target, float* wgt) {
+    if (!c) return PNB_E_ARG;
+    MlpState* s = c->mlp;
+    if (!s || s->waves == 0) return fail(c, PNB_E_STATE, "the training set is empty");
+    if (int rc = activate(c)) return rc;
+    hipStream_t st = c->stream;
+    if (feats && s->rows) PNB_HIP(c, hipMemcpyAsync(feats, s->feats.ptr, s->rows * MLP_FEAT * 4, hipMemcpyDeviceToHost, st));
+    if (scale && s->rows) PNB_HIP(c, hipMemcpyAsync(scale, s->scale.ptr, s->rows * 4, hipMemcpyDeviceToHost, st));
+    if (mask && s->rows) PNB_HIP(c, hipMemcpyAsync(mask, s->mask.ptr, s->rows * 8, hipMemcpyDeviceToHost, st));
+    if (row_wave && s->rows) PNB_HIP(c, hipMemcpyAsync(row_wave, s->rwave.ptr, s->rows * 4, hipMemcpyDeviceToHost, st));
+    if (wave_rows) PNB_HIP(c, hipMemcpyAsync(wave_rows, s->wave_rows.ptr, (s->waves + 1) * 4, hipMemcpyDeviceToHost, st));
+    if (target) PNB_HIP(c, hipMemcpyAsync(target, s->target.ptr, s->waves * 64 * 16, hipMemcpyDeviceToHost, st));
+    if (wgt) PNB_HIP(c, hipMemcpyAsync(wgt, s->wgt.ptr, s->waves * 64 * 4, hipMemcpyDeviceToHost, st));
+    PNB_HIP(c, hipStreamSynchronize(st));
+    return PNB_OK;
+}
+
+int pnb_mlp_dataset_load(pnb_ctx* c, const float* feats, const float* scale, const uint64_t* mask,
+                         const uint32_t* row_wave, const uint32_t* wave_rows, size_t rows,
+                         size_t waves, const float* target, const float* wgt) {
+    if (!c) return PNB_E_ARG;
+    if (waves == 0 || !wave_rows || !target || !wgt || (rows && (!feats || !scale || !mask || !row_wave)))
+        return fail(c, PNB_E_ARG, "incomplete training set");
+    if (int rc = activate(c)) return rc;
+    DevBuf tmp;
+    const size_t bytes = rows * (MLP_FEAT * 4 + 4 + 8 + 4) + (waves + 1) * 4 + waves * 64 * 20;
+    PNB_HIP(c, tmp.ensure(bytes));
+    char* p = tmp.as<char>();
+    float* df = reinterpret_cast<float*>(p); p += rows * MLP_FEAT * 4;
+    float* ds = reinterpret_cast<float*>(p); p += rows * 4;
+    unsigned long long* dm = reinterpret_cast<unsigned long long*>(p); p += rows * 8;
+    uint32_t* drw = reinterpret_cast<uint32_t*>(p); p += rows * 4;
+    uint32_t* dwr = reinterpret_cast<uint32_t*>(p); p += (waves + 1) * 4;
+    float* dt = reinterpret_cast<float*>(p); p += waves * 64 * 16;
+    float* dwg = reinterpret_cast<float*>(p);
+    hipStream_t st = c->stream;
+    hipError_t e = hipSuccess;
+    if (rows) {
+        if (e == hipSuccess) e = hipMemcpyAsync(df, feats, rows * MLP_FEAT * 4, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(ds, scale, rows * 4, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(dm, mask, rows * 8, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(drw, row_wave, rows * 4, hipMemcpyHostToDevice, st);
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(dwr, wave_rows, (waves + 1) * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(dt, target, waves * 64 * 16, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(dwg, wgt, waves * 64 * 4, hipMemcpyHostToDevice, st);
+    if (e != hipSuccess) {
+        tmp.release();
+        return hip_fail(c, e, "training set upload", __FILE__, __LINE__);
+    }
+    int rc = mlp_append(c, df, ds, dm, drw, dwr, rows, waves, dt, dwg, 0);
+    PNB_HIP(c, hipStreamSynchronize(st));
+    tmp.release();
+    return rc;
+}
+
+// One forward/backward over the whole training set without an update: loss and gradients.
+int pnb_mlp_grad(pnb_ctx* c, float* loss, float* grads, size_t n) {
+    if (!c) return PNB_E_ARG;
+    if (grads && n != MLP_NPARAM) return fail(c, PNB_E_ARG, "expected %d gradients", MLP_NPARAM);
+    if (!c->mlp || !c->mlp->ready) return fail(c, PNB_E_STATE, "the MLP has not been initialised");
+    if (int rc = activate(c)) return rc;
+    MlpState* s = c->mlp;
+    PNB_HIP(c, s->loss_hist.ensure(16));
+    if (int rc = step_grads(c, s->loss_hist.as<float>())) return rc;
+    if (loss) PNB_HIP(c, hipMemcpyAsync(loss, s->loss_hist.ptr, 4, hipMemcpyDeviceToHost, c->stream));
+    if (grads) PNB_HIP(c, hipMemcpyAsync(grads, s->grads.ptr, n * 4, hipMemcpyDeviceToHost, c->stream));
+    PNB_HIP(c, hipStreamSynchronize(c->stream));
+    return PNB_OK;
+}
+
+// `iterations` full-batch Adam steps; loss_history (optional) receives the loss before each step.
+int pnb_mlp_train(pnb_ctx* c, size_t iterations, float lr, float* loss_history) {
+    if (!c) return PNB_E_ARG;
+    if (!c->mlp || !c->mlp->ready) return fail(c, PNB_E_STATE, "the MLP has not been initialised");
+    if (!(lr > 0.f)) return fail(c, PNB_E_ARG, "learning rate must be > 0");
+    if (int rc = activate(c)) return rc;
+    MlpState* s = c->mlp;
+    PNB_HIP(c, s->loss_hist.ensure(std::max<size_t>(iterations, 1) * 4));
+    for (size_t it = 0; it < iterations; ++it) {
+        if (int rc = step_grads(c, s->loss_hist.as<float>() + it)) return rc;
+        s->adam_t += 1;
+        const float c1 = 1.f - std::pow(0.9f, float(s->adam_t)), c2 = 1.f - std::pow(0.999f, float(s->adam_t));
+        mlp_adam_kernel<<<div_up(MLP_NPARAM, 256), 256, 0, c->stream>>>(
+            s->params.as<float>(), s->grads.as<float>(), s->m.as<float>(), s->v.as<float>(), lr, c1, c2);
+        PNB_CHECK_LAUNCH(c);
+        if (int rc = pack(c)) return rc;
+    }
+    if (loss_history && iterations)
+        PNB_HIP(c, hipMemcpyAsync(loss_history, s->loss_hist.ptr, iterations * 4, hipMemcpyDeviceToHost, c->stream));
+    PNB_HIP(c, hipStreamSynchronize(c->stream));
+    return PNB_OK;
+}
+
+// Standalone inference: y (rows x 4, w = 0) = MLP(feats (rows x 8)); device pointers.
+int pnb_mlp_forward_dev(pnb_ctx* c, const float* feats, size_t rows, float* y) {
+    if (!c) return PNB_E_ARG;
+    if (rows && (!feats || !y)) return fail(c, PNB_E_ARG, "null pointer with non-zero length");
+    if (!c->mlp || !c->mlp->ready) return fail(c, PNB_E_STATE, "the MLP has not been initialised");
+    if (rows == 0) return PNB_OK;
+    if (int rc = activate(c)) return rc;
+    const unsigned tiles = div_up(rows, 16);
+    mlp_fwd_kernel<<<std::min<unsigned>(1024, div_up(tiles, MLP_WAVES_PER_BLOCK)), 256, 0, c->stream>>>(
+        feats, rows, c->mlp->weights.as<MlpWeights>(), y);
+    PNB_CHECK_LAUNCH(c);
+    return PNB_OK;
+}
+
+}  // extern "C"

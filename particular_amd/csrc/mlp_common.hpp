@@ -1,0 +1,200 @@
+// Far-field correction MLP: shared device definitions (bh.hip walk, mlp.hip training).
+//
+// Network: 7 inputs -> 64 -> 64 -> 3, ReLU. BF16 weights and activations feed MFMA
+// (v_mfma_f32_16x16x16_bf16, FP32 accumulate); biases and everything outside the products are
+// FP32. A "row" is one (wavefront, accepted internal node) event of the Barnes-Hut walk: its
+// features describe the node as seen from the wavefront's centroid, and its output y is a
+// correction to the node's monopole term, applied as (mass / d^2) * y to every lane of the
+// wavefront that accepted the node. Leaves are never corrected (their monopole is exact).
+//
+// MFMA 16x16x16 bf16 operand layout (lane l of 64): A[m][k] with m = l % 16, k = 4 (l / 16) + i;
+// B[k][n] with n = l % 16, k = 4 (l / 16) + i; D[m][n] with n = l % 16, m = 4 (l / 16) + i,
+// i = 0..3. Operands are read from LDS arrays that keep k contiguous, so each fragment is one
+// 8-byte read.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace pnb {
+
+constexpr int MLP_IN = 7;
+constexpr int MLP_H = 64;
+constexpr int MLP_OUT = 3;
+constexpr int MLP_KIN = 16;   // inputs padded to one K step
+constexpr int MLP_NOUT = 16;  // outputs padded to one N tile
+constexpr int MLP_OFF_W1 = 0;
+constexpr int MLP_OFF_B1 = MLP_OFF_W1 + MLP_IN * MLP_H;
+constexpr int MLP_OFF_W2 = MLP_OFF_B1 + MLP_H;
+constexpr int MLP_OFF_B2 = MLP_OFF_W2 + MLP_H * MLP_H;
+constexpr int MLP_OFF_W3 = MLP_OFF_B2 + MLP_H;
+constexpr int MLP_OFF_B3 = MLP_OFF_W3 + MLP_H * MLP_OUT;
+constexpr int MLP_NPARAM = MLP_OFF_B3 + MLP_OUT;  // 4867
+constexpr int MLP_FEAT = 8;                       // stored feature stride (7 used)
+
+typedef short bf16x4_t __attribute__((ext_vector_type(4)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+
+// BF16 operand copies of the weights (as raw bf16 bits) and FP32 biases. The forward part
+// (MlpForward) is a prefix, so inference kernels stage only that much in LDS.
+struct MlpForward {
+    uint16_t w1t[MLP_H][MLP_KIN];   // [n][k] = W1[k][n], k >= 7 zero
+    uint16_t w2t[MLP_H][MLP_H];     // [n][k] = W2[k][n]
+    uint16_t w3t[MLP_NOUT][MLP_H];  // [n][k] = W3[k][n], n >= 3 zero
+    float b1[MLP_H], b2[MLP_H], b3[MLP_NOUT];
+};
+struct MlpWeights : MlpForward {
+    uint16_t w2[MLP_H][MLP_H];      // [n][k] = W2[n][k]   (B operand of g W2^T)
+    uint16_t w3[MLP_H][MLP_NOUT];   // [n][k] = W3[n][k]   (B operand of g W3^T)
+};
+
+__device__ __forceinline__ uint16_t f2bf(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even (finite inputs)
+    return uint16_t(u >> 16);
+}
+
+__device__ __forceinline__ bf16x4_t frag(const uint16_t* p) {
+    return *reinterpret_cast<const bf16x4_t*>(p);
+}
+
+__device__ __forceinline__ f32x4_t mfma16(bf16x4_t a, bf16x4_t b, f32x4_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+// Makes this wavefront's LDS writes visible to its own later reads by other lanes. Only
+// wave-scope ordering is needed: the MLP tiles are private to one wavefront, and callers run them
+// inside per-wave loops where a workgroup barrier could deadlock.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ bf16x4_t pack_relu(const f32x4_t& z) {
+    bf16x4_t r;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) r[i] = short(f2bf(fmaxf(z[i], 0.f)));
+    return r;
+}
+
+// Forward of 16 rows computed transposed (Z^T = W^T X^T), entirely in registers: every MFMA
+// result D[m][n] (m = 4 (l/16) + i, n = l % 16) is the B fragment (k = 4 (l/16) + i, n = l % 16)
+// of the next layer's product. `bx` is the B fragment of X^T: features 4 (l/16) .. +3 of row l % 16.
+// Returns D of Y^T: lanes 0..15 hold y[row l][0..3] (component 3 is padding).
+__device__ __forceinline__ f32x4_t mlp_forward16_regs(const MlpForward& w, bf16x4_t bx) {
+    const int lane = threadIdx.x & 63;
+    const int r16 = lane & 15, g = lane >> 4;
+    const f32x4_t zero = {0.f, 0.f, 0.f, 0.f};
+    bf16x4_t h1[4], h2[4];
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+        f32x4_t z = mfma16(frag(&w.w1t[16 * mt + r16][4 * g]), bx, zero);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) z[i] += w.b1[16 * mt + 4 * g + i];
+        h1[mt] = pack_relu(z);
+    }
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+        f32x4_t z = zero;
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks) z = mfma16(frag(&w.w2t[16 * mt + r16][16 * ks + 4 * g]), h1[ks], z);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) z[i] += w.b2[16 * mt + 4 * g + i];
+        h2[mt] = pack_relu(z);
+    }
+    f32x4_t y = zero;
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) y = mfma16(frag(&w.w3t[r16][16 * ks + 4 * g]), h2[ks], y);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) y[i] += (4 * g + i < MLP_OUT) ? w.b3[4 * g + i] : 0.f;
+    return y;
+}
+
+// Per-wave scratch of a 16-row forward pass.
+struct MlpTile {
+    uint16_t x[16][MLP_KIN];   // inputs, row-major
+    uint16_t h1[16][MLP_H];    // relu(z1), row-major
+    uint16_t h2[16][MLP_H];    // relu(z2), row-major
+};
+
+// Forward of 16 rows held in t.x; writes y (16 x 3, row-major with stride 4) to `y` when y is not
+// null. z1 / z2 accumulators stay in registers (D layout) for the backward pass; h1t / h2t, when
+// given, receive the activations transposed ([hidden][row]) for weight gradients.
+__device__ __forceinline__ void mlp_forward16(const MlpForward& w, MlpTile& t, float* y,
+                                              f32x4_t (&z1)[4], f32x4_t (&z2)[4],
+                                              uint16_t (*h1t)[16] = nullptr,
+                                              uint16_t (*h2t)[16] = nullptr) {
+    const int lane = threadIdx.x & 63;
+    const int r16 = lane & 15, g = lane >> 4;
+    const f32x4_t zero = {0.f, 0.f, 0.f, 0.f};
+    const bf16x4_t ax = frag(&t.x[r16][4 * g]);
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) {
+        z1[nt] = mfma16(ax, frag(&w.w1t[16 * nt + r16][4 * g]), zero);
+        const float b = w.b1[16 * nt + r16];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            z1[nt][i] += b;
+            const uint16_t h = f2bf(fmaxf(z1[nt][i], 0.f));
+            t.h1[4 * g + i][16 * nt + r16] = h;
+            if (h1t) h1t[16 * nt + r16][4 * g + i] = h;
+        }
+    }
+    wave_lds_sync();
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) {
+        f32x4_t acc = zero;
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks)
+            acc = mfma16(frag(&t.h1[r16][16 * ks + 4 * g]), frag(&w.w2t[16 * nt + r16][16 * ks + 4 * g]), acc);
+        const float b = w.b2[16 * nt + r16];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            acc[i] += b;
+            const uint16_t h = f2bf(fmaxf(acc[i], 0.f));
+            t.h2[4 * g + i][16 * nt + r16] = h;
+            if (h2t) h2t[16 * nt + r16][4 * g + i] = h;
+        }
+        z2[nt] = acc;
+    }
+    wave_lds_sync();
+    f32x4_t acc = zero;
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks)
+        acc = mfma16(frag(&t.h2[r16][16 * ks + 4 * g]), frag(&w.w3t[r16][16 * ks + 4 * g]), acc);
+    if (y && r16 < MLP_OUT) {
+        const float b = w.b3[r16];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) y[(4 * g + i) * 4 + r16] = acc[i] + b;
+    }
+    wave_lds_sync();
+}
+
+// Features of one (wavefront, internal node) row, as seen from the wavefront centroid c:
+// unit direction to the node's centre of mass (3), node width / distance, log2 of the node's
+// mass fraction scaled to about [0, 1], wavefront rms radius / distance, accepting-lane fraction.
+// Returns the monopole scale mass / d^2 (0 when d == 0: the row is then ignored).
+__device__ __forceinline__ float mlp_features(float cx, float cy, float cz, float mass, float width,
+                                              float wx, float wy, float wz, float wrad,
+                                              float total_mass, int accepting, float* f) {
+    const float rx = cx - wx, ry = cy - wy, rz = cz - wz;
+    const float d2 = rx * rx + ry * ry + rz * rz;
+    if (!(d2 > 0.f)) {
+        for (int i = 0; i < MLP_FEAT; ++i) f[i] = 0.f;
+        return 0.f;
+    }
+    const float inv = rsqrtf(d2);
+    f[0] = rx * inv;
+    f[1] = ry * inv;
+    f[2] = rz * inv;
+    f[3] = width * inv;
+    f[4] = 1.f + log2f(fmaxf(mass / total_mass, 1e-30f)) * (1.f / 32.f);
+    f[5] = wrad * inv;
+    f[6] = float(accepting) * (1.f / 64.f);
+    f[7] = 0.f;
+    return mass / d2;
+}
+
+}  // namespace pnb

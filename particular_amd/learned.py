@@ -1,0 +1,117 @@
+"""Host side of the learned far-field correction (MLP on top of Barnes-Hut).
+
+Thin wrappers over the pnb_mlp_* entry points: parameters, training-set collection (rows and
+brute-force-minus-Barnes-Hut targets generated on the GPU), full-batch training, and export of the
+training set for checks against the CPU oracle.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+
+FEAT = 8
+
+
+def num_params() -> int:
+    return int(_lib.lib().pnb_mlp_num_params())
+
+
+def init(res, seed: int = 0) -> None:
+    res.check(_lib.lib().pnb_mlp_init(res.handle, seed))
+
+
+def set_params(res, params: np.ndarray) -> None:
+    p = np.ascontiguousarray(params, np.float32).ravel()
+    res.check(_lib.lib().pnb_mlp_set_params(res.handle, p.ctypes.data, p.size))
+
+
+def get_params(res) -> np.ndarray:
+    p = np.zeros(num_params(), np.float32)
+    res.check(_lib.lib().pnb_mlp_get_params(res.handle, p.ctypes.data, p.size))
+    return p
+
+
+def params_to_dict(p: np.ndarray) -> dict:
+    """Splits the flat vector into the oracle's W1, b1, W2, b2, W3, b3 (FP64)."""
+    p = np.asarray(p, np.float64)
+    shapes = [("W1", (7, 64)), ("b1", (64,)), ("W2", (64, 64)), ("b2", (64,)), ("W3", (64, 3)),
+              ("b3", (3,))]
+    out, o = {}, 0
+    for name, shp in shapes:
+        k = int(np.prod(shp))
+        out[name] = p[o:o + k].reshape(shp).copy()
+        o += k
+    return out
+
+
+def dict_to_params(d: dict) -> np.ndarray:
+    return np.concatenate([np.asarray(d[k], np.float64).ravel()
+                           for k in ("W1", "b1", "W2", "b2", "W3", "b3")]).astype(np.float32)
+
+
+def collect(res, particles_dev, theta: float, waves: int, seed: int, softening: float = 0.0,
+            row_cap: int = 4096) -> None:
+    """Appends the rows and targets of `waves` seeded wavefronts of one snapshot (device tensor)."""
+    res.use_torch_stream()
+    res.check(_lib.lib().pnb_mlp_collect(res.handle, particles_dev.data_ptr(),
+                                         particles_dev.shape[0], theta, softening, waves, seed,
+                                         row_cap))
+    res.synchronize()
+
+
+def clear(res) -> None:
+    res.check(_lib.lib().pnb_mlp_clear_dataset(res.handle))
+
+
+def info(res) -> tuple[int, int, int]:
+    r, w, d = ctypes.c_size_t(), ctypes.c_size_t(), ctypes.c_uint64()
+    res.check(_lib.lib().pnb_mlp_dataset_info(res.handle, ctypes.byref(r), ctypes.byref(w),
+                                              ctypes.byref(d)))
+    return r.value, w.value, d.value
+
+
+def dataset(res) -> dict:
+    rows, waves, _ = info(res)
+    d = {
+        "feats": np.zeros((rows, FEAT), np.float32), "scale": np.zeros(rows, np.float32),
+        "mask": np.zeros(rows, np.uint64), "row_wave": np.zeros(rows, np.uint32),
+        "wave_rows": np.zeros(waves + 1, np.uint32),
+        "target": np.zeros((waves, 64, 4), np.float32), "wgt": np.zeros((waves, 64), np.float32),
+    }
+    res.check(_lib.lib().pnb_mlp_dataset_read(
+        res.handle, d["feats"].ctypes.data, d["scale"].ctypes.data, d["mask"].ctypes.data,
+        d["row_wave"].ctypes.data, d["wave_rows"].ctypes.data, d["target"].ctypes.data,
+        d["wgt"].ctypes.data))
+    return d
+
+
+def load(res, d: dict) -> None:
+    f = {k: np.ascontiguousarray(v) for k, v in d.items()}
+    res.check(_lib.lib().pnb_mlp_dataset_load(
+        res.handle, f["feats"].astype(np.float32).ctypes.data,
+        f["scale"].astype(np.float32).ctypes.data, f["mask"].astype(np.uint64).ctypes.data,
+        f["row_wave"].astype(np.uint32).ctypes.data, f["wave_rows"].astype(np.uint32).ctypes.data,
+        f["feats"].shape[0], f["wave_rows"].shape[0] - 1,
+        f["target"].astype(np.float32).ctypes.data, f["wgt"].astype(np.float32).ctypes.data))
+
+
+def grad(res) -> tuple[float, np.ndarray]:
+    loss = ctypes.c_float()
+    g = np.zeros(num_params(), np.float32)
+    res.check(_lib.lib().pnb_mlp_grad(res.handle, ctypes.byref(loss), g.ctypes.data, g.size))
+    return loss.value, g
+
+
+def train(res, iterations: int, lr: float = 1e-3) -> np.ndarray:
+    hist = np.zeros(iterations, np.float32)
+    res.check(_lib.lib().pnb_mlp_train(res.handle, iterations, lr, hist.ctypes.data))
+    return hist
+
+
+def forward(res, feats_dev, y_dev) -> None:
+    res.use_torch_stream()
+    res.check(_lib.lib().pnb_mlp_forward_dev(res.handle, feats_dev.data_ptr(), feats_dev.shape[0],
+                                             y_dev.data_ptr()))
