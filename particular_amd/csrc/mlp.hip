@@ -384,7 +384,7 @@ static int grow(pnb_ctx* c, DevBuf& b, size_t used, size_t want) {
     if (want <= b.cap) return PNB_OK;
     DevBuf nb;
     PNB_HIP(c, nb.ensure(want));
-    if (used) PNB_HIP(c, hipMemcpyAsync(nb.ptr, b.ptr, used, hipMemcpyDeviceToDevice, c->stream));
+    if (used && b.ptr) PNB_HIP(c, hipMemcpyAsync(nb.ptr, b.ptr, used, hipMemcpyDeviceToDevice, c->stream));
     PNB_HIP(c, hipStreamSynchronize(c->stream));
     b.release();
     b = nb;
@@ -412,7 +412,7 @@ int mlp_append(pnb_ctx* c, const float* feats, const float* scale, const unsigne
     if (int rc = grow(c, s->rwave, s->rows * 4, cr * 4)) return rc;
     if (int rc = grow(c, s->target, s->waves * 64 * 16, cw * 64 * 16)) return rc;
     if (int rc = grow(c, s->wgt, s->waves * 64 * 4, cw * 64 * 4)) return rc;
-    if (int rc = grow(c, s->wave_rows, (s->waves + 1) * 4, (cw + 1) * 4)) return rc;
+    if (int rc = grow(c, s->wave_rows, s->waves ? (s->waves + 1) * 4 : 0, (cw + 1) * 4)) return rc;
     s->cap_rows = cr;
     s->cap_waves = cw;
     hipStream_t st = c->stream;
