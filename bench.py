@@ -185,6 +185,126 @@ def cpu_baseline_bf(s, sample: int):
     }
 
 
+def bench_bh(args):
+    """Barnes-Hut on two Plummer spheres (4,194,304 bodies, mu = 1/N, theta 0.3 / 0.5 / 0.8):
+    tree build + walk device time from HIP events, accepted pair terms and wavefront node steps
+    from the walk's counters, accuracy against the FP64 direct sum on a seeded sample."""
+    import torch
+
+    import particular_amd as pa
+    from particular_amd import _lib, metrics, workloads
+
+    log(f"generating two-Plummer workload ({args.bh_n})")
+    s = workloads.two_plummer_case(args.bh_n)
+    dev = torch.from_numpy(s.particles).cuda()
+    res = pa.HipResources(0)
+    res.use_torch_stream()
+    sample = min(16384, s.n)
+    idx = np.random.default_rng(5).choice(s.n, sample, replace=False)
+    pts = dev[torch.from_numpy(idx).cuda()].contiguous()
+    ref = torch.empty((sample, 3), dtype=torch.float64, device="cuda")
+    res.check(_lib.lib().pnb_direct_f64_dev(res.handle, pts.data_ptr(), sample, dev.data_ptr(),
+                                            s.n, 0.0, ref.data_ptr()))
+    torch.cuda.synchronize()
+    ref = ref.cpu().numpy()
+    out = []
+    for theta in (0.3, 0.5, 0.8):
+        alg = pa.BarnesHut(res, theta, pa.Acceleration.checked())
+        acc = alg.compute(pa.Between(dev, dev))
+        torch.cuda.synchronize()
+        reps = 3
+        res.profile(True)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            acc = alg.compute(pa.Between(dev, dev))
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t0) / reps
+        build_ms, nb = res.profile_read(_lib.PROF_BH_BUILD)
+        walk_ms, nw = res.profile_read(_lib.PROF_BH_WALK)
+        sort_ms, ns = res.profile_read(_lib.PROF_SORT)
+        res.profile(False)
+        steps, accepted = pa.bh_stats(res)
+        build_ms, walk_ms, sort_ms = build_ms / nb, walk_ms / nw, sort_ms / max(ns, 1)
+        nodes, _ = pa.tree_export(res)[0].shape[0], None
+        err = metrics.summary(acc.cpu().numpy()[idx], ref)
+        # the walk's counters accumulate over the reps since the last build: one build per call
+        row = {
+            "theta": theta,
+            "mparticles_per_s": s.n / wall / 1e6,
+            "ms_per_eval": wall * 1e3,
+            "build_ms": build_ms,
+            "sort_ms": sort_ms,
+            "walk_ms": walk_ms,
+            "nodes": int(nodes),
+            "accepted_terms_per_particle": accepted / s.n,
+            "wave_steps_per_wave": steps / (s.n / 64),
+            "node_record_GBps": steps * 32 / (walk_ms * 1e-3) / 1e9,
+            "accepted_G_terms_per_s": accepted / (walk_ms * 1e-3) / 1e9,
+            **err,
+        }
+        log(f"BH theta={theta}: {row['ms_per_eval']:.2f} ms ({row['mparticles_per_s']:.1f} Mp/s), "
+            f"build {build_ms:.2f} ms, walk {walk_ms:.2f} ms, rms {err['rms_rel']:.2e}")
+        out.append(row)
+    res.close()
+    return out, s
+
+
+def cpu_baseline_bh(s, theta, sample):
+    """The oracle's restatement of parallel::BarnesHut: sequential tree build (timed whole) and
+    the threaded traversal timed on a sample of affected particles, scaled to all N."""
+    from oracle import oracle
+
+    threads = cpu_threads()
+    t0 = time.perf_counter()
+    tree = oracle.Tree(s.particles)
+    build = time.perf_counter() - t0
+    aff = s.particles[np.random.default_rng(3).choice(s.n, sample, replace=False)]
+    t0 = time.perf_counter()
+    tree.barnes_hut(aff, theta, threads=threads)
+    walk = (time.perf_counter() - t0) * s.n / sample
+    return {
+        "value": s.n / (build + walk) / 1e6,
+        "unit": "M particles/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"theta={theta}: sequential build of all {s.n} bodies ({build:.2f} s) + threaded "
+                  f"traversal of {sample} affected scaled to {s.n} ({walk:.2f} s)",
+    }
+
+
+def bench_batch(args):
+    """Batched ensemble: 2048 Plummer systems (64..65,536 bodies, log-uniform), eps = 1e-3,
+    `--batch-steps` KDK steps captured in one hipGraph."""
+    import particular_amd as pa
+    from particular_amd import _lib, autotune, workloads
+
+    L = _lib.lib()
+    log("generating ensemble")
+    e = workloads.ensemble_case()
+    res = pa.HipResources(0)
+    tuned = autotune.apply(res)
+    pos, vel = np.ascontiguousarray(e.pos), np.ascontiguousarray(e.vel)
+    mu = np.ascontiguousarray(e.particles[:, 3])
+    off = np.ascontiguousarray(e.offsets, np.int64)
+    res.check(L.pnb_batch_init(res.handle, pos.ctypes.data, vel.ctypes.data, mu.ctypes.data,
+                               off.ctypes.data, e.num_systems, e.softening, 1))
+    n, items, inter = ctypes.c_size_t(), ctypes.c_size_t(), ctypes.c_ulonglong()
+    res.check(L.pnb_batch_info(res.handle, ctypes.byref(n), ctypes.byref(items), ctypes.byref(inter)))
+    steps = args.batch_steps
+    res.check(L.pnb_batch_run(res.handle, 1, 1e-3, 0))
+    res.synchronize()
+    t0 = time.perf_counter()
+    res.check(L.pnb_batch_run(res.handle, steps, 1e-3, 1))
+    res.synchronize()
+    wall = time.perf_counter() - t0
+    res.close()
+    return {
+        "systems": e.num_systems, "particles": n.value, "work_items": items.value,
+        "interactions_per_step": inter.value, "steps": steps, "graph": True, "tuned": tuned,
+        "seconds": wall, "g_interactions_per_s": inter.value * steps / wall / 1e9,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -194,13 +314,40 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--graph", action="store_true", default=True)
     ap.add_argument("--no-graph", dest="graph", action="store_false")
+    ap.add_argument("--bh", action="store_true", default=True)
+    ap.add_argument("--no-bh", dest="bh", action="store_false")
+    ap.add_argument("--bh-n", type=int, default=4194304)
+    ap.add_argument("--bh-cpu-sample", type=int, default=16384)
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="first copy a buffer of this size with torch (HBM counter calibration)")
+    ap.add_argument("--batch", action="store_true", default=False)
+    ap.add_argument("--batch-steps", type=int, default=100)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(the acceleration path is built and measured on a single MI355X)", file=sys.stderr)
         sys.exit(2)
+    if args.calibrate_copy_mib:
+        import torch
+
+        x = torch.ones(args.calibrate_copy_mib << 18, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        y.copy_(x)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {args.calibrate_copy_mib} MiB read + written")
+        del x, y
     result, s = bench_bf(args)
     result["cpu_baseline"] = None if args.no_cpu else cpu_baseline_bf(s, args.cpu_sample)
+    if args.bh:
+        bh_rows, s3 = bench_bh(args)
+        result["barnes_hut"] = {
+            "workload": f"two Plummer spheres, {s3.n} bodies (scale 1, centres x = -5/+5, "
+                        "bulk v_x = +-0.3), mu = 1/N, Acceleration::checked, FP64 centres of mass",
+            "rows": bh_rows,
+            "cpu_baseline": None if args.no_cpu else cpu_baseline_bh(s3, 0.5, args.bh_cpu_sample),
+        }
+    if args.batch:
+        result["batch"] = bench_batch(args)
     print(json.dumps(result), flush=True)
 
 

@@ -159,6 +159,14 @@ int pnb_bh_tree_export(pnb_ctx *ctx, pnb_tree_node *nodes, size_t cap, uint32_t 
 /* Counters of the walks since the last build: wavefront node steps and accepted pair terms. */
 int pnb_bh_stats(pnb_ctx *ctx, uint64_t *wave_steps, uint64_t *accepted);
 
+/* FP64 brute force for glam::DVec3 / f64 particles (CPU-only in the reference:
+ * particular/src/gravity/impls/glam.rs:234): records are 4 doubles {x, y, z, mu} (32 B); the
+ * reference's scalar kernel in FP64 with IEEE sqrt and division (impls/mod.rs:151-166). */
+int pnb_bf_accel_f64(pnb_ctx *ctx, const double *affected, size_t n, const double *affecting,
+                     size_t m, double softening, int checked, double *out);
+int pnb_bf_accel_f64_dev(pnb_ctx *ctx, const double *affected, size_t n, const double *affecting,
+                         size_t m, double softening, int checked, double *out);
+
 /* ---- accuracy judge -------------------------------------------------------------------- */
 
 /* FP64 direct sum on the GPU (device pointers): out3[3 i + k] = sum_j mu_j d / (|d|^2+eps^2)^1.5

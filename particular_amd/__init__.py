@@ -295,8 +295,25 @@ class BruteForce(_Algorithm):
     """Brute force on the GPU: replaces gpu::BruteForce (particular/src/gpu/mod.rs:149-208).
 
     Per affected particle, the sum over every affecting particle of the pair interaction, as
-    sequential::BruteForce computes it (particular/src/sequential.rs:181-194).
+    sequential::BruteForce computes it (particular/src/sequential.rs:181-194). float64 host arrays
+    take the FP64 kernel (the reference's DVec3 / f64 scalar path).
     """
+
+    def compute(self, storage):
+        kind, affected, affecting = _lower(storage)
+        if (not _is_torch(affected)) and np.asarray(affected).dtype == np.float64:
+            a = np.ascontiguousarray(affected, np.float64)
+            if kind == "reordered":
+                b = np.ascontiguousarray(a[a[:, 3] != 0])
+            else:
+                b = np.ascontiguousarray(affecting, np.float64)
+            out = np.zeros((a.shape[0], 4), np.float64)
+            eps, chk = self._params()
+            self.resources.check(_lib.lib().pnb_bf_accel_f64(
+                self.resources.handle, a.ctypes.data, a.shape[0], b.ctypes.data, b.shape[0], eps,
+                chk, out.ctypes.data))
+            return out[:, :3]
+        return super().compute(storage)
 
     def _between_host(self, a, b, out):
         eps, chk = self._params()

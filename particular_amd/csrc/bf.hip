@@ -274,6 +274,39 @@ __global__ __launch_bounds__(DF_BLOCK) void direct_f64_kernel(const float4* __re
     }
 }
 
+// ---- FP64 brute force (glam::DVec3 / f64: particular/src/gravity/impls/glam.rs:234) ------------
+// The reference's scalar kernel in FP64 (impls/mod.rs:151-166): IEEE sqrt and division, pairs
+// folded in affecting order per tile; tiles of BF_TILE bodies staged in LDS, one affected body per
+// lane. checked: zero unsoftened distance contributes nothing.
+
+__global__ __launch_bounds__(BF_BLOCK) void bf64_kernel(const double4* __restrict__ aff, uint32_t n,
+                                                        const double4* __restrict__ afg, uint32_t m,
+                                                        double eps2, int checked,
+                                                        double4* __restrict__ out) {
+    __shared__ double4 tile[BF_TILE];
+    const uint32_t i = blockIdx.x * BF_BLOCK + threadIdx.x;
+    const double4 p = aff[i < n ? i : n - 1];
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    for (uint32_t j0 = 0; j0 < m; j0 += BF_TILE) {
+        const uint32_t cnt = min(uint32_t(BF_TILE), m - j0);
+        __syncthreads();
+        if (threadIdx.x < cnt) tile[threadIdx.x] = afg[j0 + threadIdx.x];
+        __syncthreads();
+        for (uint32_t j = 0; j < cnt; ++j) {
+            const double4 q = tile[j];
+            const double dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
+            const double r2 = __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
+            if (checked && r2 == 0.0) continue;
+            const double rs = r2 + eps2;
+            const double s = q.w / (rs * sqrt(rs));
+            ax += dx * s;
+            ay += dy * s;
+            az += dz * s;
+        }
+    }
+    if (i < n) out[i] = make_double4(ax, ay, az, 0.0);
+}
+
 static int copy_in(pnb_ctx* c, DevBuf& b, const float* src, size_t count) {
     PNB_HIP(c, b.ensure(count * sizeof(float4)));
     if (count)
@@ -385,6 +418,43 @@ int pnb_direct_f64_dev(pnb_ctx* c, const float* affected, size_t n, const float*
         reinterpret_cast<const float4*>(affected), reinterpret_cast<const float4*>(affecting),
         uint32_t(m), softening * softening, out3);
     PNB_CHECK_LAUNCH(c);
+    return PNB_OK;
+}
+
+int pnb_bf_accel_f64_dev(pnb_ctx* c, const double* affected, size_t n, const double* affecting,
+                         size_t m, double softening, int checked, double* out) {
+    if (!c) return PNB_E_ARG;
+    if ((n && (!affected || !out)) || (m && !affecting))
+        return fail(c, PNB_E_ARG, "null pointer with non-zero length");
+    if (!(softening >= 0.0)) return fail(c, PNB_E_ARG, "softening must be >= 0");
+    if (n == 0) return PNB_OK;
+    if (n > 0x7fffffffu || m > 0x7fffffffu) return fail(c, PNB_E_ARG, "too many particles");
+    if (int rc = activate(c)) return rc;
+    bf64_kernel<<<div_up(n, BF_BLOCK), BF_BLOCK, 0, c->stream>>>(
+        reinterpret_cast<const double4*>(affected), uint32_t(n),
+        reinterpret_cast<const double4*>(affecting), uint32_t(m), softening * softening, checked,
+        reinterpret_cast<double4*>(out));
+    PNB_CHECK_LAUNCH(c);
+    return PNB_OK;
+}
+
+int pnb_bf_accel_f64(pnb_ctx* c, const double* affected, size_t n, const double* affecting,
+                     size_t m, double softening, int checked, double* out) {
+    if (!c) return PNB_E_ARG;
+    if ((n && (!affected || !out)) || (m && !affecting))
+        return fail(c, PNB_E_ARG, "null pointer with non-zero length");
+    if (n == 0) return PNB_OK;
+    if (int rc = activate(c)) return rc;
+    PNB_HIP(c, c->h_affected.ensure(n * 32));
+    PNB_HIP(c, c->h_affecting.ensure(std::max<size_t>(m, 1) * 32));
+    PNB_HIP(c, c->h_out.ensure(n * 32));
+    PNB_HIP(c, hipMemcpyAsync(c->h_affected.ptr, affected, n * 32, hipMemcpyHostToDevice, c->stream));
+    if (m) PNB_HIP(c, hipMemcpyAsync(c->h_affecting.ptr, affecting, m * 32, hipMemcpyHostToDevice, c->stream));
+    if (int rc = pnb_bf_accel_f64_dev(c, c->h_affected.as<double>(), n, c->h_affecting.as<double>(), m,
+                                      softening, checked, c->h_out.as<double>()))
+        return rc;
+    PNB_HIP(c, hipMemcpyAsync(out, c->h_out.ptr, n * 32, hipMemcpyDeviceToHost, c->stream));
+    PNB_HIP(c, hipStreamSynchronize(c->stream));
     return PNB_OK;
 }
 

@@ -593,9 +593,14 @@ __global__ __launch_bounds__(256) void bh_walk_kernel(const NodeRec* __restrict_
         nrows = 0;
     };
 
+    // The cursor's two possible successors (k + 1 when some lane opens, skip otherwise) are
+    // loaded while node k is evaluated, so the scalar-load latency overlaps the VALU work.
     uint32_t k = 0;
+    NodeRec r{};
+    if (total > 0) r = nodes[0];  // no tree (m = 0): nodes may not even be allocated
     while (k < total) {
-        const NodeRec r = nodes[k];
+        const NodeRec r_next = nodes[min(k + 1, total - 1)];
+        const NodeRec r_skip = nodes[min(r.skip, total - 1)];
         const bool active = k >= resume;
         const float dx = __fsub_rn(r.cx, p.x), dy = __fsub_rn(r.cy, p.y), dz = __fsub_rn(r.cz, p.z);
         const float d2 = norm2_ref(dx, dy, dz);
@@ -647,8 +652,9 @@ __global__ __launch_bounds__(256) void bh_walk_kernel(const NodeRec* __restrict_
         }
         if (active && !open) resume = r.skip;
         ++steps;
-        const uint32_t nxt = __any(open) ? k + 1 : max(r.skip, k + 1);
-        k = __builtin_amdgcn_readfirstlane(nxt);
+        const uint32_t nxt = __builtin_amdgcn_readfirstlane(__any(open) ? k + 1 : max(r.skip, k + 1));
+        r = (nxt == k + 1) ? r_next : r_skip;
+        k = nxt;
     }
     if (MODE == WALK_MLP && nrows) flush();
     if (MODE == WALK_RECORD && lane == 0) rec.count[wave_slot] = min(nrows, rec.cap);

@@ -164,3 +164,25 @@ def test_circular_orbit_gpu(res):
 
     ed, ee = circular_orbit(accel, 100)
     assert ed < 1e-2 and ee < 1e-2
+
+
+def test_f64_brute_force(res):
+    """float64 particles take the FP64 kernel (the reference's DVec3 path)."""
+    rng = np.random.default_rng(77)
+    a = rng.normal(size=(700, 4))
+    b = rng.normal(size=(900, 4))
+    b[:, 3] = rng.uniform(0.1, 2.0, 900)
+    b[:3, :3] = a[:3, :3]  # coincident pairs: checked drops them
+    got = bf(res, pa.Between(a, b), pa.AccelerationSoftened.checked(0.0))
+    d = b[None, :, :3] - a[:, None, :3]
+    r2 = (d * d).sum(-1)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        s = np.where(r2 > 0, b[None, :, 3] / (r2 * np.sqrt(r2)), 0.0)
+    ref = (d * s[..., None]).sum(1)
+    assert got.dtype == np.float64
+    assert metrics.max_rel(got, ref) < 1e-12
+    soft = bf(res, pa.Reordered.new(b), pa.AccelerationSoftened.checked(0.05))
+    d = b[None, :, :3] - b[:, None, :3]
+    rs = (d * d).sum(-1) + 0.05**2
+    ref = (d * (b[None, :, 3] / (rs * np.sqrt(rs)))[..., None]).sum(1)
+    assert metrics.max_rel(soft, ref) < 1e-12
