@@ -99,12 +99,27 @@ void bh_free(BhState* s) {
 }
 
 // ---- f32 helpers with the reference's rounding (no contraction) ------------------------------
+// hipcc contracts a*b + c into one FMA by default, also through __fadd_rn / __fmul_rn (plain
+// operators in HIP); the reference rounds every product and sum, so these switch contraction off.
 
-__device__ __forceinline__ float midpoint(float a, float b) { return __fmul_rn(__fadd_rn(a, b), 0.5f); }
+__device__ __forceinline__ float add_rn(float a, float b) {
+#pragma clang fp contract(off)
+    return a + b;
+}
+__device__ __forceinline__ float sub_rn(float a, float b) {
+#pragma clang fp contract(off)
+    return a - b;
+}
+__device__ __forceinline__ float mul_rn(float a, float b) {
+#pragma clang fp contract(off)
+    return a * b;
+}
+
+__device__ __forceinline__ float midpoint(float a, float b) { return mul_rn(add_rn(a, b), 0.5f); }
 
 // glam Vec3::length_squared: (x*x + y*y) + z*z
 __device__ __forceinline__ float norm2_ref(float x, float y, float z) {
-    return __fadd_rn(__fadd_rn(__fmul_rn(x, x), __fmul_rn(y, y)), __fmul_rn(z, z));
+    return add_rn(add_rn(mul_rn(x, x), mul_rn(y, y)), mul_rn(z, z));
 }
 
 __device__ __forceinline__ uint64_t descend_key(float x, float y, float z, float* mn, float* mx,
@@ -175,12 +190,12 @@ __global__ void bh_root_kernel(const float* __restrict__ part, int blocks, BhInf
     float centre[3], big = 0.0f;
     for (int a = 0; a < 3; ++a) {
         centre[a] = midpoint(mn[a], mx[a]);
-        big = fmaxf(big, __fsub_rn(mx[a], mn[a]));
+        big = fmaxf(big, sub_rn(mx[a], mn[a]));
     }
     const float half = midpoint(big, 0.0f);
     for (int a = 0; a < 3; ++a) {
-        info->mn[a] = __fsub_rn(centre[a], half);
-        info->mx[a] = __fadd_rn(centre[a], half);
+        info->mn[a] = sub_rn(centre[a], half);
+        info->mx[a] = add_rn(centre[a], half);
     }
 }
 
@@ -376,10 +391,10 @@ __global__ __launch_bounds__(256) void bh_node_kernel(
         }
         if (l > lp) {
             const uint32_t k = base + uint32_t(l - lp - 1);
-            const float wd = __fsub_rn(mx, mn);
+            const float wd = sub_rn(mx, mn);
             width[k] = wd;
             level[k] = uint8_t(l);
-            rec[k].w2 = (l <= I) ? __fmul_rn(wd, wd) : -INFINITY;
+            rec[k].w2 = (l <= I) ? mul_rn(wd, wd) : -INFINITY;
             rec[k].begin = uint32_t(p);
         }
     }
@@ -468,12 +483,12 @@ __global__ __launch_bounds__(256) void bh_ref_com_kernel(NodeRec* __restrict__ r
          k += size_t(gridDim.x) * blockDim.x) {
         const uint32_t b = rec[k].begin, e = rec[k].end;
         float tot = 0.0f;
-        for (uint32_t q = b; q < e; ++q) tot = __fadd_rn(tot, sp[q].w);
+        for (uint32_t q = b; q < e; ++q) tot = add_rn(tot, sp[q].w);
         float cx = 0.0f, cy = 0.0f, cz = 0.0f;
         if (tot == 0.0f) {
             for (uint32_t q = b; q < e; ++q) {
                 const float4 v = sp[q];
-                cx = __fadd_rn(cx, v.x); cy = __fadd_rn(cy, v.y); cz = __fadd_rn(cz, v.z);
+                cx = add_rn(cx, v.x); cy = add_rn(cy, v.y); cz = add_rn(cz, v.z);
             }
             const float len = float(e - b);
             cx = __fdiv_rn(cx, len); cy = __fdiv_rn(cy, len); cz = __fdiv_rn(cz, len);
@@ -481,9 +496,9 @@ __global__ __launch_bounds__(256) void bh_ref_com_kernel(NodeRec* __restrict__ r
             for (uint32_t q = b; q < e; ++q) {
                 const float4 v = sp[q];
                 const float f = __fdiv_rn(v.w, tot);
-                cx = __fadd_rn(cx, __fmul_rn(v.x, f));
-                cy = __fadd_rn(cy, __fmul_rn(v.y, f));
-                cz = __fadd_rn(cz, __fmul_rn(v.z, f));
+                cx = add_rn(cx, mul_rn(v.x, f));
+                cy = add_rn(cy, mul_rn(v.y, f));
+                cz = add_rn(cz, mul_rn(v.z, f));
             }
         }
         rec[k].cx = cx;
@@ -602,14 +617,14 @@ __global__ __launch_bounds__(256) void bh_walk_kernel(const NodeRec* __restrict_
         const NodeRec r_next = nodes[min(k + 1, total - 1)];
         const NodeRec r_skip = nodes[min(r.skip, total - 1)];
         const bool active = k >= resume;
-        const float dx = __fsub_rn(r.cx, p.x), dy = __fsub_rn(r.cy, p.y), dz = __fsub_rn(r.cz, p.z);
+        const float dx = sub_rn(r.cx, p.x), dy = sub_rn(r.cy, p.y), dz = sub_rn(r.cz, p.z);
         const float d2 = norm2_ref(dx, dy, dz);
         const bool nz = d2 != 0.0f;
-        const bool crit = __fmul_rn(theta2, d2) < r.w2;
+        const bool crit = mul_rn(theta2, d2) < r.w2;
         const bool open = active && nz && crit;
         const bool acc = active && nz && !crit;
         if (acc) {
-            const float r2 = __fadd_rn(d2, eps2);
+            const float r2 = add_rn(d2, eps2);
             const float inv = __builtin_amdgcn_rsqf(r2);
             const float s = (r.mass * inv) * (inv * inv);
             float y = fmaf(dx, s, -cx), t = ax + y;

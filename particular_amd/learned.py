@@ -88,14 +88,27 @@ def dataset(res) -> dict:
     return d
 
 
+_DATASET_DTYPES = {"feats": np.float32, "scale": np.float32, "mask": np.uint64,
+                   "row_wave": np.uint32, "wave_rows": np.uint32, "target": np.float32,
+                   "wgt": np.float32}
+
+
 def load(res, d: dict) -> None:
-    f = {k: np.ascontiguousarray(v) for k, v in d.items()}
+    """Appends a training set held in host arrays (the layout `dataset` returns)."""
+    # the converted arrays are kept referenced until the call returns
+    f = {k: np.ascontiguousarray(d[k], dt) for k, dt in _DATASET_DTYPES.items()}
+    if f["feats"].ndim != 2 or f["feats"].shape[1] != FEAT:
+        raise ValueError(f"feats must be (rows, {FEAT})")
+    if f["target"].shape[1:] != (64, 4) or f["wgt"].shape[1:] != (64,):
+        raise ValueError("target must be (waves, 64, 4) and wgt (waves, 64)")
+    rows, waves = f["feats"].shape[0], f["wave_rows"].shape[0] - 1
+    if not (f["scale"].size == f["mask"].size == f["row_wave"].size == rows
+            and f["target"].shape[0] == f["wgt"].shape[0] == waves):
+        raise ValueError("inconsistent training-set sizes")
     res.check(_lib.lib().pnb_mlp_dataset_load(
-        res.handle, f["feats"].astype(np.float32).ctypes.data,
-        f["scale"].astype(np.float32).ctypes.data, f["mask"].astype(np.uint64).ctypes.data,
-        f["row_wave"].astype(np.uint32).ctypes.data, f["wave_rows"].astype(np.uint32).ctypes.data,
-        f["feats"].shape[0], f["wave_rows"].shape[0] - 1,
-        f["target"].astype(np.float32).ctypes.data, f["wgt"].astype(np.float32).ctypes.data))
+        res.handle, f["feats"].ctypes.data, f["scale"].ctypes.data, f["mask"].ctypes.data,
+        f["row_wave"].ctypes.data, f["wave_rows"].ctypes.data, rows, waves,
+        f["target"].ctypes.data, f["wgt"].ctypes.data))
 
 
 def grad(res) -> tuple[float, np.ndarray]:

@@ -107,7 +107,11 @@ def test_collect_rows_and_targets(res, orc):
     assert np.all(d["scale"] > 0) and np.all(d["mask"] != 0)
     f = d["feats"]
     np.testing.assert_allclose(np.linalg.norm(f[:, :3], axis=1), 1.0, rtol=1e-5)
-    assert np.all((f[:, 3] > 0) & (f[:, 3] <= 0.8 * 1.5))  # accepted internal nodes: width/d <~ theta
+    # Accepted internal nodes: some lane at distance d_l >= width / theta accepted the node, and
+    # that lane is within 8 rms radii of the wavefront centre (64 lanes), so the centre distance
+    # d_c >= width / theta - 8 * wrad, i.e. width/d_c <= theta * (1 + 8 * wrad/d_c).
+    assert np.all(f[:, 3] > 0)
+    assert np.all(f[:, 3] <= 0.8 * (1 + 8 * f[:, 5]) * (1 + 1e-5))
     # targets are brute force minus Barnes-Hut: |target| relative to |a| is a monopole-sized error
     a_ref = 1.0 / np.sqrt(d["wgt"][d["wgt"] > 0])
     t = np.linalg.norm(d["target"][..., :3], axis=-1)[d["wgt"] > 0]
