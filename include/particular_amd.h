@@ -150,6 +150,10 @@ typedef struct {
     uint32_t count;   /* particles in the node */
     uint32_t level;   /* depth below the root */
     uint32_t internal;/* 1 for Node::Internal, 0 for Node::External */
+    /* Traceless quadrupole about the centre of mass, normalised by mass and width:
+     * (3 T - tr(T) I) / (mass w^2) with T = sum mu (x - com)(x - com)^T, as {xx, yy, zz, xy, xz,
+     * yz}. Filled when the tree was built for the learned correction, zero otherwise. */
+    float quad[6];
 } pnb_tree_node;
 
 int pnb_bh_tree_size(pnb_ctx *ctx, size_t *nodes, int *max_level);
@@ -215,18 +219,22 @@ int pnb_batch_info(pnb_ctx *ctx, size_t *particles, size_t *items,
                    unsigned long long *interactions_per_step);
 
 /* ---- learned far-field correction (Barnes-Hut + MLP) -----------------------------------------
- * No reference counterpart. A 7 -> 64 -> 64 -> 3 ReLU MLP (BF16 weights on MFMA, FP32
+ * No reference counterpart. A 13 -> 64 -> 64 -> 3 ReLU MLP (BF16 weights on MFMA, FP32
  * accumulate) corrects the monopole term of every internal node a wavefront accepts in the
  * Barnes-Hut walk (pnb_bh_accel_f32* with PNB_BH_MLP). One row = one (wavefront, accepted internal
- * node); its features are seen from the wavefront centroid (unit direction to the node's centre
- * of mass, width / distance, log mass fraction, wavefront radius / distance, accepting-lane
- * fraction) and its output y adds (mass / d^2) * y to every accepting lane. Training data are
- * generated by the method itself: per-lane targets are brute force minus Barnes-Hut. */
+ * node). Its features are seen from the wavefront centroid c, at distance d from the node's centre
+ * of mass along the unit vector u: u (3), width / d, 1 + log2(mass fraction) / 32, wavefront rms
+ * radius / d, accepting-lane fraction, and the node's normalised quadrupole Q (see pnb_tree_node)
+ * projected on u: Q u - (u.Q.u) u (3) and (u.Q.u) u (3). Its output y adds (mass w^2 / d^4) * y
+ * to every accepting lane (the size of the monopole term's leading error). The walk lists each wavefront's rows; a second kernel
+ * evaluates them (16 rows per MFMA tile) and adds the corrections. Training data are generated by
+ * the method itself: per-lane targets are brute force minus Barnes-Hut. */
 
 int pnb_mlp_num_params(void);
-/* He-normal initialisation from a seed (SplitMix64 + Box-Muller); resets the Adam state. */
+/* He-normal hidden layers from a seed (SplitMix64 + Box-Muller), zero biases and a zero output
+ * layer (a fresh network corrects nothing); resets the Adam state. */
 int pnb_mlp_init(pnb_ctx *ctx, uint64_t seed);
-/* Parameters as one FP32 vector: W1[7][64], b1[64], W2[64][64], b2[64], W3[64][3], b3[3]. */
+/* Parameters as one FP32 vector: W1[13][64], b1[64], W2[64][64], b2[64], W3[64][3], b3[3]. */
 int pnb_mlp_set_params(pnb_ctx *ctx, const float *params, size_t n);
 int pnb_mlp_get_params(pnb_ctx *ctx, float *params, size_t n);
 /* Builds the octree of `particles` (device, n x 4, all affecting), walks `waves` seeded
@@ -237,8 +245,9 @@ int pnb_mlp_collect(pnb_ctx *ctx, const float *particles, size_t n, float theta,
                     size_t waves, uint64_t seed, size_t row_cap);
 int pnb_mlp_clear_dataset(pnb_ctx *ctx);
 int pnb_mlp_dataset_info(pnb_ctx *ctx, size_t *rows, size_t *waves, uint64_t *dropped);
-/* Training set on the host: feats rows x 8, scale rows, mask rows (64-bit lane masks), row_wave
- * rows, wave_rows waves + 1 (row offsets), target waves x 64 x 4, wgt waves x 64. */
+/* Training set on the host: feats rows x 16 (13 used), scale rows, mask rows (64-bit lane
+ * masks), row_wave rows, wave_rows waves + 1 (row offsets), target waves x 64 x 4,
+ * wgt waves x 64. */
 int pnb_mlp_dataset_read(pnb_ctx *ctx, float *feats, float *scale, uint64_t *mask,
                          uint32_t *row_wave, uint32_t *wave_rows, float *target, float *wgt);
 int pnb_mlp_dataset_load(pnb_ctx *ctx, const float *feats, const float *scale,
@@ -249,7 +258,7 @@ int pnb_mlp_dataset_load(pnb_ctx *ctx, const float *feats, const float *scale,
 int pnb_mlp_grad(pnb_ctx *ctx, float *loss, float *grads, size_t n);
 /* Full-batch Adam steps; loss_history (optional, `iterations` floats) gets the loss before each. */
 int pnb_mlp_train(pnb_ctx *ctx, size_t iterations, float lr, float *loss_history);
-/* Inference on device rows: y (rows x 4) = MLP(feats (rows x 8)). */
+/* Inference on device rows: y (rows x 4) = MLP(feats (rows x 16, 13 used)). */
 int pnb_mlp_forward_dev(pnb_ctx *ctx, const float *feats, size_t rows, float *y);
 
 #ifdef __cplusplus

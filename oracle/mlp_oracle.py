@@ -4,7 +4,7 @@ No reference counterpart (particular has no learned component). The network and 
 defined here in plain numpy and pinned by finite differences (tests/test_mlp_oracle.py); the GPU
 forward/backward kernels are checked against it.
 
-Network: 7 -> 64 -> 64 -> 3, ReLU, y = relu(relu(x W1 + b1) W2 + b2) W3 + b3. When
+Network: 13 -> 64 -> 64 -> 3, ReLU, y = relu(relu(x W1 + b1) W2 + b2) W3 + b3. When
 ``bf16=True`` the inputs of every matrix product (x, h1, h2, g_y, g_z2, g_z1 and W1..W3) are
 rounded to bfloat16 (round to nearest even) as the MFMA kernels round them; sums are FP64.
 
@@ -17,7 +17,7 @@ from __future__ import annotations
 
 import numpy as np
 
-IN, HID, OUT = 7, 64, 3
+IN, HID, OUT = 13, 64, 3
 
 
 def bf16_round(a) -> np.ndarray:

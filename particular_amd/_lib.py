@@ -100,7 +100,7 @@ def tree_node_dtype():
 
     return np.dtype([("com", "<f4", (3,)), ("mass", "<f4"), ("mass64", "<f8"), ("width", "<f4"),
                      ("skip", "<u4"), ("begin", "<u4"), ("count", "<u4"), ("level", "<u4"),
-                     ("internal", "<u4")], align=True)
+                     ("internal", "<u4"), ("quad", "<f4", (6,))], align=True)
 
 
 _lib = None

@@ -63,7 +63,7 @@ def tune_bf(res, reps=3):
             for per_lane in (1, 2, 4, 8):
                 blocks = -(-n // (256 * per_lane))
                 for j_split in (1, 2, 4, 8, 16, 32, 64):
-                    if j_split > 1 and blocks * j_split > 16 * cus:
+                    if j_split > 1 and blocks * j_split > 64 * cus:
                         continue
                     if BF_M // j_split < 1024:
                         continue

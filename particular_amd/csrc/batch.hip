@@ -193,7 +193,7 @@ static int per_lane_for(const BatchState* s, size_t n) {
             r = t.second;
         }
     if (r) return r;
-    return n >= 16384 ? 4 : n >= 1024 ? 2 : 1;  // untuned default
+    return n >= 256 ? 4 : 1;  // untuned default (what autotuning picks on MI355X)
 }
 
 static int build_items(pnb_ctx* c, BatchState* s) {

@@ -64,11 +64,13 @@ __device__ __forceinline__ void kahan_add(float& sum, float& comp, float v) {
 
 // grid.x: affected tiles of BF_BLOCK * R particles; grid.y: affecting ranges of j_chunk bodies.
 // With gridDim.y == 1 the result goes to `out`, else to partial[blockIdx.y * n + i].
-template <int R, bool MASK, bool LDS>
+// VAR 0: LDS tile; 1: wave-uniform global reads (scalar loads).
+template <int R, bool MASK, int VAR>
 __global__ __launch_bounds__(BF_BLOCK) void bf_kernel(const float4* __restrict__ aff, uint32_t n,
                                                       const float4* __restrict__ afg, uint32_t m,
                                                       uint32_t j_chunk, float eps2,
                                                       float4* __restrict__ out) {
+    constexpr bool LDS = VAR == 0;
     __shared__ float4 tile[LDS ? BF_TILE : 1];
     const uint32_t jb = blockIdx.y * j_chunk;
     const uint32_t je = min(m, jb + j_chunk);
@@ -140,19 +142,23 @@ __global__ void fill_zero_kernel(float4* __restrict__ out, size_t n) {
 
 using BfKernel = void (*)(const float4*, uint32_t, const float4*, uint32_t, uint32_t, float, float4*);
 
-template <bool MASK, bool LDS>
+template <bool MASK, int VAR>
 static BfKernel pick_r(int r) {
     switch (r) {
-        case 1: return bf_kernel<1, MASK, LDS>;
-        case 2: return bf_kernel<2, MASK, LDS>;
-        case 4: return bf_kernel<4, MASK, LDS>;
-        default: return bf_kernel<8, MASK, LDS>;
+        case 1: return bf_kernel<1, MASK, VAR>;
+        case 2: return bf_kernel<2, MASK, VAR>;
+        case 4: return bf_kernel<4, MASK, VAR>;
+        default: return bf_kernel<8, MASK, VAR>;
     }
 }
 
+template <int VAR>
+static BfKernel pick_mask(int r, bool mask) {
+    return mask ? pick_r<true, VAR>(r) : pick_r<false, VAR>(r);
+}
+
 static BfKernel pick_kernel(int variant, int r, bool mask) {
-    if (variant == 0) return mask ? pick_r<true, true>(r) : pick_r<false, true>(r);
-    return mask ? pick_r<true, false>(r) : pick_r<false, false>(r);
+    return variant == 0 ? pick_mask<0>(r, mask) : pick_mask<1>(r, mask);
 }
 
 struct BfShape {
@@ -166,14 +172,17 @@ static BfShape choose_shape(pnb_ctx* c, size_t n, size_t m) {
             if (e.n_min <= n && (!best || e.n_min > best->n_min)) best = &e;
         if (best) return {best->variant, best->per_lane, best->j_split};
     }
-    // Default: enough affected tiles for two workgroups per CU, else split the affecting range.
+    // Default (from autotuning on MI355X): 4 affected bodies per lane with scalar-loaded
+    // affecting bodies once there are at least 64 affected tiles, else 1 per lane from LDS; the
+    // affecting range is split into up to 64 workgroups per CU (small workgroups balance the
+    // load across CUs and XCDs), keeping at least 4 tiles of affecting bodies per split.
     const size_t cus = c->cu_count > 0 ? size_t(c->cu_count) : 256;
-    int r = 4;
-    while (r > 1 && div_up(n, size_t(BF_BLOCK) * r) < 2 * cus) r >>= 1;
+    const int r = n >= size_t(64) * BF_BLOCK * 4 ? 4 : 1;
+    const int variant = r == 4 ? 1 : 0;
     const size_t blocks = div_up(n, size_t(BF_BLOCK) * r);
     int s = 1;
-    while (s < 64 && blocks * s < 4 * cus && m / (size_t(s) * 2) >= 4 * size_t(BF_TILE)) s <<= 1;
-    return {0, r, s};
+    while (s < 64 && blocks * s * 2 <= 64 * cus && m / (size_t(s) * 2) >= 4 * size_t(BF_TILE)) s <<= 1;
+    return {variant, r, s};
 }
 
 struct BfPlan {
@@ -388,7 +397,7 @@ int pnb_reordered_bf_accel_f32(pnb_ctx* c, const float* particles, size_t n, flo
 
 int pnb_bf_set_tuning(pnb_ctx* c, size_t n_min, int variant, int per_lane, int j_split) {
     if (!c) return PNB_E_ARG;
-    if ((variant != 0 && variant != 1) ||
+    if (variant < 0 || variant > 1 ||
         (per_lane != 1 && per_lane != 2 && per_lane != 4 && per_lane != 8) || j_split < 1 ||
         j_split > 64)
         return fail(c, PNB_E_ARG, "invalid brute-force shape");

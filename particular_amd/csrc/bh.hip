@@ -60,12 +60,23 @@ struct __align__(32) NodeRec {
 };
 static_assert(sizeof(NodeRec) == 32, "node record is 32 bytes");
 
+// Mass-weighted second moments of a node about its FP64 centre of mass,
+// T = sum m (x - c)(x - c)^T, as {xx, yy, zz, xy, xz, yz}; built for the learned correction only.
+struct NodeMoments {
+    double xx = 0, yy = 0, zz = 0, xy = 0, xz = 0, yz = 0;
+};
+// Traceless quadrupole normalised by mass and width: Q = (3 T - tr(T) I) / (M w^2), f32,
+// a = {xx, yy, zz, xy}, b = {xz, yz, 0, 0}.
+struct __align__(32) NodeQuad {
+    float4 a, b;
+};
+
 struct BhInfo {
     float mn[3], mx[3];  // root box
     uint32_t flags;
     int max_level;
-    uint32_t total;  // node count (S[M])
-    uint32_t pad;
+    uint32_t total;     // node count (S[M])
+    uint32_t max_rows;  // most rows one wavefront listed in the last row-listing walk
     unsigned long long steps, accepted;  // traversal counters of the last walk
 };
 
@@ -77,10 +88,14 @@ struct BhState {
     const uint64_t* skeys = nullptr;  // sorted keys (points into keys or keys_alt)
     const uint32_t* sidx = nullptr;   // tree order -> original index
     DevBuf keys, keys_alt, idx, idx_alt, sp, lcp, cnt, ip, key2, rec, sums, sums2, level, width;
+    DevBuf mom, quad;       // second moments (FP64) and normalised quadrupoles, on request
+    bool has_moments = false;
     DevBuf info, bbox_part, hist, bits, afg_tmp;
     DevBuf akeys, akeys_alt, aidx, aidx_alt, apts, h_aff, h_afg, h_out;
-    DevBuf c_starts, c_pts, c_bh, c_bf, c_feats, c_scale, c_mask, c_count, c_drop, c_target,
-        c_wgt, c_rows, c_rwave, c_cfeats, c_cscale, c_cmask;
+    DevBuf row_buf, row_count;  // row lists of the learned-correction walk
+    size_t row_cap = 1024;
+    DevBuf c_starts, c_pts, c_bh, c_bf, c_list, c_count, c_target, c_wgt, c_rows, c_rwave,
+        c_cfeats, c_cscale, c_cmask;
     float last_theta = 0.f;
     unsigned long long last_steps = 0, last_accepted = 0;
 };
@@ -89,11 +104,12 @@ void bh_free(BhState* s) {
     if (!s) return;
     for (DevBuf* b : {&s->keys, &s->keys_alt, &s->idx, &s->idx_alt, &s->sp, &s->lcp, &s->cnt,
                       &s->ip, &s->key2, &s->rec, &s->sums, &s->sums2, &s->level, &s->width,
+                      &s->mom, &s->quad,
                       &s->info, &s->bbox_part, &s->hist, &s->bits, &s->afg_tmp, &s->akeys,
                       &s->akeys_alt, &s->aidx, &s->aidx_alt, &s->apts, &s->h_aff, &s->h_afg,
-                      &s->h_out, &s->c_starts, &s->c_pts, &s->c_bh, &s->c_bf, &s->c_feats,
-                      &s->c_scale, &s->c_mask, &s->c_count, &s->c_drop, &s->c_target, &s->c_wgt,
-                      &s->c_rows, &s->c_rwave, &s->c_cfeats, &s->c_cscale, &s->c_cmask})
+                      &s->h_out, &s->row_buf, &s->row_count, &s->c_starts, &s->c_pts, &s->c_bh,
+                      &s->c_bf, &s->c_list, &s->c_count, &s->c_target, &s->c_wgt, &s->c_rows,
+                      &s->c_rwave, &s->c_cfeats, &s->c_cscale, &s->c_cmask})
         b->release();
     delete s;
 }
@@ -203,6 +219,7 @@ __global__ void bh_info_init_kernel(BhInfo* info) {
     info->flags = 0;
     info->max_level = -1;
     info->total = 0;
+    info->max_rows = 0;
     info->steps = 0;
     info->accepted = 0;
 }
@@ -373,7 +390,8 @@ __global__ __launch_bounds__(256) void bh_node_kernel(
     const uint64_t* __restrict__ keys, const float4* __restrict__ sp, const int* __restrict__ lcp,
     const int* __restrict__ ip, const uint32_t* __restrict__ S, size_t m,
     const BhInfo* __restrict__ info, NodeRec* __restrict__ rec, double4* __restrict__ sums,
-    double4* __restrict__ sums2, uint8_t* __restrict__ level, float* __restrict__ width) {
+    double4* __restrict__ sums2, uint8_t* __restrict__ level, float* __restrict__ width,
+    NodeMoments* __restrict__ mom, NodeQuad* __restrict__ quad) {
     const size_t p = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (p >= m) return;
     const int lp = lcp[p];
@@ -428,6 +446,10 @@ __global__ __launch_bounds__(256) void bh_node_kernel(
             const double cnt_d = double(e - p);
             sums[k] = make_double4(tot, tot * q.x, tot * q.y, tot * q.z);
             sums2[k] = make_double4(cnt_d * q.x, cnt_d * q.y, cnt_d * q.z, cnt_d);
+            if (mom) {  // one position: no spread about the centre
+                mom[k] = NodeMoments{};
+                quad[k] = NodeQuad{make_float4(0.f, 0.f, 0.f, 0.f), make_float4(0.f, 0.f, 0.f, 0.f)};
+            }
             rec[k].cx = q.x;
             rec[k].cy = q.y;
             rec[k].cz = q.z;
@@ -439,11 +461,15 @@ __global__ __launch_bounds__(256) void bh_node_kernel(
 // FP64 sums of the internal nodes of one level from their children (pre-order: first child at
 // k+1, next sibling at the previous child's skip), then the f32 centre of mass
 // (impls/mod.rs:126-131: mass-weighted mean, or the plain mean when the total mass is 0).
+// With `mom`, also the second moments by the parallel-axis rule
+// T = sum over children of (T_c + M_c (c_c - c)(c_c - c)^T), and the normalised quadrupole.
 __global__ __launch_bounds__(256) void bh_up_kernel(NodeRec* __restrict__ rec,
                                                     double4* __restrict__ sums,
                                                     double4* __restrict__ sums2,
                                                     const uint8_t* __restrict__ level, size_t total,
-                                                    int L) {
+                                                    int L, NodeMoments* __restrict__ mom,
+                                                    NodeQuad* __restrict__ quad,
+                                                    const float* __restrict__ width) {
     for (size_t k = size_t(blockIdx.x) * blockDim.x + threadIdx.x; k < total;
          k += size_t(gridDim.x) * blockDim.x) {
         if (level[k] != L) continue;
@@ -470,6 +496,36 @@ __global__ __launch_bounds__(256) void bh_up_kernel(NodeRec* __restrict__ rec,
         rec[k].cy = cy;
         rec[k].cz = cz;
         rec[k].mass = float(a.x);
+        if (mom) {
+            NodeMoments t;
+            NodeQuad qd{make_float4(0.f, 0.f, 0.f, 0.f), make_float4(0.f, 0.f, 0.f, 0.f)};
+            if (a.x > 0.0 && width[k] > 0.0f) {
+                const double px = a.y / a.x, py = a.z / a.x, pz = a.w / a.x;
+                for (size_t ch = k + 1; ch < r.skip;) {
+                    const double4 s1 = sums[ch];
+                    if (s1.x > 0.0) {
+                        const NodeMoments tc = mom[ch];
+                        const double dx = s1.y / s1.x - px, dy = s1.z / s1.x - py,
+                                     dz = s1.w / s1.x - pz;
+                        t.xx += tc.xx + s1.x * dx * dx;
+                        t.yy += tc.yy + s1.x * dy * dy;
+                        t.zz += tc.zz + s1.x * dz * dz;
+                        t.xy += tc.xy + s1.x * dx * dy;
+                        t.xz += tc.xz + s1.x * dx * dz;
+                        t.yz += tc.yz + s1.x * dy * dz;
+                    }
+                    const uint32_t nx = rec[ch].skip;
+                    ch = nx > ch ? nx : ch + 1;
+                }
+                const double w = width[k];
+                const double f = 1.0 / (a.x * w * w), tr = t.xx + t.yy + t.zz;
+                qd.a = make_float4(float((3.0 * t.xx - tr) * f), float((3.0 * t.yy - tr) * f),
+                                   float((3.0 * t.zz - tr) * f), float(3.0 * t.xy * f));
+                qd.b = make_float4(float(3.0 * t.xz * f), float(3.0 * t.yz * f), 0.f, 0.f);
+            }
+            mom[k] = t;
+            quad[k] = qd;
+        }
     }
 }
 
@@ -510,103 +566,86 @@ __global__ __launch_bounds__(256) void bh_ref_com_kernel(NodeRec* __restrict__ r
 
 // ---- traversal -----------------------------------------------------------------------------------
 
-constexpr int WALK_PLAIN = 0;   // plain Barnes-Hut
-constexpr int WALK_RECORD = 1;  // also write one row per (wavefront, accepted internal node)
-constexpr int WALK_MLP = 2;     // add the learned correction of each accepted internal node
-constexpr int MLP_ROWS = 64;    // rows buffered per wavefront before one MLP batch
+// ---- learned correction: row features and the corrected sum --------------------------------------
+// A row (wavefront, accepted internal node) is described as seen from the wavefront's centroid c:
+//   f0..2   unit vector u from c to the node's centre of mass, at distance d
+//   f3      w / d (w = node width)
+//   f4      1 + log2(node mass / total mass) / 32
+//   f5      wavefront rms radius / d
+//   f6      accepting lanes / 64
+//   f7..9   Q u - (u.Q.u) u      Q = node quadrupole (3T - tr(T) I) / (M w^2)
+//   f10..12 (u.Q.u) u
+// Rows with rms radius / d above ROW_SPREAD get zero features and scale (no correction).
+// The row's output y is a correction to the node's monopole term, in units of M w^2 / d^4 (the
+// size of the leading error of that term): every accepting lane adds (M w^2 / d^4) y. For
+// reference, the quadrupole term of the multipole expansion at c is (M w^2 / d^4)
+// (-f7..9 + 1.5 f10..12); the network learns its own map from the residuals.
+// Both helpers are compiled without contraction, so every kernel that calls them reproduces
+// the same values bit for bit.
 
-struct WalkRecord {
-    float* feats;                // [wave][cap][MLP_FEAT]
-    float* scale;                // [wave][cap]
-    unsigned long long* mask;    // [wave][cap]
-    uint32_t* count;             // [wave] rows written
+__device__ __forceinline__ void wave_frame(float4 p, bool valid, float& wx, float& wy, float& wz,
+                                           float& wrad) {
+#pragma clang fp contract(off)
+    float sx = valid ? p.x : 0.f, sy = valid ? p.y : 0.f, sz = valid ? p.z : 0.f;
+    float cnt = valid ? 1.f : 0.f;
+    for (int off = 32; off > 0; off >>= 1) {
+        sx += __shfl_xor(sx, off, 64);
+        sy += __shfl_xor(sy, off, 64);
+        sz += __shfl_xor(sz, off, 64);
+        cnt += __shfl_xor(cnt, off, 64);
+    }
+    wx = sx / cnt;
+    wy = sy / cnt;
+    wz = sz / cnt;
+    float r2 = valid ? (p.x - wx) * (p.x - wx) + (p.y - wy) * (p.y - wy) + (p.z - wz) * (p.z - wz) : 0.f;
+    for (int off = 32; off > 0; off >>= 1) r2 += __shfl_xor(r2, off, 64);
+    wrad = sqrtf(r2 / cnt);
+}
+
+constexpr float ROW_SPREAD = 0.15f;  // largest wavefront rms radius / d of a corrected row
+
+// d2 = squared distance from the wavefront centroid to the node's centre of mass
+__device__ __forceinline__ float centre_dist2(const NodeRec& r, float wx, float wy, float wz) {
+#pragma clang fp contract(off)
+    const float rx = r.cx - wx, ry = r.cy - wy, rz = r.cz - wz;
+    return rx * rx + ry * ry + rz * rz;
+}
+
+__device__ __forceinline__ bool row_kept(float d2, float wrad) {
+#pragma clang fp contract(off)
+    return d2 > 0.f && wrad * wrad <= (ROW_SPREAD * ROW_SPREAD) * d2;
+}
+
+constexpr int WALK_PLAIN = 0;  // plain Barnes-Hut
+constexpr int WALK_ROWS = 1;   // also list the internal nodes each wavefront accepted (rows)
+
+// Per wavefront, the internal nodes that at least one of its lanes accepted and that are far
+// enough from the wavefront (row_kept), in walk order:
+// {node index, accepting-lane count, accepting-lane mask (low, high 32 bits)}.
+struct WalkRows {
+    uint4* rows;      // [wave][cap]
+    uint32_t* count;  // [wave] rows found; only the first cap are stored
     uint32_t cap;
-    unsigned long long* dropped;
-};
-
-struct WalkRows {  // per-wavefront LDS batch of rows for the MLP
-    uint16_t x[MLP_ROWS][MLP_KIN];
-    float y[MLP_ROWS][4];
-    float scale[MLP_ROWS];
-    unsigned long long mask[MLP_ROWS];
 };
 
 template <int MODE>
 __global__ __launch_bounds__(256) void bh_walk_kernel(const NodeRec* __restrict__ nodes,
                                                       uint32_t total,
-                                                      const float* __restrict__ widths,
                                                       const float4* __restrict__ pts,
                                                       const uint32_t* __restrict__ perm,
                                                       uint32_t n, float theta2, float eps2,
                                                       float4* __restrict__ out,
-                                                      BhInfo* __restrict__ info,
-                                                      const MlpWeights* __restrict__ wg,
-                                                      WalkRecord rec) {
-    const MlpForward* mwp = nullptr;
-    WalkRows* rows_base = nullptr;
-    if constexpr (MODE == WALK_MLP) {  // LDS only in the MLP instantiation
-        __shared__ MlpForward mw;
-        __shared__ WalkRows rows_lds[4];
-        const uint32_t* src = reinterpret_cast<const uint32_t*>(wg);
-        uint32_t* dst = reinterpret_cast<uint32_t*>(&mw);
-        for (int t = threadIdx.x; t < int(sizeof(MlpForward) / 4); t += blockDim.x) dst[t] = src[t];
-        __syncthreads();
-        mwp = &mw;
-        rows_base = rows_lds;
-    }
+                                                      BhInfo* __restrict__ info, WalkRows rows) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63;
     const bool valid = i < n;
     const float4 p = pts[valid ? i : n - 1];
     float ax = 0.f, ay = 0.f, az = 0.f, cx = 0.f, cy = 0.f, cz = 0.f;
-    float kx = 0.f, ky = 0.f, kz = 0.f;  // learned correction
     uint32_t resume = valid ? 0u : 0xffffffffu;
-    uint32_t accepted = 0, steps = 0;
-
-    // wavefront centroid and rms radius (row features are relative to the wavefront)
-    float wx = 0.f, wy = 0.f, wz = 0.f, wrad = 0.f, total_mass = 1.f;
-    uint32_t nrows = 0;
+    uint32_t accepted = 0, steps = 0, nrows = 0;
     const uint32_t wave_slot = i / 64;
-    if (MODE != WALK_PLAIN) {
-        float sx = valid ? p.x : 0.f, sy = valid ? p.y : 0.f, sz = valid ? p.z : 0.f;
-        float cnt = valid ? 1.f : 0.f;
-        for (int off = 32; off > 0; off >>= 1) {
-            sx += __shfl_xor(sx, off, 64);
-            sy += __shfl_xor(sy, off, 64);
-            sz += __shfl_xor(sz, off, 64);
-            cnt += __shfl_xor(cnt, off, 64);
-        }
-        wx = sx / cnt; wy = sy / cnt; wz = sz / cnt;
-        float r2 = valid ? (p.x - wx) * (p.x - wx) + (p.y - wy) * (p.y - wy) + (p.z - wz) * (p.z - wz) : 0.f;
-        for (int off = 32; off > 0; off >>= 1) r2 += __shfl_xor(r2, off, 64);
-        wrad = sqrtf(r2 / cnt);
-        if (total > 0) total_mass = nodes[0].mass;
-    }
-    WalkRows* wr = MODE == WALK_MLP ? rows_base + (threadIdx.x >> 6) : nullptr;
-
-    auto flush = [&]() {
-        // y for the buffered rows (16-row tiles, MFMA), then each row's correction to its lanes
-        wave_lds_sync();
-        const int r16 = lane & 15, g = lane >> 4;
-        for (uint32_t t0 = 0; t0 < nrows; t0 += 16) {
-            const f32x4_t yv = mlp_forward16_regs(*mwp, frag(&wr->x[t0 + r16][4 * g]));
-            if (g == 0) {
-                wr->y[t0 + r16][0] = yv[0];
-                wr->y[t0 + r16][1] = yv[1];
-                wr->y[t0 + r16][2] = yv[2];
-            }
-        }
-        wave_lds_sync();
-        for (uint32_t r = 0; r < nrows; ++r) {
-            const float s = wr->scale[r];
-            const bool mine = (wr->mask[r] >> lane) & 1ull;
-            kx += mine ? s * wr->y[r][0] : 0.f;
-            ky += mine ? s * wr->y[r][1] : 0.f;
-            kz += mine ? s * wr->y[r][2] : 0.f;
-        }
-        wave_lds_sync();
-        nrows = 0;
-    };
+    float wx = 0.f, wy = 0.f, wz = 0.f, wrad = 0.f;
+    if (MODE == WALK_ROWS) wave_frame(p, valid, wx, wy, wz, wrad);
 
     // The cursor's two possible successors (k + 1 when some lane opens, skip otherwise) are
     // loaded while node k is evaluated, so the scalar-load latency overlaps the VALU work.
@@ -635,34 +674,15 @@ __global__ __launch_bounds__(256) void bh_walk_kernel(const NodeRec* __restrict_
             cz = (t - az) - y; az = t;
             ++accepted;
         }
-        if (MODE != WALK_PLAIN && r.w2 >= 0.0f) {  // internal node: candidate row
+        if (MODE == WALK_ROWS && r.w2 >= 0.0f) {  // internal node: a row if some lane accepted it
             const unsigned long long m = __ballot(acc);
-            if (m) {
-                float f[MLP_FEAT];
-                const float sc = mlp_features(r.cx, r.cy, r.cz, r.mass, widths[k], wx, wy, wz, wrad,
-                                              total_mass, __popcll(m), f);
-                if (MODE == WALK_RECORD) {
-                    if (lane == 0) {
-                        if (nrows < rec.cap) {
-                            const size_t row = size_t(wave_slot) * rec.cap + nrows;
-                            for (int q = 0; q < MLP_FEAT; ++q) rec.feats[row * MLP_FEAT + q] = f[q];
-                            rec.scale[row] = sc;
-                            rec.mask[row] = m;
-                        } else {
-                            atomicAdd(rec.dropped, 1ull);
-                        }
-                    }
-                    ++nrows;
-                } else {
-                    if (lane == 0) {
-#pragma unroll
-                        for (int q = 0; q < MLP_KIN; ++q)
-                            wr->x[nrows][q] = q < MLP_IN ? f2bf(f[q]) : uint16_t(0);
-                        wr->scale[nrows] = sc;
-                        wr->mask[nrows] = m;
-                    }
-                    if (++nrows == MLP_ROWS) flush();
-                }
+            // nodes close to the wavefront relative to its spread get no row: there one
+            // correction for all lanes cannot stand for the field over the whole wavefront
+            if (m && row_kept(centre_dist2(r, wx, wy, wz), wrad)) {
+                if (lane == 0 && nrows < rows.cap)
+                    rows.rows[size_t(wave_slot) * rows.cap + nrows] =
+                        make_uint4(k, uint32_t(__popcll(m)), uint32_t(m), uint32_t(m >> 32));
+                ++nrows;
             }
         }
         if (active && !open) resume = r.skip;
@@ -671,9 +691,11 @@ __global__ __launch_bounds__(256) void bh_walk_kernel(const NodeRec* __restrict_
         r = (nxt == k + 1) ? r_next : r_skip;
         k = nxt;
     }
-    if (MODE == WALK_MLP && nrows) flush();
-    if (MODE == WALK_RECORD && lane == 0) rec.count[wave_slot] = min(nrows, rec.cap);
-    if (valid) out[perm ? perm[i] : i] = make_float4(ax + kx, ay + ky, az + kz, 0.f);
+    if (MODE == WALK_ROWS && lane == 0 && wave_slot * 64 < n) {  // wavefronts holding particles
+        rows.count[wave_slot] = nrows;
+        atomicMax(&info->max_rows, nrows);
+    }
+    if (valid) out[perm ? perm[i] : i] = make_float4(ax, ay, az, 0.f);
     unsigned long long acc64 = accepted;
     for (int off = 32; off > 0; off >>= 1) acc64 += __shfl_xor(acc64, off, 64);
     if (lane == 0) {
@@ -682,12 +704,158 @@ __global__ __launch_bounds__(256) void bh_walk_kernel(const NodeRec* __restrict_
     }
 }
 
+__device__ __forceinline__ float row_features(const NodeRec& r, float width, const NodeQuad& q,
+                                              float wx, float wy, float wz, float wrad,
+                                              float total_mass, uint32_t lanes,
+                                              float (&f)[MLP_KIN]) {
+#pragma clang fp contract(off)
+#pragma unroll
+    for (int i = 0; i < MLP_KIN; ++i) f[i] = 0.f;
+    const float rx = r.cx - wx, ry = r.cy - wy, rz = r.cz - wz;
+    const float d2 = centre_dist2(r, wx, wy, wz);
+    if (!row_kept(d2, wrad)) return 0.f;
+    const float inv = 1.0f / sqrtf(d2);
+    const float ux = rx * inv, uy = ry * inv, uz = rz * inv;
+    const float s = width * inv;
+    f[0] = ux;
+    f[1] = uy;
+    f[2] = uz;
+    f[3] = s;
+    f[4] = 1.f + log2f(fmaxf(r.mass / total_mass, 1e-30f)) * (1.f / 32.f);
+    f[5] = wrad * inv;
+    f[6] = float(lanes) * (1.f / 64.f);
+    const float qx = q.a.x * ux + q.a.w * uy + q.b.x * uz;
+    const float qy = q.a.w * ux + q.a.y * uy + q.b.y * uz;
+    const float qz = q.b.x * ux + q.b.y * uy + q.a.z * uz;
+    const float quu = qx * ux + qy * uy + qz * uz;
+    f[7] = qx - quu * ux;
+    f[8] = qy - quu * uy;
+    f[9] = qz - quu * uz;
+    f[10] = quu * ux;
+    f[11] = quu * uy;
+    f[12] = quu * uz;
+    return (r.mass / d2) * (s * s);
+}
+
+struct ApplyTile {  // per wavefront: the scaled outputs and lane masks of one 16-row tile
+    float y[16][4];
+    unsigned long long mask[16];
+};
+
+// out[perm[i]] += sum over the rows of i's wavefront whose mask holds i of (M w^2 / d^4) MLP(row).
+// One wavefront per walk wavefront (particles [64 w, 64 w + 64) of pts); its rows go through
+// the MLP 16 at a time (MFMA, weights in LDS), then every lane adds the rows it accepted.
+__global__ __launch_bounds__(256) void mlp_apply_kernel(const NodeRec* __restrict__ nodes,
+                                                        const float* __restrict__ widths,
+                                                        const NodeQuad* __restrict__ quad,
+                                                        const float4* __restrict__ pts,
+                                                        const uint32_t* __restrict__ perm,
+                                                        uint32_t n, const uint4* __restrict__ rows,
+                                                        const uint32_t* __restrict__ count,
+                                                        uint32_t cap, uint32_t waves,
+                                                        const MlpWeights* __restrict__ wg,
+                                                        float4* __restrict__ out) {
+    __shared__ MlpForward w;
+    __shared__ ApplyTile tiles[4];
+    {
+        const uint32_t* src = reinterpret_cast<const uint32_t*>(wg);
+        uint32_t* dst = reinterpret_cast<uint32_t*>(&w);
+        for (int t = threadIdx.x; t < int(sizeof(MlpForward) / 4); t += blockDim.x) dst[t] = src[t];
+        __syncthreads();
+    }
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int r16 = lane & 15, g = lane >> 4;
+    ApplyTile& tile = tiles[wv];
+    const float total_mass = nodes[0].mass;
+    for (uint32_t wave = blockIdx.x * 4 + wv; wave < waves; wave += gridDim.x * 4) {
+        const uint32_t i = wave * 64 + lane;
+        const bool valid = i < n;
+        float wx, wy, wz, wrad;
+        wave_frame(pts[valid ? i : n - 1], valid, wx, wy, wz, wrad);
+        const uint32_t cnt = min(count[wave], cap);
+        const uint4* wrows = rows + size_t(wave) * cap;
+        float kx = 0.f, ky = 0.f, kz = 0.f;
+        // software pipeline: while a tile computes, the node data of the next tile and the row
+        // of the one after load (each load's address arrived one iteration earlier)
+        const uint4 zero4 = make_uint4(0u, 0u, 0u, 0u);
+        uint4 row = r16 < cnt ? wrows[r16] : zero4;
+        NodeRec nrec{};
+        float nwid = 0.f;
+        NodeQuad nq{};
+        if (r16 < cnt) {
+            nrec = nodes[row.x];
+            nwid = widths[row.x];
+            nq = quad[row.x];
+        }
+        uint4 row_next = 16 + r16 < cnt ? wrows[16 + r16] : zero4;
+        for (uint32_t t0 = 0; t0 < cnt; t0 += 16) {
+            const uint32_t ri = t0 + r16;
+            const uint4 crow = row;
+            const NodeRec crec = nrec;
+            const float cwid = nwid;
+            const NodeQuad cq = nq;
+            if (ri + 16 < cnt) {
+                nrec = nodes[row_next.x];
+                nwid = widths[row_next.x];
+                nq = quad[row_next.x];
+            }
+            row = row_next;
+            if (ri + 32 < cnt) row_next = wrows[ri + 32];
+            float f[MLP_KIN];
+            float sc = 0.f;
+            unsigned long long mk = 0ull;
+            if (ri < cnt) {
+                sc = row_features(crec, cwid, cq, wx, wy, wz, wrad, total_mass, crow.y, f);
+                mk = (unsigned long long)crow.z | ((unsigned long long)crow.w << 32);
+            } else {
+#pragma unroll
+                for (int q = 0; q < MLP_KIN; ++q) f[q] = 0.f;
+            }
+            bf16x4_t bx;  // features 4 g .. 4 g + 3 of row r16 (static indices, selected by g)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const float v = g == 0 ? f[q] : g == 1 ? f[4 + q] : g == 2 ? f[8 + q] : f[12 + q];
+                bx[q] = short(f2bf(v));
+            }
+            const f32x4_t yv = mlp_forward16_regs(w, bx);
+            if (g == 0) {
+                tile.y[r16][0] = sc * yv[0];
+                tile.y[r16][1] = sc * yv[1];
+                tile.y[r16][2] = sc * yv[2];
+                tile.mask[r16] = mk;
+            }
+            wave_lds_sync();
+            const uint32_t nt = min(16u, cnt - t0);
+            for (uint32_t q = 0; q < nt; ++q) {
+                if ((tile.mask[q] >> lane) & 1ull) {
+                    kx += tile.y[q][0];
+                    ky += tile.y[q][1];
+                    kz += tile.y[q][2];
+                }
+            }
+            wave_lds_sync();
+        }
+        if (valid) {
+            float4* o = out + (perm ? perm[i] : i);
+            float4 a = *o;
+            a.x += kx;
+            a.y += ky;
+            a.z += kz;
+            *o = a;
+        }
+    }
+}
+
+__global__ void bh_rows_reset_kernel(BhInfo* info) { info->max_rows = 0; }
+
 // ---- host side --------------------------------------------------------------------------------
 
-static int build_tree(pnb_ctx* c, const float4* afg, size_t m, bool ref_com = false) {
+static int build_tree(pnb_ctx* c, const float4* afg, size_t m, bool ref_com = false,
+                      bool moments = false) {
     if (!c->bh) c->bh = new BhState();
     BhState* s = c->bh;
     s->built = false;
+    s->has_moments = false;
     s->m = m;
     s->nodes = 0;
     s->max_level = -1;
@@ -757,19 +925,28 @@ static int build_tree(pnb_ctx* c, const float4* afg, size_t m, bool ref_com = fa
         PNB_HIP(c, s->sums2.ensure(size_t(total) * sizeof(double4)));
         PNB_HIP(c, s->level.ensure(total));
         PNB_HIP(c, s->width.ensure(size_t(total) * sizeof(float)));
+        NodeMoments* mom = nullptr;
+        NodeQuad* quad = nullptr;
+        if (moments) {
+            PNB_HIP(c, s->mom.ensure(size_t(total) * sizeof(NodeMoments)));
+            PNB_HIP(c, s->quad.ensure(size_t(total) * sizeof(NodeQuad)));
+            mom = s->mom.as<NodeMoments>();
+            quad = s->quad.as<NodeQuad>();
+        }
         NodeRec* rec = s->rec.as<NodeRec>();
         bh_node_kernel<<<g, 256, 0, c->stream>>>(sk, s->sp.as<float4>(), lcp, s->ip.as<int>(),
                                                  cnt, m, info, rec, s->sums.as<double4>(),
                                                  s->sums2.as<double4>(), s->level.as<uint8_t>(),
-                                                 s->width.as<float>());
+                                                 s->width.as<float>(), mom, quad);
         PNB_CHECK_LAUNCH(c);
         const unsigned ug = std::min<unsigned>(4 * std::max(c->cu_count, 1), div_up(total, 256));
         for (int L = h.max_level - 1; L >= 0; --L) {
             bh_up_kernel<<<ug, 256, 0, c->stream>>>(rec, s->sums.as<double4>(),
                                                     s->sums2.as<double4>(), s->level.as<uint8_t>(),
-                                                    total, L);
+                                                    total, L, mom, quad, s->width.as<float>());
             PNB_CHECK_LAUNCH(c);
         }
+        s->has_moments = moments;
         if (ref_com) {
             bh_ref_com_kernel<<<div_up(total, 256), 256, 0, c->stream>>>(rec, s->sp.as<float4>(), total);
             PNB_CHECK_LAUNCH(c);
@@ -782,40 +959,77 @@ static int build_tree(pnb_ctx* c, const float4* afg, size_t m, bool ref_com = fa
 const MlpWeights* mlp_weights(pnb_ctx* c);  // mlp.hip: nullptr until initialised
 
 static int walk(pnb_ctx* c, const float4* pts, const uint32_t* perm, size_t n, float theta,
-                float softening, float4* out, int mode = WALK_PLAIN,
-                const WalkRecord* record = nullptr) {
+                float softening, float4* out, const WalkRows* rows = nullptr) {
     BhState* s = c->bh;
     const float theta2 = theta * theta;  // (self.theta * self.theta), sequential.rs:491
     const float eps2 = softening * softening;
-    const MlpWeights* wg = nullptr;
-    if (mode == WALK_MLP) {
-        wg = mlp_weights(c);
-        if (!wg) return fail(c, PNB_E_STATE, "PNB_BH_MLP needs an initialised MLP (pnb_mlp_init)");
-    }
-    WalkRecord rec = record ? *record : WalkRecord{};
     return timed(c, PNB_PROF_BH_WALK, [&]() -> int {
         const unsigned grid = div_up(n, 256);
-        auto* kern = mode == WALK_MLP ? bh_walk_kernel<WALK_MLP>
-                     : mode == WALK_RECORD ? bh_walk_kernel<WALK_RECORD> : bh_walk_kernel<WALK_PLAIN>;
-        kern<<<grid, 256, 0, c->stream>>>(s->rec.as<NodeRec>(), uint32_t(s->nodes),
-                                          s->width.as<float>(), pts, perm, uint32_t(n), theta2,
-                                          eps2, out, s->info.as<BhInfo>(), wg, rec);
+        if (rows) {
+            bh_rows_reset_kernel<<<1, 1, 0, c->stream>>>(s->info.as<BhInfo>());
+            PNB_CHECK_LAUNCH(c);
+        }
+        auto* kern = rows ? bh_walk_kernel<WALK_ROWS> : bh_walk_kernel<WALK_PLAIN>;
+        kern<<<grid, 256, 0, c->stream>>>(s->rec.as<NodeRec>(), uint32_t(s->nodes), pts, perm,
+                                          uint32_t(n), theta2, eps2, out, s->info.as<BhInfo>(),
+                                          rows ? *rows : WalkRows{});
         PNB_CHECK_LAUNCH(c);
         return PNB_OK;
     });
 }
 
+// Walk listing rows, then add the learned correction. Row storage is sized from the largest
+// wavefront list seen so far; a walk that finds more rows than fit is repeated with room for all.
+static int walk_mlp(pnb_ctx* c, const float4* pts, const uint32_t* perm, size_t n, float theta,
+                    float softening, float4* out) {
+    BhState* s = c->bh;
+    const MlpWeights* wg = mlp_weights(c);
+    if (!wg) return fail(c, PNB_E_STATE, "PNB_BH_MLP needs an initialised MLP (pnb_mlp_init)");
+    if (!s->has_moments) return fail(c, PNB_E_STATE, "octree built without node moments");
+    const size_t waves = div_up(n, 64);
+    PNB_HIP(c, s->row_count.ensure(waves * 4));
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        const size_t cap = s->row_cap;
+        PNB_HIP(c, s->row_buf.ensure(waves * cap * sizeof(uint4)));
+        WalkRows rows{s->row_buf.as<uint4>(), s->row_count.as<uint32_t>(), uint32_t(cap)};
+        if (int rc = walk(c, pts, perm, n, theta, softening, out, &rows)) return rc;
+        uint32_t most = 0;
+        PNB_HIP(c, hipMemcpyAsync(&most, &s->info.as<BhInfo>()->max_rows, 4, hipMemcpyDeviceToHost,
+                                  c->stream));
+        PNB_HIP(c, hipStreamSynchronize(c->stream));
+        if (most <= cap) {
+            if (n == 0 || s->nodes == 0) return PNB_OK;
+            return timed(c, PNB_PROF_MLP, [&]() -> int {
+                const unsigned grid = unsigned(div_up(waves, 4));  // one wavefront per walk wavefront
+                mlp_apply_kernel<<<grid, 256, 0, c->stream>>>(
+                    s->rec.as<NodeRec>(), s->width.as<float>(), s->quad.as<NodeQuad>(), pts, perm,
+                    uint32_t(n), s->row_buf.as<uint4>(), s->row_count.as<uint32_t>(), uint32_t(cap),
+                    uint32_t(waves), wg, out);
+                PNB_CHECK_LAUNCH(c);
+                return PNB_OK;
+            });
+        }
+        s->row_cap = (size_t(most) + most / 4 + 255) / 256 * 256;
+    }
+    return fail(c, PNB_E_STATE, "row listing did not converge");
+}
+
 // Orders the affected particles along the tree's key order so that each wavefront holds
 // neighbours, walks, and scatters results back to the caller's order.
 static int accel_dev(pnb_ctx* c, const float4* affected, size_t n, const float4* afg, size_t m,
-                     float theta, float softening, float4* out, int mode = WALK_PLAIN,
+                     float theta, float softening, float4* out, bool mlp = false,
                      bool ref_com = false) {
     if (n == 0) return PNB_OK;
     if (n > 0x7fffffffu) return fail(c, PNB_E_ARG, "Barnes-Hut supports at most 2^31-1 particles");
-    if (int rc = build_tree(c, afg, m, ref_com)) return rc;
+    if (mlp && !mlp_weights(c))
+        return fail(c, PNB_E_STATE, "PNB_BH_MLP needs an initialised MLP (pnb_mlp_init)");
+    if (int rc = build_tree(c, afg, m, ref_com, mlp)) return rc;
     BhState* s = c->bh;
-    if (affected == afg && n == m)
-        return walk(c, s->sp.as<float4>(), s->sidx, n, theta, softening, out, mode);
+    auto run = [&](const float4* pts, const uint32_t* perm) -> int {
+        return (mlp && m) ? walk_mlp(c, pts, perm, n, theta, softening, out)
+                          : walk(c, pts, perm, n, theta, softening, out);
+    };
+    if (affected == afg && n == m) return run(s->sp.as<float4>(), s->sidx);
     const unsigned g = div_up(n, 256);
     PNB_HIP(c, s->akeys.ensure(n * 8));
     PNB_HIP(c, s->akeys_alt.ensure(n * 8));
@@ -824,7 +1038,7 @@ static int accel_dev(pnb_ctx* c, const float4* affected, size_t n, const float4*
     PNB_HIP(c, s->apts.ensure(n * sizeof(float4)));
     if (m == 0) {
         // no tree: every particle gets zero (the reference's root is None)
-        return walk(c, affected, nullptr, n, theta, softening, out, mode);
+        return walk(c, affected, nullptr, n, theta, softening, out);
     }
     bh_keys_kernel<<<g, 256, 0, c->stream>>>(affected, n, s->info.as<BhInfo>(),
                                              s->akeys.as<uint64_t>(), s->aidx.as<uint32_t>());
@@ -837,7 +1051,7 @@ static int accel_dev(pnb_ctx* c, const float4* affected, size_t n, const float4*
         return rc;
     bh_gather_kernel<<<g, 256, 0, c->stream>>>(affected, n, si, s->apts.as<float4>());
     PNB_CHECK_LAUNCH(c);
-    return walk(c, s->apts.as<float4>(), si, n, theta, softening, out, mode);
+    return run(s->apts.as<float4>(), si);
 }
 
 static int copy_h2d(pnb_ctx* c, DevBuf& b, const float* src, size_t count) {
@@ -855,8 +1069,6 @@ __global__ void mlp_gather_waves_kernel(const float4* __restrict__ sp, const uin
     if (i < waves * 64) dst[i] = sp[starts[i / 64] + (i % 64)];
 }
 
-__global__ void mlp_zero_u64_kernel(unsigned long long* p) { *p = 0ull; }
-
 // target = brute force - Barnes-Hut, weight = 1 / |brute force|^2 (0 when it vanishes)
 __global__ void mlp_target_kernel(const float4* __restrict__ bf, const float4* __restrict__ bh,
                                   uint32_t n, float* __restrict__ target, float* __restrict__ wgt) {
@@ -871,20 +1083,30 @@ __global__ void mlp_target_kernel(const float4* __restrict__ bf, const float4* _
     wgt[i] = n2 > 0.f ? 1.f / n2 : 0.f;
 }
 
-// Moves each wave's recorded rows [w * cap, w * cap + count) to [wave_rows[w], wave_rows[w+1]).
-__global__ void mlp_compact_rows_kernel(const float* __restrict__ feats, const float* __restrict__ scale,
-                                        const unsigned long long* __restrict__ mask,
-                                        const uint32_t* __restrict__ wave_rows, uint32_t cap,
-                                        float* __restrict__ cfeats, float* __restrict__ cscale,
-                                        unsigned long long* __restrict__ cmask,
-                                        uint32_t* __restrict__ rwave) {
+// One 64-thread block per walked wavefront: the features of its stored rows, written compactly
+// at [wave_rows[w], wave_rows[w+1]) (the same row_features / wave_frame as mlp_apply_kernel).
+__global__ __launch_bounds__(64) void mlp_row_feats_kernel(
+    const NodeRec* __restrict__ nodes, const float* __restrict__ widths,
+    const NodeQuad* __restrict__ quad, const float4* __restrict__ pts,
+    const uint4* __restrict__ rows, uint32_t cap, const uint32_t* __restrict__ wave_rows,
+    float* __restrict__ feats, float* __restrict__ scale, unsigned long long* __restrict__ mask,
+    uint32_t* __restrict__ rwave) {
     const uint32_t w = blockIdx.x;
+    const int lane = threadIdx.x;
+    float wx, wy, wz, wrad;
+    wave_frame(pts[size_t(w) * 64 + lane], true, wx, wy, wz, wrad);
+    const float total_mass = nodes[0].mass;
     const uint32_t b = wave_rows[w], cnt = wave_rows[w + 1] - b;
-    for (uint32_t r = threadIdx.x; r < cnt; r += blockDim.x) {
-        const size_t src = size_t(w) * cap + r, dst = size_t(b) + r;
-        for (int q = 0; q < MLP_FEAT; ++q) cfeats[dst * MLP_FEAT + q] = feats[src * MLP_FEAT + q];
-        cscale[dst] = scale[src];
-        cmask[dst] = mask[src];
+    for (uint32_t r = lane; r < cnt; r += 64) {
+        const uint4 row = rows[size_t(w) * cap + r];
+        float f[MLP_KIN];
+        const float sc = row_features(nodes[row.x], widths[row.x], quad[row.x], wx, wy, wz, wrad,
+                                      total_mass, row.y, f);
+        const size_t dst = size_t(b) + r;
+#pragma unroll
+        for (int q = 0; q < MLP_FEAT; ++q) feats[dst * MLP_FEAT + q] = f[q];
+        scale[dst] = sc;
+        mask[dst] = (unsigned long long)row.z | ((unsigned long long)row.w << 32);
         rwave[dst] = w;
     }
 }
@@ -895,7 +1117,7 @@ int mlp_append(pnb_ctx* c, const float* feats, const float* scale, const unsigne
 
 static int collect(pnb_ctx* c, const float4* p, size_t n, float theta, float softening,
                    size_t waves, uint64_t seed, uint32_t cap) {
-    if (int rc = build_tree(c, p, n)) return rc;
+    if (int rc = build_tree(c, p, n, false, true)) return rc;
     BhState* s = c->bh;
     const size_t full = n / 64;
     const size_t W = std::min(waves, full);
@@ -909,11 +1131,8 @@ static int collect(pnb_ctx* c, const float4* p, size_t n, float theta, float sof
     PNB_HIP(c, s->c_pts.ensure(nr * sizeof(float4)));
     PNB_HIP(c, s->c_bh.ensure(nr * sizeof(float4)));
     PNB_HIP(c, s->c_bf.ensure(nr * sizeof(float4)));
-    PNB_HIP(c, s->c_feats.ensure(W * cap * MLP_FEAT * 4));
-    PNB_HIP(c, s->c_scale.ensure(W * cap * 4));
-    PNB_HIP(c, s->c_mask.ensure(W * cap * 8));
+    PNB_HIP(c, s->c_list.ensure(W * cap * sizeof(uint4)));
     PNB_HIP(c, s->c_count.ensure(W * 4));
-    PNB_HIP(c, s->c_drop.ensure(8));
     PNB_HIP(c, s->c_target.ensure(nr * 16));
     PNB_HIP(c, s->c_wgt.ensure(nr * 4));
     PNB_HIP(c, s->c_rows.ensure((W + 1) * 4));
@@ -921,12 +1140,9 @@ static int collect(pnb_ctx* c, const float4* p, size_t n, float theta, float sof
     mlp_gather_waves_kernel<<<div_up(nr, 256), 256, 0, c->stream>>>(
         s->sp.as<float4>(), s->c_starts.as<uint32_t>(), uint32_t(W), s->c_pts.as<float4>());
     PNB_CHECK_LAUNCH(c);
-    mlp_zero_u64_kernel<<<1, 1, 0, c->stream>>>(s->c_drop.as<unsigned long long>());
-    PNB_CHECK_LAUNCH(c);
-    WalkRecord rec{s->c_feats.as<float>(), s->c_scale.as<float>(), s->c_mask.as<unsigned long long>(),
-                   s->c_count.as<uint32_t>(), cap, s->c_drop.as<unsigned long long>()};
+    WalkRows rows{s->c_list.as<uint4>(), s->c_count.as<uint32_t>(), cap};
     if (int rc = walk(c, s->c_pts.as<float4>(), nullptr, nr, theta, softening, s->c_bh.as<float4>(),
-                      WALK_RECORD, &rec))
+                      &rows))
         return rc;
     if (int rc = bf_launch(c, s->c_pts.as<float4>(), nr, s->sp.as<float4>(), n, softening, 1,
                            s->c_bf.as<float4>()))
@@ -935,22 +1151,25 @@ static int collect(pnb_ctx* c, const float4* p, size_t n, float theta, float sof
                                                             uint32_t(nr), s->c_target.as<float>(),
                                                             s->c_wgt.as<float>());
     PNB_CHECK_LAUNCH(c);
-    std::vector<uint32_t> counts(W), rows(W + 1, 0);
-    unsigned long long dropped = 0;
+    std::vector<uint32_t> counts(W), offs(W + 1, 0);
     PNB_HIP(c, hipMemcpyAsync(counts.data(), s->c_count.ptr, W * 4, hipMemcpyDeviceToHost, c->stream));
-    PNB_HIP(c, hipMemcpyAsync(&dropped, s->c_drop.ptr, 8, hipMemcpyDeviceToHost, c->stream));
     PNB_HIP(c, hipStreamSynchronize(c->stream));
-    for (size_t w = 0; w < W; ++w) rows[w + 1] = rows[w] + counts[w];
-    const size_t R = rows[W];
+    unsigned long long dropped = 0;
+    for (size_t w = 0; w < W; ++w) {
+        const uint32_t kept = std::min(counts[w], cap);
+        dropped += counts[w] - kept;
+        offs[w + 1] = offs[w] + kept;
+    }
+    const size_t R = offs[W];
     PNB_HIP(c, s->c_cfeats.ensure(std::max<size_t>(R, 1) * MLP_FEAT * 4));
     PNB_HIP(c, s->c_cscale.ensure(std::max<size_t>(R, 1) * 4));
     PNB_HIP(c, s->c_cmask.ensure(std::max<size_t>(R, 1) * 8));
     PNB_HIP(c, s->c_rwave.ensure(std::max<size_t>(R, 1) * 4));
-    PNB_HIP(c, hipMemcpyAsync(s->c_rows.ptr, rows.data(), (W + 1) * 4, hipMemcpyHostToDevice, c->stream));
-    mlp_compact_rows_kernel<<<unsigned(W), 256, 0, c->stream>>>(
-        s->c_feats.as<float>(), s->c_scale.as<float>(), s->c_mask.as<unsigned long long>(),
-        s->c_rows.as<uint32_t>(), cap, s->c_cfeats.as<float>(), s->c_cscale.as<float>(),
-        s->c_cmask.as<unsigned long long>(), s->c_rwave.as<uint32_t>());
+    PNB_HIP(c, hipMemcpyAsync(s->c_rows.ptr, offs.data(), (W + 1) * 4, hipMemcpyHostToDevice, c->stream));
+    mlp_row_feats_kernel<<<unsigned(W), 64, 0, c->stream>>>(
+        s->rec.as<NodeRec>(), s->width.as<float>(), s->quad.as<NodeQuad>(), s->c_pts.as<float4>(),
+        s->c_list.as<uint4>(), cap, s->c_rows.as<uint32_t>(), s->c_cfeats.as<float>(),
+        s->c_cscale.as<float>(), s->c_cmask.as<unsigned long long>(), s->c_rwave.as<uint32_t>());
     PNB_CHECK_LAUNCH(c);
     return mlp_append(c, s->c_cfeats.as<float>(), s->c_cscale.as<float>(),
                       s->c_cmask.as<unsigned long long>(), s->c_rwave.as<uint32_t>(),
@@ -979,7 +1198,7 @@ int pnb_bh_accel_f32_dev(pnb_ctx* c, const float* affected, size_t n, const floa
     if (int rc = activate(c)) return rc;
     return accel_dev(c, reinterpret_cast<const float4*>(affected), n,
                      reinterpret_cast<const float4*>(affecting), m, theta, softening,
-                     reinterpret_cast<float4*>(out), (flags & PNB_BH_MLP) ? WALK_MLP : WALK_PLAIN,
+                     reinterpret_cast<float4*>(out), (flags & PNB_BH_MLP) != 0,
                      (flags & PNB_BH_REF_COM) != 0);
 }
 
@@ -1023,7 +1242,7 @@ int pnb_reordered_bh_accel_f32_dev(pnb_ctx* c, const float* particles, size_t n,
     (void)checked;
     if (flags & ~(PNB_BH_MLP | PNB_BH_REF_COM)) return fail(c, PNB_E_ARG, "unknown Barnes-Hut flags");
     return accel_dev(c, p, n, afg, m, theta, softening, reinterpret_cast<float4*>(out),
-                     (flags & PNB_BH_MLP) ? WALK_MLP : WALK_PLAIN, (flags & PNB_BH_REF_COM) != 0);
+                     (flags & PNB_BH_MLP) != 0, (flags & PNB_BH_REF_COM) != 0);
 }
 
 int pnb_reordered_bh_accel_f32(pnb_ctx* c, const float* particles, size_t n, float theta,
@@ -1064,10 +1283,13 @@ int pnb_bh_tree_export(pnb_ctx* c, pnb_tree_node* out, size_t cap, uint32_t* ord
         uint8_t* lev = new uint8_t[T];
         float* wid = new float[T];
         double4* sm = new double4[T];
+        NodeQuad* qd = s->has_moments ? new NodeQuad[T] : nullptr;
         hipError_t e = hipMemcpyAsync(rec, s->rec.ptr, T * sizeof(NodeRec), hipMemcpyDeviceToHost, c->stream);
         if (e == hipSuccess) e = hipMemcpyAsync(lev, s->level.ptr, T, hipMemcpyDeviceToHost, c->stream);
         if (e == hipSuccess) e = hipMemcpyAsync(wid, s->width.ptr, T * 4, hipMemcpyDeviceToHost, c->stream);
         if (e == hipSuccess) e = hipMemcpyAsync(sm, s->sums.ptr, T * sizeof(double4), hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess && qd)
+            e = hipMemcpyAsync(qd, s->quad.ptr, T * sizeof(NodeQuad), hipMemcpyDeviceToHost, c->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
         if (e == hipSuccess)
             for (size_t k = 0; k < T; ++k) {
@@ -1081,11 +1303,16 @@ int pnb_bh_tree_export(pnb_ctx* c, pnb_tree_node* out, size_t cap, uint32_t* ord
                 o.count = rec[k].end - rec[k].begin;
                 o.level = lev[k];
                 o.internal = rec[k].w2 >= 0.0f ? 1u : 0u;
+                const float q6[6] = {qd ? qd[k].a.x : 0.f, qd ? qd[k].a.y : 0.f,
+                                     qd ? qd[k].a.z : 0.f, qd ? qd[k].a.w : 0.f,
+                                     qd ? qd[k].b.x : 0.f, qd ? qd[k].b.y : 0.f};
+                for (int q = 0; q < 6; ++q) o.quad[q] = q6[q];
             }
         delete[] rec;
         delete[] lev;
         delete[] wid;
         delete[] sm;
+        delete[] qd;
         if (e != hipSuccess) return hip_fail(c, e, "tree export copy", __FILE__, __LINE__);
     }
     if (order && s->m)

@@ -486,7 +486,8 @@ int pnb_mlp_init(pnb_ctx* c, uint64_t seed) {
     if (int rc = activate(c)) return rc;
     if (int rc = mlp_ensure(c)) return rc;
     MlpState* s = c->mlp;
-    // He-normal initialisation (Box-Muller over SplitMix64), last layer scaled by 0.1, zero biases
+    // He-normal hidden layers (Box-Muller over SplitMix64), zero biases, and a zero output layer:
+    // a fresh network predicts no correction, i.e. starts exactly at plain Barnes-Hut
     std::vector<float> p(MLP_NPARAM, 0.f);
     uint64_t st = seed;
     auto normal = [&]() {
@@ -496,7 +497,6 @@ int pnb_mlp_init(pnb_ctx* c, uint64_t seed) {
     };
     for (int k = 0; k < MLP_IN * MLP_H; ++k) p[MLP_OFF_W1 + k] = float(normal() * std::sqrt(2.0 / MLP_IN));
     for (int k = 0; k < MLP_H * MLP_H; ++k) p[MLP_OFF_W2 + k] = float(normal() * std::sqrt(2.0 / MLP_H));
-    for (int k = 0; k < MLP_H * MLP_OUT; ++k) p[MLP_OFF_W3 + k] = float(0.1 * normal() * std::sqrt(2.0 / MLP_H));
     std::vector<float> zeros(MLP_NPARAM, 0.f);
     PNB_HIP(c, hipMemcpyAsync(s->params.ptr, p.data(), MLP_NPARAM * 4, hipMemcpyHostToDevice, c->stream));
     PNB_HIP(c, hipMemcpyAsync(s->m.ptr, zeros.data(), MLP_NPARAM * 4, hipMemcpyHostToDevice, c->stream));

@@ -1,6 +1,6 @@
 // Far-field correction MLP: shared device definitions (bh.hip walk, mlp.hip training).
 //
-// Network: 7 inputs -> 64 -> 64 -> 3, ReLU. BF16 weights and activations feed MFMA
+// Network: 13 inputs -> 64 -> 64 -> 3, ReLU. BF16 weights and activations feed MFMA
 // (v_mfma_f32_16x16x16_bf16, FP32 accumulate); biases and everything outside the products are
 // FP32. A "row" is one (wavefront, accepted internal node) event of the Barnes-Hut walk: its
 // features describe the node as seen from the wavefront's centroid, and its output y is a
@@ -19,7 +19,7 @@
 
 namespace pnb {
 
-constexpr int MLP_IN = 7;
+constexpr int MLP_IN = 13;
 constexpr int MLP_H = 64;
 constexpr int MLP_OUT = 3;
 constexpr int MLP_KIN = 16;   // inputs padded to one K step
@@ -30,8 +30,8 @@ constexpr int MLP_OFF_W2 = MLP_OFF_B1 + MLP_H;
 constexpr int MLP_OFF_B2 = MLP_OFF_W2 + MLP_H * MLP_H;
 constexpr int MLP_OFF_W3 = MLP_OFF_B2 + MLP_H;
 constexpr int MLP_OFF_B3 = MLP_OFF_W3 + MLP_H * MLP_OUT;
-constexpr int MLP_NPARAM = MLP_OFF_B3 + MLP_OUT;  // 4867
-constexpr int MLP_FEAT = 8;                       // stored feature stride (7 used)
+constexpr int MLP_NPARAM = MLP_OFF_B3 + MLP_OUT;  // 5251
+constexpr int MLP_FEAT = 16;                      // stored feature stride (13 used)
 
 typedef short bf16x4_t __attribute__((ext_vector_type(4)));
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
@@ -39,7 +39,7 @@ typedef float f32x4_t __attribute__((ext_vector_type(4)));
 // BF16 operand copies of the weights (as raw bf16 bits) and FP32 biases. The forward part
 // (MlpForward) is a prefix, so inference kernels stage only that much in LDS.
 struct MlpForward {
-    uint16_t w1t[MLP_H][MLP_KIN];   // [n][k] = W1[k][n], k >= 7 zero
+    uint16_t w1t[MLP_H][MLP_KIN];   // [n][k] = W1[k][n], k >= MLP_IN zero
     uint16_t w2t[MLP_H][MLP_H];     // [n][k] = W2[k][n]
     uint16_t w3t[MLP_NOUT][MLP_H];  // [n][k] = W3[k][n], n >= 3 zero
     float b1[MLP_H], b2[MLP_H], b3[MLP_NOUT];
@@ -170,31 +170,6 @@ __device__ __forceinline__ void mlp_forward16(const MlpForward& w, MlpTile& t, f
         for (int i = 0; i < 4; ++i) y[(4 * g + i) * 4 + r16] = acc[i] + b;
     }
     wave_lds_sync();
-}
-
-// Features of one (wavefront, internal node) row, as seen from the wavefront centroid c:
-// unit direction to the node's centre of mass (3), node width / distance, log2 of the node's
-// mass fraction scaled to about [0, 1], wavefront rms radius / distance, accepting-lane fraction.
-// Returns the monopole scale mass / d^2 (0 when d == 0: the row is then ignored).
-__device__ __forceinline__ float mlp_features(float cx, float cy, float cz, float mass, float width,
-                                              float wx, float wy, float wz, float wrad,
-                                              float total_mass, int accepting, float* f) {
-    const float rx = cx - wx, ry = cy - wy, rz = cz - wz;
-    const float d2 = rx * rx + ry * ry + rz * rz;
-    if (!(d2 > 0.f)) {
-        for (int i = 0; i < MLP_FEAT; ++i) f[i] = 0.f;
-        return 0.f;
-    }
-    const float inv = rsqrtf(d2);
-    f[0] = rx * inv;
-    f[1] = ry * inv;
-    f[2] = rz * inv;
-    f[3] = width * inv;
-    f[4] = 1.f + log2f(fmaxf(mass / total_mass, 1e-30f)) * (1.f / 32.f);
-    f[5] = wrad * inv;
-    f[6] = float(accepting) * (1.f / 64.f);
-    f[7] = 0.f;
-    return mass / d2;
 }
 
 }  // namespace pnb

@@ -12,7 +12,8 @@ import numpy as np
 
 from . import _lib
 
-FEAT = 8
+FEAT = 16  # stored feature stride
+IN = 13    # features the network reads
 
 
 def num_params() -> int:
@@ -37,7 +38,7 @@ def get_params(res) -> np.ndarray:
 def params_to_dict(p: np.ndarray) -> dict:
     """Splits the flat vector into the oracle's W1, b1, W2, b2, W3, b3 (FP64)."""
     p = np.asarray(p, np.float64)
-    shapes = [("W1", (7, 64)), ("b1", (64,)), ("W2", (64, 64)), ("b2", (64,)), ("W3", (64, 3)),
+    shapes = [("W1", (IN, 64)), ("b1", (64,)), ("W2", (64, 64)), ("b2", (64,)), ("W3", (64, 3)),
               ("b3", (3,))]
     out, o = {}, 0
     for name, shp in shapes:
@@ -122,6 +123,58 @@ def train(res, iterations: int, lr: float = 1e-3) -> np.ndarray:
     hist = np.zeros(iterations, np.float32)
     res.check(_lib.lib().pnb_mlp_train(res.handle, iterations, lr, hist.ctypes.data))
     return hist
+
+
+def quadrupole_params() -> np.ndarray:
+    """Parameters that make the network output the quadrupole term of the node's multipole
+    expansion at the wavefront centroid, y = -f7..9 + 1.5 f10..12 (in units of M w^2 / d^4):
+    features 7..12 pass through 12 hidden units as relu(+f) and relu(-f) (every weight is 0,
+    +-1 or +-1.5, exact in bf16). A physics-informed starting point for training."""
+    d = params_to_dict(np.zeros(num_params(), np.float32))
+    for j in range(6):
+        d["W1"][7 + j, 2 * j] = 1.0
+        d["W1"][7 + j, 2 * j + 1] = -1.0
+    for i in range(12):
+        d["W2"][i, i] = 1.0
+    for o in range(3):
+        d["W3"][2 * o, o] = -1.0
+        d["W3"][2 * o + 1, o] = 1.0
+        d["W3"][6 + 2 * o, o] = 1.5
+        d["W3"][7 + 2 * o, o] = -1.5
+    return dict_to_params(d)
+
+
+def fit(res, iterations: int, lrs=(1e-3, 3e-4, 1e-4, 3e-5, 1e-5), probe: int = 40) -> dict:
+    """Full-batch training from the current parameters with the learning rate chosen by probes.
+
+    Every candidate in `lrs` runs `probe` Adam steps from the same start; the one with the
+    lowest loss then runs `iterations` steps in three phases (lr for half of them, lr/10 and
+    lr/100 for a quarter each). If the result fits the training set worse than the starting
+    parameters, those are restored (a fresh network is plain Barnes-Hut, the fallback).
+    Returns the loss history, the chosen rate and the start / end losses."""
+    p0 = get_params(res)
+    loss0, _ = grad(res)
+    best = None
+    for lr in lrs:
+        set_params(res, p0)
+        h = train(res, probe, lr)
+        last, _ = grad(res)
+        if np.isfinite(last) and (best is None or last < best[1]):
+            best = (lr, last)
+    set_params(res, p0)
+    if best is None:
+        return {"history": np.zeros(0, np.float32), "lr": None, "loss_start": loss0,
+                "loss_end": loss0, "kept": False}
+    lr = best[0]
+    phases = [(iterations // 2, lr), (iterations // 4, lr / 10),
+              (iterations - iterations // 2 - iterations // 4, lr / 100)]
+    hist = np.concatenate([train(res, it, r) for it, r in phases if it > 0] or [np.zeros(0)])
+    loss1, _ = grad(res)
+    kept = bool(np.isfinite(loss1) and loss1 < loss0)
+    if not kept:
+        set_params(res, p0)
+    return {"history": hist, "lr": lr, "loss_start": loss0, "loss_end": loss1 if kept else loss0,
+            "kept": kept}
 
 
 def forward(res, feats_dev, y_dev) -> None:

@@ -2,13 +2,16 @@
 
 1. Training set: `--snapshots` two-Plummer systems (seeded; each seed also seen under a few random
    rotations, which changes how the octree cuts it), `--waves` wavefronts each, rows and
-   brute-force-minus-Barnes-Hut targets generated on the GPU at `--theta`.
-2. Full-batch Adam on the GPU (`--iters`).
-3. A held-out snapshot: plain Barnes-Hut at several theta and NN-corrected Barnes-Hut at
-   `--theta`, each timed (tree build + walk, device time from HIP events) and checked against
-   the FP64 direct sum on a seeded sample. Reports the particles/s of each, the error of each,
-   the plain-BH theta with the same error as the corrected walk (log-linear interpolation) and
-   the resulting speed ratio.
+   brute-force-minus-Barnes-Hut targets generated on the GPU, at opening angles cycling through
+   `--train-thetas`.
+2. Full-batch Adam on the GPU (`--iters`), learning rate picked by short probes and stepped
+   down by 10x twice (particular_amd.learned.fit).
+3. A held-out snapshot: plain Barnes-Hut at `--thetas` and NN-corrected Barnes-Hut at
+   `--nn-thetas` (with the starting network as well when it is the quadrupole one), each timed (tree build + walk [+ correction], device time from HIP events) and
+   checked against the FP64 direct sum on a seeded sample. Reports particles/s and error of each
+   run and, by log-log interpolation of time against error on each curve, the speed-up of the
+   corrected method over plain Barnes-Hut at equal rms error (at `--target-rms` and at the error
+   of each NN run).
 Prints one JSON document.
 """
 from __future__ import annotations
@@ -53,9 +56,9 @@ def timed_bh(res, dev, theta, mlp, reps=2):
         res.profile(True)
         out = alg.compute(pa.Between(dev, dev))
         torch.cuda.synchronize()
-        b, _ = res.profile_read(_lib.PROF_BH_BUILD)
-        w, _ = res.profile_read(_lib.PROF_BH_WALK)
-        times.append(b + w)
+        parts = [res.profile_read(slot)[0]
+                 for slot in (_lib.PROF_BH_BUILD, _lib.PROF_BH_WALK, _lib.PROF_MLP)]
+        times.append((sum(parts), parts))
     res.profile(False)
     return out, min(times)
 
@@ -72,17 +75,32 @@ def fp64_reference(res, dev, idx_dev, sample):
     return out.cpu().numpy()
 
 
+def time_at_error(curve, rms):
+    """Time at a given rms error on a (rms, ms) curve, interpolated in log-log; None outside."""
+    e = np.log([q["rms_rel"] for q in curve])
+    t = np.log([q["ms"] for q in curve])
+    o = np.argsort(e)
+    x = np.log(rms)
+    if not (e.min() <= x <= e.max()):
+        return None
+    return float(np.exp(np.interp(x, e[o], t[o])))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--n", type=int, default=4194304)
     ap.add_argument("--snapshots", type=int, default=64)
     ap.add_argument("--rotations", type=int, default=8, help="rotations per seed")
     ap.add_argument("--waves", type=int, default=128)
-    ap.add_argument("--theta", type=float, default=0.8)
-    ap.add_argument("--iters", type=int, default=400)
-    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--train-thetas", default="0.8,0.6,0.45")
+    ap.add_argument("--iters", type=int, default=600)
     ap.add_argument("--sample", type=int, default=16384)
-    ap.add_argument("--thetas", default="0.3,0.4,0.5,0.6,0.7,0.8")
+    ap.add_argument("--thetas", default="0.15,0.2,0.25,0.3,0.4,0.5,0.6,0.8")
+    ap.add_argument("--nn-thetas", default="0.3,0.4,0.5,0.6,0.8")
+    ap.add_argument("--target-rms", type=float, default=1e-4)
+    ap.add_argument("--init", choices=("quadrupole", "zero"), default="quadrupole",
+                    help="starting network: the quadrupole term (learned.quadrupole_params) or a "
+                         "zero output layer")
     ap.add_argument("--params-out", default="")
     a = ap.parse_args()
     import torch
@@ -91,8 +109,11 @@ def main():
 
     res = pa.HipResources(0)
     learned.init(res, 1808)
+    if a.init == "quadrupole":
+        learned.set_params(res, learned.quadrupole_params())
     learned.clear(res)
     t0 = time.time()
+    train_thetas = [float(x) for x in a.train_thetas.split(",")]
     seeds = max(1, a.snapshots // a.rotations)
     k = 0
     for sd in range(seeds):
@@ -101,60 +122,65 @@ def main():
                 break
             p = snapshot(a.n, 7000 + sd, rot)
             dev = torch.from_numpy(p).cuda()
-            learned.collect(res, dev, a.theta, a.waves, seed=k)
+            learned.collect(res, dev, train_thetas[k % len(train_thetas)], a.waves, seed=k,
+                            row_cap=65536)
             k += 1
             del dev
-        log(f"collected {k} snapshots, rows/waves/dropped = {learned.info(res)}, {time.time() - t0:.0f} s")
+        log(f"collected {k} snapshots, rows/waves/dropped = {learned.info(res)}, "
+            f"{time.time() - t0:.0f} s")
     rows, waves, dropped = learned.info(res)
-    res.profile(True)
-    hist = learned.train(res, a.iters, a.lr)
-    train_ms, _ = res.profile_read(_lib.PROF_MLP)
-    res.profile(False)
-    log(f"trained {a.iters} iterations: loss {hist[0]:.4e} -> {hist[-1]:.4e} ({train_ms:.0f} ms)")
-    if a.params_out:
-        np.save(a.params_out, learned.get_params(res))
 
-    # held-out evaluation
+    # held-out snapshot and its FP64 reference
     p = snapshot(a.n, 9999, 0)
     dev = torch.from_numpy(p).cuda()
     idx = np.random.default_rng(5).choice(a.n, a.sample, replace=False)
     ref = fp64_reference(res, dev, torch.from_numpy(idx).cuda(), a.sample)
-    plain = []
-    for th in [float(x) for x in a.thetas.split(",")]:
-        out, ms = timed_bh(res, dev, th, False)
+
+    def run(theta, mlp):
+        out, (ms, (build, walk, corr)) = timed_bh(res, dev, theta, mlp)
         err = metrics.summary(out.cpu().numpy()[idx], ref)
-        plain.append({"theta": th, "ms": ms, "mparticles_per_s": a.n / ms / 1e3, **err})
-        log(f"plain BH theta={th}: {ms:.2f} ms, rms {err['rms_rel']:.3e}")
-    out, ms_nn = timed_bh(res, dev, a.theta, True)
-    err_nn = metrics.summary(out.cpu().numpy()[idx], ref)
-    log(f"NN BH theta={a.theta}: {ms_nn:.2f} ms, rms {err_nn['rms_rel']:.3e}")
-    # plain theta with the NN error (interpolate log(rms) and log(ms) linearly in theta)
-    th = np.array([q["theta"] for q in plain])
-    lr = np.log([q["rms_rel"] for q in plain])
-    lt = np.log([q["ms"] for q in plain])
-    o = np.argsort(lr)
-    target = np.log(err_nn["rms_rel"])
-    if lr.min() <= target <= lr.max():
-        th_eq = float(np.interp(target, lr[o], th[o]))
-        ms_eq = float(np.exp(np.interp(th_eq, th, lt)))
-    else:
-        th_eq, ms_eq = None, None
-    at04 = next((q for q in plain if abs(q["theta"] - 0.4) < 1e-9), None)
+        row = {"theta": theta, "ms": ms, "build_ms": build, "walk_ms": walk,
+               "correction_ms": corr, "mparticles_per_s": a.n / ms / 1e3, **err}
+        log(f"{'NN' if mlp else 'plain'} BH theta={theta}: {ms:.2f} ms (build {build:.2f}, walk "
+            f"{walk:.2f}, correction {corr:.2f}), rms {err['rms_rel']:.3e}")
+        return row
+
+    nn_thetas = [float(x) for x in a.nn_thetas.split(",")]
+    start = [run(th, True) for th in nn_thetas] if a.init == "quadrupole" else []
+    res.profile(True)
+    fit = learned.fit(res, a.iters)
+    train_ms, _ = res.profile_read(_lib.PROF_MLP)
+    res.profile(False)
+    log(f"trained {a.iters} iterations at lr {fit['lr']}: loss {fit['loss_start']:.4e} -> "
+        f"{fit['loss_end']:.4e} ({train_ms:.0f} ms), kept {fit['kept']}")
+    if a.params_out:
+        np.save(a.params_out, learned.get_params(res))
+
+    plain = [run(float(x), False) for x in a.thetas.split(",")]
+    nn = [run(th, True) for th in nn_thetas]
+    for q in nn + start:
+        t_plain = time_at_error(plain, q["rms_rel"])
+        q["plain_ms_at_same_error"] = t_plain
+        q["speedup_at_same_error"] = t_plain / q["ms"] if t_plain else None
+        same = next((r for r in plain if abs(r["theta"] - q["theta"]) < 1e-9), None)
+        q["error_vs_plain_same_theta"] = q["rms_rel"] / same["rms_rel"] if same else None
+    t_plain = time_at_error(plain, a.target_rms)
+    t_nn = time_at_error(nn, a.target_rms)
     result = {
-        "workload": f"two Plummer spheres, {a.n} particles, theta={a.theta}, held-out seed",
-        "training": {"snapshots": k, "waves_per_snapshot": a.waves, "rows": rows,
-                     "lane_targets": waves * 64, "dropped_rows": dropped, "iterations": a.iters,
-                     "loss_first": float(hist[0]), "loss_last": float(hist[-1]),
+        "workload": f"two Plummer spheres, {a.n} particles, held-out seed; Acceleration::checked",
+        "training": {"snapshots": k, "waves_per_snapshot": a.waves, "thetas": train_thetas,
+                     "rows": rows, "lane_targets": waves * 64, "dropped_rows": dropped,
+                     "iterations": a.iters, "lr": fit["lr"], "loss_start": float(fit["loss_start"]),
+                     "loss_end": float(fit["loss_end"]), "trained_network_kept": fit["kept"],
                      "train_ms": train_ms},
+        "init": a.init,
         "plain": plain,
-        "nn": {"theta": a.theta, "ms": ms_nn, "mparticles_per_s": a.n / ms_nn / 1e3, **err_nn},
-        "plain_theta_at_nn_error": th_eq,
-        "plain_ms_at_nn_error": ms_eq,
-        "speedup_vs_plain_at_equal_error": (ms_eq / ms_nn) if ms_eq else None,
-        "nn_error_vs_plain_same_theta": err_nn["rms_rel"] / next(
-            q["rms_rel"] for q in plain if abs(q["theta"] - a.theta) < 1e-9)
-        if any(abs(q["theta"] - a.theta) < 1e-9 for q in plain) else None,
-        "plain_theta_0.4": at04,
+        "nn_start": start,
+        "nn": nn,
+        "target_rms": a.target_rms,
+        "plain_ms_at_target": t_plain,
+        "nn_ms_at_target": t_nn,
+        "speedup_at_target": (t_plain / t_nn) if (t_plain and t_nn) else None,
     }
     print(json.dumps(result, indent=1))
 

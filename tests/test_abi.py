@@ -105,3 +105,19 @@ def test_interaction_constructors():
     s = pa.AccelerationSoftened.checked(1e-3)
     assert s.softening == pytest.approx(1e-3) and s.is_checked
     assert not pa.AccelerationSoftened.unchecked(0.5).is_checked
+
+
+def test_mlp_parameter_layout_matches_library():
+    from particular_amd import learned
+
+    L = _lib.lib()
+    assert L.pnb_mlp_num_params() == learned.IN * 64 + 64 + 64 * 64 + 64 + 64 * 3 + 3
+    flat = learned.quadrupole_params()
+    back = learned.dict_to_params(learned.params_to_dict(flat))
+    np.testing.assert_array_equal(back, flat)
+
+
+def test_tree_node_record_size():
+    # pnb_tree_node: com[3], mass, mass64 (8-aligned), width, skip, begin, count, level, internal,
+    # quad[6] -> 72 bytes
+    assert _lib.tree_node_dtype().itemsize == 72

@@ -1,13 +1,17 @@
 """Learned far-field correction on the GPU: MFMA forward/backward against the FP64 oracle with the
-same bf16 rounding points, training, training-set generation, and the corrected walk."""
+same bf16 rounding points, training, training-set generation (rows against a CPU emulation of the
+walk and the feature oracle, node quadrupoles against their particles), and the corrected walk."""
 import numpy as np
 import pytest
 
 import particular_amd as pa
+from oracle import features as F
 from oracle import mlp_oracle as M
-from particular_amd import learned, metrics, workloads
+from particular_amd import learned, workloads
 
 pytestmark = pytest.mark.gpu
+
+NF = M.IN  # features the network reads (of learned.FEAT stored per row)
 
 
 def oracle_params(seed):
@@ -27,8 +31,8 @@ def set_oracle_params(res, p):
 def random_dataset(seed, waves=40, rows_per_wave=50):
     rng = np.random.default_rng(seed)
     rows = waves * rows_per_wave
-    feats = np.zeros((rows, 8), np.float32)
-    feats[:, :7] = rng.normal(size=(rows, 7))
+    feats = np.zeros((rows, learned.FEAT), np.float32)
+    feats[:, :NF] = rng.normal(size=(rows, NF))
     return {
         "feats": feats,
         "scale": rng.uniform(0.5, 2.0, rows).astype(np.float32),
@@ -44,14 +48,14 @@ def test_forward_matches_oracle(res):
     import torch
 
     p = set_oracle_params(res, oracle_params(1))
-    x = np.random.default_rng(2).normal(size=(1000, 8)).astype(np.float32)
-    x[:, 7] = 0
+    x = np.zeros((1000, learned.FEAT), np.float32)
+    x[:, :NF] = np.random.default_rng(2).normal(size=(1000, NF))
     xd = torch.from_numpy(x).cuda()
     yd = torch.zeros((1000, 4), dtype=torch.float32, device="cuda")
     learned.forward(res, xd, yd)
     torch.cuda.synchronize()
     y = yd.cpu().numpy()[:, :3]
-    ref, _ = M.forward(p, x[:, :7].astype(np.float64), bf16=True)
+    ref, _ = M.forward(p, x[:, :NF].astype(np.float64), bf16=True)
     err = np.abs(y - ref)
     scale = np.abs(ref).max()
     print("forward max abs err", err.max(), "scale", scale, "median", np.median(err))
@@ -64,7 +68,7 @@ def test_gradients_match_oracle(res):
     learned.clear(res)
     learned.load(res, d)
     loss, g = learned.grad(res)
-    ref_loss, ref_g, _ = M.loss_and_grads(p, d["feats"][:, :7].astype(np.float64),
+    ref_loss, ref_g, _ = M.loss_and_grads(p, d["feats"][:, :NF].astype(np.float64),
                                           d["row_wave"].astype(np.int64), d["mask"],
                                           d["scale"].astype(np.float64),
                                           d["target"][..., :3].astype(np.float64),
@@ -81,7 +85,7 @@ def test_gradients_match_oracle(res):
 def test_training_fits_a_teacher(res):
     teacher = oracle_params(5)
     d = random_dataset(6, waves=64, rows_per_wave=40)
-    y, _ = M.forward(teacher, d["feats"][:, :7].astype(np.float64))
+    y, _ = M.forward(teacher, d["feats"][:, :NF].astype(np.float64))
     pred = M.predict(y, d["row_wave"].astype(np.int64), d["mask"], d["scale"].astype(np.float64), 64)
     d["target"][..., :3] = pred
     d["wgt"][:] = 1.0 / (1.0 + (pred ** 2).sum(-1))
@@ -93,7 +97,7 @@ def test_training_fits_a_teacher(res):
     assert hist[-1] < 0.05 * hist[0]
 
 
-def test_collect_rows_and_targets(res, orc):
+def test_collect_rows_and_targets(res):
     import torch
 
     s = workloads.direct_sum_case(16384, seed=41)
@@ -112,11 +116,108 @@ def test_collect_rows_and_targets(res, orc):
     # d_c >= width / theta - 8 * wrad, i.e. width/d_c <= theta * (1 + 8 * wrad/d_c).
     assert np.all(f[:, 3] > 0)
     assert np.all(f[:, 3] <= 0.8 * (1 + 8 * f[:, 5]) * (1 + 1e-5))
+    # Every body of a node lies within w of its centre of mass along each axis, so the entries
+    # of Q = (3T - tr(T) I) / (M w^2) are at most 2 (diagonal) and 3 (off-diagonal): its norm is
+    # below sqrt(66) < 9, and so are the quadrupole features
+    assert np.all(np.abs(f[:, 7:13]) <= 9.0)
+    assert not f[:, NF:].any()
     # targets are brute force minus Barnes-Hut: |target| relative to |a| is a monopole-sized error
     a_ref = 1.0 / np.sqrt(d["wgt"][d["wgt"] > 0])
     t = np.linalg.norm(d["target"][..., :3], axis=-1)[d["wgt"] > 0]
     rel = t / a_ref
     assert 1e-4 < np.sqrt(np.mean(rel**2)) < 0.2
+
+
+def cpu_rows(nodes, pts, theta):
+    """Rows of one wavefront by the reference's acceptance rule in f32 (sequential.rs:485-494):
+    the internal nodes some lane accepts, in pre-order, with the accepting-lane mask, kept when
+    the wavefront is compact relative to the node (rms radius <= ROW_SPREAD d). Also returns
+    whether a node sits so close to that threshold that f32 and f64 could disagree."""
+    f32 = np.float32
+    th2 = f32(theta) * f32(theta)
+    com = nodes["com"].astype(np.float32)
+    w2 = (nodes["width"] * nodes["width"]).astype(np.float32)
+    internal = nodes["internal"] == 1
+    masks = {}
+    for lane, p in enumerate(pts.astype(np.float32)):
+        k = 0
+        while k < len(nodes):
+            d = com[k] - p[:3]
+            d2 = f32(f32(d[0] * d[0]) + f32(d[1] * d[1])) + f32(d[2] * d[2])
+            if d2 == 0:
+                k = nodes["skip"][k]
+            elif internal[k] and f32(th2 * d2) < w2[k]:
+                k += 1
+            else:
+                if internal[k]:
+                    masks[k] = masks.get(k, 0) | (1 << lane)
+                k = nodes["skip"][k]
+    c, wrad = F.wave_frame(pts[:, :3])
+    rows, borderline = [], False
+    for k in sorted(masks):
+        r = nodes["com"][k].astype(np.float64) - c
+        ratio = wrad * wrad / float(r @ r) / F.ROW_SPREAD**2
+        borderline |= abs(ratio - 1) < 1e-4
+        if ratio <= 1:
+            rows.append((k, masks[k]))
+    return rows, borderline
+
+
+def test_collected_rows_match_cpu_walk(res):
+    """Row lists and masks exactly as the reference rule implies; features as the oracle has them."""
+    import torch
+
+    s = workloads.two_plummer_case(2048, seed=45)
+    dev = torch.from_numpy(s.particles).cuda()
+    learned.init(res, 1)
+    learned.clear(res)
+    theta = 0.6
+    learned.collect(res, dev, theta, waves=s.n // 64, seed=0)
+    d = learned.dataset(res)
+    nodes, order = pa.tree_export(res, s.n)
+    sp = s.particles[order]
+    total = float(nodes["mass"][0])
+    checked = 0
+    for w in (0, 5, 17, 31, 9, 23):
+        b, e = int(d["wave_rows"][w]), int(d["wave_rows"][w + 1])
+        pts = sp[64 * w:64 * w + 64]
+        want, borderline = cpu_rows(nodes, pts, theta)
+        if borderline:
+            continue
+        checked += 1
+        assert e - b > 0
+        assert [int(m) for m in d["mask"][b:e]] == [m for _, m in want], w
+        c, wrad = F.wave_frame(pts[:, :3])
+        for r, (k, m) in zip(range(b, e), want):
+            n = nodes[k]
+            f, sc = F.row_features(n["com"], float(n["mass"]), float(n["width"]), n["quad"], c,
+                                   wrad, total, bin(m).count("1"))
+            np.testing.assert_allclose(d["feats"][r, :NF], f, rtol=2e-5, atol=2e-6)
+            assert abs(d["scale"][r] / sc - 1) < 2e-5
+    assert checked >= 4
+
+
+def test_node_quadrupoles_match_particles(res):
+    import torch
+
+    s = workloads.two_plummer_case(8192, seed=46)
+    dev = torch.from_numpy(s.particles).cuda()
+    learned.init(res, 1)
+    pa.BarnesHut(res, 0.5, pa.Acceleration.checked(), mlp=True).compute(pa.Between(dev, dev))
+    nodes, order = pa.tree_export(res, s.n)
+    sp = s.particles[order].astype(np.float64)
+    internal = np.flatnonzero(nodes["internal"] == 1)
+    rng = np.random.default_rng(0)
+    worst = 0.0
+    for k in np.concatenate([internal[:50], rng.choice(internal, 200, replace=False)]):
+        n = nodes[k]
+        part = sp[n["begin"]:n["begin"] + n["count"]]
+        want = F.quadrupole(part[:, :3], part[:, 3], float(n["width"]))
+        err = np.abs(n["quad"] - want).max()
+        worst = max(worst, err)
+        assert err < 1e-5 + 1e-5 * np.abs(want).max(), (k, n["quad"], want)
+    assert not nodes["quad"][nodes["internal"] == 0].any()
+    print("worst quadrupole error", worst)
 
 
 def test_walk_correction_matches_rows(res):
@@ -133,7 +234,7 @@ def test_walk_correction_matches_rows(res):
     plain = pa.BarnesHut(res, 0.7, pa.Acceleration.checked()).compute(s.particles)
     nn = pa.BarnesHut(res, 0.7, pa.Acceleration.checked(), mlp=True).compute(s.particles)
     _, order = pa.tree_export(res, s.n)
-    y, _ = M.forward(p, d["feats"][:, :7].astype(np.float64), bf16=True)
+    y, _ = M.forward(p, d["feats"][:, :NF].astype(np.float64), bf16=True)
     pred = M.predict(y, d["row_wave"].astype(np.int64), d["mask"], d["scale"].astype(np.float64),
                      s.n // 64).reshape(-1, 3)
     corr = (nn - plain)[order]
@@ -141,10 +242,32 @@ def test_walk_correction_matches_rows(res):
     err = np.abs(corr - pred)
     print("correction scale", scale, "max err", err.max())
     assert err.max() < 3e-2 * scale and np.median(err) < 1e-3 * scale
-    # a zero last layer leaves the walk bitwise unchanged
-    p["W3"][:] = 0
-    p["b3"][:] = 0
-    set_oracle_params(res, p)
+
+
+def test_quadrupole_network_reduces_error(res, orc):
+    """The network set to the quadrupole term (learned.quadrupole_params) is the quadrupole
+    correction of every accepted node: it must cut the Barnes-Hut error against FP64."""
+    s = workloads.two_plummer_case(32768, seed=47)
+    idx = np.random.default_rng(1).choice(s.n, 512, replace=False)
+    ref = orc.direct_f64(s.particles[idx], s.particles)
+    learned.set_params(res, learned.quadrupole_params())
+    for theta in (0.5, 0.8):
+        plain = pa.BarnesHut(res, theta, pa.Acceleration.checked()).compute(s.particles)
+        quad = pa.BarnesHut(res, theta, pa.Acceleration.checked(), mlp=True).compute(s.particles)
+
+        def rms(a):
+            e = np.linalg.norm(a[idx, :3] - ref, axis=1) / np.linalg.norm(ref, axis=1)
+            return float(np.sqrt(np.mean(e**2)))
+
+        print(f"theta {theta}: plain rms {rms(plain):.3e}, quadrupole network rms {rms(quad):.3e}")
+        assert rms(quad) < 0.6 * rms(plain)
+
+
+def test_untrained_network_leaves_walk_unchanged(res):
+    """A fresh network has a zero output layer: the corrected walk is plain Barnes-Hut, bit for bit."""
+    s = workloads.direct_sum_case(8192, seed=44)
+    learned.init(res, 3)
+    plain = pa.BarnesHut(res, 0.7, pa.Acceleration.checked()).compute(s.particles)
     nn0 = pa.BarnesHut(res, 0.7, pa.Acceleration.checked(), mlp=True).compute(s.particles)
     np.testing.assert_array_equal(nn0, plain)
 
