@@ -714,14 +714,17 @@ __device__ __forceinline__ float row_features(const NodeRec& r, float width, con
     const float rx = r.cx - wx, ry = r.cy - wy, rz = r.cz - wz;
     const float d2 = centre_dist2(r, wx, wy, wz);
     if (!row_kept(d2, wrad)) return 0.f;
-    const float inv = 1.0f / sqrtf(d2);
+    // hardware reciprocal square root, reciprocal and log2 (about 1 ulp): the features need no
+    // more, and every caller gets the same values
+    const float inv = __builtin_amdgcn_rsqf(d2);
     const float ux = rx * inv, uy = ry * inv, uz = rz * inv;
     const float s = width * inv;
     f[0] = ux;
     f[1] = uy;
     f[2] = uz;
     f[3] = s;
-    f[4] = 1.f + log2f(fmaxf(r.mass / total_mass, 1e-30f)) * (1.f / 32.f);
+    f[4] = 1.f + __builtin_amdgcn_logf(fmaxf(r.mass * __builtin_amdgcn_rcpf(total_mass), 1e-30f)) *
+                     (1.f / 32.f);
     f[5] = wrad * inv;
     f[6] = float(lanes) * (1.f / 64.f);
     const float qx = q.a.x * ux + q.a.w * uy + q.b.x * uz;
@@ -734,7 +737,7 @@ __device__ __forceinline__ float row_features(const NodeRec& r, float width, con
     f[10] = quu * ux;
     f[11] = quu * uy;
     f[12] = quu * uz;
-    return (r.mass / d2) * (s * s);
+    return (r.mass * (inv * inv)) * (s * s);
 }
 
 struct ApplyTile {  // per wavefront: the scaled outputs and lane masks of one 16-row tile
@@ -826,12 +829,11 @@ __global__ __launch_bounds__(256) void mlp_apply_kernel(const NodeRec* __restric
             }
             wave_lds_sync();
             const uint32_t nt = min(16u, cnt - t0);
-            for (uint32_t q = 0; q < nt; ++q) {
-                if ((tile.mask[q] >> lane) & 1ull) {
-                    kx += tile.y[q][0];
-                    ky += tile.y[q][1];
-                    kz += tile.y[q][2];
-                }
+            for (uint32_t q = 0; q < nt; ++q) {  // branch-free: a lane outside the mask adds 0
+                const float sel = ((tile.mask[q] >> lane) & 1ull) ? 1.f : 0.f;
+                kx = fmaf(sel, tile.y[q][0], kx);
+                ky = fmaf(sel, tile.y[q][1], ky);
+                kz = fmaf(sel, tile.y[q][2], kz);
             }
             wave_lds_sync();
         }

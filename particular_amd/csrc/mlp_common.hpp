@@ -49,10 +49,9 @@ struct MlpWeights : MlpForward {
     uint16_t w3[MLP_H][MLP_NOUT];   // [n][k] = W3[n][k]   (B operand of g W3^T)
 };
 
+// FP32 -> BF16 bits, round to nearest even (gfx950 converts in hardware: v_cvt_pk_bf16_f32)
 __device__ __forceinline__ uint16_t f2bf(float f) {
-    uint32_t u = __float_as_uint(f);
-    u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even (finite inputs)
-    return uint16_t(u >> 16);
+    return __builtin_bit_cast(uint16_t, static_cast<__bf16>(f));
 }
 
 __device__ __forceinline__ bf16x4_t frag(const uint16_t* p) {
