@@ -152,7 +152,8 @@ typedef struct {
     uint32_t internal;/* 1 for Node::Internal, 0 for Node::External */
     /* Traceless quadrupole about the centre of mass, normalised by mass and width:
      * (3 T - tr(T) I) / (mass w^2) with T = sum mu (x - com)(x - com)^T, as {xx, yy, zz, xy, xz,
-     * yz}. Filled when the tree was built for the learned correction, zero otherwise. */
+     * yz}. Filled when the tree was built for the learned correction, zero otherwise (and for
+     * leaves). */
     float quad[6];
 } pnb_tree_node;
 
@@ -219,46 +220,47 @@ int pnb_batch_info(pnb_ctx *ctx, size_t *particles, size_t *items,
                    unsigned long long *interactions_per_step);
 
 /* ---- learned far-field correction (Barnes-Hut + MLP) -----------------------------------------
- * No reference counterpart. A 13 -> 64 -> 64 -> 3 ReLU MLP (BF16 weights on MFMA, FP32
- * accumulate) corrects the monopole term of every internal node a wavefront accepts in the
- * Barnes-Hut walk (pnb_bh_accel_f32* with PNB_BH_MLP). One row = one (wavefront, accepted internal
- * node). Its features are seen from the wavefront centroid c, at distance d from the node's centre
- * of mass along the unit vector u: u (3), width / d, 1 + log2(mass fraction) / 32, wavefront rms
- * radius / d, accepting-lane fraction, and the node's normalised quadrupole Q (see pnb_tree_node)
- * projected on u: Q u - (u.Q.u) u (3) and (u.Q.u) u (3). Its output y adds (mass w^2 / d^4) * y
- * to every accepting lane (the size of the monopole term's leading error). The walk lists each wavefront's rows; a second kernel
- * evaluates them (16 rows per MFMA tile) and adds the corrections. Training data are generated by
- * the method itself: per-lane targets are brute force minus Barnes-Hut. */
+ * No reference counterpart. An 11 -> 64 -> 64 -> 6 ReLU MLP (BF16 weights on MFMA, FP32
+ * accumulate) corrects the monopole term of every internal node a lane accepts in the Barnes-Hut
+ * walk (pnb_bh_accel_f32* with PNB_BH_MLP). Its input describes the node: the normalised
+ * quadrupole Q = (3T - tr(T) I) / (M w^2) (6, see pnb_tree_node), 1 + log2(mass / root mass) / 32,
+ * log2(bodies) / 32, depth / 64, the heaviest child's mass share and children / 8. Its output is an
+ * effective second moment in units of M w^2; every lane that accepts the node adds that moment's
+ * field at its own position, a = -Q d |d|^-5 + 2.5 (d.Q.d) d |d|^-7 (d = centre of mass - lane).
+ * The network runs once per node after the octree build. Training data are generated by the
+ * method itself: per-lane targets are brute force minus Barnes-Hut. */
 
 int pnb_mlp_num_params(void);
 /* He-normal hidden layers from a seed (SplitMix64 + Box-Muller), zero biases and a zero output
  * layer (a fresh network corrects nothing); resets the Adam state. */
 int pnb_mlp_init(pnb_ctx *ctx, uint64_t seed);
-/* Parameters as one FP32 vector: W1[13][64], b1[64], W2[64][64], b2[64], W3[64][3], b3[3]. */
+/* Parameters as one FP32 vector: W1[11][64], b1[64], W2[64][64], b2[64], W3[64][6], b3[6]. */
 int pnb_mlp_set_params(pnb_ctx *ctx, const float *params, size_t n);
 int pnb_mlp_get_params(pnb_ctx *ctx, float *params, size_t n);
 /* Builds the octree of `particles` (device, n x 4, all affecting), walks `waves` seeded
- * wavefronts of 64 consecutive particles in tree order at `theta` recording their rows (at most
- * row_cap per wavefront; the rest are counted as dropped), computes their brute-force
- * accelerations, and appends rows and targets to the training set. */
+ * wavefronts of 64 consecutive particles in tree order at `theta` recording their rows (one row =
+ * one internal node some lane of the wavefront accepted, with the accepting-lane mask; at most
+ * row_cap per wavefront, the rest counted as dropped), computes their brute-force accelerations,
+ * and appends rows and targets to the training set. */
 int pnb_mlp_collect(pnb_ctx *ctx, const float *particles, size_t n, float theta, float softening,
                     size_t waves, uint64_t seed, size_t row_cap);
 int pnb_mlp_clear_dataset(pnb_ctx *ctx);
 int pnb_mlp_dataset_info(pnb_ctx *ctx, size_t *rows, size_t *waves, uint64_t *dropped);
-/* Training set on the host: feats rows x 16 (13 used), scale rows, mask rows (64-bit lane
- * masks), row_wave rows, wave_rows waves + 1 (row offsets), target waves x 64 x 4,
- * wgt waves x 64. */
-int pnb_mlp_dataset_read(pnb_ctx *ctx, float *feats, float *scale, uint64_t *mask,
-                         uint32_t *row_wave, uint32_t *wave_rows, float *target, float *wgt);
-int pnb_mlp_dataset_load(pnb_ctx *ctx, const float *feats, const float *scale,
+/* Training set on the host: feats rows x 16 (11 used), geom rows x 4 (node centre of mass,
+ * M w^2), mask rows (64-bit lane masks), row_wave rows, wave_rows waves + 1 (row offsets), lanes
+ * waves x 64 x 4 (lane positions, w = softening^2), target waves x 64 x 4, wgt waves x 64. */
+int pnb_mlp_dataset_read(pnb_ctx *ctx, float *feats, float *geom, uint64_t *mask,
+                         uint32_t *row_wave, uint32_t *wave_rows, float *lanes, float *target,
+                         float *wgt);
+int pnb_mlp_dataset_load(pnb_ctx *ctx, const float *feats, const float *geom,
                          const uint64_t *mask, const uint32_t *row_wave,
                          const uint32_t *wave_rows, size_t rows, size_t waves,
-                         const float *target, const float *wgt);
+                         const float *lanes, const float *target, const float *wgt);
 /* Loss and gradients over the whole training set, without an update. */
 int pnb_mlp_grad(pnb_ctx *ctx, float *loss, float *grads, size_t n);
 /* Full-batch Adam steps; loss_history (optional, `iterations` floats) gets the loss before each. */
 int pnb_mlp_train(pnb_ctx *ctx, size_t iterations, float lr, float *loss_history);
-/* Inference on device rows: y (rows x 4) = MLP(feats (rows x 16, 13 used)). */
+/* Inference on device rows: y (rows x 8, 6 used) = MLP(feats (rows x 16, 11 used)). */
 int pnb_mlp_forward_dev(pnb_ctx *ctx, const float *feats, size_t rows, float *y);
 
 #ifdef __cplusplus

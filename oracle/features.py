@@ -1,25 +1,17 @@
-"""Row features of the learned Barnes-Hut correction in FP64 -- TEST INFRASTRUCTURE ONLY.
+"""Node features and the quadrupole field of the learned Barnes-Hut correction in FP64 -- TEST
+INFRASTRUCTURE ONLY.
 
 No reference counterpart (particular has no learned component). Restates, in numpy, what the
-library computes for one (wavefront, accepted internal node) row (particular_amd/csrc/bh.hip,
-row_features and wave_frame), so the GPU's training rows can be checked against the exported
-octree; and the node moments from the node's particles, so the exported quadrupoles can be
-checked. The multipole claim behind features 7..12 (the quadrupole term of a node seen from c is
-(M w^2 / d^4) (-f7..9 + 1.5 f10..12)) is pinned against exact sums in tests/test_features_oracle.py.
+library computes per internal octree node (particular_amd/csrc/bh.hip: node_features, the
+moments of bh_up_kernel) and the field every accepting lane adds (csrc/mlp_common.hpp:
+quad_field), so that the GPU's training rows and node moments can be checked against an exported
+octree. The field formula is pinned against exact sums over a cluster in
+tests/test_features_oracle.py: with the node's own quadrupole it is the monopole's residual up to
+the next (octupole) order.
 """
 from __future__ import annotations
 
 import numpy as np
-
-
-ROW_SPREAD = 0.15  # rows with wavefront rms radius / d above this are not corrected
-
-
-def wave_frame(pos: np.ndarray):
-    """Centroid and rms radius of a wavefront's particles (k x 3)."""
-    pos = np.asarray(pos, np.float64)
-    c = pos.mean(0)
-    return c, float(np.sqrt(((pos - c) ** 2).sum(1).mean()))
 
 
 def quadrupole(pos: np.ndarray, mass: np.ndarray, width: float):
@@ -40,31 +32,36 @@ def quad_matrix(q6) -> np.ndarray:
     return np.array([[xx, xy, xz], [xy, yy, yz], [xz, yz, zz]], np.float64)
 
 
-def row_features(com, mass, width, q6, centre, wrad, total_mass, lanes):
-    """Features f[0..12] and correction scale M w^2 / d^4 of one row (zeros and 0 when d == 0 or
-    the wavefront is too spread out for the node: rms radius > ROW_SPREAD d)."""
-    r = np.asarray(com, np.float64) - np.asarray(centre, np.float64)
-    d2 = float(r @ r)
-    f = np.zeros(13)
-    if not d2 > 0 or wrad * wrad > ROW_SPREAD * ROW_SPREAD * d2:
-        return f, 0.0
-    d = np.sqrt(d2)
-    u = r / d
-    s = width / d
-    Q = quad_matrix(q6)
-    qu = Q @ u
-    quu = float(u @ qu)
-    f[0:3] = u
-    f[3] = s
-    f[4] = 1.0 + np.log2(max(mass / total_mass, 1e-30)) / 32.0
-    f[5] = wrad / d
-    f[6] = lanes / 64.0
-    f[7:10] = qu - quu * u
-    f[10:13] = quu * u
-    return f, mass / d2 * s * s
+def quad_field(q6_abs, d, eps2: float = 0.0) -> np.ndarray:
+    """Field at a point of a second moment Q (absolute units), d = centre of mass - point:
+    -Q d |d|^-5 + 2.5 (d.Q.d) d |d|^-7 with |d| softened as sqrt(|d|^2 + eps2)."""
+    Q = quad_matrix(q6_abs)
+    d = np.asarray(d, np.float64)
+    r = np.sqrt(d @ d + eps2)
+    qd = Q @ d
+    return -qd / r**5 + 2.5 * (d @ qd) * d / r**7
 
 
-def quadrupole_term(f, scale):
-    """The quadrupole correction implied by a row's features and scale M w^2 / d^4:
-    scale (-f7..9 + 1.5 f10..12)."""
-    return scale * (-np.asarray(f[7:10]) + 1.5 * np.asarray(f[10:13]))
+def node_children(nodes, k: int):
+    """Indices of the children of node k in a pre-order node array with skip pointers."""
+    out, c = [], k + 1
+    while c < nodes["skip"][k]:
+        out.append(c)
+        c = max(int(nodes["skip"][c]), c + 1)
+    return out
+
+
+def node_features(nodes, k: int) -> np.ndarray:
+    """Features f[0..10] of node k of an exported tree (zeros for a leaf or a massless node)."""
+    n = nodes[k]
+    f = np.zeros(11)
+    if not n["internal"] or not n["mass"] > 0:
+        return f
+    f[0:6] = n["quad"]
+    f[6] = 1.0 + np.log2(max(float(n["mass"]) / float(nodes["mass"][0]), 1e-30)) / 32.0
+    f[7] = np.log2(float(n["count"])) / 32.0
+    f[8] = float(n["level"]) / 64.0
+    ch = node_children(nodes, k)
+    f[9] = max(float(nodes["mass64"][c]) for c in ch) / float(n["mass64"])
+    f[10] = len(ch) / 8.0
+    return f

@@ -4,20 +4,23 @@ No reference counterpart (particular has no learned component). The network and 
 defined here in plain numpy and pinned by finite differences (tests/test_mlp_oracle.py); the GPU
 forward/backward kernels are checked against it.
 
-Network: 13 -> 64 -> 64 -> 3, ReLU, y = relu(relu(x W1 + b1) W2 + b2) W3 + b3. When
+Network: 11 -> 64 -> 64 -> 6, ReLU, y = relu(relu(x W1 + b1) W2 + b2) W3 + b3. When
 ``bf16=True`` the inputs of every matrix product (x, h1, h2, g_y, g_z2, g_z1 and W1..W3) are
 rounded to bfloat16 (round to nearest even) as the MFMA kernels round them; sums are FP64.
 
-Rows are grouped by wavefront: row r belongs to wave ``wave[r]``, applies to the lanes set in
-``mask[r]`` (64-bit) with scale ``scale[r]``; lane l of wave w predicts
-``pred[w, l] = sum_{r in w, l in mask[r]} scale[r] * y[r]`` for target ``t[w, l]`` with weight
+A row is an internal node accepted by some lanes of a wavefront: row r belongs to wave
+``wave[r]``, applies to the lanes set in ``mask[r]`` (64-bit), and has geometry ``geom[r]`` =
+(centre of mass, M w^2). Its output y[r] is a second moment Q = M w^2 y[r] ({xx, yy, zz, xy, xz,
+yz}); lane l of wave w at ``lanes[w, l, :3]`` (softening^2 in ``lanes[w, l, 3]``) predicts
+``pred[w, l] = sum_{r in w, l in mask[r]} field(Q_r, d)`` with d = centre - lane and
+field(Q, d) = -Q d |d|^-5 + 2.5 (d.Q.d) d |d|^-7, for target ``t[w, l]`` with weight
 ``wgt[w, l]``; loss = sum wgt * |pred - t|^2 / (number of lanes with wgt > 0).
 """
 from __future__ import annotations
 
 import numpy as np
 
-IN, HID, OUT = 13, 64, 3
+IN, HID, OUT = 11, 64, 6
 
 
 def bf16_round(a) -> np.ndarray:
@@ -58,24 +61,49 @@ def lane_bits(mask: np.ndarray) -> np.ndarray:
     return ((m[:, None] >> np.arange(64, dtype=np.uint64)[None, :]) & np.uint64(1)).astype(bool)
 
 
-def predict(y, wave, mask, scale, num_waves):
+def field_basis(d: np.ndarray, inv: np.ndarray) -> np.ndarray:
+    """B (..., 3, 6) with field(Q, d) = B @ q for q = {xx, yy, zz, xy, xz, yz}; inv = 1 / |d|
+    (softened)."""
+    dx, dy, dz = d[..., 0], d[..., 1], d[..., 2]
+    inv2 = inv * inv
+    inv5 = inv2 * inv2 * inv
+    z = np.zeros_like(dx)
+    c = np.stack([dx * dx, dy * dy, dz * dz, 2 * dx * dy, 2 * dx * dz, 2 * dy * dz], -1)
+    m = np.stack([np.stack([dx, z, z, dy, dz, z], -1),
+                  np.stack([z, dy, z, dx, z, dz], -1),
+                  np.stack([z, z, dz, z, dx, dy], -1)], -2)
+    return inv5[..., None, None] * (2.5 * inv2[..., None, None] * d[..., :, None] * c[..., None, :] - m)
+
+
+def _row_lane_basis(wave, mask, geom, lanes):
+    """Per (row, lane): the basis scaled by M w^2 (rows x 64 x 3 x 6) and the lane bits."""
     bits = lane_bits(mask)
-    pred = np.zeros((num_waves, 64, OUT))
-    contrib = (scale[:, None] * y)[:, None, :] * bits[:, :, None]
+    d = np.asarray(geom, np.float64)[:, None, :3] - np.asarray(lanes, np.float64)[wave][:, :, :3]
+    r2 = (d * d).sum(-1) + np.asarray(lanes, np.float64)[wave][:, :, 3]
+    inv = np.where(bits, 1.0 / np.sqrt(np.where(r2 > 0, r2, 1.0)), 0.0)
+    B = field_basis(d, inv) * np.asarray(geom, np.float64)[:, 3, None, None, None]
+    return B * bits[:, :, None, None], bits
+
+
+def predict(y, wave, mask, geom, lanes, num_waves):
+    B, _ = _row_lane_basis(wave, mask, geom, lanes)
+    contrib = np.einsum("rlij,rj->rli", B, y)
+    pred = np.zeros((num_waves, 64, 3))
     np.add.at(pred, wave, contrib)
     return pred
 
 
-def loss_and_grads(p, x, wave, mask, scale, target, wgt, bf16=False):
+def loss_and_grads(p, x, wave, mask, geom, lanes, target, wgt, bf16=False):
     num_waves = target.shape[0]
     y, (z1, h1, z2, h2) = forward(p, x, bf16)
-    pred = predict(y, wave, mask, scale, num_waves)
+    B, _ = _row_lane_basis(wave, mask, geom, lanes)
+    pred = np.zeros((num_waves, 64, 3))
+    np.add.at(pred, wave, np.einsum("rlij,rj->rli", B, y))
     count = max(int((wgt > 0).sum()), 1)
     diff = pred - target
     loss = float((wgt[..., None] * diff * diff).sum() / count)
     g_pred = 2.0 * wgt[..., None] * diff / count
-    bits = lane_bits(mask)
-    g_y = scale[:, None] * (bits[:, :, None] * g_pred[wave]).sum(1)
+    g_y = np.einsum("rlij,rli->rj", B, g_pred[wave])
     r = bf16_round if bf16 else (lambda a: np.asarray(a, np.float64))
     g = {}
     g["W3"] = r(h2).T @ r(g_y)

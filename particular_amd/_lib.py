@@ -80,8 +80,8 @@ SIGNATURES = {
     "pnb_mlp_collect": (_i, [_vp, _vp, _sz, _f, _f, _sz, _u64, _sz]),
     "pnb_mlp_clear_dataset": (_i, [_vp]),
     "pnb_mlp_dataset_info": (_i, [_vp, _c.POINTER(_sz), _c.POINTER(_sz), _pu64]),
-    "pnb_mlp_dataset_read": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
-    "pnb_mlp_dataset_load": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _sz, _sz, _vp, _vp]),
+    "pnb_mlp_dataset_read": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "pnb_mlp_dataset_load": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _sz, _sz, _vp, _vp, _vp]),
     "pnb_mlp_grad": (_i, [_vp, _vp, _vp, _sz]),
     "pnb_mlp_train": (_i, [_vp, _sz, _f, _vp]),
     "pnb_mlp_forward_dev": (_i, [_vp, _vp, _sz, _vp]),

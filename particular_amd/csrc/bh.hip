@@ -71,6 +71,12 @@ struct __align__(32) NodeQuad {
     float4 a, b;
 };
 
+// Per internal node, from its children (built with the moments).
+struct NodeDesc {
+    float max_child;  // heaviest child mass / node mass
+    float children;   // children / 8
+};
+
 struct BhInfo {
     float mn[3], mx[3];  // root box
     uint32_t flags;
@@ -88,14 +94,13 @@ struct BhState {
     const uint64_t* skeys = nullptr;  // sorted keys (points into keys or keys_alt)
     const uint32_t* sidx = nullptr;   // tree order -> original index
     DevBuf keys, keys_alt, idx, idx_alt, sp, lcp, cnt, ip, key2, rec, sums, sums2, level, width;
-    DevBuf mom, quad;       // second moments (FP64) and normalised quadrupoles, on request
+    DevBuf mom, quad, desc;  // second moments (FP64), normalised quadrupoles, child shares
+    DevBuf qeff;             // per node: M w^2 MLP(node), the learned correction's moment
     bool has_moments = false;
     DevBuf info, bbox_part, hist, bits, afg_tmp;
     DevBuf akeys, akeys_alt, aidx, aidx_alt, apts, h_aff, h_afg, h_out;
-    DevBuf row_buf, row_count;  // row lists of the learned-correction walk
-    size_t row_cap = 1024;
     DevBuf c_starts, c_pts, c_bh, c_bf, c_list, c_count, c_target, c_wgt, c_rows, c_rwave,
-        c_cfeats, c_cscale, c_cmask;
+        c_cfeats, c_cgeom, c_cmask, c_lanes;
     float last_theta = 0.f;
     unsigned long long last_steps = 0, last_accepted = 0;
 };
@@ -104,12 +109,12 @@ void bh_free(BhState* s) {
     if (!s) return;
     for (DevBuf* b : {&s->keys, &s->keys_alt, &s->idx, &s->idx_alt, &s->sp, &s->lcp, &s->cnt,
                       &s->ip, &s->key2, &s->rec, &s->sums, &s->sums2, &s->level, &s->width,
-                      &s->mom, &s->quad,
+                      &s->mom, &s->quad, &s->desc, &s->qeff,
                       &s->info, &s->bbox_part, &s->hist, &s->bits, &s->afg_tmp, &s->akeys,
                       &s->akeys_alt, &s->aidx, &s->aidx_alt, &s->apts, &s->h_aff, &s->h_afg,
-                      &s->h_out, &s->row_buf, &s->row_count, &s->c_starts, &s->c_pts, &s->c_bh,
-                      &s->c_bf, &s->c_list, &s->c_count, &s->c_target, &s->c_wgt, &s->c_rows,
-                      &s->c_rwave, &s->c_cfeats, &s->c_cscale, &s->c_cmask})
+                      &s->h_out, &s->c_starts, &s->c_pts, &s->c_bh, &s->c_bf, &s->c_list,
+                      &s->c_count, &s->c_target, &s->c_wgt, &s->c_rows, &s->c_rwave, &s->c_cfeats,
+                      &s->c_cgeom, &s->c_cmask, &s->c_lanes})
         b->release();
     delete s;
 }
@@ -469,6 +474,7 @@ __global__ __launch_bounds__(256) void bh_up_kernel(NodeRec* __restrict__ rec,
                                                     const uint8_t* __restrict__ level, size_t total,
                                                     int L, NodeMoments* __restrict__ mom,
                                                     NodeQuad* __restrict__ quad,
+                                                    NodeDesc* __restrict__ desc,
                                                     const float* __restrict__ width) {
     for (size_t k = size_t(blockIdx.x) * blockDim.x + threadIdx.x; k < total;
          k += size_t(gridDim.x) * blockDim.x) {
@@ -499,10 +505,15 @@ __global__ __launch_bounds__(256) void bh_up_kernel(NodeRec* __restrict__ rec,
         if (mom) {
             NodeMoments t;
             NodeQuad qd{make_float4(0.f, 0.f, 0.f, 0.f), make_float4(0.f, 0.f, 0.f, 0.f)};
+            NodeDesc ds{0.f, 0.f};
             if (a.x > 0.0 && width[k] > 0.0f) {
                 const double px = a.y / a.x, py = a.z / a.x, pz = a.w / a.x;
+                double heaviest = 0.0;
+                int children = 0;
                 for (size_t ch = k + 1; ch < r.skip;) {
                     const double4 s1 = sums[ch];
+                    ++children;
+                    heaviest = fmax(heaviest, s1.x);
                     if (s1.x > 0.0) {
                         const NodeMoments tc = mom[ch];
                         const double dx = s1.y / s1.x - px, dy = s1.z / s1.x - py,
@@ -522,9 +533,12 @@ __global__ __launch_bounds__(256) void bh_up_kernel(NodeRec* __restrict__ rec,
                 qd.a = make_float4(float((3.0 * t.xx - tr) * f), float((3.0 * t.yy - tr) * f),
                                    float((3.0 * t.zz - tr) * f), float(3.0 * t.xy * f));
                 qd.b = make_float4(float(3.0 * t.xz * f), float(3.0 * t.yz * f), 0.f, 0.f);
+                ds.max_child = float(heaviest / a.x);
+                ds.children = float(children) * 0.125f;
             }
             mom[k] = t;
             quad[k] = qd;
+            desc[k] = ds;
         }
     }
 }
@@ -564,63 +578,95 @@ __global__ __launch_bounds__(256) void bh_ref_com_kernel(NodeRec* __restrict__ r
     }
 }
 
-// ---- traversal -----------------------------------------------------------------------------------
+// ---- learned correction: node features and the node network ------------------------------------
+// Features of an internal node (the network's input; zero for leaves):
+//   f0..5  normalised quadrupole Q~ = (3T - tr(T) I) / (M w^2), {xx, yy, zz, xy, xz, yz}
+//   f6     1 + log2(node mass / root mass) / 32
+//   f7     log2(bodies in the node) / 32
+//   f8     depth / 64
+//   f9     heaviest child's share of the node's mass
+//   f10    non-empty children / 8
+// The network's output y is an effective second moment in units of M w^2: every lane that
+// accepts the node adds quad_field(M w^2 y) at its own position. With y = f0..5 this is the
+// quadrupole term of the node's multipole expansion (learned.quadrupole_params sets that network).
 
-// ---- learned correction: row features and the corrected sum --------------------------------------
-// A row (wavefront, accepted internal node) is described as seen from the wavefront's centroid c:
-//   f0..2   unit vector u from c to the node's centre of mass, at distance d
-//   f3      w / d (w = node width)
-//   f4      1 + log2(node mass / total mass) / 32
-//   f5      wavefront rms radius / d
-//   f6      accepting lanes / 64
-//   f7..9   Q u - (u.Q.u) u      Q = node quadrupole (3T - tr(T) I) / (M w^2)
-//   f10..12 (u.Q.u) u
-// Rows with rms radius / d above ROW_SPREAD get zero features and scale (no correction).
-// The row's output y is a correction to the node's monopole term, in units of M w^2 / d^4 (the
-// size of the leading error of that term): every accepting lane adds (M w^2 / d^4) y. For
-// reference, the quadrupole term of the multipole expansion at c is (M w^2 / d^4)
-// (-f7..9 + 1.5 f10..12); the network learns its own map from the residuals.
-// Both helpers are compiled without contraction, so every kernel that calls them reproduces
-// the same values bit for bit.
+__device__ __forceinline__ void node_features(const NodeRec& r, const NodeQuad& q, uint8_t level,
+                                              const NodeDesc& dsc, float root_mass,
+                                              float (&f)[MLP_KIN]) {
+#pragma unroll
+    for (int i = 0; i < MLP_KIN; ++i) f[i] = 0.f;
+    if (!(r.w2 >= 0.0f) || !(r.mass > 0.0f)) return;  // leaf or massless: no correction
+    f[0] = q.a.x;
+    f[1] = q.a.y;
+    f[2] = q.a.z;
+    f[3] = q.a.w;
+    f[4] = q.b.x;
+    f[5] = q.b.y;
+    f[6] = 1.f + __builtin_amdgcn_logf(fmaxf(r.mass * __builtin_amdgcn_rcpf(root_mass), 1e-30f)) *
+                     (1.f / 32.f);
+    f[7] = __builtin_amdgcn_logf(float(r.end - r.begin)) * (1.f / 32.f);
+    f[8] = float(level) * (1.f / 64.f);
+    f[9] = dsc.max_child;
+    f[10] = dsc.children;
+}
 
-__device__ __forceinline__ void wave_frame(float4 p, bool valid, float& wx, float& wy, float& wz,
-                                           float& wrad) {
-#pragma clang fp contract(off)
-    float sx = valid ? p.x : 0.f, sy = valid ? p.y : 0.f, sz = valid ? p.z : 0.f;
-    float cnt = valid ? 1.f : 0.f;
-    for (int off = 32; off > 0; off >>= 1) {
-        sx += __shfl_xor(sx, off, 64);
-        sy += __shfl_xor(sy, off, 64);
-        sz += __shfl_xor(sz, off, 64);
-        cnt += __shfl_xor(cnt, off, 64);
+// qeff[k] = M w^2 MLP(features of node k) for every node (zero for leaves): 16 nodes per MFMA
+// tile, one tile per wavefront and iteration, weights staged in LDS.
+__global__ __launch_bounds__(256) void mlp_node_kernel(const NodeRec* __restrict__ rec,
+                                                       const NodeQuad* __restrict__ quad,
+                                                       const uint8_t* __restrict__ level,
+                                                       const NodeDesc* __restrict__ desc,
+                                                       const float* __restrict__ widths,
+                                                       uint32_t total,
+                                                       const MlpWeights* __restrict__ wg,
+                                                       NodeQuad* __restrict__ qeff) {
+    __shared__ MlpForward w;
+    {
+        const uint32_t* src = reinterpret_cast<const uint32_t*>(wg);
+        uint32_t* dst = reinterpret_cast<uint32_t*>(&w);
+        for (int t = threadIdx.x; t < int(sizeof(MlpForward) / 4); t += blockDim.x) dst[t] = src[t];
+        __syncthreads();
     }
-    wx = sx / cnt;
-    wy = sy / cnt;
-    wz = sz / cnt;
-    float r2 = valid ? (p.x - wx) * (p.x - wx) + (p.y - wy) * (p.y - wy) + (p.z - wz) * (p.z - wz) : 0.f;
-    for (int off = 32; off > 0; off >>= 1) r2 += __shfl_xor(r2, off, 64);
-    wrad = sqrtf(r2 / cnt);
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int r16 = lane & 15, g = lane >> 4;
+    const float root_mass = rec[0].mass;
+    for (uint32_t t0 = (blockIdx.x * 4 + wv) * 16; t0 < total; t0 += gridDim.x * 4 * 16) {
+        const uint32_t k = t0 + r16;
+        float f[MLP_KIN];
+        NodeRec r{};
+        r.w2 = -INFINITY;
+        float wd = 0.f;
+        if (k < total) {
+            r = rec[k];
+            wd = widths[k];
+            node_features(r, quad[k], level[k], desc[k], root_mass, f);
+        } else {
+#pragma unroll
+            for (int q = 0; q < MLP_KIN; ++q) f[q] = 0.f;
+        }
+        bf16x4_t bx;  // features 4 g .. 4 g + 3 of node k (static indices, selected by g)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const float v = g == 0 ? f[q] : g == 1 ? f[4 + q] : g == 2 ? f[8 + q] : f[12 + q];
+            bx[q] = short(f2bf(v));
+        }
+        const f32x4_t yv = mlp_forward16_regs(w, bx);  // lanes of group g: outputs 4 g .. 4 g + 3
+        const bool internal = r.w2 >= 0.0f && r.mass > 0.0f;
+        const float s = internal ? r.mass * wd * wd : 0.f;
+        if (k < total) {
+            if (g == 0) qeff[k].a = make_float4(s * yv[0], s * yv[1], s * yv[2], s * yv[3]);
+            if (g == 1) qeff[k].b = make_float4(s * yv[0], s * yv[1], 0.f, 0.f);
+        }
+    }
 }
 
-constexpr float ROW_SPREAD = 0.15f;  // largest wavefront rms radius / d of a corrected row
-
-// d2 = squared distance from the wavefront centroid to the node's centre of mass
-__device__ __forceinline__ float centre_dist2(const NodeRec& r, float wx, float wy, float wz) {
-#pragma clang fp contract(off)
-    const float rx = r.cx - wx, ry = r.cy - wy, rz = r.cz - wz;
-    return rx * rx + ry * ry + rz * rz;
-}
-
-__device__ __forceinline__ bool row_kept(float d2, float wrad) {
-#pragma clang fp contract(off)
-    return d2 > 0.f && wrad * wrad <= (ROW_SPREAD * ROW_SPREAD) * d2;
-}
+// ---- traversal -----------------------------------------------------------------------------------
 
 constexpr int WALK_PLAIN = 0;  // plain Barnes-Hut
 constexpr int WALK_ROWS = 1;   // also list the internal nodes each wavefront accepted (rows)
+constexpr int WALK_QUAD = 2;   // add quad_field(qeff[k]) for every accepted internal node
 
-// Per wavefront, the internal nodes that at least one of its lanes accepted and that are far
-// enough from the wavefront (row_kept), in walk order:
+// Per wavefront, the internal nodes that at least one of its lanes accepted, in walk order:
 // {node index, accepting-lane count, accepting-lane mask (low, high 32 bits)}.
 struct WalkRows {
     uint4* rows;      // [wave][cap]
@@ -635,26 +681,35 @@ __global__ __launch_bounds__(256) void bh_walk_kernel(const NodeRec* __restrict_
                                                       const uint32_t* __restrict__ perm,
                                                       uint32_t n, float theta2, float eps2,
                                                       float4* __restrict__ out,
-                                                      BhInfo* __restrict__ info, WalkRows rows) {
+                                                      BhInfo* __restrict__ info, WalkRows rows,
+                                                      const NodeQuad* __restrict__ qeff) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63;
     const bool valid = i < n;
     const float4 p = pts[valid ? i : n - 1];
     float ax = 0.f, ay = 0.f, az = 0.f, cx = 0.f, cy = 0.f, cz = 0.f;
+    float kx = 0.f, ky = 0.f, kz = 0.f;  // learned correction (WALK_QUAD)
     uint32_t resume = valid ? 0u : 0xffffffffu;
     uint32_t accepted = 0, steps = 0, nrows = 0;
     const uint32_t wave_slot = i / 64;
-    float wx = 0.f, wy = 0.f, wz = 0.f, wrad = 0.f;
-    if (MODE == WALK_ROWS) wave_frame(p, valid, wx, wy, wz, wrad);
 
     // The cursor's two possible successors (k + 1 when some lane opens, skip otherwise) are
     // loaded while node k is evaluated, so the scalar-load latency overlaps the VALU work.
     uint32_t k = 0;
     NodeRec r{};
-    if (total > 0) r = nodes[0];  // no tree (m = 0): nodes may not even be allocated
+    NodeQuad qk{};
+    if (total > 0) {  // no tree (m = 0): nodes may not even be allocated
+        r = nodes[0];
+        if (MODE == WALK_QUAD) qk = qeff[0];
+    }
     while (k < total) {
         const NodeRec r_next = nodes[min(k + 1, total - 1)];
         const NodeRec r_skip = nodes[min(r.skip, total - 1)];
+        NodeQuad q_next{}, q_skip{};
+        if (MODE == WALK_QUAD) {
+            q_next = qeff[min(k + 1, total - 1)];
+            q_skip = qeff[min(r.skip, total - 1)];
+        }
         const bool active = k >= resume;
         const float dx = sub_rn(r.cx, p.x), dy = sub_rn(r.cy, p.y), dz = sub_rn(r.cz, p.z);
         const float d2 = norm2_ref(dx, dy, dz);
@@ -673,12 +728,14 @@ __global__ __launch_bounds__(256) void bh_walk_kernel(const NodeRec* __restrict_
             y = fmaf(dz, s, -cz); t = az + y;
             cz = (t - az) - y; az = t;
             ++accepted;
+            if (MODE == WALK_QUAD && r.w2 >= 0.0f) {
+                const float q[6] = {qk.a.x, qk.a.y, qk.a.z, qk.a.w, qk.b.x, qk.b.y};
+                quad_field(q, dx, dy, dz, inv, kx, ky, kz);
+            }
         }
         if (MODE == WALK_ROWS && r.w2 >= 0.0f) {  // internal node: a row if some lane accepted it
             const unsigned long long m = __ballot(acc);
-            // nodes close to the wavefront relative to its spread get no row: there one
-            // correction for all lanes cannot stand for the field over the whole wavefront
-            if (m && row_kept(centre_dist2(r, wx, wy, wz), wrad)) {
+            if (m) {
                 if (lane == 0 && nrows < rows.cap)
                     rows.rows[size_t(wave_slot) * rows.cap + nrows] =
                         make_uint4(k, uint32_t(__popcll(m)), uint32_t(m), uint32_t(m >> 32));
@@ -689,13 +746,14 @@ __global__ __launch_bounds__(256) void bh_walk_kernel(const NodeRec* __restrict_
         ++steps;
         const uint32_t nxt = __builtin_amdgcn_readfirstlane(__any(open) ? k + 1 : max(r.skip, k + 1));
         r = (nxt == k + 1) ? r_next : r_skip;
+        if (MODE == WALK_QUAD) qk = (nxt == k + 1) ? q_next : q_skip;
         k = nxt;
     }
     if (MODE == WALK_ROWS && lane == 0 && wave_slot * 64 < n) {  // wavefronts holding particles
         rows.count[wave_slot] = nrows;
         atomicMax(&info->max_rows, nrows);
     }
-    if (valid) out[perm ? perm[i] : i] = make_float4(ax, ay, az, 0.f);
+    if (valid) out[perm ? perm[i] : i] = make_float4(ax + kx, ay + ky, az + kz, 0.f);
     unsigned long long acc64 = accepted;
     for (int off = 32; off > 0; off >>= 1) acc64 += __shfl_xor(acc64, off, 64);
     if (lane == 0) {
@@ -703,152 +761,6 @@ __global__ __launch_bounds__(256) void bh_walk_kernel(const NodeRec* __restrict_
         atomicAdd(&info->accepted, acc64);
     }
 }
-
-__device__ __forceinline__ float row_features(const NodeRec& r, float width, const NodeQuad& q,
-                                              float wx, float wy, float wz, float wrad,
-                                              float total_mass, uint32_t lanes,
-                                              float (&f)[MLP_KIN]) {
-#pragma clang fp contract(off)
-#pragma unroll
-    for (int i = 0; i < MLP_KIN; ++i) f[i] = 0.f;
-    const float rx = r.cx - wx, ry = r.cy - wy, rz = r.cz - wz;
-    const float d2 = centre_dist2(r, wx, wy, wz);
-    if (!row_kept(d2, wrad)) return 0.f;
-    // hardware reciprocal square root, reciprocal and log2 (about 1 ulp): the features need no
-    // more, and every caller gets the same values
-    const float inv = __builtin_amdgcn_rsqf(d2);
-    const float ux = rx * inv, uy = ry * inv, uz = rz * inv;
-    const float s = width * inv;
-    f[0] = ux;
-    f[1] = uy;
-    f[2] = uz;
-    f[3] = s;
-    f[4] = 1.f + __builtin_amdgcn_logf(fmaxf(r.mass * __builtin_amdgcn_rcpf(total_mass), 1e-30f)) *
-                     (1.f / 32.f);
-    f[5] = wrad * inv;
-    f[6] = float(lanes) * (1.f / 64.f);
-    const float qx = q.a.x * ux + q.a.w * uy + q.b.x * uz;
-    const float qy = q.a.w * ux + q.a.y * uy + q.b.y * uz;
-    const float qz = q.b.x * ux + q.b.y * uy + q.a.z * uz;
-    const float quu = qx * ux + qy * uy + qz * uz;
-    f[7] = qx - quu * ux;
-    f[8] = qy - quu * uy;
-    f[9] = qz - quu * uz;
-    f[10] = quu * ux;
-    f[11] = quu * uy;
-    f[12] = quu * uz;
-    return (r.mass * (inv * inv)) * (s * s);
-}
-
-struct ApplyTile {  // per wavefront: the scaled outputs and lane masks of one 16-row tile
-    float y[16][4];
-    unsigned long long mask[16];
-};
-
-// out[perm[i]] += sum over the rows of i's wavefront whose mask holds i of (M w^2 / d^4) MLP(row).
-// One wavefront per walk wavefront (particles [64 w, 64 w + 64) of pts); its rows go through
-// the MLP 16 at a time (MFMA, weights in LDS), then every lane adds the rows it accepted.
-__global__ __launch_bounds__(256) void mlp_apply_kernel(const NodeRec* __restrict__ nodes,
-                                                        const float* __restrict__ widths,
-                                                        const NodeQuad* __restrict__ quad,
-                                                        const float4* __restrict__ pts,
-                                                        const uint32_t* __restrict__ perm,
-                                                        uint32_t n, const uint4* __restrict__ rows,
-                                                        const uint32_t* __restrict__ count,
-                                                        uint32_t cap, uint32_t waves,
-                                                        const MlpWeights* __restrict__ wg,
-                                                        float4* __restrict__ out) {
-    __shared__ MlpForward w;
-    __shared__ ApplyTile tiles[4];
-    {
-        const uint32_t* src = reinterpret_cast<const uint32_t*>(wg);
-        uint32_t* dst = reinterpret_cast<uint32_t*>(&w);
-        for (int t = threadIdx.x; t < int(sizeof(MlpForward) / 4); t += blockDim.x) dst[t] = src[t];
-        __syncthreads();
-    }
-    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int r16 = lane & 15, g = lane >> 4;
-    ApplyTile& tile = tiles[wv];
-    const float total_mass = nodes[0].mass;
-    for (uint32_t wave = blockIdx.x * 4 + wv; wave < waves; wave += gridDim.x * 4) {
-        const uint32_t i = wave * 64 + lane;
-        const bool valid = i < n;
-        float wx, wy, wz, wrad;
-        wave_frame(pts[valid ? i : n - 1], valid, wx, wy, wz, wrad);
-        const uint32_t cnt = min(count[wave], cap);
-        const uint4* wrows = rows + size_t(wave) * cap;
-        float kx = 0.f, ky = 0.f, kz = 0.f;
-        // software pipeline: while a tile computes, the node data of the next tile and the row
-        // of the one after load (each load's address arrived one iteration earlier)
-        const uint4 zero4 = make_uint4(0u, 0u, 0u, 0u);
-        uint4 row = r16 < cnt ? wrows[r16] : zero4;
-        NodeRec nrec{};
-        float nwid = 0.f;
-        NodeQuad nq{};
-        if (r16 < cnt) {
-            nrec = nodes[row.x];
-            nwid = widths[row.x];
-            nq = quad[row.x];
-        }
-        uint4 row_next = 16 + r16 < cnt ? wrows[16 + r16] : zero4;
-        for (uint32_t t0 = 0; t0 < cnt; t0 += 16) {
-            const uint32_t ri = t0 + r16;
-            const uint4 crow = row;
-            const NodeRec crec = nrec;
-            const float cwid = nwid;
-            const NodeQuad cq = nq;
-            if (ri + 16 < cnt) {
-                nrec = nodes[row_next.x];
-                nwid = widths[row_next.x];
-                nq = quad[row_next.x];
-            }
-            row = row_next;
-            if (ri + 32 < cnt) row_next = wrows[ri + 32];
-            float f[MLP_KIN];
-            float sc = 0.f;
-            unsigned long long mk = 0ull;
-            if (ri < cnt) {
-                sc = row_features(crec, cwid, cq, wx, wy, wz, wrad, total_mass, crow.y, f);
-                mk = (unsigned long long)crow.z | ((unsigned long long)crow.w << 32);
-            } else {
-#pragma unroll
-                for (int q = 0; q < MLP_KIN; ++q) f[q] = 0.f;
-            }
-            bf16x4_t bx;  // features 4 g .. 4 g + 3 of row r16 (static indices, selected by g)
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const float v = g == 0 ? f[q] : g == 1 ? f[4 + q] : g == 2 ? f[8 + q] : f[12 + q];
-                bx[q] = short(f2bf(v));
-            }
-            const f32x4_t yv = mlp_forward16_regs(w, bx);
-            if (g == 0) {
-                tile.y[r16][0] = sc * yv[0];
-                tile.y[r16][1] = sc * yv[1];
-                tile.y[r16][2] = sc * yv[2];
-                tile.mask[r16] = mk;
-            }
-            wave_lds_sync();
-            const uint32_t nt = min(16u, cnt - t0);
-            for (uint32_t q = 0; q < nt; ++q) {  // branch-free: a lane outside the mask adds 0
-                const float sel = ((tile.mask[q] >> lane) & 1ull) ? 1.f : 0.f;
-                kx = fmaf(sel, tile.y[q][0], kx);
-                ky = fmaf(sel, tile.y[q][1], ky);
-                kz = fmaf(sel, tile.y[q][2], kz);
-            }
-            wave_lds_sync();
-        }
-        if (valid) {
-            float4* o = out + (perm ? perm[i] : i);
-            float4 a = *o;
-            a.x += kx;
-            a.y += ky;
-            a.z += kz;
-            *o = a;
-        }
-    }
-}
-
-__global__ void bh_rows_reset_kernel(BhInfo* info) { info->max_rows = 0; }
 
 // ---- host side --------------------------------------------------------------------------------
 
@@ -929,11 +841,14 @@ static int build_tree(pnb_ctx* c, const float4* afg, size_t m, bool ref_com = fa
         PNB_HIP(c, s->width.ensure(size_t(total) * sizeof(float)));
         NodeMoments* mom = nullptr;
         NodeQuad* quad = nullptr;
+        NodeDesc* desc = nullptr;
         if (moments) {
             PNB_HIP(c, s->mom.ensure(size_t(total) * sizeof(NodeMoments)));
             PNB_HIP(c, s->quad.ensure(size_t(total) * sizeof(NodeQuad)));
+            PNB_HIP(c, s->desc.ensure(size_t(total) * sizeof(NodeDesc)));
             mom = s->mom.as<NodeMoments>();
             quad = s->quad.as<NodeQuad>();
+            desc = s->desc.as<NodeDesc>();
         }
         NodeRec* rec = s->rec.as<NodeRec>();
         bh_node_kernel<<<g, 256, 0, c->stream>>>(sk, s->sp.as<float4>(), lcp, s->ip.as<int>(),
@@ -945,7 +860,7 @@ static int build_tree(pnb_ctx* c, const float4* afg, size_t m, bool ref_com = fa
         for (int L = h.max_level - 1; L >= 0; --L) {
             bh_up_kernel<<<ug, 256, 0, c->stream>>>(rec, s->sums.as<double4>(),
                                                     s->sums2.as<double4>(), s->level.as<uint8_t>(),
-                                                    total, L, mom, quad, s->width.as<float>());
+                                                    total, L, mom, quad, desc, s->width.as<float>());
             PNB_CHECK_LAUNCH(c);
         }
         s->has_moments = moments;
@@ -961,59 +876,42 @@ static int build_tree(pnb_ctx* c, const float4* afg, size_t m, bool ref_com = fa
 const MlpWeights* mlp_weights(pnb_ctx* c);  // mlp.hip: nullptr until initialised
 
 static int walk(pnb_ctx* c, const float4* pts, const uint32_t* perm, size_t n, float theta,
-                float softening, float4* out, const WalkRows* rows = nullptr) {
+                float softening, float4* out, const WalkRows* rows = nullptr,
+                const NodeQuad* qeff = nullptr) {
     BhState* s = c->bh;
     const float theta2 = theta * theta;  // (self.theta * self.theta), sequential.rs:491
     const float eps2 = softening * softening;
     return timed(c, PNB_PROF_BH_WALK, [&]() -> int {
         const unsigned grid = div_up(n, 256);
-        if (rows) {
-            bh_rows_reset_kernel<<<1, 1, 0, c->stream>>>(s->info.as<BhInfo>());
-            PNB_CHECK_LAUNCH(c);
-        }
-        auto* kern = rows ? bh_walk_kernel<WALK_ROWS> : bh_walk_kernel<WALK_PLAIN>;
+        auto* kern = rows ? bh_walk_kernel<WALK_ROWS>
+                     : qeff ? bh_walk_kernel<WALK_QUAD> : bh_walk_kernel<WALK_PLAIN>;
         kern<<<grid, 256, 0, c->stream>>>(s->rec.as<NodeRec>(), uint32_t(s->nodes), pts, perm,
                                           uint32_t(n), theta2, eps2, out, s->info.as<BhInfo>(),
-                                          rows ? *rows : WalkRows{});
+                                          rows ? *rows : WalkRows{}, qeff);
         PNB_CHECK_LAUNCH(c);
         return PNB_OK;
     });
 }
 
-// Walk listing rows, then add the learned correction. Row storage is sized from the largest
-// wavefront list seen so far; a walk that finds more rows than fit is repeated with room for all.
-static int walk_mlp(pnb_ctx* c, const float4* pts, const uint32_t* perm, size_t n, float theta,
-                    float softening, float4* out) {
+// The learned correction's per-node moments: qeff[k] = M w^2 MLP(node k) for the current tree.
+static int node_network(pnb_ctx* c) {
     BhState* s = c->bh;
     const MlpWeights* wg = mlp_weights(c);
     if (!wg) return fail(c, PNB_E_STATE, "PNB_BH_MLP needs an initialised MLP (pnb_mlp_init)");
     if (!s->has_moments) return fail(c, PNB_E_STATE, "octree built without node moments");
-    const size_t waves = div_up(n, 64);
-    PNB_HIP(c, s->row_count.ensure(waves * 4));
-    for (int attempt = 0; attempt < 2; ++attempt) {
-        const size_t cap = s->row_cap;
-        PNB_HIP(c, s->row_buf.ensure(waves * cap * sizeof(uint4)));
-        WalkRows rows{s->row_buf.as<uint4>(), s->row_count.as<uint32_t>(), uint32_t(cap)};
-        if (int rc = walk(c, pts, perm, n, theta, softening, out, &rows)) return rc;
-        uint32_t most = 0;
-        PNB_HIP(c, hipMemcpyAsync(&most, &s->info.as<BhInfo>()->max_rows, 4, hipMemcpyDeviceToHost,
-                                  c->stream));
-        PNB_HIP(c, hipStreamSynchronize(c->stream));
-        if (most <= cap) {
-            if (n == 0 || s->nodes == 0) return PNB_OK;
-            return timed(c, PNB_PROF_MLP, [&]() -> int {
-                const unsigned grid = unsigned(div_up(waves, 4));  // one wavefront per walk wavefront
-                mlp_apply_kernel<<<grid, 256, 0, c->stream>>>(
-                    s->rec.as<NodeRec>(), s->width.as<float>(), s->quad.as<NodeQuad>(), pts, perm,
-                    uint32_t(n), s->row_buf.as<uint4>(), s->row_count.as<uint32_t>(), uint32_t(cap),
-                    uint32_t(waves), wg, out);
-                PNB_CHECK_LAUNCH(c);
-                return PNB_OK;
-            });
-        }
-        s->row_cap = (size_t(most) + most / 4 + 255) / 256 * 256;
-    }
-    return fail(c, PNB_E_STATE, "row listing did not converge");
+    const size_t total = s->nodes;
+    PNB_HIP(c, s->qeff.ensure(std::max<size_t>(total, 1) * sizeof(NodeQuad)));
+    if (total == 0) return PNB_OK;
+    return timed(c, PNB_PROF_MLP, [&]() -> int {
+        const size_t tiles = div_up(total, 16);
+        const unsigned grid = unsigned(std::min<size_t>(div_up(tiles, 4), 16 * size_t(std::max(c->cu_count, 1))));
+        mlp_node_kernel<<<grid, 256, 0, c->stream>>>(s->rec.as<NodeRec>(), s->quad.as<NodeQuad>(),
+                                                     s->level.as<uint8_t>(), s->desc.as<NodeDesc>(),
+                                                     s->width.as<float>(), uint32_t(total), wg,
+                                                     s->qeff.as<NodeQuad>());
+        PNB_CHECK_LAUNCH(c);
+        return PNB_OK;
+    });
 }
 
 // Orders the affected particles along the tree's key order so that each wavefront holds
@@ -1027,11 +925,13 @@ static int accel_dev(pnb_ctx* c, const float4* affected, size_t n, const float4*
         return fail(c, PNB_E_STATE, "PNB_BH_MLP needs an initialised MLP (pnb_mlp_init)");
     if (int rc = build_tree(c, afg, m, ref_com, mlp)) return rc;
     BhState* s = c->bh;
-    auto run = [&](const float4* pts, const uint32_t* perm) -> int {
-        return (mlp && m) ? walk_mlp(c, pts, perm, n, theta, softening, out)
-                          : walk(c, pts, perm, n, theta, softening, out);
-    };
-    if (affected == afg && n == m) return run(s->sp.as<float4>(), s->sidx);
+    const NodeQuad* qeff = nullptr;
+    if (mlp && m) {
+        if (int rc = node_network(c)) return rc;
+        qeff = s->qeff.as<NodeQuad>();
+    }
+    if (affected == afg && n == m)
+        return walk(c, s->sp.as<float4>(), s->sidx, n, theta, softening, out, nullptr, qeff);
     const unsigned g = div_up(n, 256);
     PNB_HIP(c, s->akeys.ensure(n * 8));
     PNB_HIP(c, s->akeys_alt.ensure(n * 8));
@@ -1053,7 +953,7 @@ static int accel_dev(pnb_ctx* c, const float4* affected, size_t n, const float4*
         return rc;
     bh_gather_kernel<<<g, 256, 0, c->stream>>>(affected, n, si, s->apts.as<float4>());
     PNB_CHECK_LAUNCH(c);
-    return run(s->apts.as<float4>(), si);
+    return walk(c, s->apts.as<float4>(), si, n, theta, softening, out, nullptr, qeff);
 }
 
 static int copy_h2d(pnb_ctx* c, DevBuf& b, const float* src, size_t count) {
@@ -1066,9 +966,14 @@ static int copy_h2d(pnb_ctx* c, DevBuf& b, const float* src, size_t count) {
 // ---- training data for the learned correction -------------------------------------------------
 
 __global__ void mlp_gather_waves_kernel(const float4* __restrict__ sp, const uint32_t* __restrict__ starts,
-                                        uint32_t waves, float4* __restrict__ dst) {
+                                        uint32_t waves, float eps2, float4* __restrict__ dst,
+                                        float4* __restrict__ lanes) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < waves * 64) dst[i] = sp[starts[i / 64] + (i % 64)];
+    if (i < waves * 64) {
+        const float4 p = sp[starts[i / 64] + (i % 64)];
+        dst[i] = p;
+        lanes[i] = make_float4(p.x, p.y, p.z, eps2);
+    }
 }
 
 // target = brute force - Barnes-Hut, weight = 1 / |brute force|^2 (0 when it vanishes)
@@ -1085,37 +990,41 @@ __global__ void mlp_target_kernel(const float4* __restrict__ bf, const float4* _
     wgt[i] = n2 > 0.f ? 1.f / n2 : 0.f;
 }
 
-// One 64-thread block per walked wavefront: the features of its stored rows, written compactly
-// at [wave_rows[w], wave_rows[w+1]) (the same row_features / wave_frame as mlp_apply_kernel).
-__global__ __launch_bounds__(64) void mlp_row_feats_kernel(
-    const NodeRec* __restrict__ nodes, const float* __restrict__ widths,
-    const NodeQuad* __restrict__ quad, const float4* __restrict__ pts,
-    const uint4* __restrict__ rows, uint32_t cap, const uint32_t* __restrict__ wave_rows,
-    float* __restrict__ feats, float* __restrict__ scale, unsigned long long* __restrict__ mask,
-    uint32_t* __restrict__ rwave) {
-    const uint32_t w = blockIdx.x;
-    const int lane = threadIdx.x;
-    float wx, wy, wz, wrad;
-    wave_frame(pts[size_t(w) * 64 + lane], true, wx, wy, wz, wrad);
-    const float total_mass = nodes[0].mass;
+// One thread per stored row: the node's features (node_features, as mlp_node_kernel computes
+// them) and geometry {centre of mass, M w^2}, written compactly at wave_rows[w] + r.
+__global__ void mlp_row_feats_kernel(const NodeRec* __restrict__ nodes,
+                                     const float* __restrict__ widths,
+                                     const NodeQuad* __restrict__ quad,
+                                     const uint8_t* __restrict__ level,
+                                     const NodeDesc* __restrict__ desc,
+                                     const uint4* __restrict__ rows, uint32_t cap,
+                                     const uint32_t* __restrict__ wave_rows, uint32_t waves,
+                                     float* __restrict__ feats, float4* __restrict__ geom,
+                                     unsigned long long* __restrict__ mask,
+                                     uint32_t* __restrict__ rwave) {
+    const uint32_t w = blockIdx.y;
+    if (w >= waves) return;
     const uint32_t b = wave_rows[w], cnt = wave_rows[w + 1] - b;
-    for (uint32_t r = lane; r < cnt; r += 64) {
+    const float root_mass = nodes[0].mass;
+    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < cnt; r += gridDim.x * blockDim.x) {
         const uint4 row = rows[size_t(w) * cap + r];
+        const NodeRec nr = nodes[row.x];
         float f[MLP_KIN];
-        const float sc = row_features(nodes[row.x], widths[row.x], quad[row.x], wx, wy, wz, wrad,
-                                      total_mass, row.y, f);
+        node_features(nr, quad[row.x], level[row.x], desc[row.x], root_mass, f);
         const size_t dst = size_t(b) + r;
 #pragma unroll
         for (int q = 0; q < MLP_FEAT; ++q) feats[dst * MLP_FEAT + q] = f[q];
-        scale[dst] = sc;
+        const float wd = widths[row.x];
+        geom[dst] = make_float4(nr.cx, nr.cy, nr.cz, nr.mass * wd * wd);
         mask[dst] = (unsigned long long)row.z | ((unsigned long long)row.w << 32);
         rwave[dst] = w;
     }
 }
 
-int mlp_append(pnb_ctx* c, const float* feats, const float* scale, const unsigned long long* mask,
+int mlp_append(pnb_ctx* c, const float* feats, const float4* geom, const unsigned long long* mask,
                const uint32_t* row_wave_local, const uint32_t* wave_rows_local, size_t rows,
-               size_t waves, const float* target, const float* wgt, unsigned long long dropped);
+               size_t waves, const float4* lanes, const float* target, const float* wgt,
+               unsigned long long dropped);
 
 static int collect(pnb_ctx* c, const float4* p, size_t n, float theta, float softening,
                    size_t waves, uint64_t seed, uint32_t cap) {
@@ -1131,6 +1040,7 @@ static int collect(pnb_ctx* c, const float4* p, size_t n, float theta, float sof
     const size_t nr = W * 64;
     PNB_HIP(c, s->c_starts.ensure(W * 4));
     PNB_HIP(c, s->c_pts.ensure(nr * sizeof(float4)));
+    PNB_HIP(c, s->c_lanes.ensure(nr * sizeof(float4)));
     PNB_HIP(c, s->c_bh.ensure(nr * sizeof(float4)));
     PNB_HIP(c, s->c_bf.ensure(nr * sizeof(float4)));
     PNB_HIP(c, s->c_list.ensure(W * cap * sizeof(uint4)));
@@ -1140,7 +1050,8 @@ static int collect(pnb_ctx* c, const float4* p, size_t n, float theta, float sof
     PNB_HIP(c, s->c_rows.ensure((W + 1) * 4));
     PNB_HIP(c, hipMemcpyAsync(s->c_starts.ptr, starts.data(), W * 4, hipMemcpyHostToDevice, c->stream));
     mlp_gather_waves_kernel<<<div_up(nr, 256), 256, 0, c->stream>>>(
-        s->sp.as<float4>(), s->c_starts.as<uint32_t>(), uint32_t(W), s->c_pts.as<float4>());
+        s->sp.as<float4>(), s->c_starts.as<uint32_t>(), uint32_t(W), softening * softening,
+        s->c_pts.as<float4>(), s->c_lanes.as<float4>());
     PNB_CHECK_LAUNCH(c);
     WalkRows rows{s->c_list.as<uint4>(), s->c_count.as<uint32_t>(), cap};
     if (int rc = walk(c, s->c_pts.as<float4>(), nullptr, nr, theta, softening, s->c_bh.as<float4>(),
@@ -1157,26 +1068,32 @@ static int collect(pnb_ctx* c, const float4* p, size_t n, float theta, float sof
     PNB_HIP(c, hipMemcpyAsync(counts.data(), s->c_count.ptr, W * 4, hipMemcpyDeviceToHost, c->stream));
     PNB_HIP(c, hipStreamSynchronize(c->stream));
     unsigned long long dropped = 0;
+    uint32_t most = 0;
     for (size_t w = 0; w < W; ++w) {
         const uint32_t kept = std::min(counts[w], cap);
         dropped += counts[w] - kept;
+        most = std::max(most, kept);
         offs[w + 1] = offs[w] + kept;
     }
     const size_t R = offs[W];
     PNB_HIP(c, s->c_cfeats.ensure(std::max<size_t>(R, 1) * MLP_FEAT * 4));
-    PNB_HIP(c, s->c_cscale.ensure(std::max<size_t>(R, 1) * 4));
+    PNB_HIP(c, s->c_cgeom.ensure(std::max<size_t>(R, 1) * 16));
     PNB_HIP(c, s->c_cmask.ensure(std::max<size_t>(R, 1) * 8));
     PNB_HIP(c, s->c_rwave.ensure(std::max<size_t>(R, 1) * 4));
     PNB_HIP(c, hipMemcpyAsync(s->c_rows.ptr, offs.data(), (W + 1) * 4, hipMemcpyHostToDevice, c->stream));
-    mlp_row_feats_kernel<<<unsigned(W), 64, 0, c->stream>>>(
-        s->rec.as<NodeRec>(), s->width.as<float>(), s->quad.as<NodeQuad>(), s->c_pts.as<float4>(),
-        s->c_list.as<uint4>(), cap, s->c_rows.as<uint32_t>(), s->c_cfeats.as<float>(),
-        s->c_cscale.as<float>(), s->c_cmask.as<unsigned long long>(), s->c_rwave.as<uint32_t>());
-    PNB_CHECK_LAUNCH(c);
-    return mlp_append(c, s->c_cfeats.as<float>(), s->c_cscale.as<float>(),
+    if (R) {
+        const dim3 grid(unsigned(std::min<uint32_t>(div_up(most, 256), 64)), unsigned(W));
+        mlp_row_feats_kernel<<<grid, 256, 0, c->stream>>>(
+            s->rec.as<NodeRec>(), s->width.as<float>(), s->quad.as<NodeQuad>(),
+            s->level.as<uint8_t>(), s->desc.as<NodeDesc>(), s->c_list.as<uint4>(), cap,
+            s->c_rows.as<uint32_t>(), uint32_t(W), s->c_cfeats.as<float>(), s->c_cgeom.as<float4>(),
+            s->c_cmask.as<unsigned long long>(), s->c_rwave.as<uint32_t>());
+        PNB_CHECK_LAUNCH(c);
+    }
+    return mlp_append(c, s->c_cfeats.as<float>(), s->c_cgeom.as<float4>(),
                       s->c_cmask.as<unsigned long long>(), s->c_rwave.as<uint32_t>(),
-                      s->c_rows.as<uint32_t>(), R, W, s->c_target.as<float>(), s->c_wgt.as<float>(),
-                      dropped);
+                      s->c_rows.as<uint32_t>(), R, W, s->c_lanes.as<float4>(),
+                      s->c_target.as<float>(), s->c_wgt.as<float>(), dropped);
 }
 
 }  // namespace pnb

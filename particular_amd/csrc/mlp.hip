@@ -1,12 +1,14 @@
 // Training and standalone inference of the far-field correction MLP (no reference counterpart).
 //
 // Parameters live as an FP32 master vector (Adam state beside it) and are packed into BF16 MFMA
-// operand copies (MlpWeights) after every update. A training set holds rows (features, monopole
-// scale, accepting-lane mask, wavefront id) and per-lane targets (brute force minus Barnes-Hut
-// acceleration) with weights 1 / |a_bf|^2. One full-batch step:
+// operand copies (MlpWeights) after every update. A training set holds rows (node features, the
+// node's centre of mass and M w^2, accepting-lane mask, wavefront id), per wavefront the lanes'
+// positions (w = softening^2), and per-lane targets (brute force minus Barnes-Hut acceleration)
+// with weights 1 / |a_bf|^2. One full-batch step:
 //   mlp_fwd_kernel    y = MLP(x) for every row (MFMA, 16-row tiles per wavefront)
-//   mlp_lane_kernel   per wavefront: pred_l = sum over its rows of scale * y for lanes in the
-//                     mask; loss; g_pred; g_y(row) = scale * sum over its lanes of g_pred
+//   mlp_lane_kernel   per wavefront: pred_l = sum over its rows holding lane l of the field of
+//                     Q = M w^2 y at the lane (quad_field); loss; g_pred; g_y(row) = sum over its
+//                     lanes of M w^2 (d field / d Q)^T g_pred (quad_field_grad)
 //   mlp_bwd_kernel    recomputes the forward per 16-row tile and back-propagates with MFMA;
 //                     weight gradients accumulate in registers, one partial per wavefront
 //   mlp_reduce_kernel fixed-order sum of the partials (deterministic)
@@ -30,14 +32,14 @@ struct MlpState {
     // training set
     size_t rows = 0, waves = 0, cap_rows = 0, cap_waves = 0;
     unsigned long long dropped = 0;
-    DevBuf feats, scale, mask, rwave, target, wgt, wave_rows;
+    DevBuf feats, geom, mask, rwave, lanes, target, wgt, wave_rows;
     DevBuf y, gy, loss_part;
 };
 
 void mlp_free(MlpState* s) {
     if (!s) return;
     for (DevBuf* b : {&s->params, &s->m, &s->v, &s->grads, &s->weights, &s->partial, &s->loss_hist,
-                      &s->feats, &s->scale, &s->mask, &s->rwave, &s->target, &s->wgt,
+                      &s->feats, &s->geom, &s->mask, &s->rwave, &s->lanes, &s->target, &s->wgt,
                       &s->wave_rows, &s->y, &s->gy, &s->loss_part})
         b->release();
     delete s;
@@ -93,7 +95,7 @@ __device__ __forceinline__ void load_x16(MlpTile& t, const float* __restrict__ f
     wave_lds_sync();
 }
 
-// y (rows x 4, w = 0) = MLP(feats); the register-resident forward used by the Barnes-Hut walk.
+// y (rows x MLP_YS, the first MLP_OUT used) = MLP(feats), register-resident forward.
 __global__ __launch_bounds__(256) void mlp_fwd_kernel(const float* __restrict__ feats, size_t rows,
                                                       const MlpWeights* __restrict__ wg,
                                                       float* __restrict__ y) {
@@ -112,11 +114,9 @@ __global__ __launch_bounds__(256) void mlp_fwd_kernel(const float* __restrict__ 
             bx[i] = short((r < rows && k < MLP_IN) ? f2bf(feats[r * MLP_FEAT + k]) : uint16_t(0));
         }
         const f32x4_t yv = mlp_forward16_regs(w, bx);
-        if (g == 0 && r < rows) {
-            y[r * 4] = yv[0];
-            y[r * 4 + 1] = yv[1];
-            y[r * 4 + 2] = yv[2];
-            y[r * 4 + 3] = 0.f;
+        if (g < MLP_YS / 4 && r < rows) {  // lanes of group g hold outputs 4 g .. 4 g + 3
+#pragma unroll
+            for (int i = 0; i < 4; ++i) y[r * MLP_YS + 4 * g + i] = (4 * g + i < MLP_OUT) ? yv[i] : 0.f;
         }
     }
 }
@@ -126,7 +126,8 @@ __global__ __launch_bounds__(256) void mlp_fwd_kernel(const float* __restrict__ 
 __global__ __launch_bounds__(64) void mlp_lane_kernel(const float* __restrict__ y,
                                                       const uint32_t* __restrict__ wave_rows,
                                                       const unsigned long long* __restrict__ mask,
-                                                      const float* __restrict__ scale,
+                                                      const float4* __restrict__ geom,
+                                                      const float4* __restrict__ lanes,
                                                       const float* __restrict__ target,
                                                       const float* __restrict__ wgt, float inv_count,
                                                       float* __restrict__ gy,
@@ -134,13 +135,17 @@ __global__ __launch_bounds__(64) void mlp_lane_kernel(const float* __restrict__ 
     const size_t w = blockIdx.x;
     const int lane = threadIdx.x;
     const uint32_t rb = wave_rows[w], re = wave_rows[w + 1];
+    const float4 p = lanes[w * 64 + lane];  // w = softening^2
     float px = 0.f, py = 0.f, pz = 0.f;
     for (uint32_t r = rb; r < re; ++r) {
         if ((mask[r] >> lane) & 1ull) {
-            const float s = scale[r];
-            px = fmaf(s, y[4 * size_t(r)], px);
-            py = fmaf(s, y[4 * size_t(r) + 1], py);
-            pz = fmaf(s, y[4 * size_t(r) + 2], pz);
+            const float4 gm = geom[r];
+            const float dx = gm.x - p.x, dy = gm.y - p.y, dz = gm.z - p.z;
+            const float inv = rsqrtf(dx * dx + dy * dy + dz * dz + p.w);
+            float q[6];
+#pragma unroll
+            for (int j = 0; j < 6; ++j) q[j] = gm.w * y[size_t(r) * MLP_YS + j];
+            quad_field(q, dx, dy, dz, inv, px, py, pz);
         }
     }
     const size_t tl = w * 64 + lane;
@@ -152,19 +157,19 @@ __global__ __launch_bounds__(64) void mlp_lane_kernel(const float* __restrict__ 
     for (int off = 32; off > 0; off >>= 1) loss += __shfl_xor(loss, off, 64);
     if (lane == 0) loss_part[w] = loss;
     for (uint32_t r = rb; r < re; ++r) {
-        const bool in = (mask[r] >> lane) & 1ull;
-        float sx = in ? gx : 0.f, sy = in ? gyv : 0.f, sz = in ? gz : 0.f;
-        for (int off = 32; off > 0; off >>= 1) {
-            sx += __shfl_xor(sx, off, 64);
-            sy += __shfl_xor(sy, off, 64);
-            sz += __shfl_xor(sz, off, 64);
+        float g6[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        const float4 gm = geom[r];
+        if ((mask[r] >> lane) & 1ull) {
+            const float ex = gm.x - p.x, ey = gm.y - p.y, ez = gm.z - p.z;
+            const float inv = rsqrtf(ex * ex + ey * ey + ez * ez + p.w);
+            quad_field_grad(ex, ey, ez, inv, gx, gyv, gz, g6);
         }
+#pragma unroll
+        for (int j = 0; j < 6; ++j)
+            for (int off = 32; off > 0; off >>= 1) g6[j] += __shfl_xor(g6[j], off, 64);
         if (lane == 0) {
-            const float s = scale[r];
-            gy[4 * size_t(r)] = s * sx;
-            gy[4 * size_t(r) + 1] = s * sy;
-            gy[4 * size_t(r) + 2] = s * sz;
-            gy[4 * size_t(r) + 3] = 0.f;
+#pragma unroll
+            for (int j = 0; j < MLP_YS; ++j) gy[size_t(r) * MLP_YS + j] = j < 6 ? gm.w * g6[j] : 0.f;
         }
     }
 }
@@ -195,7 +200,7 @@ __global__ __launch_bounds__(256) void mlp_bwd_kernel(const float* __restrict__ 
     BwdTile& t = tiles[wv];
     const f32x4_t zero = {0.f, 0.f, 0.f, 0.f};
     f32x4_t gw1[4], gw2[4][4], gw3[4];
-    float gb1[4] = {0, 0, 0, 0}, gb2[4] = {0, 0, 0, 0}, gb3[3] = {0, 0, 0};
+    float gb1[4] = {0, 0, 0, 0}, gb2[4] = {0, 0, 0, 0}, gb3[MLP_OUT] = {};
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
         gw1[a] = zero;
@@ -213,7 +218,7 @@ __global__ __launch_bounds__(256) void mlp_bwd_kernel(const float* __restrict__ 
             const bool valid = r < rows;
 #pragma unroll
             for (int n = 0; n < MLP_NOUT; ++n) {
-                const float gv = (valid && n < MLP_OUT) ? gyin[4 * r + n] : 0.f;
+                const float gv = (valid && n < MLP_OUT) ? gyin[MLP_YS * r + n] : 0.f;
                 if (n < MLP_OUT) gb3[n] += gv;
                 const uint16_t b = f2bf(gv);
                 t.gyr[lane][n] = b;
@@ -375,9 +380,10 @@ const MlpWeights* mlp_weights(pnb_ctx* c) {
 }
 
 // Appends `rows` rows and `waves` wavefronts (already on the device) to the training set.
-int mlp_append(pnb_ctx* c, const float* feats, const float* scale, const unsigned long long* mask,
+int mlp_append(pnb_ctx* c, const float* feats, const float4* geom, const unsigned long long* mask,
                const uint32_t* row_wave_local, const uint32_t* wave_rows_local, size_t rows,
-               size_t waves, const float* target, const float* wgt, unsigned long long dropped);
+               size_t waves, const float4* lanes, const float* target, const float* wgt,
+               unsigned long long dropped);
 
 // Grows a device array, keeping its first `used` bytes.
 static int grow(pnb_ctx* c, DevBuf& b, size_t used, size_t want) {
@@ -397,9 +403,10 @@ __global__ void mlp_offset_kernel(const uint32_t* __restrict__ src, size_t n, ui
     if (i < n) dst[i] = src[i] + add;
 }
 
-int mlp_append(pnb_ctx* c, const float* feats, const float* scale, const unsigned long long* mask,
+int mlp_append(pnb_ctx* c, const float* feats, const float4* geom, const unsigned long long* mask,
                const uint32_t* row_wave_local, const uint32_t* wave_rows_local, size_t rows,
-               size_t waves, const float* target, const float* wgt, unsigned long long dropped) {
+               size_t waves, const float4* lanes, const float* target, const float* wgt,
+               unsigned long long dropped) {
     if (int rc = mlp_ensure(c)) return rc;
     MlpState* s = c->mlp;
     const size_t nr = s->rows + rows, nwv = s->waves + waves;
@@ -407,9 +414,10 @@ int mlp_append(pnb_ctx* c, const float* feats, const float* scale, const unsigne
     auto cap = [](size_t need, size_t have) { return need <= have ? have : std::max(need, 2 * have); };
     const size_t cr = cap(nr, s->cap_rows), cw = cap(nwv, s->cap_waves);
     if (int rc = grow(c, s->feats, s->rows * MLP_FEAT * 4, cr * MLP_FEAT * 4)) return rc;
-    if (int rc = grow(c, s->scale, s->rows * 4, cr * 4)) return rc;
+    if (int rc = grow(c, s->geom, s->rows * 16, cr * 16)) return rc;
     if (int rc = grow(c, s->mask, s->rows * 8, cr * 8)) return rc;
     if (int rc = grow(c, s->rwave, s->rows * 4, cr * 4)) return rc;
+    if (int rc = grow(c, s->lanes, s->waves * 64 * 16, cw * 64 * 16)) return rc;
     if (int rc = grow(c, s->target, s->waves * 64 * 16, cw * 64 * 16)) return rc;
     if (int rc = grow(c, s->wgt, s->waves * 64 * 4, cw * 64 * 4)) return rc;
     if (int rc = grow(c, s->wave_rows, s->waves ? (s->waves + 1) * 4 : 0, (cw + 1) * 4)) return rc;
@@ -418,12 +426,13 @@ int mlp_append(pnb_ctx* c, const float* feats, const float* scale, const unsigne
     hipStream_t st = c->stream;
     if (rows) {
         PNB_HIP(c, hipMemcpyAsync(s->feats.as<float>() + s->rows * MLP_FEAT, feats, rows * MLP_FEAT * 4, hipMemcpyDeviceToDevice, st));
-        PNB_HIP(c, hipMemcpyAsync(s->scale.as<float>() + s->rows, scale, rows * 4, hipMemcpyDeviceToDevice, st));
+        PNB_HIP(c, hipMemcpyAsync(s->geom.as<float4>() + s->rows, geom, rows * 16, hipMemcpyDeviceToDevice, st));
         PNB_HIP(c, hipMemcpyAsync(s->mask.as<unsigned long long>() + s->rows, mask, rows * 8, hipMemcpyDeviceToDevice, st));
         mlp_offset_kernel<<<div_up(rows, 256), 256, 0, st>>>(row_wave_local, rows, uint32_t(s->waves),
                                                              s->rwave.as<uint32_t>() + s->rows);
         PNB_CHECK_LAUNCH(c);
     }
+    PNB_HIP(c, hipMemcpyAsync(s->lanes.as<float4>() + s->waves * 64, lanes, waves * 64 * 16, hipMemcpyDeviceToDevice, st));
     PNB_HIP(c, hipMemcpyAsync(s->target.as<float>() + s->waves * 64 * 4, target, waves * 64 * 16, hipMemcpyDeviceToDevice, st));
     PNB_HIP(c, hipMemcpyAsync(s->wgt.as<float>() + s->waves * 64, wgt, waves * 64 * 4, hipMemcpyDeviceToDevice, st));
     mlp_offset_kernel<<<div_up(waves + 1, 256), 256, 0, st>>>(wave_rows_local, waves + 1, uint32_t(s->rows),
@@ -439,8 +448,8 @@ int mlp_append(pnb_ctx* c, const float* feats, const float* scale, const unsigne
 static int step_grads(pnb_ctx* c, float* loss_dev) {
     MlpState* s = c->mlp;
     if (s->waves == 0) return fail(c, PNB_E_STATE, "the training set is empty");
-    PNB_HIP(c, s->y.ensure(std::max<size_t>(1, s->rows) * 16));
-    PNB_HIP(c, s->gy.ensure(std::max<size_t>(1, s->rows) * 16));
+    PNB_HIP(c, s->y.ensure(std::max<size_t>(1, s->rows) * MLP_YS * 4));
+    PNB_HIP(c, s->gy.ensure(std::max<size_t>(1, s->rows) * MLP_YS * 4));
     PNB_HIP(c, s->loss_part.ensure(s->waves * 4));
     PNB_HIP(c, s->partial.ensure(size_t(MLP_BWD_BLOCKS) * MLP_WAVES_PER_BLOCK * MLP_NPARAM * 4));
     // lanes with a positive weight, counted on the host once per call
@@ -460,7 +469,7 @@ static int step_grads(pnb_ctx* c, float* loss_dev) {
         }
         mlp_lane_kernel<<<unsigned(s->waves), 64, 0, c->stream>>>(
             s->y.as<float>(), s->wave_rows.as<uint32_t>(), s->mask.as<unsigned long long>(),
-            s->scale.as<float>(), s->target.as<float>(), s->wgt.as<float>(), inv_count,
+            s->geom.as<float4>(), s->lanes.as<float4>(), s->target.as<float>(), s->wgt.as<float>(), inv_count,
             s->gy.as<float>(), s->loss_part.as<float>());
         PNB_CHECK_LAUNCH(c);
         mlp_loss_sum_kernel<<<1, 256, 0, c->stream>>>(s->loss_part.as<float>(), s->waves, loss_dev);
@@ -554,58 +563,62 @@ int pnb_mlp_dataset_info(pnb_ctx* c, size_t* rows, size_t* waves, uint64_t* drop
     return PNB_OK;
 }
 
-int pnb_mlp_dataset_read(pnb_ctx* c, float* feats, float* scale, uint64_t* mask, uint32_t* row_wave,
-                         uint32_t* wave_rows, float* target, float* wgt) {
+int pnb_mlp_dataset_read(pnb_ctx* c, float* feats, float* geom, uint64_t* mask, uint32_t* row_wave,
+                         uint32_t* wave_rows, float* lanes, float* target, float* wgt) {
     if (!c) return PNB_E_ARG;
     MlpState* s = c->mlp;
     if (!s || s->waves == 0) return fail(c, PNB_E_STATE, "the training set is empty");
     if (int rc = activate(c)) return rc;
     hipStream_t st = c->stream;
     if (feats && s->rows) PNB_HIP(c, hipMemcpyAsync(feats, s->feats.ptr, s->rows * MLP_FEAT * 4, hipMemcpyDeviceToHost, st));
-    if (scale && s->rows) PNB_HIP(c, hipMemcpyAsync(scale, s->scale.ptr, s->rows * 4, hipMemcpyDeviceToHost, st));
+    if (geom && s->rows) PNB_HIP(c, hipMemcpyAsync(geom, s->geom.ptr, s->rows * 16, hipMemcpyDeviceToHost, st));
     if (mask && s->rows) PNB_HIP(c, hipMemcpyAsync(mask, s->mask.ptr, s->rows * 8, hipMemcpyDeviceToHost, st));
     if (row_wave && s->rows) PNB_HIP(c, hipMemcpyAsync(row_wave, s->rwave.ptr, s->rows * 4, hipMemcpyDeviceToHost, st));
     if (wave_rows) PNB_HIP(c, hipMemcpyAsync(wave_rows, s->wave_rows.ptr, (s->waves + 1) * 4, hipMemcpyDeviceToHost, st));
+    if (lanes) PNB_HIP(c, hipMemcpyAsync(lanes, s->lanes.ptr, s->waves * 64 * 16, hipMemcpyDeviceToHost, st));
     if (target) PNB_HIP(c, hipMemcpyAsync(target, s->target.ptr, s->waves * 64 * 16, hipMemcpyDeviceToHost, st));
     if (wgt) PNB_HIP(c, hipMemcpyAsync(wgt, s->wgt.ptr, s->waves * 64 * 4, hipMemcpyDeviceToHost, st));
     PNB_HIP(c, hipStreamSynchronize(st));
     return PNB_OK;
 }
 
-int pnb_mlp_dataset_load(pnb_ctx* c, const float* feats, const float* scale, const uint64_t* mask,
+int pnb_mlp_dataset_load(pnb_ctx* c, const float* feats, const float* geom, const uint64_t* mask,
                          const uint32_t* row_wave, const uint32_t* wave_rows, size_t rows,
-                         size_t waves, const float* target, const float* wgt) {
+                         size_t waves, const float* lanes, const float* target, const float* wgt) {
     if (!c) return PNB_E_ARG;
-    if (waves == 0 || !wave_rows || !target || !wgt || (rows && (!feats || !scale || !mask || !row_wave)))
+    if (waves == 0 || !wave_rows || !lanes || !target || !wgt ||
+        (rows && (!feats || !geom || !mask || !row_wave)))
         return fail(c, PNB_E_ARG, "incomplete training set");
     if (int rc = activate(c)) return rc;
     DevBuf tmp;
-    const size_t bytes = rows * (MLP_FEAT * 4 + 4 + 8 + 4) + (waves + 1) * 4 + waves * 64 * 20;
+    const size_t bytes = rows * (MLP_FEAT * 4 + 16 + 8 + 4) + (waves + 1) * 4 + waves * 64 * 36;
     PNB_HIP(c, tmp.ensure(bytes));
     char* p = tmp.as<char>();
+    float4* dg = reinterpret_cast<float4*>(p); p += rows * 16;  // 16-byte records first
+    float4* dl = reinterpret_cast<float4*>(p); p += waves * 64 * 16;
+    float* dt = reinterpret_cast<float*>(p); p += waves * 64 * 16;
     float* df = reinterpret_cast<float*>(p); p += rows * MLP_FEAT * 4;
-    float* ds = reinterpret_cast<float*>(p); p += rows * 4;
     unsigned long long* dm = reinterpret_cast<unsigned long long*>(p); p += rows * 8;
     uint32_t* drw = reinterpret_cast<uint32_t*>(p); p += rows * 4;
     uint32_t* dwr = reinterpret_cast<uint32_t*>(p); p += (waves + 1) * 4;
-    float* dt = reinterpret_cast<float*>(p); p += waves * 64 * 16;
     float* dwg = reinterpret_cast<float*>(p);
     hipStream_t st = c->stream;
     hipError_t e = hipSuccess;
     if (rows) {
         if (e == hipSuccess) e = hipMemcpyAsync(df, feats, rows * MLP_FEAT * 4, hipMemcpyHostToDevice, st);
-        if (e == hipSuccess) e = hipMemcpyAsync(ds, scale, rows * 4, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(dg, geom, rows * 16, hipMemcpyHostToDevice, st);
         if (e == hipSuccess) e = hipMemcpyAsync(dm, mask, rows * 8, hipMemcpyHostToDevice, st);
         if (e == hipSuccess) e = hipMemcpyAsync(drw, row_wave, rows * 4, hipMemcpyHostToDevice, st);
     }
     if (e == hipSuccess) e = hipMemcpyAsync(dwr, wave_rows, (waves + 1) * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(dl, lanes, waves * 64 * 16, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemcpyAsync(dt, target, waves * 64 * 16, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemcpyAsync(dwg, wgt, waves * 64 * 4, hipMemcpyHostToDevice, st);
     if (e != hipSuccess) {
         tmp.release();
         return hip_fail(c, e, "training set upload", __FILE__, __LINE__);
     }
-    int rc = mlp_append(c, df, ds, dm, drw, dwr, rows, waves, dt, dwg, 0);
+    int rc = mlp_append(c, df, dg, dm, drw, dwr, rows, waves, dl, dt, dwg, 0);
     PNB_HIP(c, hipStreamSynchronize(st));
     tmp.release();
     return rc;
@@ -649,7 +662,7 @@ int pnb_mlp_train(pnb_ctx* c, size_t iterations, float lr, float* loss_history) 
     return PNB_OK;
 }
 
-// Standalone inference: y (rows x 4, w = 0) = MLP(feats (rows x 8)); device pointers.
+// Standalone inference: y (rows x 8, 6 used) = MLP(feats (rows x 16, 11 used)); device pointers.
 int pnb_mlp_forward_dev(pnb_ctx* c, const float* feats, size_t rows, float* y) {
     if (!c) return PNB_E_ARG;
     if (rows && (!feats || !y)) return fail(c, PNB_E_ARG, "null pointer with non-zero length");

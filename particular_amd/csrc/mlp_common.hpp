@@ -1,11 +1,12 @@
-// Far-field correction MLP: shared device definitions (bh.hip walk, mlp.hip training).
+// Far-field correction MLP: shared device definitions (bh.hip build and walk, mlp.hip training).
 //
-// Network: 13 inputs -> 64 -> 64 -> 3, ReLU. BF16 weights and activations feed MFMA
+// Network: 11 inputs -> 64 -> 64 -> 6, ReLU. BF16 weights and activations feed MFMA
 // (v_mfma_f32_16x16x16_bf16, FP32 accumulate); biases and everything outside the products are
-// FP32. A "row" is one (wavefront, accepted internal node) event of the Barnes-Hut walk: its
-// features describe the node as seen from the wavefront's centroid, and its output y is a
-// correction to the node's monopole term, applied as (mass / d^2) * y to every lane of the
-// wavefront that accepted the node. Leaves are never corrected (their monopole is exact).
+// FP32. The network maps a description of an internal octree node (its normalised quadrupole,
+// mass fraction, size in bodies, depth and how its mass splits over its children) to an
+// effective second moment Q (6 components, in units of the node's M w^2). Every lane that
+// accepts the node in the Barnes-Hut walk adds the field of Q at its own position (quad_field)
+// to the node's monopole term. Leaves are never corrected (their monopole is exact).
 //
 // MFMA 16x16x16 bf16 operand layout (lane l of 64): A[m][k] with m = l % 16, k = 4 (l / 16) + i;
 // B[k][n] with n = l % 16, k = 4 (l / 16) + i; D[m][n] with n = l % 16, m = 4 (l / 16) + i,
@@ -19,9 +20,9 @@
 
 namespace pnb {
 
-constexpr int MLP_IN = 13;
+constexpr int MLP_IN = 11;
 constexpr int MLP_H = 64;
-constexpr int MLP_OUT = 3;
+constexpr int MLP_OUT = 6;
 constexpr int MLP_KIN = 16;   // inputs padded to one K step
 constexpr int MLP_NOUT = 16;  // outputs padded to one N tile
 constexpr int MLP_OFF_W1 = 0;
@@ -30,8 +31,9 @@ constexpr int MLP_OFF_W2 = MLP_OFF_B1 + MLP_H;
 constexpr int MLP_OFF_B2 = MLP_OFF_W2 + MLP_H * MLP_H;
 constexpr int MLP_OFF_W3 = MLP_OFF_B2 + MLP_H;
 constexpr int MLP_OFF_B3 = MLP_OFF_W3 + MLP_H * MLP_OUT;
-constexpr int MLP_NPARAM = MLP_OFF_B3 + MLP_OUT;  // 5251
-constexpr int MLP_FEAT = 16;                      // stored feature stride (13 used)
+constexpr int MLP_NPARAM = MLP_OFF_B3 + MLP_OUT;  // 5318
+constexpr int MLP_FEAT = 16;                      // stored feature stride (11 used)
+constexpr int MLP_YS = 8;                         // stored output stride (6 used)
 
 typedef short bf16x4_t __attribute__((ext_vector_type(4)));
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
@@ -169,6 +171,38 @@ __device__ __forceinline__ void mlp_forward16(const MlpForward& w, MlpTile& t, f
         for (int i = 0; i < 4; ++i) y[(4 * g + i) * 4 + r16] = acc[i] + b;
     }
     wave_lds_sync();
+}
+
+// Field of an effective second moment at a point: q = {xx, yy, zz, xy, xz, yz} in absolute units
+// (M w^2 folded in), d = centre of mass - point, inv = 1 / |d| (softened like the monopole term).
+// The quadrupole term of the multipole expansion, a = -Q d |d|^-5 + 2.5 (d.Q.d) d |d|^-7, is
+// added to (ax, ay, az).
+__device__ __forceinline__ void quad_field(const float (&q)[6], float dx, float dy, float dz,
+                                           float inv, float& ax, float& ay, float& az) {
+    const float qx = q[0] * dx + q[3] * dy + q[4] * dz;
+    const float qy = q[3] * dx + q[1] * dy + q[5] * dz;
+    const float qz = q[4] * dx + q[5] * dy + q[2] * dz;
+    const float dqd = dx * qx + dy * qy + dz * qz;
+    const float inv2 = inv * inv;
+    const float inv5 = inv2 * inv2 * inv;
+    const float t = 2.5f * dqd * inv2;
+    ax += inv5 * (t * dx - qx);
+    ay += inv5 * (t * dy - qy);
+    az += inv5 * (t * dz - qz);
+}
+
+// g . (d a / d q_j) for the six components of quad_field's q, added to g6.
+__device__ __forceinline__ void quad_field_grad(float dx, float dy, float dz, float inv, float gx,
+                                                float gy, float gz, float (&g6)[6]) {
+    const float inv2 = inv * inv;
+    const float inv5 = inv2 * inv2 * inv;
+    const float gd = 2.5f * (gx * dx + gy * dy + gz * dz) * inv2;  // 2.5 (g.d) / |d|^2
+    g6[0] += inv5 * (gd * dx * dx - dx * gx);
+    g6[1] += inv5 * (gd * dy * dy - dy * gy);
+    g6[2] += inv5 * (gd * dz * dz - dz * gz);
+    g6[3] += inv5 * (2.f * gd * dx * dy - (dy * gx + dx * gy));
+    g6[4] += inv5 * (2.f * gd * dx * dz - (dz * gx + dx * gz));
+    g6[5] += inv5 * (2.f * gd * dy * dz - (dz * gy + dy * gz));
 }
 
 }  // namespace pnb

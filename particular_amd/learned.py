@@ -13,7 +13,9 @@ import numpy as np
 from . import _lib
 
 FEAT = 16  # stored feature stride
-IN = 13    # features the network reads
+IN = 11    # features the network reads
+OUT = 6    # outputs: an effective second moment {xx, yy, zz, xy, xz, yz} in units of M w^2
+YS = 8     # stored output stride
 
 
 def num_params() -> int:
@@ -38,8 +40,8 @@ def get_params(res) -> np.ndarray:
 def params_to_dict(p: np.ndarray) -> dict:
     """Splits the flat vector into the oracle's W1, b1, W2, b2, W3, b3 (FP64)."""
     p = np.asarray(p, np.float64)
-    shapes = [("W1", (IN, 64)), ("b1", (64,)), ("W2", (64, 64)), ("b2", (64,)), ("W3", (64, 3)),
-              ("b3", (3,))]
+    shapes = [("W1", (IN, 64)), ("b1", (64,)), ("W2", (64, 64)), ("b2", (64,)), ("W3", (64, OUT)),
+              ("b3", (OUT,))]
     out, o = {}, 0
     for name, shp in shapes:
         k = int(np.prod(shp))
@@ -77,21 +79,21 @@ def info(res) -> tuple[int, int, int]:
 def dataset(res) -> dict:
     rows, waves, _ = info(res)
     d = {
-        "feats": np.zeros((rows, FEAT), np.float32), "scale": np.zeros(rows, np.float32),
+        "feats": np.zeros((rows, FEAT), np.float32), "geom": np.zeros((rows, 4), np.float32),
         "mask": np.zeros(rows, np.uint64), "row_wave": np.zeros(rows, np.uint32),
-        "wave_rows": np.zeros(waves + 1, np.uint32),
+        "wave_rows": np.zeros(waves + 1, np.uint32), "lanes": np.zeros((waves, 64, 4), np.float32),
         "target": np.zeros((waves, 64, 4), np.float32), "wgt": np.zeros((waves, 64), np.float32),
     }
     res.check(_lib.lib().pnb_mlp_dataset_read(
-        res.handle, d["feats"].ctypes.data, d["scale"].ctypes.data, d["mask"].ctypes.data,
-        d["row_wave"].ctypes.data, d["wave_rows"].ctypes.data, d["target"].ctypes.data,
-        d["wgt"].ctypes.data))
+        res.handle, d["feats"].ctypes.data, d["geom"].ctypes.data, d["mask"].ctypes.data,
+        d["row_wave"].ctypes.data, d["wave_rows"].ctypes.data, d["lanes"].ctypes.data,
+        d["target"].ctypes.data, d["wgt"].ctypes.data))
     return d
 
 
-_DATASET_DTYPES = {"feats": np.float32, "scale": np.float32, "mask": np.uint64,
-                   "row_wave": np.uint32, "wave_rows": np.uint32, "target": np.float32,
-                   "wgt": np.float32}
+_DATASET_DTYPES = {"feats": np.float32, "geom": np.float32, "mask": np.uint64,
+                   "row_wave": np.uint32, "wave_rows": np.uint32, "lanes": np.float32,
+                   "target": np.float32, "wgt": np.float32}
 
 
 def load(res, d: dict) -> None:
@@ -100,16 +102,17 @@ def load(res, d: dict) -> None:
     f = {k: np.ascontiguousarray(d[k], dt) for k, dt in _DATASET_DTYPES.items()}
     if f["feats"].ndim != 2 or f["feats"].shape[1] != FEAT:
         raise ValueError(f"feats must be (rows, {FEAT})")
-    if f["target"].shape[1:] != (64, 4) or f["wgt"].shape[1:] != (64,):
-        raise ValueError("target must be (waves, 64, 4) and wgt (waves, 64)")
+    if (f["target"].shape[1:] != (64, 4) or f["lanes"].shape[1:] != (64, 4)
+            or f["wgt"].shape[1:] != (64,)):
+        raise ValueError("lanes and target must be (waves, 64, 4) and wgt (waves, 64)")
     rows, waves = f["feats"].shape[0], f["wave_rows"].shape[0] - 1
-    if not (f["scale"].size == f["mask"].size == f["row_wave"].size == rows
-            and f["target"].shape[0] == f["wgt"].shape[0] == waves):
+    if not (f["geom"].shape == (rows, 4) and f["mask"].size == f["row_wave"].size == rows
+            and f["target"].shape[0] == f["wgt"].shape[0] == f["lanes"].shape[0] == waves):
         raise ValueError("inconsistent training-set sizes")
     res.check(_lib.lib().pnb_mlp_dataset_load(
-        res.handle, f["feats"].ctypes.data, f["scale"].ctypes.data, f["mask"].ctypes.data,
+        res.handle, f["feats"].ctypes.data, f["geom"].ctypes.data, f["mask"].ctypes.data,
         f["row_wave"].ctypes.data, f["wave_rows"].ctypes.data, rows, waves,
-        f["target"].ctypes.data, f["wgt"].ctypes.data))
+        f["lanes"].ctypes.data, f["target"].ctypes.data, f["wgt"].ctypes.data))
 
 
 def grad(res) -> tuple[float, np.ndarray]:
@@ -126,21 +129,18 @@ def train(res, iterations: int, lr: float = 1e-3) -> np.ndarray:
 
 
 def quadrupole_params() -> np.ndarray:
-    """Parameters that make the network output the quadrupole term of the node's multipole
-    expansion at the wavefront centroid, y = -f7..9 + 1.5 f10..12 (in units of M w^2 / d^4):
-    features 7..12 pass through 12 hidden units as relu(+f) and relu(-f) (every weight is 0,
-    +-1 or +-1.5, exact in bf16). A physics-informed starting point for training."""
+    """Parameters that make the network output the node's own normalised quadrupole (features
+    0..5), so every accepting lane adds the quadrupole term of the node's multipole expansion:
+    the six features pass through 12 hidden units as relu(+f) and relu(-f) (every weight is 0 or
+    +-1, exact in bf16). A physics-informed starting point for training."""
     d = params_to_dict(np.zeros(num_params(), np.float32))
-    for j in range(6):
-        d["W1"][7 + j, 2 * j] = 1.0
-        d["W1"][7 + j, 2 * j + 1] = -1.0
-    for i in range(12):
-        d["W2"][i, i] = 1.0
-    for o in range(3):
-        d["W3"][2 * o, o] = -1.0
-        d["W3"][2 * o + 1, o] = 1.0
-        d["W3"][6 + 2 * o, o] = 1.5
-        d["W3"][7 + 2 * o, o] = -1.5
+    for j in range(OUT):
+        d["W1"][j, 2 * j] = 1.0
+        d["W1"][j, 2 * j + 1] = -1.0
+        d["W2"][2 * j, 2 * j] = 1.0
+        d["W2"][2 * j + 1, 2 * j + 1] = 1.0
+        d["W3"][2 * j, j] = 1.0
+        d["W3"][2 * j + 1, j] = -1.0
     return dict_to_params(d)
 
 
@@ -178,6 +178,7 @@ def fit(res, iterations: int, lrs=(1e-3, 3e-4, 1e-4, 3e-5, 1e-5), probe: int = 4
 
 
 def forward(res, feats_dev, y_dev) -> None:
+    """y_dev (rows x YS) = MLP(feats_dev (rows x FEAT)); device tensors."""
     res.use_torch_stream()
     res.check(_lib.lib().pnb_mlp_forward_dev(res.handle, feats_dev.data_ptr(), feats_dev.shape[0],
                                              y_dev.data_ptr()))

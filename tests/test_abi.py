@@ -111,7 +111,7 @@ def test_mlp_parameter_layout_matches_library():
     from particular_amd import learned
 
     L = _lib.lib()
-    assert L.pnb_mlp_num_params() == learned.IN * 64 + 64 + 64 * 64 + 64 + 64 * 3 + 3
+    assert L.pnb_mlp_num_params() == learned.IN * 64 + 64 + 64 * 64 + 64 + 64 * learned.OUT + learned.OUT
     flat = learned.quadrupole_params()
     back = learned.dict_to_params(learned.params_to_dict(flat))
     np.testing.assert_array_equal(back, flat)

@@ -18,7 +18,7 @@ def oracle_params(seed):
     p = M.init_params(seed)
     p["b1"] = np.random.default_rng(seed + 100).normal(0, 0.1, 64)
     p["b2"] = np.random.default_rng(seed + 200).normal(0, 0.1, 64)
-    p["b3"] = np.random.default_rng(seed + 300).normal(0, 0.1, 3)
+    p["b3"] = np.random.default_rng(seed + 300).normal(0, 0.1, M.OUT)
     return p
 
 
@@ -33,15 +33,28 @@ def random_dataset(seed, waves=40, rows_per_wave=50):
     rows = waves * rows_per_wave
     feats = np.zeros((rows, learned.FEAT), np.float32)
     feats[:, :NF] = rng.normal(size=(rows, NF))
+    u = rng.normal(size=(rows, 3))
+    geom = np.zeros((rows, 4), np.float32)
+    geom[:, :3] = 3.0 * u / np.linalg.norm(u, axis=1, keepdims=True)
+    geom[:, 3] = rng.uniform(0.5, 2.0, rows)
+    lanes = np.zeros((waves, 64, 4), np.float32)
+    lanes[..., :3] = rng.normal(size=(waves, 64, 3)) * 0.3
+    lanes[..., 3] = 1e-4
     return {
         "feats": feats,
-        "scale": rng.uniform(0.5, 2.0, rows).astype(np.float32),
+        "geom": geom,
         "mask": rng.integers(0, 2**63, rows, dtype=np.int64).astype(np.uint64),
         "row_wave": np.repeat(np.arange(waves), rows_per_wave).astype(np.uint32),
         "wave_rows": (np.arange(waves + 1) * rows_per_wave).astype(np.uint32),
-        "target": rng.normal(size=(waves, 64, 4)).astype(np.float32),
+        "lanes": lanes,
+        "target": (rng.normal(size=(waves, 64, 4)) * 0.01).astype(np.float32),
         "wgt": rng.uniform(0.1, 1.0, (waves, 64)).astype(np.float32),
     }
+
+
+def oracle_args(d):
+    return (d["row_wave"].astype(np.int64), d["mask"], d["geom"].astype(np.float64),
+            d["lanes"].astype(np.float64))
 
 
 def test_forward_matches_oracle(res):
@@ -51,15 +64,16 @@ def test_forward_matches_oracle(res):
     x = np.zeros((1000, learned.FEAT), np.float32)
     x[:, :NF] = np.random.default_rng(2).normal(size=(1000, NF))
     xd = torch.from_numpy(x).cuda()
-    yd = torch.zeros((1000, 4), dtype=torch.float32, device="cuda")
+    yd = torch.zeros((1000, learned.YS), dtype=torch.float32, device="cuda")
     learned.forward(res, xd, yd)
     torch.cuda.synchronize()
-    y = yd.cpu().numpy()[:, :3]
+    y = yd.cpu().numpy()
     ref, _ = M.forward(p, x[:, :NF].astype(np.float64), bf16=True)
-    err = np.abs(y - ref)
+    err = np.abs(y[:, :M.OUT] - ref)
     scale = np.abs(ref).max()
     print("forward max abs err", err.max(), "scale", scale, "median", np.median(err))
     assert np.median(err) < 1e-4 * scale and err.max() < 3e-2 * scale
+    assert not y[:, M.OUT:].any()
 
 
 def test_gradients_match_oracle(res):
@@ -69,8 +83,7 @@ def test_gradients_match_oracle(res):
     learned.load(res, d)
     loss, g = learned.grad(res)
     ref_loss, ref_g, _ = M.loss_and_grads(p, d["feats"][:, :NF].astype(np.float64),
-                                          d["row_wave"].astype(np.int64), d["mask"],
-                                          d["scale"].astype(np.float64),
+                                          *oracle_args(d),
                                           d["target"][..., :3].astype(np.float64),
                                           d["wgt"].astype(np.float64), bf16=True)
     gd = learned.params_to_dict(g)
@@ -84,11 +97,12 @@ def test_gradients_match_oracle(res):
 
 def test_training_fits_a_teacher(res):
     teacher = oracle_params(5)
+    teacher["W3"] = np.random.default_rng(6).normal(0, 0.2, teacher["W3"].shape)
     d = random_dataset(6, waves=64, rows_per_wave=40)
     y, _ = M.forward(teacher, d["feats"][:, :NF].astype(np.float64))
-    pred = M.predict(y, d["row_wave"].astype(np.int64), d["mask"], d["scale"].astype(np.float64), 64)
+    pred = M.predict(y, *oracle_args(d), 64)
     d["target"][..., :3] = pred
-    d["wgt"][:] = 1.0 / (1.0 + (pred ** 2).sum(-1))
+    d["wgt"][:] = 1.0 / (1e-4 + (pred ** 2).sum(-1))
     learned.init(res, 7)
     learned.clear(res)
     learned.load(res, d)
@@ -108,19 +122,18 @@ def test_collect_rows_and_targets(res):
     assert waves == 16 and rows > 16 and dropped == 0
     d = learned.dataset(res)
     assert np.all(np.diff(d["wave_rows"].astype(np.int64)) >= 0) and d["wave_rows"][-1] == rows
-    assert np.all(d["scale"] > 0) and np.all(d["mask"] != 0)
+    assert np.all(d["geom"][:, 3] > 0) and np.all(d["mask"] != 0)
     f = d["feats"]
-    np.testing.assert_allclose(np.linalg.norm(f[:, :3], axis=1), 1.0, rtol=1e-5)
-    # Accepted internal nodes: some lane at distance d_l >= width / theta accepted the node, and
-    # that lane is within 8 rms radii of the wavefront centre (64 lanes), so the centre distance
-    # d_c >= width / theta - 8 * wrad, i.e. width/d_c <= theta * (1 + 8 * wrad/d_c).
-    assert np.all(f[:, 3] > 0)
-    assert np.all(f[:, 3] <= 0.8 * (1 + 8 * f[:, 5]) * (1 + 1e-5))
     # Every body of a node lies within w of its centre of mass along each axis, so the entries
-    # of Q = (3T - tr(T) I) / (M w^2) are at most 2 (diagonal) and 3 (off-diagonal): its norm is
-    # below sqrt(66) < 9, and so are the quadrupole features
-    assert np.all(np.abs(f[:, 7:13]) <= 9.0)
+    # of Q = (3T - tr(T) I) / (M w^2) are at most 2 (diagonal) and 3 (off-diagonal)
+    assert np.all(np.abs(f[:, 0:3]) <= 2.0) and np.all(np.abs(f[:, 3:6]) <= 3.0)
+    np.testing.assert_allclose(f[:, 0] + f[:, 1] + f[:, 2], 0.0, atol=1e-5)  # traceless
+    assert np.all((f[:, 6] > 0) & (f[:, 6] <= 1.0 + 1e-6))  # mass fraction
+    assert np.all(f[:, 7] >= 1.0 / 32.0 - 1e-6)             # at least two bodies
+    assert np.all((f[:, 9] > 0) & (f[:, 9] <= 1.0 + 1e-6))  # heaviest child's share
+    assert np.all((f[:, 10] >= 0.25 - 1e-6) & (f[:, 10] <= 1.0))  # 2..8 children
     assert not f[:, NF:].any()
+    np.testing.assert_allclose(d["lanes"][..., 3], 0.0)  # softening^2
     # targets are brute force minus Barnes-Hut: |target| relative to |a| is a monopole-sized error
     a_ref = 1.0 / np.sqrt(d["wgt"][d["wgt"] > 0])
     t = np.linalg.norm(d["target"][..., :3], axis=-1)[d["wgt"] > 0]
@@ -130,9 +143,7 @@ def test_collect_rows_and_targets(res):
 
 def cpu_rows(nodes, pts, theta):
     """Rows of one wavefront by the reference's acceptance rule in f32 (sequential.rs:485-494):
-    the internal nodes some lane accepts, in pre-order, with the accepting-lane mask, kept when
-    the wavefront is compact relative to the node (rms radius <= ROW_SPREAD d). Also returns
-    whether a node sits so close to that threshold that f32 and f64 could disagree."""
+    the internal nodes some lane accepts, in pre-order, with the accepting-lane mask."""
     f32 = np.float32
     th2 = f32(theta) * f32(theta)
     com = nodes["com"].astype(np.float32)
@@ -152,19 +163,12 @@ def cpu_rows(nodes, pts, theta):
                 if internal[k]:
                     masks[k] = masks.get(k, 0) | (1 << lane)
                 k = nodes["skip"][k]
-    c, wrad = F.wave_frame(pts[:, :3])
-    rows, borderline = [], False
-    for k in sorted(masks):
-        r = nodes["com"][k].astype(np.float64) - c
-        ratio = wrad * wrad / float(r @ r) / F.ROW_SPREAD**2
-        borderline |= abs(ratio - 1) < 1e-4
-        if ratio <= 1:
-            rows.append((k, masks[k]))
-    return rows, borderline
+    return [(k, masks[k]) for k in sorted(masks)]
 
 
 def test_collected_rows_match_cpu_walk(res):
-    """Row lists and masks exactly as the reference rule implies; features as the oracle has them."""
+    """Row lists and masks exactly as the reference rule implies; node features and geometry as
+    the oracle computes them from the exported tree."""
     import torch
 
     s = workloads.two_plummer_case(2048, seed=45)
@@ -176,25 +180,19 @@ def test_collected_rows_match_cpu_walk(res):
     d = learned.dataset(res)
     nodes, order = pa.tree_export(res, s.n)
     sp = s.particles[order]
-    total = float(nodes["mass"][0])
-    checked = 0
-    for w in (0, 5, 17, 31, 9, 23):
+    for w in (0, 5, 17, 31):
         b, e = int(d["wave_rows"][w]), int(d["wave_rows"][w + 1])
         pts = sp[64 * w:64 * w + 64]
-        want, borderline = cpu_rows(nodes, pts, theta)
-        if borderline:
-            continue
-        checked += 1
-        assert e - b > 0
+        np.testing.assert_array_equal(d["lanes"][w, :, :3], pts[:, :3])
+        want = cpu_rows(nodes, pts, theta)
+        assert e - b == len(want) > 0
         assert [int(m) for m in d["mask"][b:e]] == [m for _, m in want], w
-        c, wrad = F.wave_frame(pts[:, :3])
         for r, (k, m) in zip(range(b, e), want):
             n = nodes[k]
-            f, sc = F.row_features(n["com"], float(n["mass"]), float(n["width"]), n["quad"], c,
-                                   wrad, total, bin(m).count("1"))
-            np.testing.assert_allclose(d["feats"][r, :NF], f, rtol=2e-5, atol=2e-6)
-            assert abs(d["scale"][r] / sc - 1) < 2e-5
-    assert checked >= 4
+            np.testing.assert_allclose(d["feats"][r, :NF], F.node_features(nodes, k),
+                                       rtol=2e-5, atol=2e-6)
+            np.testing.assert_array_equal(d["geom"][r, :3], n["com"])
+            assert abs(d["geom"][r, 3] / (float(n["mass"]) * float(n["width"]) ** 2) - 1) < 1e-6
 
 
 def test_node_quadrupoles_match_particles(res):
@@ -221,12 +219,15 @@ def test_node_quadrupoles_match_particles(res):
 
 
 def test_walk_correction_matches_rows(res):
-    """The corrected walk adds exactly sum(scale * MLP(row)) over each lane's rows."""
+    """The corrected walk adds, for every lane, the field of M w^2 MLP(node) of each internal node
+    it accepted: the oracle's prediction from the recorded rows."""
     import torch
 
     s = workloads.direct_sum_case(8192, seed=43)
     dev = torch.from_numpy(s.particles).cuda()
-    p = set_oracle_params(res, oracle_params(9))
+    p = oracle_params(9)
+    p["W3"] = np.random.default_rng(10).normal(0, 0.2, p["W3"].shape)
+    p = set_oracle_params(res, p)
     # the recorded rows of every wavefront (all 128 waves, in tree order)
     learned.clear(res)
     learned.collect(res, dev, 0.7, waves=s.n // 64, seed=0)
@@ -235,32 +236,32 @@ def test_walk_correction_matches_rows(res):
     nn = pa.BarnesHut(res, 0.7, pa.Acceleration.checked(), mlp=True).compute(s.particles)
     _, order = pa.tree_export(res, s.n)
     y, _ = M.forward(p, d["feats"][:, :NF].astype(np.float64), bf16=True)
-    pred = M.predict(y, d["row_wave"].astype(np.int64), d["mask"], d["scale"].astype(np.float64),
-                     s.n // 64).reshape(-1, 3)
-    corr = (nn - plain)[order]
+    pred = M.predict(y, *oracle_args(d), s.n // 64).reshape(-1, 3)
+    corr = (nn - plain)[order][:, :3]
     scale = np.abs(corr).max()
     err = np.abs(corr - pred)
     print("correction scale", scale, "max err", err.max())
+    assert scale > 0
     assert err.max() < 3e-2 * scale and np.median(err) < 1e-3 * scale
 
 
 def test_quadrupole_network_reduces_error(res, orc):
-    """The network set to the quadrupole term (learned.quadrupole_params) is the quadrupole
-    correction of every accepted node: it must cut the Barnes-Hut error against FP64."""
+    """The network set to output each node's own quadrupole (learned.quadrupole_params) turns the
+    walk into quadrupole Barnes-Hut: it must cut the error against FP64 well below plain."""
     s = workloads.two_plummer_case(32768, seed=47)
     idx = np.random.default_rng(1).choice(s.n, 512, replace=False)
     ref = orc.direct_f64(s.particles[idx], s.particles)
     learned.set_params(res, learned.quadrupole_params())
+
+    def rms(a):
+        e = np.linalg.norm(a[idx, :3] - ref, axis=1) / np.linalg.norm(ref, axis=1)
+        return float(np.sqrt(np.mean(e**2)))
+
     for theta in (0.5, 0.8):
         plain = pa.BarnesHut(res, theta, pa.Acceleration.checked()).compute(s.particles)
         quad = pa.BarnesHut(res, theta, pa.Acceleration.checked(), mlp=True).compute(s.particles)
-
-        def rms(a):
-            e = np.linalg.norm(a[idx, :3] - ref, axis=1) / np.linalg.norm(ref, axis=1)
-            return float(np.sqrt(np.mean(e**2)))
-
         print(f"theta {theta}: plain rms {rms(plain):.3e}, quadrupole network rms {rms(quad):.3e}")
-        assert rms(quad) < 0.6 * rms(plain)
+        assert rms(quad) < 0.5 * rms(plain)
 
 
 def test_untrained_network_leaves_walk_unchanged(res):

@@ -7,16 +7,23 @@ from oracle import mlp_oracle as M
 
 
 def problem(seed=0, waves=3, rows_per_wave=9):
+    """Rows of nodes a few units from lanes clustered near the origin."""
     rng = np.random.default_rng(seed)
     rows = waves * rows_per_wave
     x = rng.normal(size=(rows, M.IN))
     wave = np.repeat(np.arange(waves), rows_per_wave)
     mask = rng.integers(0, 2**63, rows, dtype=np.int64).astype(np.uint64) | np.uint64(1)
-    scale = rng.uniform(0.5, 2.0, rows)
-    target = rng.normal(size=(waves, 64, M.OUT))
+    geom = np.zeros((rows, 4))
+    u = rng.normal(size=(rows, 3))
+    geom[:, :3] = 3.0 * u / np.linalg.norm(u, axis=1, keepdims=True)
+    geom[:, 3] = rng.uniform(0.5, 2.0, rows)  # M w^2
+    lanes = np.zeros((waves, 64, 4))
+    lanes[..., :3] = rng.normal(size=(waves, 64, 3)) * 0.3
+    lanes[..., 3] = 1e-4
+    target = rng.normal(size=(waves, 64, 3)) * 0.01
     wgt = rng.uniform(0.0, 1.0, (waves, 64))
     wgt[0, :5] = 0.0
-    return x, wave, mask, scale, target, wgt
+    return x, wave, mask, geom, lanes, target, wgt
 
 
 def test_bf16_rounding():
@@ -29,24 +36,29 @@ def test_bf16_rounding():
 
 
 def test_aggregation_matches_loop():
-    x, wave, mask, scale, target, wgt = problem(1)
+    from oracle import features as F
+
+    x, wave, mask, geom, lanes, target, wgt = problem(1)
     p = M.init_params(1)
+    p["W3"] = np.random.default_rng(5).normal(size=p["W3"].shape)
     y, _ = M.forward(p, x)
-    pred = M.predict(y, wave, mask, scale, target.shape[0])
+    pred = M.predict(y, wave, mask, geom, lanes, target.shape[0])
     ref = np.zeros_like(pred)
     for r in range(x.shape[0]):
         for lane in range(64):
             if (int(mask[r]) >> lane) & 1:
-                ref[wave[r], lane] += scale[r] * y[r]
-    np.testing.assert_allclose(pred, ref, rtol=1e-13, atol=1e-13)
+                lp = lanes[wave[r], lane]
+                ref[wave[r], lane] += F.quad_field(geom[r, 3] * y[r], geom[r, :3] - lp[:3], lp[3])
+    assert np.abs(ref).max() > 0
+    np.testing.assert_allclose(pred, ref, rtol=1e-12, atol=1e-13)
 
 
 def test_gradients_match_finite_differences():
-    x, wave, mask, scale, target, wgt = problem(2)
+    x, wave, mask, geom, lanes, target, wgt = problem(2)
     p = M.init_params(2)
     p["b1"] += 0.05  # keep pre-activations away from the ReLU kink for the probes
     p["b2"] += 0.05
-    _, g, _ = M.loss_and_grads(p, x, wave, mask, scale, target, wgt)
+    _, g, _ = M.loss_and_grads(p, x, wave, mask, geom, lanes, target, wgt)
     rng = np.random.default_rng(3)
     h = 1e-6
     for name in ("W1", "b1", "W2", "b2", "W3", "b3"):
@@ -54,21 +66,21 @@ def test_gradients_match_finite_differences():
             idx = tuple(rng.integers(0, s) for s in p[name].shape)
             q = {k: v.copy() for k, v in p.items()}
             q[name][idx] += h
-            lp, _, _ = M.loss_and_grads(q, x, wave, mask, scale, target, wgt)
+            lp, _, _ = M.loss_and_grads(q, x, wave, mask, geom, lanes, target, wgt)
             q[name][idx] -= 2 * h
-            lm, _, _ = M.loss_and_grads(q, x, wave, mask, scale, target, wgt)
+            lm, _, _ = M.loss_and_grads(q, x, wave, mask, geom, lanes, target, wgt)
             fd = (lp - lm) / (2 * h)
             assert abs(fd - g[name][idx]) <= 1e-6 * max(1.0, abs(fd)), (name, idx, fd, g[name][idx])
 
 
 def test_adam_reduces_loss():
-    x, wave, mask, scale, target, wgt = problem(4)
+    x, wave, mask, geom, lanes, target, wgt = problem(4)
     target *= 0.01
     p = M.init_params(4)
     st = {}
-    l0, g, _ = M.loss_and_grads(p, x, wave, mask, scale, target, wgt)
+    l0, g, _ = M.loss_and_grads(p, x, wave, mask, geom, lanes, target, wgt)
     for _ in range(60):
-        _, g, _ = M.loss_and_grads(p, x, wave, mask, scale, target, wgt)
+        _, g, _ = M.loss_and_grads(p, x, wave, mask, geom, lanes, target, wgt)
         p = M.adam_step(p, g, st, lr=3e-3)
-    l1, _, _ = M.loss_and_grads(p, x, wave, mask, scale, target, wgt)
+    l1, _, _ = M.loss_and_grads(p, x, wave, mask, geom, lanes, target, wgt)
     assert l1 < 0.5 * l0
