@@ -31,6 +31,7 @@
 // reaches skip[k]. Each lane therefore applies exactly the reference's decisions; node records
 // are wave-uniform, so they arrive through scalar loads.
 #include "pnb_internal.hpp"
+#include <hip/hip_fp16.h>
 #include "mlp_common.hpp"
 
 #include <algorithm>
@@ -59,6 +60,7 @@ struct __align__(32) NodeRec {
     uint32_t begin, end;     // particle range in tree order
 };
 static_assert(sizeof(NodeRec) == 32, "node record is 32 bytes");
+
 
 // Mass-weighted second moments of a node about its FP64 centre of mass,
 // T = sum m (x - c)(x - c)^T, as {xx, yy, zz, xy, xz, yz}; built for the learned correction only.
@@ -95,7 +97,7 @@ struct BhState {
     const uint32_t* sidx = nullptr;   // tree order -> original index
     DevBuf keys, keys_alt, idx, idx_alt, sp, lcp, cnt, ip, key2, rec, sums, sums2, level, width;
     DevBuf mom, quad, desc;  // second moments (FP64), normalised quadrupoles, child shares
-    DevBuf qeff;             // per node: M w^2 MLP(node), the learned correction's moment
+    DevBuf qh;               // per node: MLP(node) as halves, the learned correction's moment
     bool has_moments = false;
     DevBuf info, bbox_part, hist, bits, afg_tmp;
     DevBuf akeys, akeys_alt, aidx, aidx_alt, apts, h_aff, h_afg, h_out;
@@ -109,7 +111,7 @@ void bh_free(BhState* s) {
     if (!s) return;
     for (DevBuf* b : {&s->keys, &s->keys_alt, &s->idx, &s->idx_alt, &s->sp, &s->lcp, &s->cnt,
                       &s->ip, &s->key2, &s->rec, &s->sums, &s->sums2, &s->level, &s->width,
-                      &s->mom, &s->quad, &s->desc, &s->qeff,
+                      &s->mom, &s->quad, &s->desc, &s->qh,
                       &s->info, &s->bbox_part, &s->hist, &s->bits, &s->afg_tmp, &s->akeys,
                       &s->akeys_alt, &s->aidx, &s->aidx_alt, &s->apts, &s->h_aff, &s->h_afg,
                       &s->h_out, &s->c_starts, &s->c_pts, &s->c_bh, &s->c_bf, &s->c_list,
@@ -610,16 +612,23 @@ __device__ __forceinline__ void node_features(const NodeRec& r, const NodeQuad& 
     f[10] = dsc.children;
 }
 
-// qeff[k] = M w^2 MLP(features of node k) for every node (zero for leaves): 16 nodes per MFMA
-// tile, one tile per wavefront and iteration, weights staged in LDS.
+// The learned moment of a node, y = MLP(node) in units of the node's M w^2, as six halves
+// (12 bytes). y is O(1) and the correction it feeds is itself a small fraction of the force, so
+// 11-bit mantissas cost nothing measurable, while the walk's scalar loads (which bound it) grow
+// by 12 bytes instead of 24 per corrected step.
+struct __align__(4) QuadH {
+    __half q[6];
+};
+
+// qh[k] = MLP(features of node k) for every node (zero for leaves and massless nodes): 16 nodes
+// per MFMA tile, one tile per wavefront and iteration, weights staged in LDS.
 __global__ __launch_bounds__(256) void mlp_node_kernel(const NodeRec* __restrict__ rec,
                                                        const NodeQuad* __restrict__ quad,
                                                        const uint8_t* __restrict__ level,
                                                        const NodeDesc* __restrict__ desc,
-                                                       const float* __restrict__ widths,
                                                        uint32_t total,
                                                        const MlpWeights* __restrict__ wg,
-                                                       NodeQuad* __restrict__ qeff) {
+                                                       QuadH* __restrict__ qh) {
     __shared__ MlpForward w;
     {
         const uint32_t* src = reinterpret_cast<const uint32_t*>(wg);
@@ -635,10 +644,8 @@ __global__ __launch_bounds__(256) void mlp_node_kernel(const NodeRec* __restrict
         float f[MLP_KIN];
         NodeRec r{};
         r.w2 = -INFINITY;
-        float wd = 0.f;
         if (k < total) {
             r = rec[k];
-            wd = widths[k];
             node_features(r, quad[k], level[k], desc[k], root_mass, f);
         } else {
 #pragma unroll
@@ -651,11 +658,15 @@ __global__ __launch_bounds__(256) void mlp_node_kernel(const NodeRec* __restrict
             bx[q] = short(f2bf(v));
         }
         const f32x4_t yv = mlp_forward16_regs(w, bx);  // lanes of group g: outputs 4 g .. 4 g + 3
-        const bool internal = r.w2 >= 0.0f && r.mass > 0.0f;
-        const float s = internal ? r.mass * wd * wd : 0.f;
-        if (k < total) {
-            if (g == 0) qeff[k].a = make_float4(s * yv[0], s * yv[1], s * yv[2], s * yv[3]);
-            if (g == 1) qeff[k].b = make_float4(s * yv[0], s * yv[1], 0.f, 0.f);
+        const float keep = (r.w2 >= 0.0f && r.mass > 0.0f) ? 1.f : 0.f;
+        if (k < total && g < 2) {
+            __half* dst = qh[k].q + 4 * g;
+            dst[0] = __float2half(keep * yv[0]);
+            dst[1] = __float2half(keep * yv[1]);
+            if (g == 0) {
+                dst[2] = __float2half(keep * yv[2]);
+                dst[3] = __float2half(keep * yv[3]);
+            }
         }
     }
 }
@@ -664,7 +675,7 @@ __global__ __launch_bounds__(256) void mlp_node_kernel(const NodeRec* __restrict
 
 constexpr int WALK_PLAIN = 0;  // plain Barnes-Hut
 constexpr int WALK_ROWS = 1;   // also list the internal nodes each wavefront accepted (rows)
-constexpr int WALK_QUAD = 2;   // add quad_field(qeff[k]) for every accepted internal node
+constexpr int WALK_QUAD = 2;   // add quad_field(q) of every accepted internal node
 
 // Per wavefront, the internal nodes that at least one of its lanes accepted, in walk order:
 // {node index, accepting-lane count, accepting-lane mask (low, high 32 bits)}.
@@ -674,6 +685,11 @@ struct WalkRows {
     uint32_t cap;
 };
 
+// One wavefront walks the pre-order node records with one cursor (k, in SGPRs); lane decisions
+// are the reference's (see the file comment). The walk is bound by the throughput of its scalar
+// loads (bytes per step), so each step loads what it needs: the records of both possible next
+// nodes (k + 1, skip) are requested while node k is evaluated, and WALK_QUAD fetches the learned
+// moment of node k only when some lane accepts it as an internal node.
 template <int MODE>
 __global__ __launch_bounds__(256) void bh_walk_kernel(const NodeRec* __restrict__ nodes,
                                                       uint32_t total,
@@ -682,7 +698,7 @@ __global__ __launch_bounds__(256) void bh_walk_kernel(const NodeRec* __restrict_
                                                       uint32_t n, float theta2, float eps2,
                                                       float4* __restrict__ out,
                                                       BhInfo* __restrict__ info, WalkRows rows,
-                                                      const NodeQuad* __restrict__ qeff) {
+                                                      const QuadH* __restrict__ qh) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63;
     const bool valid = i < n;
@@ -693,23 +709,12 @@ __global__ __launch_bounds__(256) void bh_walk_kernel(const NodeRec* __restrict_
     uint32_t accepted = 0, steps = 0, nrows = 0;
     const uint32_t wave_slot = i / 64;
 
-    // The cursor's two possible successors (k + 1 when some lane opens, skip otherwise) are
-    // loaded while node k is evaluated, so the scalar-load latency overlaps the VALU work.
     uint32_t k = 0;
     NodeRec r{};
-    NodeQuad qk{};
-    if (total > 0) {  // no tree (m = 0): nodes may not even be allocated
-        r = nodes[0];
-        if (MODE == WALK_QUAD) qk = qeff[0];
-    }
+    if (total > 0) r = nodes[0];  // no tree (m = 0): nodes may not even be allocated
     while (k < total) {
         const NodeRec r_next = nodes[min(k + 1, total - 1)];
         const NodeRec r_skip = nodes[min(r.skip, total - 1)];
-        NodeQuad q_next{}, q_skip{};
-        if (MODE == WALK_QUAD) {
-            q_next = qeff[min(k + 1, total - 1)];
-            q_skip = qeff[min(r.skip, total - 1)];
-        }
         const bool active = k >= resume;
         const float dx = sub_rn(r.cx, p.x), dy = sub_rn(r.cy, p.y), dz = sub_rn(r.cz, p.z);
         const float d2 = norm2_ref(dx, dy, dz);
@@ -729,7 +734,11 @@ __global__ __launch_bounds__(256) void bh_walk_kernel(const NodeRec* __restrict_
             cz = (t - az) - y; az = t;
             ++accepted;
             if (MODE == WALK_QUAD && r.w2 >= 0.0f) {
-                const float q[6] = {qk.a.x, qk.a.y, qk.a.z, qk.a.w, qk.b.x, qk.b.y};
+                const QuadH h = qh[k];
+                const float mw2 = r.mass * r.w2;  // w2 = width^2 of an internal node
+                const float q[6] = {mw2 * __half2float(h.q[0]), mw2 * __half2float(h.q[1]),
+                                    mw2 * __half2float(h.q[2]), mw2 * __half2float(h.q[3]),
+                                    mw2 * __half2float(h.q[4]), mw2 * __half2float(h.q[5])};
                 quad_field(q, dx, dy, dz, inv, kx, ky, kz);
             }
         }
@@ -746,7 +755,6 @@ __global__ __launch_bounds__(256) void bh_walk_kernel(const NodeRec* __restrict_
         ++steps;
         const uint32_t nxt = __builtin_amdgcn_readfirstlane(__any(open) ? k + 1 : max(r.skip, k + 1));
         r = (nxt == k + 1) ? r_next : r_skip;
-        if (MODE == WALK_QUAD) qk = (nxt == k + 1) ? q_next : q_skip;
         k = nxt;
     }
     if (MODE == WALK_ROWS && lane == 0 && wave_slot * 64 < n) {  // wavefronts holding particles
@@ -868,6 +876,7 @@ static int build_tree(pnb_ctx* c, const float4* afg, size_t m, bool ref_com = fa
             bh_ref_com_kernel<<<div_up(total, 256), 256, 0, c->stream>>>(rec, s->sp.as<float4>(), total);
             PNB_CHECK_LAUNCH(c);
         }
+
         s->built = true;
         return PNB_OK;
     });
@@ -875,40 +884,48 @@ static int build_tree(pnb_ctx* c, const float4* afg, size_t m, bool ref_com = fa
 
 const MlpWeights* mlp_weights(pnb_ctx* c);  // mlp.hip: nullptr until initialised
 
+// quad: walk the WalkRecQ records (learned correction) instead of the plain ones.
 static int walk(pnb_ctx* c, const float4* pts, const uint32_t* perm, size_t n, float theta,
-                float softening, float4* out, const WalkRows* rows = nullptr,
-                const NodeQuad* qeff = nullptr) {
+                float softening, float4* out, const WalkRows* rows = nullptr, bool quad = false) {
     BhState* s = c->bh;
     const float theta2 = theta * theta;  // (self.theta * self.theta), sequential.rs:491
     const float eps2 = softening * softening;
     return timed(c, PNB_PROF_BH_WALK, [&]() -> int {
         const unsigned grid = div_up(n, 256);
-        auto* kern = rows ? bh_walk_kernel<WALK_ROWS>
-                     : qeff ? bh_walk_kernel<WALK_QUAD> : bh_walk_kernel<WALK_PLAIN>;
-        kern<<<grid, 256, 0, c->stream>>>(s->rec.as<NodeRec>(), uint32_t(s->nodes), pts, perm,
-                                          uint32_t(n), theta2, eps2, out, s->info.as<BhInfo>(),
-                                          rows ? *rows : WalkRows{}, qeff);
+        const uint32_t total = uint32_t(s->nodes);
+        BhInfo* info = s->info.as<BhInfo>();
+        const NodeRec* nr = s->rec.as<NodeRec>();
+        if (quad) {
+            bh_walk_kernel<WALK_QUAD><<<grid, 256, 0, c->stream>>>(
+                nr, total, pts, perm, uint32_t(n), theta2, eps2, out, info, WalkRows{},
+                s->qh.as<QuadH>());
+        } else if (rows) {
+            bh_walk_kernel<WALK_ROWS><<<grid, 256, 0, c->stream>>>(
+                nr, total, pts, perm, uint32_t(n), theta2, eps2, out, info, *rows, nullptr);
+        } else {
+            bh_walk_kernel<WALK_PLAIN><<<grid, 256, 0, c->stream>>>(
+                nr, total, pts, perm, uint32_t(n), theta2, eps2, out, info, WalkRows{}, nullptr);
+        }
         PNB_CHECK_LAUNCH(c);
         return PNB_OK;
     });
 }
 
-// The learned correction's per-node moments: qeff[k] = M w^2 MLP(node k) for the current tree.
+// The learned correction's per-node moments: qh[k] = MLP(node k) for the current tree.
 static int node_network(pnb_ctx* c) {
     BhState* s = c->bh;
     const MlpWeights* wg = mlp_weights(c);
     if (!wg) return fail(c, PNB_E_STATE, "PNB_BH_MLP needs an initialised MLP (pnb_mlp_init)");
     if (!s->has_moments) return fail(c, PNB_E_STATE, "octree built without node moments");
     const size_t total = s->nodes;
-    PNB_HIP(c, s->qeff.ensure(std::max<size_t>(total, 1) * sizeof(NodeQuad)));
+    PNB_HIP(c, s->qh.ensure(std::max<size_t>(total, 1) * sizeof(QuadH)));
     if (total == 0) return PNB_OK;
     return timed(c, PNB_PROF_MLP, [&]() -> int {
         const size_t tiles = div_up(total, 16);
         const unsigned grid = unsigned(std::min<size_t>(div_up(tiles, 4), 16 * size_t(std::max(c->cu_count, 1))));
         mlp_node_kernel<<<grid, 256, 0, c->stream>>>(s->rec.as<NodeRec>(), s->quad.as<NodeQuad>(),
                                                      s->level.as<uint8_t>(), s->desc.as<NodeDesc>(),
-                                                     s->width.as<float>(), uint32_t(total), wg,
-                                                     s->qeff.as<NodeQuad>());
+                                                     uint32_t(total), wg, s->qh.as<QuadH>());
         PNB_CHECK_LAUNCH(c);
         return PNB_OK;
     });
@@ -925,13 +942,11 @@ static int accel_dev(pnb_ctx* c, const float4* affected, size_t n, const float4*
         return fail(c, PNB_E_STATE, "PNB_BH_MLP needs an initialised MLP (pnb_mlp_init)");
     if (int rc = build_tree(c, afg, m, ref_com, mlp)) return rc;
     BhState* s = c->bh;
-    const NodeQuad* qeff = nullptr;
-    if (mlp && m) {
+    const bool quad = mlp && m;
+    if (quad)
         if (int rc = node_network(c)) return rc;
-        qeff = s->qeff.as<NodeQuad>();
-    }
     if (affected == afg && n == m)
-        return walk(c, s->sp.as<float4>(), s->sidx, n, theta, softening, out, nullptr, qeff);
+        return walk(c, s->sp.as<float4>(), s->sidx, n, theta, softening, out, nullptr, quad);
     const unsigned g = div_up(n, 256);
     PNB_HIP(c, s->akeys.ensure(n * 8));
     PNB_HIP(c, s->akeys_alt.ensure(n * 8));
@@ -953,7 +968,7 @@ static int accel_dev(pnb_ctx* c, const float4* affected, size_t n, const float4*
         return rc;
     bh_gather_kernel<<<g, 256, 0, c->stream>>>(affected, n, si, s->apts.as<float4>());
     PNB_CHECK_LAUNCH(c);
-    return walk(c, s->apts.as<float4>(), si, n, theta, softening, out, nullptr, qeff);
+    return walk(c, s->apts.as<float4>(), si, n, theta, softening, out, nullptr, quad);
 }
 
 static int copy_h2d(pnb_ctx* c, DevBuf& b, const float* src, size_t count) {

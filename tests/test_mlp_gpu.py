@@ -131,7 +131,7 @@ def test_collect_rows_and_targets(res):
     assert np.all((f[:, 6] > 0) & (f[:, 6] <= 1.0 + 1e-6))  # mass fraction
     assert np.all(f[:, 7] >= 1.0 / 32.0 - 1e-6)             # at least two bodies
     assert np.all((f[:, 9] > 0) & (f[:, 9] <= 1.0 + 1e-6))  # heaviest child's share
-    assert np.all((f[:, 10] >= 0.25 - 1e-6) & (f[:, 10] <= 1.0))  # 2..8 children
+    assert np.all((f[:, 10] >= 0.125 - 1e-6) & (f[:, 10] <= 1.0))  # 1..8 children
     assert not f[:, NF:].any()
     np.testing.assert_allclose(d["lanes"][..., 3], 0.0)  # softening^2
     # targets are brute force minus Barnes-Hut: |target| relative to |a| is a monopole-sized error
