@@ -185,14 +185,29 @@ def cpu_baseline_bf(s, sample: int):
     }
 
 
+def time_at_error(rows, rms):
+    """Time per evaluation at a given rms error on a (rms, ms) curve, log-log interpolation;
+    None outside the measured range."""
+    e = np.log([r["rms_rel"] for r in rows])
+    t = np.log([r["ms_per_eval"] for r in rows])
+    o = np.argsort(e)
+    x = np.log(rms)
+    if not (e.min() <= x <= e.max()):
+        return None
+    return float(np.exp(np.interp(x, e[o], t[o])))
+
+
 def bench_bh(args):
-    """Barnes-Hut on two Plummer spheres (4,194,304 bodies, mu = 1/N, theta 0.3 / 0.5 / 0.8):
-    tree build + walk device time from HIP events, accepted pair terms and wavefront node steps
-    from the walk's counters, accuracy against the FP64 direct sum on a seeded sample."""
+    """Barnes-Hut on two Plummer spheres (4,194,304 bodies, mu = 1/N): plain at theta 0.2 ... 0.8
+    and with the learned correction at theta 0.3 / 0.4 (the network that outputs each node's own
+    quadrupole, learned.quadrupole_params). Tree build + walk (+ node network) device time from
+    HIP events, accepted pair terms and wavefront node steps from the walk's counters, accuracy
+    against the FP64 direct sum on a seeded sample, and the particles/s ratio of the corrected to
+    the plain method at equal rms error (1e-4), interpolated on both curves."""
     import torch
 
     import particular_amd as pa
-    from particular_amd import _lib, metrics, workloads
+    from particular_amd import _lib, learned, metrics, workloads
 
     log(f"generating two-Plummer workload ({args.bh_n})")
     s = workloads.two_plummer_case(args.bh_n)
@@ -207,9 +222,11 @@ def bench_bh(args):
                                             s.n, 0.0, ref.data_ptr()))
     torch.cuda.synchronize()
     ref = ref.cpu().numpy()
-    out = []
-    for theta in (0.3, 0.5, 0.8):
-        alg = pa.BarnesHut(res, theta, pa.Acceleration.checked())
+    learned.init(res, 0)
+    learned.set_params(res, learned.quadrupole_params())
+
+    def run(theta, mlp):
+        alg = pa.BarnesHut(res, theta, pa.Acceleration.checked(), mlp=mlp)
         acc = alg.compute(pa.Between(dev, dev))
         torch.cuda.synchronize()
         reps = 3
@@ -222,19 +239,23 @@ def bench_bh(args):
         build_ms, nb = res.profile_read(_lib.PROF_BH_BUILD)
         walk_ms, nw = res.profile_read(_lib.PROF_BH_WALK)
         sort_ms, ns = res.profile_read(_lib.PROF_SORT)
+        net_ms, nn = res.profile_read(_lib.PROF_MLP)
         res.profile(False)
         steps, accepted = pa.bh_stats(res)
         build_ms, walk_ms, sort_ms = build_ms / nb, walk_ms / nw, sort_ms / max(ns, 1)
+        net_ms = net_ms / max(nn, 1)
         nodes, _ = pa.tree_export(res)[0].shape[0], None
         err = metrics.summary(acc.cpu().numpy()[idx], ref)
         # the walk's counters accumulate over the reps since the last build: one build per call
         row = {
             "theta": theta,
+            "learned_correction": mlp,
             "mparticles_per_s": s.n / wall / 1e6,
             "ms_per_eval": wall * 1e3,
             "build_ms": build_ms,
             "sort_ms": sort_ms,
             "walk_ms": walk_ms,
+            "node_network_ms": net_ms if mlp else 0.0,
             "nodes": int(nodes),
             "accepted_terms_per_particle": accepted / s.n,
             "wave_steps_per_wave": steps / (s.n / 64),
@@ -242,11 +263,29 @@ def bench_bh(args):
             "accepted_G_terms_per_s": accepted / (walk_ms * 1e-3) / 1e9,
             **err,
         }
-        log(f"BH theta={theta}: {row['ms_per_eval']:.2f} ms ({row['mparticles_per_s']:.1f} Mp/s), "
-            f"build {build_ms:.2f} ms, walk {walk_ms:.2f} ms, rms {err['rms_rel']:.2e}")
-        out.append(row)
+        log(f"BH{' + correction' if mlp else ''} theta={theta}: {row['ms_per_eval']:.2f} ms "
+            f"({row['mparticles_per_s']:.1f} Mp/s), build {build_ms:.2f} ms, walk {walk_ms:.2f} ms, "
+            f"rms {err['rms_rel']:.2e}")
+        return row
+
+    plain = [run(float(th), False) for th in args.bh_thetas.split(",") if th]
+    nn = [run(float(th), True) for th in args.bh_nn_thetas.split(",") if th]
     res.close()
-    return out, s
+    target = 1e-4
+    t_plain = time_at_error(plain, target) if plain else None
+    t_nn = time_at_error(nn, target) if nn else None
+    comparison = {
+        "target_rms": target,
+        "plain_ms_at_target": t_plain,
+        "corrected_ms_at_target": t_nn,
+        "plain_mparticles_per_s_at_target": s.n / t_plain / 1e3 if t_plain else None,
+        "corrected_mparticles_per_s_at_target": s.n / t_nn / 1e3 if t_nn else None,
+        "speedup_at_equal_error": (t_plain / t_nn) if (t_plain and t_nn) else None,
+        "network": "learned.quadrupole_params (outputs each node's own quadrupole; on held-out "
+                   "validation the trained networks did not beat it, see particular_amd/nn_bh.py)",
+    }
+    log(f"equal error {target}: plain {t_plain} ms, corrected {t_nn} ms")
+    return plain + nn, comparison, s
 
 
 def cpu_baseline_bh(s, theta, sample):
@@ -317,6 +356,9 @@ def main():
     ap.add_argument("--bh", action="store_true", default=True)
     ap.add_argument("--no-bh", dest="bh", action="store_false")
     ap.add_argument("--bh-n", type=int, default=4194304)
+    ap.add_argument("--bh-thetas", default="0.2,0.25,0.3,0.5,0.8", help="plain Barnes-Hut angles")
+    ap.add_argument("--bh-nn-thetas", default="0.3,0.4",
+                    help="angles with the learned correction (empty: none)")
     ap.add_argument("--bh-cpu-sample", type=int, default=16384)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="first copy a buffer of this size with torch (HBM counter calibration)")
@@ -339,11 +381,12 @@ def main():
     result, s = bench_bf(args)
     result["cpu_baseline"] = None if args.no_cpu else cpu_baseline_bf(s, args.cpu_sample)
     if args.bh:
-        bh_rows, s3 = bench_bh(args)
+        bh_rows, comparison, s3 = bench_bh(args)
         result["barnes_hut"] = {
             "workload": f"two Plummer spheres, {s3.n} bodies (scale 1, centres x = -5/+5, "
                         "bulk v_x = +-0.3), mu = 1/N, Acceleration::checked, FP64 centres of mass",
             "rows": bh_rows,
+            "corrected_vs_plain": comparison,
             "cpu_baseline": None if args.no_cpu else cpu_baseline_bh(s3, 0.5, args.bh_cpu_sample),
         }
     if args.batch:

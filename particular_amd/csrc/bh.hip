@@ -734,12 +734,11 @@ __global__ __launch_bounds__(256) void bh_walk_kernel(const NodeRec* __restrict_
             cz = (t - az) - y; az = t;
             ++accepted;
             if (MODE == WALK_QUAD && r.w2 >= 0.0f) {
+                // field of M w^2 y (w2 = width^2 of an internal node): M w^2 folds into |d|^-5
                 const QuadH h = qh[k];
-                const float mw2 = r.mass * r.w2;  // w2 = width^2 of an internal node
-                const float q[6] = {mw2 * __half2float(h.q[0]), mw2 * __half2float(h.q[1]),
-                                    mw2 * __half2float(h.q[2]), mw2 * __half2float(h.q[3]),
-                                    mw2 * __half2float(h.q[4]), mw2 * __half2float(h.q[5])};
-                quad_field(q, dx, dy, dz, inv, kx, ky, kz);
+                const float y[6] = {__half2float(h.q[0]), __half2float(h.q[1]), __half2float(h.q[2]),
+                                    __half2float(h.q[3]), __half2float(h.q[4]), __half2float(h.q[5])};
+                quad_field(y, dx, dy, dz, inv, r.mass * r.w2, kx, ky, kz);
             }
         }
         if (MODE == WALK_ROWS && r.w2 >= 0.0f) {  // internal node: a row if some lane accepted it

@@ -144,8 +144,8 @@ __global__ __launch_bounds__(64) void mlp_lane_kernel(const float* __restrict__ 
             const float inv = rsqrtf(dx * dx + dy * dy + dz * dz + p.w);
             float q[6];
 #pragma unroll
-            for (int j = 0; j < 6; ++j) q[j] = gm.w * y[size_t(r) * MLP_YS + j];
-            quad_field(q, dx, dy, dz, inv, px, py, pz);
+            for (int j = 0; j < 6; ++j) q[j] = y[size_t(r) * MLP_YS + j];
+            quad_field(q, dx, dy, dz, inv, gm.w, px, py, pz);
         }
     }
     const size_t tl = w * 64 + lane;

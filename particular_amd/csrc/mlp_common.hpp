@@ -173,22 +173,21 @@ __device__ __forceinline__ void mlp_forward16(const MlpForward& w, MlpTile& t, f
     wave_lds_sync();
 }
 
-// Field of an effective second moment at a point: q = {xx, yy, zz, xy, xz, yz} in absolute units
-// (M w^2 folded in), d = centre of mass - point, inv = 1 / |d| (softened like the monopole term).
-// The quadrupole term of the multipole expansion, a = -Q d |d|^-5 + 2.5 (d.Q.d) d |d|^-7, is
-// added to (ax, ay, az).
+// Field of a second moment Q = scale * q at a point: q = {xx, yy, zz, xy, xz, yz}, d = centre of
+// mass - point, inv = 1 / |d| (softened like the monopole term). The quadrupole term of the
+// multipole expansion, a = -Q d |d|^-5 + 2.5 (d.Q.d) d |d|^-7, is added to (ax, ay, az).
 __device__ __forceinline__ void quad_field(const float (&q)[6], float dx, float dy, float dz,
-                                           float inv, float& ax, float& ay, float& az) {
+                                           float inv, float scale, float& ax, float& ay, float& az) {
     const float qx = q[0] * dx + q[3] * dy + q[4] * dz;
     const float qy = q[3] * dx + q[1] * dy + q[5] * dz;
     const float qz = q[4] * dx + q[5] * dy + q[2] * dz;
     const float dqd = dx * qx + dy * qy + dz * qz;
     const float inv2 = inv * inv;
-    const float inv5 = inv2 * inv2 * inv;
+    const float s5 = scale * (inv2 * inv2 * inv);
     const float t = 2.5f * dqd * inv2;
-    ax += inv5 * (t * dx - qx);
-    ay += inv5 * (t * dy - qy);
-    az += inv5 * (t * dz - qz);
+    ax += s5 * (t * dx - qx);
+    ay += s5 * (t * dy - qy);
+    az += s5 * (t * dz - qz);
 }
 
 // g . (d a / d q_j) for the six components of quad_field's q, added to g6.

@@ -5,7 +5,8 @@
    brute-force-minus-Barnes-Hut targets generated on the GPU, at opening angles cycling through
    `--train-thetas`.
 2. Full-batch Adam on the GPU (`--iters`), learning rate picked by short probes and stepped
-   down by 10x twice (particular_amd.learned.fit).
+   down by 10x twice (particular_amd.learned.fit); the trained network replaces the starting one
+   only if it also lowers the error on a separate validation snapshot.
 3. A held-out snapshot: plain Barnes-Hut at `--thetas` and NN-corrected Barnes-Hut at
    `--nn-thetas` (with the starting network as well when it is the quadrupole one), each timed (tree build + walk [+ correction], device time from HIP events) and
    checked against the FP64 direct sum on a seeded sample. Reports particles/s and error of each
@@ -92,7 +93,7 @@ def main():
     ap.add_argument("--snapshots", type=int, default=64)
     ap.add_argument("--rotations", type=int, default=8, help="rotations per seed")
     ap.add_argument("--waves", type=int, default=128)
-    ap.add_argument("--train-thetas", default="0.8,0.6,0.45")
+    ap.add_argument("--train-thetas", default="0.3,0.4,0.5,0.6,0.8")
     ap.add_argument("--iters", type=int, default=600)
     ap.add_argument("--sample", type=int, default=16384)
     ap.add_argument("--thetas", default="0.15,0.2,0.25,0.3,0.4,0.5,0.6,0.8")
@@ -147,12 +148,33 @@ def main():
 
     nn_thetas = [float(x) for x in a.nn_thetas.split(",")]
     start = [run(th, True) for th in nn_thetas] if a.init == "quadrupole" else []
+
+    # validation snapshot (its own seed): the trained network is kept only if it beats the
+    # starting one there, at the middle training angle
+    pv = snapshot(a.n, 8888, 1)
+    dv = torch.from_numpy(pv).cuda()
+    iv = np.random.default_rng(6).choice(a.n, a.sample, replace=False)
+    rv = fp64_reference(res, dv, torch.from_numpy(iv).cuda(), a.sample)
+    th_val = sorted(train_thetas)[len(train_thetas) // 2]
+
+    def val_error():
+        out = pa.BarnesHut(res, th_val, pa.Acceleration.checked(), mlp=True).compute(pa.Between(dv, dv))
+        return metrics.summary(out.cpu().numpy()[iv], rv)["rms_rel"]
+
+    p_start = learned.get_params(res)
+    val_start = val_error()
     res.profile(True)
     fit = learned.fit(res, a.iters)
     train_ms, _ = res.profile_read(_lib.PROF_MLP)
     res.profile(False)
+    val_end = val_error()
+    kept = bool(fit["kept"] and val_end < val_start)
+    if not kept:
+        learned.set_params(res, p_start)
     log(f"trained {a.iters} iterations at lr {fit['lr']}: loss {fit['loss_start']:.4e} -> "
-        f"{fit['loss_end']:.4e} ({train_ms:.0f} ms), kept {fit['kept']}")
+        f"{fit['loss_end']:.4e} ({train_ms:.0f} ms); validation rms at theta {th_val}: "
+        f"{val_start:.4e} -> {val_end:.4e}; trained network kept: {kept}")
+    del dv
     if a.params_out:
         np.save(a.params_out, learned.get_params(res))
 
@@ -171,8 +193,9 @@ def main():
         "training": {"snapshots": k, "waves_per_snapshot": a.waves, "thetas": train_thetas,
                      "rows": rows, "lane_targets": waves * 64, "dropped_rows": dropped,
                      "iterations": a.iters, "lr": fit["lr"], "loss_start": float(fit["loss_start"]),
-                     "loss_end": float(fit["loss_end"]), "trained_network_kept": fit["kept"],
-                     "train_ms": train_ms},
+                     "loss_end": float(fit["loss_end"]), "validation_theta": th_val,
+                     "validation_rms_start": val_start, "validation_rms_trained": val_end,
+                     "trained_network_kept": kept, "train_ms": train_ms},
         "init": a.init,
         "plain": plain,
         "nn_start": start,
