@@ -164,6 +164,23 @@ int pnb_bh_tree_export(pnb_ctx *ctx, pnb_tree_node *nodes, size_t cap, uint32_t 
 /* Counters of the walks since the last build: wavefront node steps and accepted pair terms. */
 int pnb_bh_stats(pnb_ctx *ctx, uint64_t *wave_steps, uint64_t *accepted);
 
+/* Two-dimensional particles (glam::Vec2 / f32; the reference's 2D GPU layout,
+ * particular/src/gravity/impls/mod.rs:381-418): records {x, y, mu} (12 bytes, the repr(C)
+ * GravitationalField<Vec2, f32>), outputs {ax, ay} (8 bytes). Same semantics as the 3D entry
+ * points (brute force: sequential.rs:181-194; Barnes-Hut on the reference's quadtree:
+ * sequential.rs:456-506, flags as pnb_bh_accel_f32). Computed on the 3D path with z = 0, which
+ * reproduces the 2D arithmetic and the quadtree exactly. */
+int pnb_bf_accel_f32_2d(pnb_ctx *ctx, const float *affected, size_t n, const float *affecting,
+                        size_t m, float softening, int checked, float *out);
+int pnb_bf_accel_f32_2d_dev(pnb_ctx *ctx, const float *affected, size_t n, const float *affecting,
+                            size_t m, float softening, int checked, float *out);
+int pnb_bh_accel_f32_2d(pnb_ctx *ctx, const float *affected, size_t n, const float *affecting,
+                        size_t m, float theta, float softening, int checked, unsigned flags,
+                        float *out);
+int pnb_bh_accel_f32_2d_dev(pnb_ctx *ctx, const float *affected, size_t n, const float *affecting,
+                            size_t m, float theta, float softening, int checked, unsigned flags,
+                            float *out);
+
 /* FP64 brute force for glam::DVec3 / f64 particles (CPU-only in the reference:
  * particular/src/gravity/impls/glam.rs:234): records are 4 doubles {x, y, z, mu} (32 B); the
  * reference's scalar kernel in FP64 with IEEE sqrt and division (impls/mod.rs:151-166). */

@@ -82,7 +82,7 @@ class AccelerationSoftened:
 
 def _is_affecting(particles) -> np.ndarray:
     """GravitationalField::is_affecting: mu != 0 (particular/src/gravity/mod.rs:27-34)."""
-    return np.asarray(particles)[:, 3] != 0
+    return np.asarray(particles)[:, -1] != 0  # mu is the last field (3D {x,y,z,mu}, 2D {x,y,mu})
 
 
 class Ordered:
@@ -94,7 +94,7 @@ class Ordered:
 
     @staticmethod
     def new(unordered, is_affecting_fn: Callable | None = None) -> "Ordered":
-        unordered = _as_host(unordered)
+        unordered = _as_host(unordered, allow_2d=True)
         mask = _is_affecting(unordered) if is_affecting_fn is None else np.asarray(
             [bool(is_affecting_fn(p)) for p in unordered])
         parts = np.concatenate([unordered[mask], unordered[~mask]])
@@ -227,12 +227,13 @@ def _is_torch(x) -> bool:
     return type(x).__module__.startswith("torch")
 
 
-def _as_host(x) -> np.ndarray:
+def _as_host(x, allow_2d: bool = False) -> np.ndarray:
     if _is_torch(x):
         x = x.detach().cpu().numpy()
     a = np.ascontiguousarray(x, dtype=np.float32)
-    if a.ndim != 2 or a.shape[1] != 4:
-        raise ValueError("particles must be an (n, 4) float32 array {x, y, z, mu}")
+    if a.ndim != 2 or not (a.shape[1] == 4 or (allow_2d and a.shape[1] == 3)):
+        raise ValueError("particles must be an (n, 4) float32 array {x, y, z, mu}"
+                         + (" or (n, 3) {x, y, mu}" if allow_2d else ""))
     return a
 
 
@@ -269,6 +270,8 @@ class _Algorithm:
 
     def compute(self, storage):
         kind, affected, affecting = _lower(storage)
+        if affected.shape[1] == 3:  # {x, y, mu}: two-dimensional particles
+            return self._compute_2d(kind, affected, affecting)
         dev = _is_torch(affected) and affected.is_cuda
         if dev:
             import torch
@@ -290,6 +293,32 @@ class _Algorithm:
             self._between_host(affected, _as_host(affecting), out)
         return out[:, :3]
 
+    def _compute_2d(self, kind, affected, affecting):
+        """Two-dimensional particles {x, y, mu} (n x 3): accelerations (n x 2). The Reordered
+        split (mu != 0) is taken here, then Between(affected, affecting)."""
+        if _is_torch(affected) and affected.is_cuda:
+            import torch
+
+            a = affected.to(torch.float32).contiguous()
+            b = a[a[:, 2] != 0].contiguous() if kind == "reordered" else \
+                affecting.to(torch.float32).contiguous()
+            out = torch.empty((a.shape[0], 2), dtype=torch.float32, device=a.device)
+            self.resources.use_torch_stream()
+            self._between_2d(a.data_ptr(), a.shape[0], b.data_ptr(), b.shape[0], out.data_ptr(),
+                             dev=True)
+            return out
+        a = np.ascontiguousarray(affected, np.float32)
+        if kind == "reordered":
+            b = np.ascontiguousarray(a[a[:, 2] != 0])
+        elif affecting is affected:
+            b = a
+        else:
+            b = np.ascontiguousarray(affecting, np.float32)
+        out = np.zeros((a.shape[0], 2), np.float32)
+        self._between_2d(a.ctypes.data, a.shape[0], b.ctypes.data, b.shape[0], out.ctypes.data,
+                         dev=False)
+        return out
+
 
 class BruteForce(_Algorithm):
     """Brute force on the GPU: replaces gpu::BruteForce (particular/src/gpu/mod.rs:149-208).
@@ -301,7 +330,8 @@ class BruteForce(_Algorithm):
 
     def compute(self, storage):
         kind, affected, affecting = _lower(storage)
-        if (not _is_torch(affected)) and np.asarray(affected).dtype == np.float64:
+        if ((not _is_torch(affected)) and np.asarray(affected).dtype == np.float64
+                and np.asarray(affected).shape[1] == 4):
             a = np.ascontiguousarray(affected, np.float64)
             if kind == "reordered":
                 b = np.ascontiguousarray(a[a[:, 3] != 0])
@@ -336,6 +366,11 @@ class BruteForce(_Algorithm):
         eps, chk = self._params()
         self.resources.check(_lib.lib().pnb_reordered_bf_accel_f32_dev(
             self.resources.handle, a.data_ptr(), a.shape[0], eps, chk, out.data_ptr()))
+
+    def _between_2d(self, a, n, b, m, out, dev):
+        eps, chk = self._params()
+        fn = _lib.lib().pnb_bf_accel_f32_2d_dev if dev else _lib.lib().pnb_bf_accel_f32_2d
+        self.resources.check(fn(self.resources.handle, a, n, b, m, eps, chk, out))
 
 
 class BarnesHut(_Algorithm):
@@ -381,6 +416,12 @@ class BarnesHut(_Algorithm):
         self.resources.check(_lib.lib().pnb_reordered_bh_accel_f32_dev(
             self.resources.handle, a.data_ptr(), a.shape[0], self.theta, eps, chk,
             self._flags(), out.data_ptr()))
+
+    def _between_2d(self, a, n, b, m, out, dev):
+        eps, chk = self._params()
+        fn = _lib.lib().pnb_bh_accel_f32_2d_dev if dev else _lib.lib().pnb_bh_accel_f32_2d
+        self.resources.check(fn(self.resources.handle, a, n, b, m, self.theta, eps, chk,
+                                self._flags(), out))
 
 
 def tree_export(resources: HipResources, m: int | None = None):

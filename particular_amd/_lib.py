@@ -61,6 +61,10 @@ SIGNATURES = {
     "pnb_direct_f64_dev": (_i, [_vp, _vp, _sz, _vp, _sz, _d, _vp]),
     "pnb_bf_accel_f64": (_i, [_vp, _vp, _sz, _vp, _sz, _d, _i, _vp]),
     "pnb_bf_accel_f64_dev": (_i, [_vp, _vp, _sz, _vp, _sz, _d, _i, _vp]),
+    "pnb_bf_accel_f32_2d": (_i, [_vp, _vp, _sz, _vp, _sz, _f, _i, _vp]),
+    "pnb_bf_accel_f32_2d_dev": (_i, [_vp, _vp, _sz, _vp, _sz, _f, _i, _vp]),
+    "pnb_bh_accel_f32_2d": (_i, [_vp, _vp, _sz, _vp, _sz, _f, _f, _i, _u32, _vp]),
+    "pnb_bh_accel_f32_2d_dev": (_i, [_vp, _vp, _sz, _vp, _sz, _f, _f, _i, _u32, _vp]),
     "pnb_bh_accel_f32": (_i, [_vp, _vp, _sz, _vp, _sz, _f, _f, _i, _u32, _vp]),
     "pnb_bh_accel_f32_dev": (_i, [_vp, _vp, _sz, _vp, _sz, _f, _f, _i, _u32, _vp]),
     "pnb_reordered_bh_accel_f32": (_i, [_vp, _vp, _sz, _f, _f, _i, _u32, _vp]),
