@@ -36,6 +36,13 @@ FLAGS = [
     f"-I{os.path.join(ROOT, 'include')}",
 ]
 
+# Per-file additions. bh.hip: the Barnes-Hut walk requests the records of both possible next nodes
+# at the top of each step so that their latency overlaps the step's arithmetic; machine sinking
+# would move those scalar loads down to their only use at the end of the step.
+FILE_FLAGS = {
+    "bh.hip": ["-mllvm", "-disable-machine-sink"],
+}
+
 
 def _headers():
     return glob.glob(os.path.join(CSRC, "*.hpp")) + glob.glob(os.path.join(ROOT, "include", "*.h"))
@@ -46,7 +53,7 @@ def _compile(src: str, force: bool, verbose: bool) -> str:
     newest = max([os.path.getmtime(src)] + [os.path.getmtime(h) for h in _headers()])
     if not force and os.path.exists(obj) and os.path.getmtime(obj) >= newest:
         return obj
-    cmd = [HIPCC, *FLAGS, "-c", src, "-o", obj]
+    cmd = [HIPCC, *FLAGS, *FILE_FLAGS.get(os.path.basename(src), []), "-c", src, "-o", obj]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)

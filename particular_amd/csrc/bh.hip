@@ -612,12 +612,12 @@ __device__ __forceinline__ void node_features(const NodeRec& r, const NodeQuad& 
     f[10] = dsc.children;
 }
 
-// The learned moment of a node, y = MLP(node) in units of the node's M w^2, as six halves
-// (12 bytes). y is O(1) and the correction it feeds is itself a small fraction of the force, so
-// 11-bit mantissas cost nothing measurable, while the walk's scalar loads (which bound it) grow
-// by 12 bytes instead of 24 per corrected step.
-struct __align__(4) QuadH {
-    __half q[6];
+// The learned moment of a node, y = MLP(node) in units of the node's M w^2, as six halves and
+// two zero halves of padding (16 bytes, one scalar load when a lane accepts the node). y is O(1)
+// and the correction it feeds is itself a small fraction of the force, so 11-bit mantissas cost
+// nothing measurable, while the walk's scalar loads grow by half of what floats would take.
+struct __align__(16) QuadH {
+    __half q[8];
 };
 
 // qh[k] = MLP(features of node k) for every node (zero for leaves and massless nodes): 16 nodes
@@ -663,10 +663,8 @@ __global__ __launch_bounds__(256) void mlp_node_kernel(const NodeRec* __restrict
             __half* dst = qh[k].q + 4 * g;
             dst[0] = __float2half(keep * yv[0]);
             dst[1] = __float2half(keep * yv[1]);
-            if (g == 0) {
-                dst[2] = __float2half(keep * yv[2]);
-                dst[3] = __float2half(keep * yv[3]);
-            }
+            dst[2] = __float2half(g == 0 ? keep * yv[2] : 0.f);  // q[6], q[7]: padding
+            dst[3] = __float2half(g == 0 ? keep * yv[3] : 0.f);
         }
     }
 }
@@ -684,6 +682,53 @@ struct WalkRows {
     uint32_t* count;  // [wave] rows found; only the first cap are stored
     uint32_t cap;
 };
+
+// Advances the wavefront's cursor: (k, r) = (open != 0) ? (k + 1, rn) : (r.skip, rs), with the
+// operands and results in SGPRs (one scalar compare and four scalar selects). Left to itself the
+// compiler copies both candidate records into VGPRs and selects them with v_cndmask.
+__device__ __forceinline__ uint64_t pack2(uint32_t lo, uint32_t hi) {
+    return uint64_t(lo) | (uint64_t(hi) << 32);
+}
+
+__device__ __forceinline__ void advance(uint64_t open, uint32_t& k, NodeRec& r, const NodeRec& rn,
+                                        const NodeRec& rs) {
+    const uint64_t n0 = pack2(__float_as_uint(rn.cx), __float_as_uint(rn.cy));
+    const uint64_t s0 = pack2(__float_as_uint(rs.cx), __float_as_uint(rs.cy));
+    const uint64_t n1 = pack2(__float_as_uint(rn.cz), __float_as_uint(rn.mass));
+    const uint64_t s1 = pack2(__float_as_uint(rs.cz), __float_as_uint(rs.mass));
+    const uint64_t n2 = pack2(__float_as_uint(rn.w2), rn.skip);
+    const uint64_t s2 = pack2(__float_as_uint(rs.w2), rs.skip);
+    // begin/end are not needed, but naming them makes each record one 32-byte scalar load
+    const uint64_t n3 = pack2(rn.begin, rn.end), s3 = pack2(rs.begin, rs.end);
+    const uint32_t kn = k + 1, ks = r.skip;
+    uint64_t o0, o1, o2;
+    uint32_t ok;
+    asm volatile(
+        "s_cmp_lg_u64 %4, 0\n\t"
+        "s_cselect_b64 %0, %5, %6\n\t"
+        "s_cselect_b64 %1, %7, %8\n\t"
+        "s_cselect_b64 %2, %9, %10\n\t"
+        "s_cselect_b32 %3, %11, %12"
+        : "=&s"(o0), "=&s"(o1), "=&s"(o2), "=&s"(ok)
+        : "s"(open), "s"(n0), "s"(s0), "s"(n1), "s"(s1), "s"(n2), "s"(s2), "s"(kn), "s"(ks),
+          "s"(n3), "s"(s3)
+        : "scc");
+    r.cx = __uint_as_float(uint32_t(o0));
+    r.cy = __uint_as_float(uint32_t(o0 >> 32));
+    r.cz = __uint_as_float(uint32_t(o1));
+    r.mass = __uint_as_float(uint32_t(o1 >> 32));
+    r.w2 = __uint_as_float(uint32_t(o2));
+    r.skip = uint32_t(o2 >> 32);
+    k = ok;
+}
+
+__device__ __forceinline__ float half_lo(uint32_t v) {
+    return __half2float(__ushort_as_half(uint16_t(v)));
+}
+__device__ __forceinline__ float half_hi(uint32_t v) {
+    return __half2float(__ushort_as_half(uint16_t(v >> 16)));
+}
+
 
 // One wavefront walks the pre-order node records with one cursor (k, in SGPRs); lane decisions
 // are the reference's (see the file comment). The walk is bound by the throughput of its scalar
@@ -704,17 +749,18 @@ __global__ __launch_bounds__(256) void bh_walk_kernel(const NodeRec* __restrict_
     const bool valid = i < n;
     const float4 p = pts[valid ? i : n - 1];
     float ax = 0.f, ay = 0.f, az = 0.f, cx = 0.f, cy = 0.f, cz = 0.f;
-    float kx = 0.f, ky = 0.f, kz = 0.f;  // learned correction (WALK_QUAD)
     uint32_t resume = valid ? 0u : 0xffffffffu;
     uint32_t accepted = 0, steps = 0, nrows = 0;
     const uint32_t wave_slot = i / 64;
 
     uint32_t k = 0;
     NodeRec r{};
+    const uint4* q4 = reinterpret_cast<const uint4*>(qh);
     if (total > 0) r = nodes[0];  // no tree (m = 0): nodes may not even be allocated
     while (k < total) {
-        const NodeRec r_next = nodes[min(k + 1, total - 1)];
-        const NodeRec r_skip = nodes[min(r.skip, total - 1)];
+        // in bounds: k + 1 <= total and skip <= total, and the array holds total + 1 records
+        const NodeRec r_next = nodes[k + 1];
+        const NodeRec r_skip = nodes[r.skip];
         const bool active = k >= resume;
         const float dx = sub_rn(r.cx, p.x), dy = sub_rn(r.cy, p.y), dz = sub_rn(r.cz, p.z);
         const float d2 = norm2_ref(dx, dy, dz);
@@ -722,24 +768,44 @@ __global__ __launch_bounds__(256) void bh_walk_kernel(const NodeRec* __restrict_
         const bool crit = mul_rn(theta2, d2) < r.w2;
         const bool open = active && nz && crit;
         const bool acc = active && nz && !crit;
+        // the wavefront's open mask from the three compares' own lane masks (a ballot of the
+        // combined bool would first copy it into a VGPR)
+        const uint64_t open_mask = __builtin_amdgcn_ballot_w64(active) &
+                                   __builtin_amdgcn_ballot_w64(nz) &
+                                   __builtin_amdgcn_fcmpf(mul_rn(theta2, d2), r.w2, 4 /*olt*/);
         if (acc) {
             const float r2 = add_rn(d2, eps2);
             const float inv = __builtin_amdgcn_rsqf(r2);
             const float s = (r.mass * inv) * (inv * inv);
-            float y = fmaf(dx, s, -cx), t = ax + y;
-            cx = (t - ax) - y; ax = t;
-            y = fmaf(dy, s, -cy); t = ay + y;
-            cy = (t - ay) - y; ay = t;
-            y = fmaf(dz, s, -cz); t = az + y;
-            cz = (t - az) - y; az = t;
-            ++accepted;
+            float yx, yy, yz;
             if (MODE == WALK_QUAD && r.w2 >= 0.0f) {
-                // field of M w^2 y (w2 = width^2 of an internal node): M w^2 folds into |d|^-5
-                const QuadH h = qh[k];
-                const float y[6] = {__half2float(h.q[0]), __half2float(h.q[1]), __half2float(h.q[2]),
-                                    __half2float(h.q[3]), __half2float(h.q[4]), __half2float(h.q[5])};
-                quad_field(y, dx, dy, dz, inv, r.mass * r.w2, kx, ky, kz);
+                // internal node: the monopole plus the field of Q = M w^2 y (quad_field's
+                // formula, w2 = width^2), summed as d (s + s5 t) - s5 Q d in one compensated term
+                const uint4 h = q4[k];
+                const float q[6] = {half_lo(h.x), half_hi(h.x), half_lo(h.y),
+                                    half_hi(h.y), half_lo(h.z), half_hi(h.z)};
+                const float qx = q[0] * dx + q[3] * dy + q[4] * dz;
+                const float qy = q[3] * dx + q[1] * dy + q[5] * dz;
+                const float qz = q[4] * dx + q[5] * dy + q[2] * dz;
+                const float dqd = dx * qx + dy * qy + dz * qz;
+                const float inv2 = inv * inv;
+                const float s5 = (r.mass * r.w2) * (inv2 * inv2 * inv);
+                const float st = fmaf(s5, 2.5f * dqd * inv2, s);
+                yx = fmaf(-s5, qx, fmaf(dx, st, -cx));
+                yy = fmaf(-s5, qy, fmaf(dy, st, -cy));
+                yz = fmaf(-s5, qz, fmaf(dz, st, -cz));
+            } else {
+                yx = fmaf(dx, s, -cx);
+                yy = fmaf(dy, s, -cy);
+                yz = fmaf(dz, s, -cz);
             }
+            float t = ax + yx;
+            cx = (t - ax) - yx; ax = t;
+            t = ay + yy;
+            cy = (t - ay) - yy; ay = t;
+            t = az + yz;
+            cz = (t - az) - yz; az = t;
+            ++accepted;
         }
         if (MODE == WALK_ROWS && r.w2 >= 0.0f) {  // internal node: a row if some lane accepted it
             const unsigned long long m = __ballot(acc);
@@ -752,15 +818,14 @@ __global__ __launch_bounds__(256) void bh_walk_kernel(const NodeRec* __restrict_
         }
         if (active && !open) resume = r.skip;
         ++steps;
-        const uint32_t nxt = __builtin_amdgcn_readfirstlane(__any(open) ? k + 1 : max(r.skip, k + 1));
-        r = (nxt == k + 1) ? r_next : r_skip;
-        k = nxt;
+        // skip > k for every node, and skip = k + 1 for a leaf, so the two candidates agree there
+        advance(open_mask, k, r, r_next, r_skip);
     }
     if (MODE == WALK_ROWS && lane == 0 && wave_slot * 64 < n) {  // wavefronts holding particles
         rows.count[wave_slot] = nrows;
         atomicMax(&info->max_rows, nrows);
     }
-    if (valid) out[perm ? perm[i] : i] = make_float4(ax + kx, ay + ky, az + kz, 0.f);
+    if (valid) out[perm ? perm[i] : i] = make_float4(ax, ay, az, 0.f);
     unsigned long long acc64 = accepted;
     for (int off = 32; off > 0; off >>= 1) acc64 += __shfl_xor(acc64, off, 64);
     if (lane == 0) {
@@ -841,7 +906,8 @@ static int build_tree(pnb_ctx* c, const float4* afg, size_t m, bool ref_com = fa
             return fail(c, PNB_E_DEPTH, "octree deeper than %d levels", MAX_LEVEL);
         s->nodes = total;
         s->max_level = h.max_level;
-        PNB_HIP(c, s->rec.ensure(size_t(total) * sizeof(NodeRec)));
+        // one record past the last node: the walk prefetches records k + 1 and skip <= total
+        PNB_HIP(c, s->rec.ensure(size_t(total + 1) * sizeof(NodeRec)));
         PNB_HIP(c, s->sums.ensure(size_t(total) * sizeof(double4)));
         PNB_HIP(c, s->sums2.ensure(size_t(total) * sizeof(double4)));
         PNB_HIP(c, s->level.ensure(total));
@@ -883,7 +949,7 @@ static int build_tree(pnb_ctx* c, const float4* afg, size_t m, bool ref_com = fa
 
 const MlpWeights* mlp_weights(pnb_ctx* c);  // mlp.hip: nullptr until initialised
 
-// quad: walk the WalkRecQ records (learned correction) instead of the plain ones.
+// quad: add the learned correction (the per-node moments in s->qh).
 static int walk(pnb_ctx* c, const float4* pts, const uint32_t* perm, size_t n, float theta,
                 float softening, float4* out, const WalkRows* rows = nullptr, bool quad = false) {
     BhState* s = c->bh;
@@ -917,7 +983,8 @@ static int node_network(pnb_ctx* c) {
     if (!wg) return fail(c, PNB_E_STATE, "PNB_BH_MLP needs an initialised MLP (pnb_mlp_init)");
     if (!s->has_moments) return fail(c, PNB_E_STATE, "octree built without node moments");
     const size_t total = s->nodes;
-    PNB_HIP(c, s->qh.ensure(std::max<size_t>(total, 1) * sizeof(QuadH)));
+    // one entry past the last node, as for the records (the walk prefetches k + 1 and skip)
+    PNB_HIP(c, s->qh.ensure((size_t(total) + 1) * sizeof(QuadH)));
     if (total == 0) return PNB_OK;
     return timed(c, PNB_PROF_MLP, [&]() -> int {
         const size_t tiles = div_up(total, 16);
