@@ -683,15 +683,15 @@ struct WalkRows {
     uint32_t cap;
 };
 
-// Advances the wavefront's cursor: (k, r) = (open != 0) ? (k + 1, rn) : (r.skip, rs), with the
+// Advances the wavefront's cursor: (k, r) = (open != 0) ? (kn, rn) : (ks, rs), with the
 // operands and results in SGPRs (one scalar compare and four scalar selects). Left to itself the
 // compiler copies both candidate records into VGPRs and selects them with v_cndmask.
 __device__ __forceinline__ uint64_t pack2(uint32_t lo, uint32_t hi) {
     return uint64_t(lo) | (uint64_t(hi) << 32);
 }
 
-__device__ __forceinline__ void advance(uint64_t open, uint32_t& k, NodeRec& r, const NodeRec& rn,
-                                        const NodeRec& rs) {
+__device__ __forceinline__ void advance(uint64_t open, uint32_t kn, uint32_t ks, uint32_t& k,
+                                        NodeRec& r, const NodeRec& rn, const NodeRec& rs) {
     const uint64_t n0 = pack2(__float_as_uint(rn.cx), __float_as_uint(rn.cy));
     const uint64_t s0 = pack2(__float_as_uint(rs.cx), __float_as_uint(rs.cy));
     const uint64_t n1 = pack2(__float_as_uint(rn.cz), __float_as_uint(rn.mass));
@@ -700,7 +700,6 @@ __device__ __forceinline__ void advance(uint64_t open, uint32_t& k, NodeRec& r, 
     const uint64_t s2 = pack2(__float_as_uint(rs.w2), rs.skip);
     // begin/end are not needed, but naming them makes each record one 32-byte scalar load
     const uint64_t n3 = pack2(rn.begin, rn.end), s3 = pack2(rs.begin, rs.end);
-    const uint32_t kn = k + 1, ks = r.skip;
     uint64_t o0, o1, o2;
     uint32_t ok;
     asm volatile(
@@ -753,15 +752,19 @@ __global__ __launch_bounds__(256) void bh_walk_kernel(const NodeRec* __restrict_
     uint32_t accepted = 0, steps = 0, nrows = 0;
     const uint32_t wave_slot = i / 64;
 
-    uint32_t k = 0;
+    // The cursor is a byte offset, ko = 32 k (the host keeps 32 (total + 1) below 2^32), so each
+    // record address is the array base plus one 32-bit scalar offset.
+    const char* base = reinterpret_cast<const char*>(nodes);
+    const uint32_t end = total * uint32_t(sizeof(NodeRec));
+    uint32_t ko = 0;
     NodeRec r{};
-    const uint4* q4 = reinterpret_cast<const uint4*>(qh);
     if (total > 0) r = nodes[0];  // no tree (m = 0): nodes may not even be allocated
-    while (k < total) {
+    while (ko < end) {
         // in bounds: k + 1 <= total and skip <= total, and the array holds total + 1 records
-        const NodeRec r_next = nodes[k + 1];
-        const NodeRec r_skip = nodes[r.skip];
-        const bool active = k >= resume;
+        const uint32_t so = r.skip * uint32_t(sizeof(NodeRec));
+        const NodeRec r_next = *reinterpret_cast<const NodeRec*>(base + (ko + 32u));
+        const NodeRec r_skip = *reinterpret_cast<const NodeRec*>(base + so);
+        const bool active = ko >= resume;
         const float dx = sub_rn(r.cx, p.x), dy = sub_rn(r.cy, p.y), dz = sub_rn(r.cz, p.z);
         const float d2 = norm2_ref(dx, dy, dz);
         const bool nz = d2 != 0.0f;
@@ -781,7 +784,8 @@ __global__ __launch_bounds__(256) void bh_walk_kernel(const NodeRec* __restrict_
             if (MODE == WALK_QUAD && r.w2 >= 0.0f) {
                 // internal node: the monopole plus the field of Q = M w^2 y (quad_field's
                 // formula, w2 = width^2), summed as d (s + s5 t) - s5 Q d in one compensated term
-                const uint4 h = q4[k];
+                const uint4 h = *reinterpret_cast<const uint4*>(
+                    reinterpret_cast<const char*>(qh) + (ko >> 1));  // 16-byte entries
                 const float q[6] = {half_lo(h.x), half_hi(h.x), half_lo(h.y),
                                     half_hi(h.y), half_lo(h.z), half_hi(h.z)};
                 const float qx = q[0] * dx + q[3] * dy + q[4] * dz;
@@ -812,14 +816,14 @@ __global__ __launch_bounds__(256) void bh_walk_kernel(const NodeRec* __restrict_
             if (m) {
                 if (lane == 0 && nrows < rows.cap)
                     rows.rows[size_t(wave_slot) * rows.cap + nrows] =
-                        make_uint4(k, uint32_t(__popcll(m)), uint32_t(m), uint32_t(m >> 32));
+                        make_uint4(ko >> 5, uint32_t(__popcll(m)), uint32_t(m), uint32_t(m >> 32));
                 ++nrows;
             }
         }
-        if (active && !open) resume = r.skip;
+        if (active && !open) resume = so;
         ++steps;
         // skip > k for every node, and skip = k + 1 for a leaf, so the two candidates agree there
-        advance(open_mask, k, r, r_next, r_skip);
+        advance(open_mask, ko + 32u, so, ko, r, r_next, r_skip);
     }
     if (MODE == WALK_ROWS && lane == 0 && wave_slot * 64 < n) {  // wavefronts holding particles
         rows.count[wave_slot] = nrows;
@@ -904,6 +908,10 @@ static int build_tree(pnb_ctx* c, const float4* afg, size_t m, bool ref_com = fa
         if (h.flags & FLAG_NONFINITE) return fail(c, PNB_E_ARG, "non-finite affecting position");
         if (h.flags & FLAG_DEPTH)
             return fail(c, PNB_E_DEPTH, "octree deeper than %d levels", MAX_LEVEL);
+        // the walk addresses records by 32-bit byte offsets, up to and including record total
+        if (uint64_t(total) + 1 > (uint64_t(1) << 27))
+            return fail(c, PNB_E_ARG, "octree of %u nodes exceeds the walk's limit of 2^27 - 1",
+                        total);
         s->nodes = total;
         s->max_level = h.max_level;
         // one record past the last node: the walk prefetches records k + 1 and skip <= total
