@@ -788,12 +788,13 @@ __global__ __launch_bounds__(256) void bh_walk_kernel(const NodeRec* __restrict_
                     reinterpret_cast<const char*>(qh) + (ko >> 1));  // 16-byte entries
                 const float q[6] = {half_lo(h.x), half_hi(h.x), half_lo(h.y),
                                     half_hi(h.y), half_lo(h.z), half_hi(h.z)};
-                const float qx = q[0] * dx + q[3] * dy + q[4] * dz;
-                const float qy = q[3] * dx + q[1] * dy + q[5] * dz;
-                const float qz = q[4] * dx + q[5] * dy + q[2] * dz;
+                // fma chains from +0 so that every half-precision product is one v_fma_mix
+                const float qx = fmaf(q[4], dz, fmaf(q[3], dy, fmaf(q[0], dx, 0.f)));
+                const float qy = fmaf(q[5], dz, fmaf(q[1], dy, fmaf(q[3], dx, 0.f)));
+                const float qz = fmaf(q[2], dz, fmaf(q[5], dy, fmaf(q[4], dx, 0.f)));
                 const float dqd = dx * qx + dy * qy + dz * qz;
                 const float inv2 = inv * inv;
-                const float s5 = (r.mass * r.w2) * (inv2 * inv2 * inv);
+                const float s5 = s * (r.w2 * inv2);  // M w^2 |d|^-5 from the monopole's M |d|^-3
                 const float st = fmaf(s5, 2.5f * dqd * inv2, s);
                 yx = fmaf(-s5, qx, fmaf(dx, st, -cx));
                 yy = fmaf(-s5, qy, fmaf(dy, st, -cy));
