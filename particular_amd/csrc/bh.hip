@@ -821,7 +821,9 @@ __global__ __launch_bounds__(256) void bh_walk_kernel(const NodeRec* __restrict_
                 ++nrows;
             }
         }
-        if (active && !open) resume = so;
+        // active && !open: resume at skip. An inactive lane sits inside a skipped subtree that
+        // contains node k, so its resume point is already >= skip(k) and the max keeps it.
+        if (!open) resume = max(resume, so);
         ++steps;
         // skip > k for every node, and skip = k + 1 for a leaf, so the two candidates agree there
         advance(open_mask, ko + 32u, so, ko, r, r_next, r_skip);
