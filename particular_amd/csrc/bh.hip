@@ -86,7 +86,13 @@ struct BhInfo {
     uint32_t total;     // node count (S[M])
     uint32_t max_rows;  // most rows one wavefront listed in the last row-listing walk
     unsigned long long steps, accepted;  // traversal counters of the last walk
+    uint32_t lvl_count[MAX_LEVEL + 2];   // internal nodes per level
 };
+
+// Per-level counters that many workgroups update: one 128-byte line each, so that atomics on
+// different levels do not queue behind each other in one L2 line.
+constexpr int LVL_STRIDE = 32;  // uint32 per counter
+constexpr int LVL_SLOTS = MAX_LEVEL + 2;
 
 struct BhState {
     size_t m = 0;
@@ -96,6 +102,8 @@ struct BhState {
     const uint64_t* skeys = nullptr;  // sorted keys (points into keys or keys_alt)
     const uint32_t* sidx = nullptr;   // tree order -> original index
     DevBuf keys, keys_alt, idx, idx_alt, sp, lcp, cnt, ip, key2, rec, sums, sums2, level, width;
+    DevBuf ilist;            // internal nodes grouped by level (bottom-up pass)
+    DevBuf lvl;              // padded per-level counts and list cursors (2 x LVL_SLOTS lines)
     DevBuf mom, quad, desc;  // second moments (FP64), normalised quadrupoles, child shares
     DevBuf qh;               // per node: MLP(node) as halves, the learned correction's moment
     bool has_moments = false;
@@ -111,7 +119,7 @@ void bh_free(BhState* s) {
     if (!s) return;
     for (DevBuf* b : {&s->keys, &s->keys_alt, &s->idx, &s->idx_alt, &s->sp, &s->lcp, &s->cnt,
                       &s->ip, &s->key2, &s->rec, &s->sums, &s->sums2, &s->level, &s->width,
-                      &s->mom, &s->quad, &s->desc, &s->qh,
+                      &s->ilist, &s->lvl, &s->mom, &s->quad, &s->desc, &s->qh,
                       &s->info, &s->bbox_part, &s->hist, &s->bits, &s->afg_tmp, &s->akeys,
                       &s->akeys_alt, &s->aidx, &s->aidx_alt, &s->apts, &s->h_aff, &s->h_afg,
                       &s->h_out, &s->c_starts, &s->c_pts, &s->c_bh, &s->c_bf, &s->c_list,
@@ -222,13 +230,17 @@ __global__ void bh_root_kernel(const float* __restrict__ part, int blocks, BhInf
     }
 }
 
-__global__ void bh_info_init_kernel(BhInfo* info) {
-    info->flags = 0;
-    info->max_level = -1;
-    info->total = 0;
-    info->max_rows = 0;
-    info->steps = 0;
-    info->accepted = 0;
+__global__ void bh_info_init_kernel(BhInfo* info, uint32_t* __restrict__ lvl) {
+    for (int l = threadIdx.x; l < 2 * LVL_SLOTS; l += blockDim.x) lvl[l * LVL_STRIDE] = 0;
+    if (threadIdx.x == 0) {
+        info->flags = 0;
+        info->max_level = -1;
+        info->total = 0;
+        info->max_rows = 0;
+        info->steps = 0;
+        info->accepted = 0;
+    }
+    for (int l = threadIdx.x; l < MAX_LEVEL + 2; l += blockDim.x) info->lvl_count[l] = 0;
 }
 
 // ---- 2. keys ------------------------------------------------------------------------------------
@@ -369,22 +381,117 @@ __global__ __launch_bounds__(256) void bh_deep_kernel(const uint64_t* __restrict
 __global__ __launch_bounds__(256) void bh_count_kernel(const int* __restrict__ lcp, size_t m,
                                                        uint32_t* __restrict__ cnt,
                                                        int* __restrict__ ip,
-                                                       BhInfo* __restrict__ info) {
+                                                       BhInfo* __restrict__ info,
+                                                       uint32_t* __restrict__ lvl_count) {
+    __shared__ uint32_t hist[LVL_SLOTS];
+    for (int l = threadIdx.x; l < LVL_SLOTS; l += blockDim.x) hist[l] = 0;
+    __syncthreads();
     const size_t p = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (p >= m) return;
-    const int lp = lcp[p];  // -1 at p = 0
-    if (lp == LCP_SAME) {
-        cnt[p] = 0;
-        return;
+    int top = -1;  // the deepest level (+ 1) of the nodes this particle starts
+    if (p < m) {
+        const int lp = lcp[p];  // -1 at p = 0
+        if (lp == LCP_SAME) {
+            cnt[p] = 0;
+        } else {
+            size_t r1 = p + 1;
+            while (r1 < m && lcp[r1] == LCP_SAME) ++r1;
+            const int nxt = r1 < m ? lcp[r1] : -1;
+            const int I = max(lp, nxt);
+            cnt[p] = uint32_t(I + 1 - lp);
+            ip[p] = I;
+            top = I + 1;
+        }
     }
-    size_t r1 = p + 1;
-    while (r1 < m && lcp[r1] == LCP_SAME) ++r1;
-    const int nxt = r1 < m ? lcp[r1] : -1;
-    const int I = max(lp, nxt);
-    cnt[p] = uint32_t(I + 1 - lp);
-    ip[p] = I;
-    atomicMax(&info->max_level, I + 1);
-    if (I + 1 > MAX_LEVEL) atomicOr(&info->flags, FLAG_DEPTH);
+    // levels lo .. hi of this run are internal nodes (its node at level top is the leaf)
+    const int lo = top >= 0 ? lcp[p] + 1 : MAX_LEVEL + 1, hi = top - 1;
+    int wlo = lo;
+    for (int off = 32; off > 0; off >>= 1) {
+        top = max(top, __shfl_xor(top, off, 64));
+        wlo = min(wlo, __shfl_xor(wlo, off, 64));
+    }
+    // one atomic per wavefront, and only when it raises the maximum seen so far (every thread
+    // of the grid updating one address serialised this kernel at 0.75 ms for 4M particles)
+    const bool lead = (threadIdx.x & 63) == 0;
+    if (lead && top >= 0) {
+        if (top > __atomic_load_n(&info->max_level, __ATOMIC_RELAXED))
+            atomicMax(&info->max_level, top);
+        if (top > MAX_LEVEL) atomicOr(&info->flags, FLAG_DEPTH);
+    }
+    // internal nodes per level: wavefront counts into the workgroup's histogram, then one
+    // atomic per workgroup and level (levels above MAX_LEVEL are not counted: the build fails
+    // on such a tree)
+    const int whi = min(top - 1, MAX_LEVEL);
+    for (int l = wlo; l <= whi; ++l) {
+        const uint64_t in = __ballot(lo <= l && l <= hi);
+        if (lead && in) atomicAdd(&hist[l], uint32_t(__popcll(in)));
+    }
+    __syncthreads();
+    for (int l = threadIdx.x; l < LVL_SLOTS; l += blockDim.x)
+        if (hist[l]) atomicAdd(&lvl_count[l * LVL_STRIDE], hist[l]);
+}
+
+// Copies the per-level counts into info (for the host) and sets the list cursors to their
+// exclusive scan (one thread; LVL_SLOTS levels).
+__global__ void bh_level_offsets_kernel(BhInfo* __restrict__ info,
+                                        const uint32_t* __restrict__ lvl_count,
+                                        uint32_t* __restrict__ lvl_cur) {
+    uint32_t acc = 0;
+    for (int l = 0; l < LVL_SLOTS; ++l) {
+        const uint32_t n_l = lvl_count[l * LVL_STRIDE];
+        info->lvl_count[l] = n_l;
+        lvl_cur[l * LVL_STRIDE] = acc;
+        acc += n_l;
+    }
+}
+
+// ilist: the internal nodes of each level together (level l from the exclusive scan of the
+// counts). The run starting at particle p holds internal nodes S[p] + (l - lcp[p] - 1) for the
+// levels lcp[p] < l <= ip[p]. Slots: each wavefront reserves its share of a level in the
+// workgroup's LDS counter, the workgroup one range per level with a global atomic. The order
+// inside a level is arbitrary: each node's sums depend only on its own children.
+__global__ __launch_bounds__(256) void bh_level_list_kernel(const int* __restrict__ lcp,
+                                                            const int* __restrict__ ip,
+                                                            const uint32_t* __restrict__ S,
+                                                            size_t m,
+                                                            uint32_t* __restrict__ lvl_cur,
+                                                            uint32_t* __restrict__ ilist) {
+    __shared__ uint32_t cnt[LVL_SLOTS];
+    __shared__ int wrange[2][4];  // per wavefront: lowest, highest level
+    __shared__ uint32_t woff[4][LVL_SLOTS];
+    for (int l = threadIdx.x; l < LVL_SLOTS; l += blockDim.x) cnt[l] = 0;
+    const size_t p = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    int lo = MAX_LEVEL + 1, hi = -1;
+    uint32_t base = 0;
+    if (p < m) {
+        const int lp = lcp[p];
+        if (lp != LCP_SAME) {
+            lo = lp + 1;
+            hi = min(ip[p], MAX_LEVEL);
+            base = S[p];
+        }
+    }
+    int wlo = lo, whi = hi;
+    for (int off = 32; off > 0; off >>= 1) {
+        wlo = min(wlo, __shfl_xor(wlo, off, 64));
+        whi = max(whi, __shfl_xor(whi, off, 64));
+    }
+    __syncthreads();  // cnt cleared
+    for (int l = wlo; l <= whi; ++l) {
+        const uint64_t in = __ballot(lo <= l && l <= hi);
+        if (lane == 0 && in) woff[wv][l] = atomicAdd(&cnt[l], uint32_t(__popcll(in)));
+    }
+    __syncthreads();  // workgroup counts complete
+    for (int l = threadIdx.x; l < LVL_SLOTS; l += blockDim.x)
+        if (cnt[l]) cnt[l] = atomicAdd(&lvl_cur[l * LVL_STRIDE], cnt[l]);  // now: the range start
+    __syncthreads();
+    const uint64_t below = (uint64_t(1) << lane) - 1;
+    for (int l = wlo; l <= whi; ++l) {
+        const bool mine = lo <= l && l <= hi;
+        const uint64_t in = __ballot(mine);
+        if (mine)
+            ilist[cnt[l] + woff[wv][l] + uint32_t(__popcll(in & below))] = base + uint32_t(l - lo);
+    }
 }
 
 __device__ __forceinline__ bool same_prefix(uint64_t a, uint64_t b, int level) {
@@ -473,16 +580,15 @@ __global__ __launch_bounds__(256) void bh_node_kernel(
 __global__ __launch_bounds__(256) void bh_up_kernel(NodeRec* __restrict__ rec,
                                                     double4* __restrict__ sums,
                                                     double4* __restrict__ sums2,
-                                                    const uint8_t* __restrict__ level, size_t total,
-                                                    int L, NodeMoments* __restrict__ mom,
+                                                    const uint32_t* __restrict__ nodes_at_level,
+                                                    uint32_t count, NodeMoments* __restrict__ mom,
                                                     NodeQuad* __restrict__ quad,
                                                     NodeDesc* __restrict__ desc,
                                                     const float* __restrict__ width) {
-    for (size_t k = size_t(blockIdx.x) * blockDim.x + threadIdx.x; k < total;
-         k += size_t(gridDim.x) * blockDim.x) {
-        if (level[k] != L) continue;
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < count;
+         t += gridDim.x * blockDim.x) {
+        const uint32_t k = nodes_at_level[t];  // an internal node of this level
         const NodeRec r = rec[k];
-        if (!(r.w2 >= 0.0f)) continue;  // leaf
         double4 a = make_double4(0, 0, 0, 0), b = make_double4(0, 0, 0, 0);
         size_t c = k + 1;
         while (c < r.skip) {
@@ -854,7 +960,10 @@ static int build_tree(pnb_ctx* c, const float4* afg, size_t m, bool ref_com = fa
     s->max_level = -1;
     PNB_HIP(c, s->info.ensure(sizeof(BhInfo)));
     BhInfo* info = s->info.as<BhInfo>();
-    bh_info_init_kernel<<<1, 1, 0, c->stream>>>(info);
+    PNB_HIP(c, s->lvl.ensure(size_t(2) * LVL_SLOTS * LVL_STRIDE * sizeof(uint32_t)));
+    uint32_t* lvl_count = s->lvl.as<uint32_t>();
+    uint32_t* lvl_cur = lvl_count + LVL_SLOTS * LVL_STRIDE;
+    bh_info_init_kernel<<<1, 256, 0, c->stream>>>(info, lvl_count);
     PNB_CHECK_LAUNCH(c);
     if (m == 0) {
         s->built = true;
@@ -900,7 +1009,9 @@ static int build_tree(pnb_ctx* c, const float4* afg, size_t m, bool ref_com = fa
                                                  s->key2.as<uint64_t>(), info);
         PNB_CHECK_LAUNCH(c);
         uint32_t* cnt = s->cnt.as<uint32_t>();
-        bh_count_kernel<<<g, 256, 0, c->stream>>>(lcp, m, cnt, s->ip.as<int>(), info);
+        bh_count_kernel<<<g, 256, 0, c->stream>>>(lcp, m, cnt, s->ip.as<int>(), info, lvl_count);
+        PNB_CHECK_LAUNCH(c);
+        bh_level_offsets_kernel<<<1, 1, 0, c->stream>>>(info, lvl_count, lvl_cur);
         PNB_CHECK_LAUNCH(c);
         if ((rc = scan_u32(c, cnt, cnt, m, true))) return rc;
         BhInfo h;
@@ -940,11 +1051,21 @@ static int build_tree(pnb_ctx* c, const float4* afg, size_t m, bool ref_com = fa
                                                  s->sums2.as<double4>(), s->level.as<uint8_t>(),
                                                  s->width.as<float>(), mom, quad);
         PNB_CHECK_LAUNCH(c);
-        const unsigned ug = std::min<unsigned>(4 * std::max(c->cu_count, 1), div_up(total, 256));
-        for (int L = h.max_level - 1; L >= 0; --L) {
-            bh_up_kernel<<<ug, 256, 0, c->stream>>>(rec, s->sums.as<double4>(),
-                                                    s->sums2.as<double4>(), s->level.as<uint8_t>(),
-                                                    total, L, mom, quad, desc, s->width.as<float>());
+        // bottom-up sums, one launch per level over that level's internal nodes
+        const int levels = std::max(h.max_level, 0);  // internal levels 0 .. max_level - 1
+        std::vector<uint32_t> off(size_t(levels) + 1, 0);
+        for (int L = 0; L < levels; ++L) off[L + 1] = off[L] + h.lvl_count[L];
+        PNB_HIP(c, s->ilist.ensure(std::max<size_t>(off[levels], 1) * sizeof(uint32_t)));
+        uint32_t* ilist = s->ilist.as<uint32_t>();
+        bh_level_list_kernel<<<g, 256, 0, c->stream>>>(lcp, s->ip.as<int>(), cnt, m, lvl_cur,
+                                                       ilist);
+        PNB_CHECK_LAUNCH(c);
+        for (int L = levels - 1; L >= 0; --L) {
+            const uint32_t n_l = h.lvl_count[L];
+            if (n_l == 0) continue;
+            bh_up_kernel<<<div_up(n_l, 256), 256, 0, c->stream>>>(
+                rec, s->sums.as<double4>(), s->sums2.as<double4>(), ilist + off[L], n_l, mom, quad,
+                desc, s->width.as<float>());
             PNB_CHECK_LAUNCH(c);
         }
         s->has_moments = moments;
