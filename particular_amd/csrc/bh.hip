@@ -97,6 +97,7 @@ constexpr int LVL_SLOTS = MAX_LEVEL + 2;
 struct BhState {
     size_t m = 0;
     size_t nodes = 0;
+    size_t internal = 0;  // internal nodes (listed by level in ilist)
     int max_level = -1;
     bool built = false;
     const uint64_t* skeys = nullptr;  // sorted keys (points into keys or keys_alt)
@@ -634,11 +635,39 @@ __global__ __launch_bounds__(256) void bh_up_kernel(NodeRec* __restrict__ rec,
         const uint32_t k = nodes_at_level[t];  // an internal node of this level
         const NodeRec r = rec[k];
         double4 a = make_double4(0, 0, 0, 0), b = make_double4(0, 0, 0, 0);
+        // with moments, in the same pass over the children: second moments about o (the first
+        // massive child's centre of mass) and first moments D = sum M_c (c_c - o), shifted to
+        // the centre of mass below
+        NodeMoments so;
+        double ox = 0, oy = 0, oz = 0, dx_ = 0, dy_ = 0, dz_ = 0, mpos = 0, heaviest = 0;
+        bool have_o = false;
+        int children = 0;
         size_t c = k + 1;
         while (c < r.skip) {
             const double4 s1 = sums[c], s2 = sums2[c];
             a.x += s1.x; a.y += s1.y; a.z += s1.z; a.w += s1.w;
             b.x += s2.x; b.y += s2.y; b.z += s2.z; b.w += s2.w;
+            if (mom) {
+                ++children;
+                heaviest = fmax(heaviest, s1.x);
+                if (s1.x > 0.0) {
+                    const NodeMoments tc = mom[c];
+                    const double cx = s1.y / s1.x, cy = s1.z / s1.x, cz = s1.w / s1.x;
+                    if (!have_o) {
+                        ox = cx; oy = cy; oz = cz;
+                        have_o = true;
+                    }
+                    const double ux = cx - ox, uy = cy - oy, uz = cz - oz;
+                    so.xx += tc.xx + s1.x * ux * ux;
+                    so.yy += tc.yy + s1.x * uy * uy;
+                    so.zz += tc.zz + s1.x * uz * uz;
+                    so.xy += tc.xy + s1.x * ux * uy;
+                    so.xz += tc.xz + s1.x * ux * uz;
+                    so.yz += tc.yz + s1.x * uy * uz;
+                    dx_ += s1.x * ux; dy_ += s1.x * uy; dz_ += s1.x * uz;
+                    mpos += s1.x;
+                }
+            }
             const uint32_t nx = rec[c].skip;
             c = nx > c ? nx : c + 1;
         }
@@ -659,27 +688,16 @@ __global__ __launch_bounds__(256) void bh_up_kernel(NodeRec* __restrict__ rec,
             NodeQuad qd{make_float4(0.f, 0.f, 0.f, 0.f), make_float4(0.f, 0.f, 0.f, 0.f)};
             NodeDesc ds{0.f, 0.f};
             if (a.x > 0.0 && width[k] > 0.0f) {
-                const double px = a.y / a.x, py = a.z / a.x, pz = a.w / a.x;
-                double heaviest = 0.0;
-                int children = 0;
-                for (size_t ch = k + 1; ch < r.skip;) {
-                    const double4 s1 = sums[ch];
-                    ++children;
-                    heaviest = fmax(heaviest, s1.x);
-                    if (s1.x > 0.0) {
-                        const NodeMoments tc = mom[ch];
-                        const double dx = s1.y / s1.x - px, dy = s1.z / s1.x - py,
-                                     dz = s1.w / s1.x - pz;
-                        t.xx += tc.xx + s1.x * dx * dx;
-                        t.yy += tc.yy + s1.x * dy * dy;
-                        t.zz += tc.zz + s1.x * dz * dz;
-                        t.xy += tc.xy + s1.x * dx * dy;
-                        t.xz += tc.xz + s1.x * dx * dz;
-                        t.yz += tc.yz + s1.x * dy * dz;
-                    }
-                    const uint32_t nx = rec[ch].skip;
-                    ch = nx > ch ? nx : ch + 1;
-                }
+                // T = sum over massive children of T_c + M_c (c_c - p)(c_c - p)^T with p the
+                // centre of mass: with u_c = c_c - o and v = p - o,
+                // T = S_o - v D^T - D v^T + M v v^T (M, D over the same children)
+                const double vx = a.y / a.x - ox, vy = a.z / a.x - oy, vz = a.w / a.x - oz;
+                t.xx = so.xx - 2.0 * vx * dx_ + mpos * vx * vx;
+                t.yy = so.yy - 2.0 * vy * dy_ + mpos * vy * vy;
+                t.zz = so.zz - 2.0 * vz * dz_ + mpos * vz * vz;
+                t.xy = so.xy - vx * dy_ - dx_ * vy + mpos * vx * vy;
+                t.xz = so.xz - vx * dz_ - dx_ * vz + mpos * vx * vz;
+                t.yz = so.yz - vy * dz_ - dy_ * vz + mpos * vy * vz;
                 const double w = width[k];
                 const double f = 1.0 / (a.x * w * w), tr = t.xx + t.yy + t.zz;
                 qd.a = make_float4(float((3.0 * t.xx - tr) * f), float((3.0 * t.yy - tr) * f),
@@ -770,13 +788,15 @@ struct __align__(16) QuadH {
     __half q[8];
 };
 
-// qh[k] = MLP(features of node k) for every node (zero for leaves and massless nodes): 16 nodes
-// per MFMA tile, one tile per wavefront and iteration, weights staged in LDS.
+// qh[k] = MLP(features of node k) for the internal nodes listed in inodes (zero for massless
+// ones; the walk reads qh only for internal nodes): 16 nodes per MFMA tile, one tile per
+// wavefront and iteration, weights staged in LDS.
 __global__ __launch_bounds__(256) void mlp_node_kernel(const NodeRec* __restrict__ rec,
                                                        const NodeQuad* __restrict__ quad,
                                                        const uint8_t* __restrict__ level,
                                                        const NodeDesc* __restrict__ desc,
-                                                       uint32_t total,
+                                                       const uint32_t* __restrict__ inodes,
+                                                       uint32_t count,
                                                        const MlpWeights* __restrict__ wg,
                                                        QuadH* __restrict__ qh) {
     __shared__ MlpForward w;
@@ -789,12 +809,13 @@ __global__ __launch_bounds__(256) void mlp_node_kernel(const NodeRec* __restrict
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int r16 = lane & 15, g = lane >> 4;
     const float root_mass = rec[0].mass;
-    for (uint32_t t0 = (blockIdx.x * 4 + wv) * 16; t0 < total; t0 += gridDim.x * 4 * 16) {
-        const uint32_t k = t0 + r16;
+    for (uint32_t t0 = (blockIdx.x * 4 + wv) * 16; t0 < count; t0 += gridDim.x * 4 * 16) {
+        const bool listed = t0 + r16 < count;
+        const uint32_t k = listed ? inodes[t0 + r16] : 0u;
         float f[MLP_KIN];
         NodeRec r{};
         r.w2 = -INFINITY;
-        if (k < total) {
+        if (listed) {
             r = rec[k];
             node_features(r, quad[k], level[k], desc[k], root_mass, f);
         } else {
@@ -809,7 +830,7 @@ __global__ __launch_bounds__(256) void mlp_node_kernel(const NodeRec* __restrict
         }
         const f32x4_t yv = mlp_forward16_regs(w, bx);  // lanes of group g: outputs 4 g .. 4 g + 3
         const float keep = (r.w2 >= 0.0f && r.mass > 0.0f) ? 1.f : 0.f;
-        if (k < total && g < 2) {
+        if (listed && g < 2) {
             __half* dst = qh[k].q + 4 * g;
             dst[0] = __float2half(keep * yv[0]);
             dst[1] = __float2half(keep * yv[1]);
@@ -1001,6 +1022,7 @@ static int build_tree(pnb_ctx* c, const float4* afg, size_t m, bool ref_com = fa
     s->has_moments = false;
     s->m = m;
     s->nodes = 0;
+    s->internal = 0;
     s->max_level = -1;
     PNB_HIP(c, s->info.ensure(sizeof(BhInfo)));
     BhInfo* info = s->info.as<BhInfo>();
@@ -1101,6 +1123,7 @@ static int build_tree(pnb_ctx* c, const float4* afg, size_t m, bool ref_com = fa
         std::vector<uint32_t> off(size_t(levels) + 1, 0);
         for (int L = 0; L < levels; ++L) off[L + 1] = off[L] + h.lvl_count[L];
         PNB_HIP(c, s->ilist.ensure(std::max<size_t>(off[levels], 1) * sizeof(uint32_t)));
+        s->internal = off[levels];
         uint32_t* ilist = s->ilist.as<uint32_t>();
         bh_level_list_kernel<<<lg, 256, 0, c->stream>>>(lcp, s->ip.as<int>(), cnt, m, lvl_cur,
                                                         ilist);
@@ -1163,12 +1186,14 @@ static int node_network(pnb_ctx* c) {
     // one entry past the last node, as for the records (the walk prefetches k + 1 and skip)
     PNB_HIP(c, s->qh.ensure((size_t(total) + 1) * sizeof(QuadH)));
     if (total == 0) return PNB_OK;
+    if (s->internal == 0) return PNB_OK;
     return timed(c, PNB_PROF_MLP, [&]() -> int {
-        const size_t tiles = div_up(total, 16);
+        const size_t tiles = div_up(s->internal, 16);
         const unsigned grid = unsigned(std::min<size_t>(div_up(tiles, 4), 16 * size_t(std::max(c->cu_count, 1))));
         mlp_node_kernel<<<grid, 256, 0, c->stream>>>(s->rec.as<NodeRec>(), s->quad.as<NodeQuad>(),
                                                      s->level.as<uint8_t>(), s->desc.as<NodeDesc>(),
-                                                     uint32_t(total), wg, s->qh.as<QuadH>());
+                                                     s->ilist.as<uint32_t>(), uint32_t(s->internal),
+                                                     wg, s->qh.as<QuadH>());
         PNB_CHECK_LAUNCH(c);
         return PNB_OK;
     });
