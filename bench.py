@@ -203,7 +203,9 @@ def bench_bh(args):
     quadrupole, learned.quadrupole_params). Tree build + walk (+ node network) device time from
     HIP events, accepted pair terms and wavefront node steps from the walk's counters, accuracy
     against the FP64 direct sum on a seeded sample, and the particles/s ratio of the corrected to
-    the plain method at equal rms error (1e-4), interpolated on both curves."""
+    the plain method at equal rms error (1e-4): each method is also run at the angle where its
+    error meets 1e-4 (secant steps on the measured curve), and the times are interpolated
+    between the rows closest to the target."""
     import torch
 
     import particular_amd as pa
@@ -268,10 +270,29 @@ def bench_bh(args):
             f"rms {err['rms_rel']:.2e}")
         return row
 
+    target = 1e-4
+
+    def refine(rows, mlp):
+        """Adds runs at the angle where the rms error meets the target: secant steps of
+        log(rms) against log(theta) between the closest bracketing rows (at most two)."""
+        for _ in range(2):
+            lo = [r for r in rows if r["rms_rel"] <= target]
+            hi = [r for r in rows if r["rms_rel"] > target]
+            if not lo or not hi:
+                return
+            a = max(lo, key=lambda r: r["rms_rel"])
+            b = min(hi, key=lambda r: r["rms_rel"])
+            if abs(np.log(a["rms_rel"] / target)) < 0.02:
+                return
+            f = np.log(target / a["rms_rel"]) / np.log(b["rms_rel"] / a["rms_rel"])
+            th = float(np.exp(np.log(a["theta"]) + f * (np.log(b["theta"]) - np.log(a["theta"]))))
+            rows.append(run(round(th, 4), mlp))
+
     plain = [run(float(th), False) for th in args.bh_thetas.split(",") if th]
     nn = [run(float(th), True) for th in args.bh_nn_thetas.split(",") if th]
+    refine(plain, False)
+    refine(nn, True)
     res.close()
-    target = 1e-4
     t_plain = time_at_error(plain, target) if plain else None
     t_nn = time_at_error(nn, target) if nn else None
     comparison = {
