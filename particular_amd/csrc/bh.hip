@@ -36,6 +36,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
+#include <type_traits>
 #include <vector>
 
 namespace pnb {
@@ -861,8 +863,9 @@ __device__ __forceinline__ uint64_t pack2(uint32_t lo, uint32_t hi) {
     return uint64_t(lo) | (uint64_t(hi) << 32);
 }
 
-__device__ __forceinline__ void advance(uint64_t open, uint32_t kn, uint32_t ks, uint32_t& k,
-                                        NodeRec& r, const NodeRec& rn, const NodeRec& rs) {
+template <typename Off>
+__device__ __forceinline__ void advance(uint64_t open, Off kn, Off ks, Off& k, NodeRec& r,
+                                        const NodeRec& rn, const NodeRec& rs) {
     const uint64_t n0 = pack2(__float_as_uint(rn.cx), __float_as_uint(rn.cy));
     const uint64_t s0 = pack2(__float_as_uint(rs.cx), __float_as_uint(rs.cy));
     const uint64_t n1 = pack2(__float_as_uint(rn.cz), __float_as_uint(rn.mass));
@@ -872,17 +875,30 @@ __device__ __forceinline__ void advance(uint64_t open, uint32_t kn, uint32_t ks,
     // begin/end are not needed, but naming them makes each record one 32-byte scalar load
     const uint64_t n3 = pack2(rn.begin, rn.end), s3 = pack2(rs.begin, rs.end);
     uint64_t o0, o1, o2;
-    uint32_t ok;
-    asm volatile(
-        "s_cmp_lg_u64 %4, 0\n\t"
-        "s_cselect_b64 %0, %5, %6\n\t"
-        "s_cselect_b64 %1, %7, %8\n\t"
-        "s_cselect_b64 %2, %9, %10\n\t"
-        "s_cselect_b32 %3, %11, %12"
-        : "=&s"(o0), "=&s"(o1), "=&s"(o2), "=&s"(ok)
-        : "s"(open), "s"(n0), "s"(s0), "s"(n1), "s"(s1), "s"(n2), "s"(s2), "s"(kn), "s"(ks),
-          "s"(n3), "s"(s3)
-        : "scc");
+    Off ok;
+    if constexpr (sizeof(Off) == 4) {
+        asm volatile(
+            "s_cmp_lg_u64 %4, 0\n\t"
+            "s_cselect_b64 %0, %5, %6\n\t"
+            "s_cselect_b64 %1, %7, %8\n\t"
+            "s_cselect_b64 %2, %9, %10\n\t"
+            "s_cselect_b32 %3, %11, %12"
+            : "=&s"(o0), "=&s"(o1), "=&s"(o2), "=&s"(ok)
+            : "s"(open), "s"(n0), "s"(s0), "s"(n1), "s"(s1), "s"(n2), "s"(s2), "s"(kn), "s"(ks),
+              "s"(n3), "s"(s3)
+            : "scc");
+    } else {
+        asm volatile(
+            "s_cmp_lg_u64 %4, 0\n\t"
+            "s_cselect_b64 %0, %5, %6\n\t"
+            "s_cselect_b64 %1, %7, %8\n\t"
+            "s_cselect_b64 %2, %9, %10\n\t"
+            "s_cselect_b64 %3, %11, %12"
+            : "=&s"(o0), "=&s"(o1), "=&s"(o2), "=&s"(ok)
+            : "s"(open), "s"(n0), "s"(s0), "s"(n1), "s"(s1), "s"(n2), "s"(s2), "s"(kn), "s"(ks),
+              "s"(n3), "s"(s3)
+            : "scc");
+    }
     r.cx = __uint_as_float(uint32_t(o0));
     r.cy = __uint_as_float(uint32_t(o0 >> 32));
     r.cz = __uint_as_float(uint32_t(o1));
@@ -905,7 +921,8 @@ __device__ __forceinline__ float half_hi(uint32_t v) {
 // loads (bytes per step), so each step loads what it needs: the records of both possible next
 // nodes (k + 1, skip) are requested while node k is evaluated, and WALK_QUAD fetches the learned
 // moment of node k only when some lane accepts it as an internal node.
-template <int MODE>
+// WIDE: 64-bit byte offsets, for trees whose records span 4 GiB or more (2^27 nodes).
+template <int MODE, bool WIDE>
 __global__ __launch_bounds__(256) void bh_walk_kernel(const NodeRec* __restrict__ nodes,
                                                       uint32_t total,
                                                       const float4* __restrict__ pts,
@@ -919,21 +936,22 @@ __global__ __launch_bounds__(256) void bh_walk_kernel(const NodeRec* __restrict_
     const bool valid = i < n;
     const float4 p = pts[valid ? i : n - 1];
     float ax = 0.f, ay = 0.f, az = 0.f, cx = 0.f, cy = 0.f, cz = 0.f;
-    uint32_t resume = valid ? 0u : 0xffffffffu;
+    using Off = std::conditional_t<WIDE, uint64_t, uint32_t>;
+    Off resume = valid ? Off(0) : ~Off(0);
     uint32_t accepted = 0, steps = 0, nrows = 0;
     const uint32_t wave_slot = i / 64;
 
-    // The cursor is a byte offset, ko = 32 k (the host keeps 32 (total + 1) below 2^32), so each
-    // record address is the array base plus one 32-bit scalar offset.
+    // The cursor is a byte offset, ko = 32 k, so each record address is the array base plus one
+    // scalar offset (32 bits unless WIDE: the host takes WIDE when 32 (total + 1) >= 2^32).
     const char* base = reinterpret_cast<const char*>(nodes);
-    const uint32_t end = total * uint32_t(sizeof(NodeRec));
-    uint32_t ko = 0;
+    const Off end = Off(total) * Off(sizeof(NodeRec));
+    Off ko = 0;
     NodeRec r{};
     if (total > 0) r = nodes[0];  // no tree (m = 0): nodes may not even be allocated
     while (ko < end) {
         // in bounds: k + 1 <= total and skip <= total, and the array holds total + 1 records
-        const uint32_t so = r.skip * uint32_t(sizeof(NodeRec));
-        const NodeRec r_next = *reinterpret_cast<const NodeRec*>(base + (ko + 32u));
+        const Off so = Off(r.skip) * Off(sizeof(NodeRec));
+        const NodeRec r_next = *reinterpret_cast<const NodeRec*>(base + (ko + Off(32)));
         const NodeRec r_skip = *reinterpret_cast<const NodeRec*>(base + so);
         const bool active = ko >= resume;
         const float dx = sub_rn(r.cx, p.x), dy = sub_rn(r.cy, p.y), dz = sub_rn(r.cz, p.z);
@@ -988,7 +1006,8 @@ __global__ __launch_bounds__(256) void bh_walk_kernel(const NodeRec* __restrict_
             if (m) {
                 if (lane == 0 && nrows < rows.cap)
                     rows.rows[size_t(wave_slot) * rows.cap + nrows] =
-                        make_uint4(ko >> 5, uint32_t(__popcll(m)), uint32_t(m), uint32_t(m >> 32));
+                        make_uint4(uint32_t(ko >> 5), uint32_t(__popcll(m)), uint32_t(m),
+                                   uint32_t(m >> 32));
                 ++nrows;
             }
         }
@@ -997,7 +1016,7 @@ __global__ __launch_bounds__(256) void bh_walk_kernel(const NodeRec* __restrict_
         if (!open) resume = max(resume, so);
         ++steps;
         // skip > k for every node, and skip = k + 1 for a leaf, so the two candidates agree there
-        advance(open_mask, ko + 32u, so, ko, r, r_next, r_skip);
+        advance<Off>(open_mask, ko + Off(32), so, ko, r, r_next, r_skip);
     }
     if (MODE == WALK_ROWS && lane == 0 && wave_slot * 64 < n) {  // wavefronts holding particles
         rows.count[wave_slot] = nrows;
@@ -1089,10 +1108,6 @@ static int build_tree(pnb_ctx* c, const float4* afg, size_t m, bool ref_com = fa
         if (h.flags & FLAG_NONFINITE) return fail(c, PNB_E_ARG, "non-finite affecting position");
         if (h.flags & FLAG_DEPTH)
             return fail(c, PNB_E_DEPTH, "octree deeper than %d levels", MAX_LEVEL);
-        // the walk addresses records by 32-bit byte offsets, up to and including record total
-        if (uint64_t(total) + 1 > (uint64_t(1) << 27))
-            return fail(c, PNB_E_ARG, "octree of %u nodes exceeds the walk's limit of 2^27 - 1",
-                        total);
         s->nodes = total;
         s->max_level = h.max_level;
         // one record past the last node: the walk prefetches records k + 1 and skip <= total
@@ -1149,6 +1164,19 @@ static int build_tree(pnb_ctx* c, const float4* afg, size_t m, bool ref_com = fa
 
 const MlpWeights* mlp_weights(pnb_ctx* c);  // mlp.hip: nullptr until initialised
 
+template <int MODE>
+static void launch_walk(bool wide, unsigned grid, hipStream_t st, const NodeRec* nr, uint32_t total,
+                        const float4* pts, const uint32_t* perm, uint32_t n, float theta2,
+                        float eps2, float4* out, BhInfo* info, const WalkRows& rows,
+                        const QuadH* qh) {
+    if (wide)
+        bh_walk_kernel<MODE, true><<<grid, 256, 0, st>>>(nr, total, pts, perm, n, theta2, eps2, out,
+                                                         info, rows, qh);
+    else
+        bh_walk_kernel<MODE, false><<<grid, 256, 0, st>>>(nr, total, pts, perm, n, theta2, eps2,
+                                                          out, info, rows, qh);
+}
+
 // quad: add the learned correction (the per-node moments in s->qh).
 static int walk(pnb_ctx* c, const float4* pts, const uint32_t* perm, size_t n, float theta,
                 float softening, float4* out, const WalkRows* rows = nullptr, bool quad = false) {
@@ -1160,17 +1188,20 @@ static int walk(pnb_ctx* c, const float4* pts, const uint32_t* perm, size_t n, f
         const uint32_t total = uint32_t(s->nodes);
         BhInfo* info = s->info.as<BhInfo>();
         const NodeRec* nr = s->rec.as<NodeRec>();
-        if (quad) {
-            bh_walk_kernel<WALK_QUAD><<<grid, 256, 0, c->stream>>>(
-                nr, total, pts, perm, uint32_t(n), theta2, eps2, out, info, WalkRows{},
-                s->qh.as<QuadH>());
-        } else if (rows) {
-            bh_walk_kernel<WALK_ROWS><<<grid, 256, 0, c->stream>>>(
-                nr, total, pts, perm, uint32_t(n), theta2, eps2, out, info, *rows, nullptr);
-        } else {
-            bh_walk_kernel<WALK_PLAIN><<<grid, 256, 0, c->stream>>>(
-                nr, total, pts, perm, uint32_t(n), theta2, eps2, out, info, WalkRows{}, nullptr);
-        }
+        // 64-bit record offsets once the records (and the one past the last) reach 4 GiB;
+        // PNB_WIDE_WALK=1 takes them for any tree (tests compare both forms)
+        const char* force = std::getenv("PNB_WIDE_WALK");
+        const bool wide = (uint64_t(total) + 1) * sizeof(NodeRec) >= (uint64_t(1) << 32) ||
+                          (force && force[0] == '1');
+        if (quad)
+            launch_walk<WALK_QUAD>(wide, grid, c->stream, nr, total, pts, perm, uint32_t(n), theta2,
+                                   eps2, out, info, WalkRows{}, s->qh.as<QuadH>());
+        else if (rows)
+            launch_walk<WALK_ROWS>(wide, grid, c->stream, nr, total, pts, perm, uint32_t(n), theta2,
+                                   eps2, out, info, *rows, nullptr);
+        else
+            launch_walk<WALK_PLAIN>(wide, grid, c->stream, nr, total, pts, perm, uint32_t(n),
+                                    theta2, eps2, out, info, WalkRows{}, nullptr);
         PNB_CHECK_LAUNCH(c);
         return PNB_OK;
     });

@@ -181,3 +181,21 @@ def test_stats_count_accepted_terms(res, orc):
     _, visits = orc.Tree(s.particles).barnes_hut(s.particles, 0.6, com64=True, return_visits=True)
     assert accepted == int(visits.sum())
     assert steps >= accepted / 64
+
+
+def test_wide_record_offsets_match_narrow(res, monkeypatch):
+    """Trees of 2^27 nodes or more are walked with 64-bit record offsets; PNB_WIDE_WALK=1 takes
+    that form for a small tree, which must give bit-identical results (plain and corrected)."""
+    from particular_amd import learned
+
+    s = workloads.direct_sum_case(30000, seed=61)
+    learned.init(res, 0)
+    learned.set_params(res, learned.quadrupole_params())
+    runs = {}
+    for wide in ("0", "1"):
+        monkeypatch.setenv("PNB_WIDE_WALK", wide)
+        runs[wide] = [pa.BarnesHut(res, th, pa.Acceleration.checked(), mlp=m).compute(s.particles)
+                      for th, m in ((0.5, False), (0.8, False), (0.5, True))]
+    for a, b in zip(runs["0"], runs["1"]):
+        np.testing.assert_array_equal(a, b)
+    assert np.abs(runs["0"][2] - runs["0"][0]).max() > 0  # the correction did something
