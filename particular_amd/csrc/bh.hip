@@ -962,15 +962,17 @@ __global__ __launch_bounds__(256) void bh_walk_kernel(const NodeRec* __restrict_
         const bool acc = active && nz && !crit;
         // the wavefront's open mask from the three compares' own lane masks (a ballot of the
         // combined bool would first copy it into a VGPR)
-        const uint64_t open_mask = __builtin_amdgcn_ballot_w64(active) &
-                                   __builtin_amdgcn_ballot_w64(nz) &
-                                   __builtin_amdgcn_fcmpf(mul_rn(theta2, d2), r.w2, 4 /*olt*/);
+        const uint64_t live_mask = __builtin_amdgcn_ballot_w64(active) &
+                                   __builtin_amdgcn_ballot_w64(nz);
+        const uint64_t crit_mask = __builtin_amdgcn_fcmpf(mul_rn(theta2, d2), r.w2, 4 /*olt*/);
+        const uint64_t open_mask = live_mask & crit_mask;
+        const uint64_t acc_mask = live_mask & ~crit_mask;
         if (acc) {
             const float r2 = add_rn(d2, eps2);
             const float inv = __builtin_amdgcn_rsqf(r2);
             const float s = (r.mass * inv) * (inv * inv);
             float yx, yy, yz;
-            if (MODE == WALK_QUAD && r.w2 >= 0.0f) {
+            if (MODE == WALK_QUAD && __float_as_int(r.w2) >= 0) {  // internal (leaves: -inf)
                 // internal node: the monopole plus the field of Q = M w^2 y (quad_field's
                 // formula, w2 = width^2), summed as d (s + s5 t) - s5 Q d in one compensated term
                 const uint4 h = *reinterpret_cast<const uint4*>(
@@ -999,10 +1001,10 @@ __global__ __launch_bounds__(256) void bh_walk_kernel(const NodeRec* __restrict_
             cy = (t - ay) - yy; ay = t;
             t = az + yz;
             cz = (t - az) - yz; az = t;
-            ++accepted;
+            ++accepted;  // per lane (a wave-wide popcount costs scalar issue slots the walk lacks)
         }
-        if (MODE == WALK_ROWS && r.w2 >= 0.0f) {  // internal node: a row if some lane accepted it
-            const unsigned long long m = __ballot(acc);
+        if (MODE == WALK_ROWS && __float_as_int(r.w2) >= 0) {  // internal: a row if accepted
+            const unsigned long long m = acc_mask;
             if (m) {
                 if (lane == 0 && nrows < rows.cap)
                     rows.rows[size_t(wave_slot) * rows.cap + nrows] =
