@@ -117,7 +117,10 @@ int pnb_bf_clear_tuning(pnb_ctx *ctx);
  * zero distance, open an internal node when (theta*theta)*d^2 < width^2, else add its pair term.
  * theta = 0 degenerates to brute force over the leaves (sequential.rs:441-442). `checked` is
  * accepted for symmetry with the reference signature; the traversal's zero-distance skip makes
- * both forms identical. flags: a combination of PNB_BH_MLP and PNB_BH_REF_COM. */
+ * both forms identical. flags: a combination of PNB_BH_MLP and PNB_BH_REF_COM. Sizes: n up to
+ * 2^31 - 1; any octree the node counts (uint32) allow, trees of 2^27 nodes or more being walked
+ * with 64-bit record offsets (the environment variable PNB_WIDE_WALK=1 forces that form, for
+ * tests). */
 #define PNB_BH_MLP 1u /* add the learned correction of each accepted internal node (see below) */
 /* Node data exactly as the reference computes it: f32 sequential folds of mass and mass-weighted
  * position per node (particular/src/gravity/impls/mod.rs:126-131), instead of the FP64 reduction.
