@@ -282,3 +282,42 @@ def test_mlp_flag_needs_initialised_network():
         assert e.value.status == pa._lib.PNB_E_STATE
     finally:
         r.close()
+
+
+def test_trained_network_beats_plain_barnes_hut(res, orc):
+    """The whole learned path from scratch: a fresh network (zero output layer, i.e. plain
+    Barnes-Hut) trained with the GPU forward/backward/Adam kernels on the brute-force minus
+    Barnes-Hut residuals the walk itself collects, then judged on a held-out snapshot against the
+    FP64 direct sum. It must cut the error of the method without it well down, towards what the
+    node's own quadrupole gives."""
+    import torch
+
+    n = 16384
+    learned.init(res, 0)
+    learned.clear(res)
+    for seed in (71, 72, 73):
+        s = workloads.two_plummer_case(n, seed=seed)
+        dev = torch.from_numpy(s.particles).cuda()
+        for theta in (0.5, 0.8):
+            learned.collect(res, dev, theta, waves=n // 64, seed=seed)
+    out = learned.fit(res, 300)
+    assert out["kept"] and out["loss_end"] < 0.5 * out["loss_start"], out["loss_end"]
+    trained = learned.get_params(res)
+
+    h = workloads.two_plummer_case(n, seed=80)
+    idx = np.random.default_rng(3).choice(n, 512, replace=False)
+    ref = orc.direct_f64(h.particles[idx], h.particles)
+
+    def rms(a):
+        e = np.linalg.norm(a[idx, :3] - ref, axis=1) / np.linalg.norm(ref, axis=1)
+        return float(np.sqrt(np.mean(e**2)))
+
+    for theta, bar in ((0.5, 0.5), (0.8, 0.75)):
+        plain = pa.BarnesHut(res, theta, pa.Acceleration.checked()).compute(h.particles)
+        learned.set_params(res, trained)
+        nn = pa.BarnesHut(res, theta, pa.Acceleration.checked(), mlp=True).compute(h.particles)
+        learned.set_params(res, learned.quadrupole_params())
+        quad = pa.BarnesHut(res, theta, pa.Acceleration.checked(), mlp=True).compute(h.particles)
+        print(f"theta {theta}: plain {rms(plain):.3e}, trained {rms(nn):.3e}, "
+              f"quadrupole {rms(quad):.3e}")
+        assert rms(nn) < bar * rms(plain)
