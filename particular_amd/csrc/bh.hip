@@ -25,7 +25,9 @@
 //     node's subtree ends at the first particle with lcp < level, and skip = index of the first
 //     node starting there;
 //  5. FP64 mass / mass-weighted position sums: leaves directly, internal nodes from their
-//     children, one launch per level from the deepest up; node records get the f32 centre of mass.
+//     children, one launch per level from the deepest up over that level's list of internal
+//     nodes (built from the same per-particle level ranges); node records get the f32 centre of
+//     mass.
 // Traversal: one wavefront walks the pre-order node array with one shared cursor. A lane that
 // opens node k lets the wave step to k+1; a lane that accepts or skips k waits until the cursor
 // reaches skip[k]. Each lane therefore applies exactly the reference's decisions; node records
@@ -917,10 +919,12 @@ __device__ __forceinline__ float half_hi(uint32_t v) {
 
 
 // One wavefront walks the pre-order node records with one cursor (k, in SGPRs); lane decisions
-// are the reference's (see the file comment). The walk is bound by the throughput of its scalar
-// loads (bytes per step), so each step loads what it needs: the records of both possible next
-// nodes (k + 1, skip) are requested while node k is evaluated, and WALK_QUAD fetches the learned
-// moment of node k only when some lane accepts it as an internal node.
+// are the reference's (see the file comment). The walk is bound by VALU issue (about 34 VALU,
+// 17 SALU and 2 scalar loads per step for the plain walk), so the step keeps everything
+// wave-uniform in SGPRs: the records of both possible next nodes (k + 1, skip) are requested at
+// the top of the step and picked with scalar selects at its end, and WALK_QUAD fetches the
+// learned moment of node k only when some lane accepts it as an internal node (prefetching it for
+// both candidates costs more scalar-cache misses than it hides).
 // WIDE: 64-bit byte offsets, for trees whose records span 4 GiB or more (2^27 nodes).
 template <int MODE, bool WIDE>
 __global__ __launch_bounds__(256) void bh_walk_kernel(const NodeRec* __restrict__ nodes,
