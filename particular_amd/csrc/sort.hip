@@ -105,7 +105,18 @@ __global__ __launch_bounds__(256) void rs_bits_kernel(const uint64_t* __restrict
         o |= __shfl_xor(o, off, 64);
         a &= __shfl_xor(a, off, 64);
     }
+    // one pair of atomics per workgroup (per wavefront, they queued on the two words: 0.1 ms)
+    __shared__ unsigned long long wo[4], wa[4];
     if ((threadIdx.x & 63) == 0) {
+        wo[threadIdx.x >> 6] = o;
+        wa[threadIdx.x >> 6] = a;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < int(blockDim.x >> 6); ++w) {
+            o |= wo[w];
+            a &= wa[w];
+        }
         atomicOr(&or_and[0], o);
         atomicAnd(&or_and[1], a);
     }
