@@ -68,6 +68,13 @@ static_assert(sizeof(NodeRec) == 32, "node record is 32 bytes");
 
 // Mass-weighted second moments of a node about its FP64 centre of mass,
 // T = sum m (x - c)(x - c)^T, as {xx, yy, zz, xy, xz, yz}; built for the learned correction only.
+// FP64 node sums in one 64-byte record: m = (mass, mass-weighted x, y, z) and p = (x, y, z,
+// count) of the positions, for the centre of a massless node. One record per child keeps the
+// bottom-up pass at one cache line per child for the sums.
+struct __align__(64) NodeSums {
+    double4 m, p;
+};
+
 struct NodeMoments {
     double xx = 0, yy = 0, zz = 0, xy = 0, xz = 0, yz = 0;
 };
@@ -106,7 +113,7 @@ struct BhState {
     bool built = false;
     const uint64_t* skeys = nullptr;  // sorted keys (points into keys or keys_alt)
     const uint32_t* sidx = nullptr;   // tree order -> original index
-    DevBuf keys, keys_alt, idx, idx_alt, sp, lcp, cnt, ip, key2, rec, sums, sums2, level, width;
+    DevBuf keys, keys_alt, idx, idx_alt, sp, lcp, cnt, ip, key2, rec, sums, level, width;
     DevBuf ilist;            // internal nodes grouped by level (bottom-up pass)
     DevBuf lvl;              // padded per-level counts and list cursors (2 x LVL_SLOTS lines)
     DevBuf mom, quad, desc;  // second moments (FP64), normalised quadrupoles, child shares
@@ -123,7 +130,7 @@ struct BhState {
 void bh_free(BhState* s) {
     if (!s) return;
     for (DevBuf* b : {&s->keys, &s->keys_alt, &s->idx, &s->idx_alt, &s->sp, &s->lcp, &s->cnt,
-                      &s->ip, &s->key2, &s->rec, &s->sums, &s->sums2, &s->level, &s->width,
+                      &s->ip, &s->key2, &s->rec, &s->sums, &s->level, &s->width,
                       &s->ilist, &s->lvl, &s->mom, &s->quad, &s->desc, &s->qh,
                       &s->info, &s->bbox_part, &s->hist, &s->bits, &s->afg_tmp, &s->akeys,
                       &s->akeys_alt, &s->aidx, &s->aidx_alt, &s->apts, &s->h_aff, &s->h_afg,
@@ -552,8 +559,8 @@ __device__ __forceinline__ bool same_prefix(uint64_t a, uint64_t b, int level) {
 __global__ __launch_bounds__(256) void bh_node_kernel(
     const uint64_t* __restrict__ keys, const float4* __restrict__ sp, const int* __restrict__ lcp,
     const int* __restrict__ ip, const uint32_t* __restrict__ S, size_t m,
-    const BhInfo* __restrict__ info, NodeRec* __restrict__ rec, double4* __restrict__ sums,
-    double4* __restrict__ sums2, uint8_t* __restrict__ level, float* __restrict__ width,
+    const BhInfo* __restrict__ info, NodeRec* __restrict__ rec, NodeSums* __restrict__ sums,
+    uint8_t* __restrict__ level, float* __restrict__ width,
     NodeMoments* __restrict__ mom, NodeQuad* __restrict__ quad) {
     const size_t p = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (p >= m) return;
@@ -607,8 +614,8 @@ __global__ __launch_bounds__(256) void bh_node_kernel(
             double tot = 0.0;
             for (size_t r = p; r < e; ++r) tot += double(sp[r].w);
             const double cnt_d = double(e - p);
-            sums[k] = make_double4(tot, tot * q.x, tot * q.y, tot * q.z);
-            sums2[k] = make_double4(cnt_d * q.x, cnt_d * q.y, cnt_d * q.z, cnt_d);
+            sums[k].m = make_double4(tot, tot * q.x, tot * q.y, tot * q.z);
+            sums[k].p = make_double4(cnt_d * q.x, cnt_d * q.y, cnt_d * q.z, cnt_d);
             if (mom) {  // one position: no spread about the centre
                 mom[k] = NodeMoments{};
                 quad[k] = NodeQuad{make_float4(0.f, 0.f, 0.f, 0.f), make_float4(0.f, 0.f, 0.f, 0.f)};
@@ -627,8 +634,7 @@ __global__ __launch_bounds__(256) void bh_node_kernel(
 // With `mom`, also the second moments by the parallel-axis rule
 // T = sum over children of (T_c + M_c (c_c - c)(c_c - c)^T), and the normalised quadrupole.
 __global__ __launch_bounds__(256) void bh_up_kernel(NodeRec* __restrict__ rec,
-                                                    double4* __restrict__ sums,
-                                                    double4* __restrict__ sums2,
+                                                    NodeSums* __restrict__ sums,
                                                     const uint32_t* __restrict__ nodes_at_level,
                                                     uint32_t count, NodeMoments* __restrict__ mom,
                                                     NodeQuad* __restrict__ quad,
@@ -648,7 +654,8 @@ __global__ __launch_bounds__(256) void bh_up_kernel(NodeRec* __restrict__ rec,
         int children = 0;
         size_t c = k + 1;
         while (c < r.skip) {
-            const double4 s1 = sums[c], s2 = sums2[c];
+            const NodeSums sc = sums[c];
+            const double4 s1 = sc.m, s2 = sc.p;
             a.x += s1.x; a.y += s1.y; a.z += s1.z; a.w += s1.w;
             b.x += s2.x; b.y += s2.y; b.z += s2.z; b.w += s2.w;
             if (mom) {
@@ -675,8 +682,8 @@ __global__ __launch_bounds__(256) void bh_up_kernel(NodeRec* __restrict__ rec,
             const uint32_t nx = rec[c].skip;
             c = nx > c ? nx : c + 1;
         }
-        sums[k] = a;
-        sums2[k] = b;
+        sums[k].m = a;
+        sums[k].p = b;
         float cx, cy, cz;
         if (a.x != 0.0) {
             cx = float(a.y / a.x); cy = float(a.z / a.x); cz = float(a.w / a.x);
@@ -1118,8 +1125,7 @@ static int build_tree(pnb_ctx* c, const float4* afg, size_t m, bool ref_com = fa
         s->max_level = h.max_level;
         // one record past the last node: the walk prefetches records k + 1 and skip <= total
         PNB_HIP(c, s->rec.ensure(size_t(total + 1) * sizeof(NodeRec)));
-        PNB_HIP(c, s->sums.ensure(size_t(total) * sizeof(double4)));
-        PNB_HIP(c, s->sums2.ensure(size_t(total) * sizeof(double4)));
+        PNB_HIP(c, s->sums.ensure(size_t(total) * sizeof(NodeSums)));
         PNB_HIP(c, s->level.ensure(total));
         PNB_HIP(c, s->width.ensure(size_t(total) * sizeof(float)));
         NodeMoments* mom = nullptr;
@@ -1135,8 +1141,8 @@ static int build_tree(pnb_ctx* c, const float4* afg, size_t m, bool ref_com = fa
         }
         NodeRec* rec = s->rec.as<NodeRec>();
         bh_node_kernel<<<g, 256, 0, c->stream>>>(sk, s->sp.as<float4>(), lcp, s->ip.as<int>(),
-                                                 cnt, m, info, rec, s->sums.as<double4>(),
-                                                 s->sums2.as<double4>(), s->level.as<uint8_t>(),
+                                                 cnt, m, info, rec, s->sums.as<NodeSums>(),
+                                                 s->level.as<uint8_t>(),
                                                  s->width.as<float>(), mom, quad);
         PNB_CHECK_LAUNCH(c);
         // bottom-up sums, one launch per level over that level's internal nodes
@@ -1153,7 +1159,7 @@ static int build_tree(pnb_ctx* c, const float4* afg, size_t m, bool ref_com = fa
             const uint32_t n_l = h.lvl_count[L];
             if (n_l == 0) continue;
             bh_up_kernel<<<div_up(n_l, 256), 256, 0, c->stream>>>(
-                rec, s->sums.as<double4>(), s->sums2.as<double4>(), ilist + off[L], n_l, mom, quad,
+                rec, s->sums.as<NodeSums>(), ilist + off[L], n_l, mom, quad,
                 desc, s->width.as<float>());
             PNB_CHECK_LAUNCH(c);
         }
@@ -1521,12 +1527,12 @@ int pnb_bh_tree_export(pnb_ctx* c, pnb_tree_node* out, size_t cap, uint32_t* ord
         NodeRec* rec = new NodeRec[T];
         uint8_t* lev = new uint8_t[T];
         float* wid = new float[T];
-        double4* sm = new double4[T];
+        NodeSums* sm = new NodeSums[T];
         NodeQuad* qd = s->has_moments ? new NodeQuad[T] : nullptr;
         hipError_t e = hipMemcpyAsync(rec, s->rec.ptr, T * sizeof(NodeRec), hipMemcpyDeviceToHost, c->stream);
         if (e == hipSuccess) e = hipMemcpyAsync(lev, s->level.ptr, T, hipMemcpyDeviceToHost, c->stream);
         if (e == hipSuccess) e = hipMemcpyAsync(wid, s->width.ptr, T * 4, hipMemcpyDeviceToHost, c->stream);
-        if (e == hipSuccess) e = hipMemcpyAsync(sm, s->sums.ptr, T * sizeof(double4), hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(sm, s->sums.ptr, T * sizeof(NodeSums), hipMemcpyDeviceToHost, c->stream);
         if (e == hipSuccess && qd)
             e = hipMemcpyAsync(qd, s->quad.ptr, T * sizeof(NodeQuad), hipMemcpyDeviceToHost, c->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
@@ -1535,7 +1541,7 @@ int pnb_bh_tree_export(pnb_ctx* c, pnb_tree_node* out, size_t cap, uint32_t* ord
                 pnb_tree_node& o = out[k];
                 o.com[0] = rec[k].cx; o.com[1] = rec[k].cy; o.com[2] = rec[k].cz;
                 o.mass = rec[k].mass;
-                o.mass64 = sm[k].x;
+                o.mass64 = sm[k].m.x;
                 o.width = wid[k];
                 o.skip = rec[k].skip;
                 o.begin = rec[k].begin;
