@@ -222,14 +222,21 @@ __global__ __launch_bounds__(256) void bh_bbox_kernel(const float4* __restrict__
 
 // square_with (partition.rs:136-153): centre = (min+max)/2, half = max(0, sizes...)/2,
 // box = centre -+ half, all in f32.
+// One wavefront: the partial boxes strided over the lanes, then a lane reduction (min and max
+// are exact, so the order does not matter).
 __global__ void bh_root_kernel(const float* __restrict__ part, int blocks, BhInfo* __restrict__ info) {
-    if (threadIdx.x != 0) return;
     float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (int b = 0; b < blocks; ++b)
+    for (int b = threadIdx.x; b < blocks; b += 64)
         for (int a = 0; a < 3; ++a) {
             mn[a] = fminf(mn[a], part[b * 6 + a]);
             mx[a] = fmaxf(mx[a], part[b * 6 + 3 + a]);
         }
+    for (int off = 32; off > 0; off >>= 1)
+        for (int a = 0; a < 3; ++a) {
+            mn[a] = fminf(mn[a], __shfl_xor(mn[a], off, 64));
+            mx[a] = fmaxf(mx[a], __shfl_xor(mx[a], off, 64));
+        }
+    if (threadIdx.x != 0) return;
     float centre[3], big = 0.0f;
     for (int a = 0; a < 3; ++a) {
         centre[a] = midpoint(mn[a], mx[a]);
