@@ -934,7 +934,7 @@ __device__ __forceinline__ float half_hi(uint32_t v) {
 
 // One wavefront walks the pre-order node records with one cursor (k, in SGPRs); lane decisions
 // are the reference's (see the file comment). The walk is bound by VALU issue (about 34 VALU,
-// 17 SALU and 2 scalar loads per step for the plain walk), so the step keeps everything
+// 15 SALU and 2 scalar loads per step for the plain walk), so the step keeps everything
 // wave-uniform in SGPRs: the records of both possible next nodes (k + 1, skip) are requested at
 // the top of the step and picked with scalar selects at its end, and WALK_QUAD fetches the
 // learned moment of node k only when some lane accepts it as an internal node (prefetching it for
