@@ -199,3 +199,30 @@ def test_wide_record_offsets_match_narrow(res, monkeypatch):
     for a, b in zip(runs["0"], runs["1"]):
         np.testing.assert_array_equal(a, b)
     assert np.abs(runs["0"][2] - runs["0"][0]).max() > 0  # the correction did something
+
+
+@pytest.mark.parametrize("m", [5, 64, 65, 257])
+def test_small_trees_match_oracle(res, orc, m):
+    """Trees from a handful of bodies: level lists with one or two internal levels, a root that
+    is the only internal node, and affected sets larger than the tree."""
+    a = workloads.direct_sum_case(300, seed=90 + m).particles
+    src = a[:m].copy()
+    bh(res, src, 0.5)
+    assert_same_tree(res, orc, src)
+    for theta in (0.0, 0.5):
+        got = bh(res, pa.Between(a, src), theta)
+        ref = orc.Tree(src).barnes_hut(a, theta, com64=True, accum64=True)[:, :3]
+        np.testing.assert_allclose(got, ref, rtol=2e-5, atol=1e-6)
+
+
+def test_all_bodies_at_one_position(res, orc):
+    """The whole affecting set at one point is one leaf (no internal node, no level list)."""
+    src = np.zeros((50, 4), np.float32)
+    src[:, :3] = (0.25, -1.0, 3.0)
+    src[:, 3] = 1.0 / 50
+    a = workloads.direct_sum_case(200, seed=97).particles
+    got = bh(res, pa.Between(a, src), 0.5)
+    ref = orc.Tree(src).barnes_hut(a, 0.5, com64=True, accum64=True)[:, :3]
+    np.testing.assert_allclose(got, ref, rtol=1e-6, atol=1e-7)
+    self_acc = bh(res, src, 0.5)
+    np.testing.assert_array_equal(self_acc, np.zeros((50, 3)))
