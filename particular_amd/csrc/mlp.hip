@@ -6,9 +6,11 @@
 // positions (w = softening^2), and per-lane targets (brute force minus Barnes-Hut acceleration)
 // with weights 1 / |a_bf|^2. One full-batch step:
 //   mlp_fwd_kernel    y = MLP(x) for every row (MFMA, 16-row tiles per wavefront)
-//   mlp_lane_kernel   per wavefront: pred_l = sum over its rows holding lane l of the field of
-//                     Q = M w^2 y at the lane (quad_field); loss; g_pred; g_y(row) = sum over its
-//                     lanes of M w^2 (d field / d Q)^T g_pred (quad_field_grad)
+//   mlp_pred_kernel   per (128-row chunk of a wavefront, lane l): the partial sum over the chunk's
+//                     rows holding l of the field of Q = M w^2 y at the lane (quad_field)
+//   mlp_resid_kernel  per (wavefront, lane): pred_l from the partials, loss, g_pred
+//   mlp_rowgrad_kernel per row: g_y = sum over its lanes of M w^2 (d field / d Q)^T g_pred
+//                     (quad_field_grad)
 //   mlp_bwd_kernel    recomputes the forward per 16-row tile and back-propagates with MFMA;
 //                     weight gradients accumulate in registers, one partial per wavefront
 //   mlp_reduce_kernel fixed-order sum of the partials (deterministic)
@@ -34,13 +36,20 @@ struct MlpState {
     unsigned long long dropped = 0;
     DevBuf feats, geom, mask, rwave, lanes, target, wgt, wave_rows;
     DevBuf y, gy, loss_part;
+    // the loss in three parallel passes over fixed chunks of each wavefront's rows (derived from
+    // wave_rows when the training set changes)
+    bool chunks_ok = false;
+    size_t chunks = 0;
+    float inv_count = 0.f;  // 1 / lanes with a positive weight
+    DevBuf chunk_wave, chunk_rb, wave_chunk, part, lane_g;
 };
 
 void mlp_free(MlpState* s) {
     if (!s) return;
     for (DevBuf* b : {&s->params, &s->m, &s->v, &s->grads, &s->weights, &s->partial, &s->loss_hist,
                       &s->feats, &s->geom, &s->mask, &s->rwave, &s->lanes, &s->target, &s->wgt,
-                      &s->wave_rows, &s->y, &s->gy, &s->loss_part})
+                      &s->wave_rows, &s->y, &s->gy, &s->loss_part, &s->chunk_wave, &s->chunk_rb,
+                      &s->wave_chunk, &s->part, &s->lane_g})
         b->release();
     delete s;
 }
@@ -123,19 +132,30 @@ __global__ __launch_bounds__(256) void mlp_fwd_kernel(const float* __restrict__ 
 
 // ---- per-lane aggregation, loss, gradient of y ---------------------------------------------------
 
-__global__ __launch_bounds__(64) void mlp_lane_kernel(const float* __restrict__ y,
+// The loss over the lanes, L = sum_lanes w |pred - target|^2 / count with
+// pred = sum over the lane's accepted rows of quad_field(y_row), and its gradient with respect to
+// each row's six outputs, in three passes that each keep every lane of the GPU busy:
+//  1. mlp_pred_kernel: per (chunk of MLP_CHUNK rows of one wavefront, lane), the partial pred;
+//  2. mlp_resid_kernel: per (wavefront, lane), the chunks' partials summed in chunk order, the
+//     residual, the loss share and g = 2 w (pred - target) / count;
+//  3. mlp_rowgrad_kernel: per row, sum over its masked lanes (in lane order) of
+//     g . d quad_field / d y, times the row's scale.
+// All sums run in a fixed order, so the loss and the gradients do not depend on scheduling.
+constexpr uint32_t MLP_CHUNK = 128;
+
+__global__ __launch_bounds__(64) void mlp_pred_kernel(const float* __restrict__ y,
                                                       const uint32_t* __restrict__ wave_rows,
+                                                      const uint32_t* __restrict__ chunk_wave,
+                                                      const uint32_t* __restrict__ chunk_rb,
                                                       const unsigned long long* __restrict__ mask,
                                                       const float4* __restrict__ geom,
                                                       const float4* __restrict__ lanes,
-                                                      const float* __restrict__ target,
-                                                      const float* __restrict__ wgt, float inv_count,
-                                                      float* __restrict__ gy,
-                                                      float* __restrict__ loss_part) {
-    const size_t w = blockIdx.x;
+                                                      float4* __restrict__ part) {
+    const uint32_t c = blockIdx.x;
     const int lane = threadIdx.x;
-    const uint32_t rb = wave_rows[w], re = wave_rows[w + 1];
-    const float4 p = lanes[w * 64 + lane];  // w = softening^2
+    const uint32_t w = chunk_wave[c], rb = chunk_rb[c];
+    const uint32_t re = min(rb + MLP_CHUNK, wave_rows[w + 1]);
+    const float4 p = lanes[size_t(w) * 64 + lane];  // w = softening^2
     float px = 0.f, py = 0.f, pz = 0.f;
     for (uint32_t r = rb; r < re; ++r) {
         if ((mask[r] >> lane) & 1ull) {
@@ -148,30 +168,59 @@ __global__ __launch_bounds__(64) void mlp_lane_kernel(const float* __restrict__ 
             quad_field(q, dx, dy, dz, inv, gm.w, px, py, pz);
         }
     }
+    part[size_t(c) * 64 + lane] = make_float4(px, py, pz, 0.f);
+}
+
+__global__ __launch_bounds__(64) void mlp_resid_kernel(const uint32_t* __restrict__ wave_chunk,
+                                                       const float4* __restrict__ part,
+                                                       const float* __restrict__ target,
+                                                       const float* __restrict__ wgt,
+                                                       float inv_count,
+                                                       float4* __restrict__ lane_g,
+                                                       float* __restrict__ loss_part) {
+    const size_t w = blockIdx.x;
+    const int lane = threadIdx.x;
+    float px = 0.f, py = 0.f, pz = 0.f;
+    for (uint32_t c = wave_chunk[w]; c < wave_chunk[w + 1]; ++c) {
+        const float4 v = part[size_t(c) * 64 + lane];
+        px += v.x;
+        py += v.y;
+        pz += v.z;
+    }
     const size_t tl = w * 64 + lane;
     const float dx = px - target[4 * tl], dy = py - target[4 * tl + 1], dz = pz - target[4 * tl + 2];
     const float wl = wgt[tl];
     float loss = wl * (dx * dx + dy * dy + dz * dz) * inv_count;
     const float k = 2.f * wl * inv_count;
-    const float gx = k * dx, gyv = k * dy, gz = k * dz;
+    lane_g[tl] = make_float4(k * dx, k * dy, k * dz, 0.f);
     for (int off = 32; off > 0; off >>= 1) loss += __shfl_xor(loss, off, 64);
     if (lane == 0) loss_part[w] = loss;
-    for (uint32_t r = rb; r < re; ++r) {
-        float g6[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-        const float4 gm = geom[r];
-        if ((mask[r] >> lane) & 1ull) {
-            const float ex = gm.x - p.x, ey = gm.y - p.y, ez = gm.z - p.z;
-            const float inv = rsqrtf(ex * ex + ey * ey + ez * ez + p.w);
-            quad_field_grad(ex, ey, ez, inv, gx, gyv, gz, g6);
-        }
-#pragma unroll
-        for (int j = 0; j < 6; ++j)
-            for (int off = 32; off > 0; off >>= 1) g6[j] += __shfl_xor(g6[j], off, 64);
-        if (lane == 0) {
-#pragma unroll
-            for (int j = 0; j < MLP_YS; ++j) gy[size_t(r) * MLP_YS + j] = j < 6 ? gm.w * g6[j] : 0.f;
-        }
+}
+
+__global__ __launch_bounds__(256) void mlp_rowgrad_kernel(size_t rows,
+                                                          const uint32_t* __restrict__ rwave,
+                                                          const unsigned long long* __restrict__ mask,
+                                                          const float4* __restrict__ geom,
+                                                          const float4* __restrict__ lanes,
+                                                          const float4* __restrict__ lane_g,
+                                                          float* __restrict__ gy) {
+    const size_t r = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    const size_t w = rwave[r];
+    unsigned long long m = mask[r];
+    const float4 gm = geom[r];
+    float g6[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    while (m) {  // the row's lanes in ascending order
+        const int l = __builtin_ctzll(m);
+        m &= m - 1;
+        const float4 p = lanes[w * 64 + l];
+        const float4 g = lane_g[w * 64 + l];
+        const float ex = gm.x - p.x, ey = gm.y - p.y, ez = gm.z - p.z;
+        const float inv = rsqrtf(ex * ex + ey * ey + ez * ez + p.w);
+        quad_field_grad(ex, ey, ez, inv, g.x, g.y, g.z, g6);
     }
+#pragma unroll
+    for (int j = 0; j < MLP_YS; ++j) gy[r * MLP_YS + j] = j < 6 ? gm.w * g6[j] : 0.f;
 }
 
 // ---- backward ------------------------------------------------------------------------------------
@@ -442,6 +491,7 @@ int mlp_append(pnb_ctx* c, const float* feats, const float4* geom, const unsigne
     s->rows = nr;
     s->waves = nwv;
     s->dropped += dropped;
+    s->chunks_ok = false;
     return PNB_OK;
 }
 
@@ -452,13 +502,41 @@ static int step_grads(pnb_ctx* c, float* loss_dev) {
     PNB_HIP(c, s->gy.ensure(std::max<size_t>(1, s->rows) * MLP_YS * 4));
     PNB_HIP(c, s->loss_part.ensure(s->waves * 4));
     PNB_HIP(c, s->partial.ensure(size_t(MLP_BWD_BLOCKS) * MLP_WAVES_PER_BLOCK * MLP_NPARAM * 4));
-    // lanes with a positive weight, counted on the host once per call
-    std::vector<float> hw(s->waves * 64);
-    PNB_HIP(c, hipMemcpyAsync(hw.data(), s->wgt.ptr, hw.size() * 4, hipMemcpyDeviceToHost, c->stream));
-    PNB_HIP(c, hipStreamSynchronize(c->stream));
-    size_t cnt = 0;
-    for (float x : hw) cnt += x > 0.f;
-    const float inv_count = 1.f / float(std::max<size_t>(cnt, 1));
+    if (!s->chunks_ok) {
+        // once per training set: lanes with a positive weight, and the row chunks
+        std::vector<float> hw(s->waves * 64);
+        std::vector<uint32_t> wr(s->waves + 1);
+        PNB_HIP(c, hipMemcpyAsync(hw.data(), s->wgt.ptr, hw.size() * 4, hipMemcpyDeviceToHost, c->stream));
+        PNB_HIP(c, hipMemcpyAsync(wr.data(), s->wave_rows.ptr, wr.size() * 4, hipMemcpyDeviceToHost, c->stream));
+        PNB_HIP(c, hipStreamSynchronize(c->stream));
+        size_t cnt = 0;
+        for (float x : hw) cnt += x > 0.f;
+        s->inv_count = 1.f / float(std::max<size_t>(cnt, 1));
+        std::vector<uint32_t> cw, crb, wc(s->waves + 1, 0);
+        for (size_t w = 0; w < s->waves; ++w) {
+            wc[w] = uint32_t(cw.size());
+            // at least one chunk per wavefront, so that every lane gets a prediction (0 if no row)
+            uint32_t r = wr[w];
+            do {
+                cw.push_back(uint32_t(w));
+                crb.push_back(r);
+                r += MLP_CHUNK;
+            } while (r < wr[w + 1]);
+        }
+        wc[s->waves] = uint32_t(cw.size());
+        s->chunks = cw.size();
+        PNB_HIP(c, s->chunk_wave.ensure(s->chunks * 4));
+        PNB_HIP(c, s->chunk_rb.ensure(s->chunks * 4));
+        PNB_HIP(c, s->wave_chunk.ensure((s->waves + 1) * 4));
+        PNB_HIP(c, s->part.ensure(s->chunks * 64 * sizeof(float4)));
+        PNB_HIP(c, s->lane_g.ensure(s->waves * 64 * sizeof(float4)));
+        PNB_HIP(c, hipMemcpyAsync(s->chunk_wave.ptr, cw.data(), s->chunks * 4, hipMemcpyHostToDevice, c->stream));
+        PNB_HIP(c, hipMemcpyAsync(s->chunk_rb.ptr, crb.data(), s->chunks * 4, hipMemcpyHostToDevice, c->stream));
+        PNB_HIP(c, hipMemcpyAsync(s->wave_chunk.ptr, wc.data(), wc.size() * 4, hipMemcpyHostToDevice, c->stream));
+        PNB_HIP(c, hipStreamSynchronize(c->stream));
+        s->chunks_ok = true;
+    }
+    const float inv_count = s->inv_count;
     const MlpWeights* wg = s->weights.as<MlpWeights>();
     const unsigned tiles = div_up(std::max<size_t>(s->rows, 1), 16);
     return timed(c, PNB_PROF_MLP, [&]() -> int {
@@ -467,11 +545,22 @@ static int step_grads(pnb_ctx* c, float* loss_dev) {
                 s->feats.as<float>(), s->rows, wg, s->y.as<float>());
             PNB_CHECK_LAUNCH(c);
         }
-        mlp_lane_kernel<<<unsigned(s->waves), 64, 0, c->stream>>>(
-            s->y.as<float>(), s->wave_rows.as<uint32_t>(), s->mask.as<unsigned long long>(),
-            s->geom.as<float4>(), s->lanes.as<float4>(), s->target.as<float>(), s->wgt.as<float>(), inv_count,
-            s->gy.as<float>(), s->loss_part.as<float>());
+        mlp_pred_kernel<<<unsigned(s->chunks), 64, 0, c->stream>>>(
+            s->y.as<float>(), s->wave_rows.as<uint32_t>(), s->chunk_wave.as<uint32_t>(),
+            s->chunk_rb.as<uint32_t>(), s->mask.as<unsigned long long>(), s->geom.as<float4>(),
+            s->lanes.as<float4>(), s->part.as<float4>());
         PNB_CHECK_LAUNCH(c);
+        mlp_resid_kernel<<<unsigned(s->waves), 64, 0, c->stream>>>(
+            s->wave_chunk.as<uint32_t>(), s->part.as<float4>(), s->target.as<float>(),
+            s->wgt.as<float>(), inv_count, s->lane_g.as<float4>(), s->loss_part.as<float>());
+        PNB_CHECK_LAUNCH(c);
+        if (s->rows) {
+            mlp_rowgrad_kernel<<<div_up(s->rows, 256), 256, 0, c->stream>>>(
+                s->rows, s->rwave.as<uint32_t>(), s->mask.as<unsigned long long>(),
+                s->geom.as<float4>(), s->lanes.as<float4>(), s->lane_g.as<float4>(),
+                s->gy.as<float>());
+            PNB_CHECK_LAUNCH(c);
+        }
         mlp_loss_sum_kernel<<<1, 256, 0, c->stream>>>(s->loss_part.as<float>(), s->waves, loss_dev);
         PNB_CHECK_LAUNCH(c);
         mlp_bwd_kernel<<<MLP_BWD_BLOCKS, 256, 0, c->stream>>>(s->feats.as<float>(), s->gy.as<float>(),
@@ -550,6 +639,7 @@ int pnb_mlp_clear_dataset(pnb_ctx* c) {
         c->mlp->rows = 0;
         c->mlp->waves = 0;
         c->mlp->dropped = 0;
+        c->mlp->chunks_ok = false;
     }
     return PNB_OK;
 }
