@@ -363,13 +363,27 @@ __global__ __launch_bounds__(256) void mlp_bwd_kernel(const float* __restrict__ 
     }
 }
 
-__global__ void mlp_reduce_kernel(const float* __restrict__ partial, int parts,
-                                  float* __restrict__ grads) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= MLP_NPARAM) return;
+// grads[k] = sum over the partials in a fixed order: 8 groups of 32 threads per workgroup, group
+// g summing its contiguous eighth of the partials for 32 parameters, then the eight group sums
+// added in group order.
+constexpr int MLP_RED_PARAMS = 32;
+__global__ __launch_bounds__(256) void mlp_reduce_kernel(const float* __restrict__ partial,
+                                                         int parts, float* __restrict__ grads) {
+    __shared__ float acc[8][MLP_RED_PARAMS];
+    const int g = threadIdx.x / MLP_RED_PARAMS, t = threadIdx.x % MLP_RED_PARAMS;
+    const int k = blockIdx.x * MLP_RED_PARAMS + t;
+    const int per = (parts + 7) / 8, p0 = g * per, p1 = min(parts, p0 + per);
     float s = 0.f;
-    for (int p = 0; p < parts; ++p) s += partial[size_t(p) * MLP_NPARAM + k];
-    grads[k] = s;
+    if (k < MLP_NPARAM)
+        for (int p = p0; p < p1; ++p) s += partial[size_t(p) * MLP_NPARAM + k];
+    acc[g][t] = s;
+    __syncthreads();
+    if (g == 0 && k < MLP_NPARAM) {
+        float total = acc[0][t];
+#pragma unroll
+        for (int q = 1; q < 8; ++q) total += acc[q][t];
+        grads[k] = total;
+    }
 }
 
 __global__ void mlp_adam_kernel(float* __restrict__ p, const float* __restrict__ g,
@@ -566,7 +580,7 @@ static int step_grads(pnb_ctx* c, float* loss_dev) {
         mlp_bwd_kernel<<<MLP_BWD_BLOCKS, 256, 0, c->stream>>>(s->feats.as<float>(), s->gy.as<float>(),
                                                              s->rows, wg, s->partial.as<float>());
         PNB_CHECK_LAUNCH(c);
-        mlp_reduce_kernel<<<div_up(MLP_NPARAM, 256), 256, 0, c->stream>>>(
+        mlp_reduce_kernel<<<div_up(MLP_NPARAM, MLP_RED_PARAMS), 256, 0, c->stream>>>(
             s->partial.as<float>(), MLP_BWD_BLOCKS * MLP_WAVES_PER_BLOCK, s->grads.as<float>());
         PNB_CHECK_LAUNCH(c);
         return PNB_OK;
