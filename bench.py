@@ -200,12 +200,15 @@ def time_at_error(rows, rms):
 def bench_bh(args):
     """Barnes-Hut on two Plummer spheres (4,194,304 bodies, mu = 1/N): plain at theta 0.2 ... 0.8
     and with the learned correction at theta 0.3 / 0.4 (the network that outputs each node's own
-    quadrupole, learned.quadrupole_params). Tree build + walk (+ node network) device time from
-    HIP events, accepted pair terms and wavefront node steps from the walk's counters, accuracy
-    against the FP64 direct sum on a seeded sample, and the particles/s ratio of the corrected to
-    the plain method at equal rms error (1e-4): each method is also run at the angle where its
-    error meets 1e-4 (secant steps on the measured curve), and the times are interpolated
-    between the rows closest to the target."""
+    quadrupole, learned.quadrupole_params), both with the reference's geometric criterion; then
+    both with the relative criterion (a node also opens when its first neglected term exceeds
+    alpha times the particle's estimated acceleration; theta = 1 as geometric floor) at several
+    alpha. Tree build + walk (+ node network) device time from HIP events (the relative walk's
+    time includes its estimate pass), accepted pair terms and wavefront node steps from the walk's
+    counters, accuracy against the FP64 direct sum on a seeded sample, and the particles/s ratios
+    at equal rms error (1e-4): each curve is also run at the knob (theta or alpha) where its error
+    meets 1e-4 (secant steps on the measured curve), and the times are interpolated between the
+    rows closest to the target."""
     import torch
 
     import particular_amd as pa
@@ -227,8 +230,9 @@ def bench_bh(args):
     learned.init(res, 0)
     learned.set_params(res, learned.quadrupole_params())
 
-    def run(theta, mlp):
-        alg = pa.BarnesHut(res, theta, pa.Acceleration.checked(), mlp=mlp)
+    def run(knob, mlp, rel=False):
+        theta, alpha = (1.0, knob) if rel else (knob, None)
+        alg = pa.BarnesHut(res, theta, pa.Acceleration.checked(), mlp=mlp, relative=alpha)
         acc = alg.compute(pa.Between(dev, dev))
         torch.cuda.synchronize()
         reps = 3
@@ -251,6 +255,9 @@ def bench_bh(args):
         # the walk's counters accumulate over the reps since the last build: one build per call
         row = {
             "theta": theta,
+            "criterion": "relative" if rel else "geometric",
+            "alpha": alpha,
+            "knob": knob,
             "learned_correction": mlp,
             "mparticles_per_s": s.n / wall / 1e6,
             "ms_per_eval": wall * 1e3,
@@ -265,16 +272,18 @@ def bench_bh(args):
             "accepted_G_terms_per_s": accepted / (walk_ms * 1e-3) / 1e9,
             **err,
         }
-        log(f"BH{' + correction' if mlp else ''} theta={theta}: {row['ms_per_eval']:.2f} ms "
+        log(f"BH{' + correction' if mlp else ''} theta={theta}"
+            f"{f' alpha={alpha}' if rel else ''}: {row['ms_per_eval']:.2f} ms "
             f"({row['mparticles_per_s']:.1f} Mp/s), build {build_ms:.2f} ms, walk {walk_ms:.2f} ms, "
             f"rms {err['rms_rel']:.2e}")
         return row
 
     target = 1e-4
 
-    def refine(rows, mlp):
-        """Adds runs at the angle where the rms error meets the target: secant steps of
-        log(rms) against log(theta) between the closest bracketing rows (at most two)."""
+    def refine(rows, mlp, rel=False):
+        """Adds runs at the knob (theta, or alpha for the relative criterion) where the rms
+        error meets the target: secant steps of log(rms) against log(knob) between the closest
+        bracketing rows (at most two)."""
         for _ in range(2):
             lo = [r for r in rows if r["rms_rel"] <= target]
             hi = [r for r in rows if r["rms_rel"] > target]
@@ -285,28 +294,49 @@ def bench_bh(args):
             if abs(np.log(a["rms_rel"] / target)) < 0.02:
                 return
             f = np.log(target / a["rms_rel"]) / np.log(b["rms_rel"] / a["rms_rel"])
-            th = float(np.exp(np.log(a["theta"]) + f * (np.log(b["theta"]) - np.log(a["theta"]))))
-            rows.append(run(round(th, 4), mlp))
+            k = float(np.exp(np.log(a["knob"]) + f * (np.log(b["knob"]) - np.log(a["knob"]))))
+            rows.append(run(float(f"{k:.4g}"), mlp, rel))
 
-    plain = [run(float(th), False) for th in args.bh_thetas.split(",") if th]
-    nn = [run(float(th), True) for th in args.bh_nn_thetas.split(",") if th]
-    refine(plain, False)
-    refine(nn, True)
+    def knobs(text):
+        return [float(v) for v in text.split(",") if v]
+
+    plain = [run(th, False) for th in knobs(args.bh_thetas)]
+    nn = [run(th, True) for th in knobs(args.bh_nn_thetas)]
+    plain_rel = [run(a, False, True) for a in knobs(args.bh_rel_alphas)]
+    nn_rel = [run(a, True, True) for a in knobs(args.bh_nn_rel_alphas)]
+    for rows, mlp, rel in ((plain, False, False), (nn, True, False), (plain_rel, False, True),
+                           (nn_rel, True, True)):
+        refine(rows, mlp, rel)
     res.close()
-    t_plain = time_at_error(plain, target) if plain else None
-    t_nn = time_at_error(nn, target) if nn else None
+
+    def at(rows):
+        return time_at_error(rows, target) if rows else None
+
+    def ratio(a, b):
+        return (a / b) if (a and b) else None
+
+    t_plain, t_nn, t_prel, t_nrel = at(plain), at(nn), at(plain_rel), at(nn_rel)
     comparison = {
         "target_rms": target,
         "plain_ms_at_target": t_plain,
         "corrected_ms_at_target": t_nn,
         "plain_mparticles_per_s_at_target": s.n / t_plain / 1e3 if t_plain else None,
         "corrected_mparticles_per_s_at_target": s.n / t_nn / 1e3 if t_nn else None,
-        "speedup_at_equal_error": (t_plain / t_nn) if (t_plain and t_nn) else None,
+        "speedup_at_equal_error": ratio(t_plain, t_nn),
+        "relative_criterion": {
+            "plain_ms_at_target": t_prel,
+            "corrected_ms_at_target": t_nrel,
+            "corrected_mparticles_per_s_at_target": s.n / t_nrel / 1e3 if t_nrel else None,
+            "speedup_vs_plain_geometric": ratio(t_plain, t_nrel),
+            "speedup_vs_plain_relative": ratio(t_prel, t_nrel),
+            "plain_relative_vs_plain_geometric": ratio(t_plain, t_prel),
+        },
         "network": "learned.quadrupole_params (outputs each node's own quadrupole; on held-out "
                    "validation the trained networks did not beat it, see particular_amd/nn_bh.py)",
     }
-    log(f"equal error {target}: plain {t_plain} ms, corrected {t_nn} ms")
-    return plain + nn, comparison, s
+    log(f"equal error {target}: plain {t_plain} ms, corrected {t_nn} ms; relative criterion: "
+        f"plain {t_prel} ms, corrected {t_nrel} ms")
+    return plain + nn + plain_rel + nn_rel, comparison, s
 
 
 def cpu_baseline_bh(s, theta, sample):
@@ -380,6 +410,10 @@ def main():
     ap.add_argument("--bh-thetas", default="0.2,0.25,0.3,0.5,0.8", help="plain Barnes-Hut angles")
     ap.add_argument("--bh-nn-thetas", default="0.3,0.4",
                     help="angles with the learned correction (empty: none)")
+    ap.add_argument("--bh-rel-alphas", default="3e-5,1e-4",
+                    help="plain Barnes-Hut with the relative criterion (empty: none)")
+    ap.add_argument("--bh-nn-rel-alphas", default="1e-4,3e-4",
+                    help="relative criterion with the learned correction (empty: none)")
     ap.add_argument("--bh-cpu-sample", type=int, default=16384)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="first copy a buffer of this size with torch (HBM counter calibration)")

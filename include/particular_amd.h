@@ -127,6 +127,22 @@ int pnb_bf_clear_tuning(pnb_ctx *ctx);
  * The folds are sequential, so a node of k bodies costs k dependent steps (the root: all of
  * them); meant for reference parity checks, not for speed. */
 #define PNB_BH_REF_COM 2u
+/* Relative opening criterion (no reference counterpart; the criterion of tree codes that bound
+ * each node's force error by a fraction of the particle's own acceleration, e.g. GADGET-2,
+ * Springel 2005, section 3.1). An internal node is opened when the geometric rule above opens it
+ * OR when M w^p / r^(p+2) > alpha |a~|: M the node's mass, w its width, r^2 = d^2 + eps^2, |a~|
+ * the particle's acceleration from an estimate pass (a plain walk at estimate_theta over the same
+ * tree; its time is part of the walk's), and p the order of the first term the walk leaves out:
+ * 2 for the plain walk (the quadrupole), 3 with PNB_BH_MLP (whose learned moment stands in for
+ * the quadrupole). Each node's error is then bounded relative to the particle's own acceleration;
+ * `theta` stays a geometric floor (1 opens every node whose width exceeds the distance). alpha
+ * and estimate_theta come from pnb_bh_set_relative (defaults 1e-4 and 1.2). Combines with the
+ * other flags. */
+#define PNB_BH_RELATIVE 4u
+#define PNB_BH_ALL_FLAGS (PNB_BH_MLP | PNB_BH_REF_COM | PNB_BH_RELATIVE)
+/* Parameters of PNB_BH_RELATIVE for the context's later calls: alpha >= 0 (0 opens every internal
+ * node with mass), estimate_theta > 0. */
+int pnb_bh_set_relative(pnb_ctx *ctx, float alpha, float estimate_theta);
 
 int pnb_bh_accel_f32(pnb_ctx *ctx, const float *affected, size_t n, const float *affecting,
                      size_t m, float theta, float softening, int checked, unsigned flags,

@@ -353,3 +353,82 @@ int orc_bh_f32(const orc_tree *t, const float *affected, size_t n, float theta, 
     }
     return rc;
 }
+
+/* The relative opening criterion of the GPU backend (no reference counterpart): the traversal of
+ * orc_bh_f32 (sequential.rs:456-506) in which an internal node is also opened when
+ * lhs > (A * r2) * r2, with r2 = d2 + softening^2, w2 = width * width, A = est[i] (alpha |a~| of
+ * affected particle i) and lhs = mass * w2 (order 2) or ((mass * w2) * sqrt(w2)) * (1 / sqrt(r2))
+ * (order 3), every f32 operation rounded (IEEE sqrt and division). est = +inf gives orc_bh_f32's
+ * decisions; est = 0 opens every internal node of positive mass. */
+int orc_bh_rel_f32(const orc_tree *t, const float *affected, size_t n, float theta,
+                   float softening, const float *est, int order, int com64, int accum64,
+                   float *out, uint32_t *visits, int threads) {
+    if (!t || (n && (!affected || !out || !est)) || (order != 2 && order != 3)) return 1;
+    set_threads(threads);
+    const orc_node *nodes = t->nodes;
+    size_t stack_cap = (size_t)(t->max_level + 2) * 8 + 8;
+    float th2 = theta * theta;
+    float eps2 = softening * softening;
+    long long nn = (long long)n;
+    int rc = 0;
+#pragma omp parallel
+    {
+        int32_t *stack = (int32_t *)malloc(stack_cap * sizeof(int32_t));
+        if (!stack) {
+#pragma omp atomic write
+            rc = 3;
+        }
+#pragma omp for schedule(dynamic, 64)
+        for (long long i = 0; i < nn; ++i) {
+            if (!stack) continue;
+            const float *a = affected + 4 * i;
+            const float A = est[i];
+            float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+            double dx64 = 0.0, dy64 = 0.0, dz64 = 0.0;
+            uint32_t acc = 0;
+            size_t sp = 0;
+            if (t->len) stack[sp++] = 0;
+            while (sp) {
+                int32_t id = stack[--sp];
+                if (id < 0) continue;
+                const orc_node *nd = &nodes[id];
+                float cx, cy, cz;
+                if (com64) { cx = (float)nd->com64[0]; cy = (float)nd->com64[1]; cz = (float)nd->com64[2]; }
+                else { cx = nd->com[0]; cy = nd->com[1]; cz = nd->com[2]; }
+                float ms = com64 ? (float)nd->mass64 : nd->mass;
+                float d2 = norm2f(cx - a[0], cy - a[1], cz - a[2]);
+                if (d2 == 0.0f) continue;
+                float w = nd->bmax[0] - nd->bmin[0];
+                float w2 = w * w;
+                int open = 0;
+                if (nd->internal) {
+                    float r2 = d2 + eps2;
+                    float lhs = ms * w2;
+                    if (order == 3) lhs = (lhs * sqrtf(w2)) * (1.0f / sqrtf(r2));
+                    float rhs = (A * r2) * r2;
+                    open = (th2 * d2) < w2 || rhs < lhs;
+                }
+                if (open) {
+                    for (int c = 0; c < 8; ++c) stack[sp++] = nd->child[c];
+                } else {
+                    if (accum64) {
+                        double ex = (double)cx - a[0], ey = (double)cy - a[1], ez = (double)cz - a[2];
+                        double q2 = ex * ex + ey * ey + ez * ez + (double)softening * softening;
+                        double s = (double)ms / (q2 * sqrt(q2));
+                        dx64 += ex * s; dy64 += ey * s; dz64 += ez * s;
+                    } else {
+                        float ox, oy, oz;
+                        pair(cx, cy, cz, ms, a[0], a[1], a[2], softening, 1, &ox, &oy, &oz);
+                        sx = sx + ox; sy = sy + oy; sz = sz + oz;
+                    }
+                    acc++;
+                }
+            }
+            if (accum64) { sx = (float)dx64; sy = (float)dy64; sz = (float)dz64; }
+            out[4 * i] = sx; out[4 * i + 1] = sy; out[4 * i + 2] = sz; out[4 * i + 3] = 0.0f;
+            if (visits) visits[i] = acc;
+        }
+        free(stack);
+    }
+    return rc;
+}

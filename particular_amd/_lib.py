@@ -26,6 +26,10 @@ PROF_BATCH = 4
 PROF_MLP = 5
 PROF_SORT = 6
 
+BH_MLP = 1
+BH_REF_COM = 2
+BH_RELATIVE = 4
+
 _c = ctypes
 _vp = _c.c_void_p
 _sz = _c.c_size_t
@@ -72,6 +76,7 @@ SIGNATURES = {
     "pnb_bh_tree_size": (_i, [_vp, _c.POINTER(_sz), _pi]),
     "pnb_bh_tree_export": (_i, [_vp, _vp, _sz, _vp, _sz]),
     "pnb_bh_stats": (_i, [_vp, _pu64, _pu64]),
+    "pnb_bh_set_relative": (_i, [_vp, _f, _f]),
     "pnb_batch_set_tuning": (_i, [_vp, _sz, _i]),
     "pnb_batch_init": (_i, [_vp, _vp, _vp, _vp, _vp, _sz, _f, _i]),
     "pnb_batch_run": (_i, [_vp, _sz, _d, _i]),

@@ -119,6 +119,12 @@ struct BhState {
     DevBuf mom, quad, desc;  // second moments (FP64), normalised quadrupoles, child shares
     DevBuf qh;               // per node: MLP(node) as halves, the learned correction's moment
     bool has_moments = false;
+    // relative opening criterion (PNB_BH_RELATIVE): walk records carrying M w^3, per affected
+    // particle alpha |a| from the estimate pass, and the criterion's parameters
+    DevBuf rrec, est;
+    bool rrec_ok = false;
+    int rrec_cubic = 0;
+    float rel_alpha = 1e-4f, rel_theta_est = 1.2f;
     DevBuf info, bbox_part, hist, bits, afg_tmp;
     DevBuf akeys, akeys_alt, aidx, aidx_alt, apts, h_aff, h_afg, h_out;
     DevBuf c_starts, c_pts, c_bh, c_bf, c_list, c_count, c_target, c_wgt, c_rows, c_rwave,
@@ -131,7 +137,7 @@ void bh_free(BhState* s) {
     if (!s) return;
     for (DevBuf* b : {&s->keys, &s->keys_alt, &s->idx, &s->idx_alt, &s->sp, &s->lcp, &s->cnt,
                       &s->ip, &s->key2, &s->rec, &s->sums, &s->level, &s->width,
-                      &s->ilist, &s->lvl, &s->mom, &s->quad, &s->desc, &s->qh,
+                      &s->ilist, &s->lvl, &s->mom, &s->quad, &s->desc, &s->qh, &s->rrec, &s->est,
                       &s->info, &s->bbox_part, &s->hist, &s->bits, &s->afg_tmp, &s->akeys,
                       &s->akeys_alt, &s->aidx, &s->aidx_alt, &s->apts, &s->h_aff, &s->h_afg,
                       &s->h_out, &s->c_starts, &s->c_pts, &s->c_bh, &s->c_bf, &s->c_list,
@@ -879,7 +885,7 @@ __device__ __forceinline__ uint64_t pack2(uint32_t lo, uint32_t hi) {
     return uint64_t(lo) | (uint64_t(hi) << 32);
 }
 
-template <typename Off>
+template <typename Off, bool SEL3>
 __device__ __forceinline__ void advance(uint64_t open, Off kn, Off ks, Off& k, NodeRec& r,
                                         const NodeRec& rn, const NodeRec& rs) {
     const uint64_t n0 = pack2(__float_as_uint(rn.cx), __float_as_uint(rn.cy));
@@ -888,11 +894,38 @@ __device__ __forceinline__ void advance(uint64_t open, Off kn, Off ks, Off& k, N
     const uint64_t s1 = pack2(__float_as_uint(rs.cz), __float_as_uint(rs.mass));
     const uint64_t n2 = pack2(__float_as_uint(rn.w2), rn.skip);
     const uint64_t s2 = pack2(__float_as_uint(rs.w2), rs.skip);
-    // begin/end are not needed, but naming them makes each record one 32-byte scalar load
+    // begin/end: selected only when SEL3 (the relative criterion keeps M w^3 in begin); otherwise
+    // naming them makes each record one 32-byte scalar load
     const uint64_t n3 = pack2(rn.begin, rn.end), s3 = pack2(rs.begin, rs.end);
-    uint64_t o0, o1, o2;
+    uint64_t o0, o1, o2, o3 = 0;
     Off ok;
-    if constexpr (sizeof(Off) == 4) {
+    if constexpr (SEL3) {
+        if constexpr (sizeof(Off) == 4) {
+            asm volatile(
+                "s_cmp_lg_u64 %5, 0\n\t"
+                "s_cselect_b64 %0, %6, %7\n\t"
+                "s_cselect_b64 %1, %8, %9\n\t"
+                "s_cselect_b64 %2, %10, %11\n\t"
+                "s_cselect_b64 %3, %14, %15\n\t"
+                "s_cselect_b32 %4, %12, %13"
+                : "=&s"(o0), "=&s"(o1), "=&s"(o2), "=&s"(o3), "=&s"(ok)
+                : "s"(open), "s"(n0), "s"(s0), "s"(n1), "s"(s1), "s"(n2), "s"(s2), "s"(kn),
+                  "s"(ks), "s"(n3), "s"(s3)
+                : "scc");
+        } else {
+            asm volatile(
+                "s_cmp_lg_u64 %5, 0\n\t"
+                "s_cselect_b64 %0, %6, %7\n\t"
+                "s_cselect_b64 %1, %8, %9\n\t"
+                "s_cselect_b64 %2, %10, %11\n\t"
+                "s_cselect_b64 %3, %14, %15\n\t"
+                "s_cselect_b64 %4, %12, %13"
+                : "=&s"(o0), "=&s"(o1), "=&s"(o2), "=&s"(o3), "=&s"(ok)
+                : "s"(open), "s"(n0), "s"(s0), "s"(n1), "s"(s1), "s"(n2), "s"(s2), "s"(kn),
+                  "s"(ks), "s"(n3), "s"(s3)
+                : "scc");
+        }
+    } else if constexpr (sizeof(Off) == 4) {
         asm volatile(
             "s_cmp_lg_u64 %4, 0\n\t"
             "s_cselect_b64 %0, %5, %6\n\t"
@@ -921,6 +954,10 @@ __device__ __forceinline__ void advance(uint64_t open, Off kn, Off ks, Off& k, N
     r.mass = __uint_as_float(uint32_t(o1 >> 32));
     r.w2 = __uint_as_float(uint32_t(o2));
     r.skip = uint32_t(o2 >> 32);
+    if constexpr (SEL3) {
+        r.begin = uint32_t(o3);
+        r.end = uint32_t(o3 >> 32);
+    }
     k = ok;
 }
 
@@ -940,7 +977,13 @@ __device__ __forceinline__ float half_hi(uint32_t v) {
 // learned moment of node k only when some lane accepts it as an internal node (prefetching it for
 // both candidates costs more scalar-cache misses than it hides).
 // WIDE: 64-bit byte offsets, for trees whose records span 4 GiB or more (2^27 nodes).
-template <int MODE, bool WIDE>
+// REL: the relative criterion. Each lane holds A = alpha |a~| (est_in, from an estimate pass) and
+// the records carry K = M w^p in `begin` (0 for leaves); an internal node is opened when the
+// geometric rule opens it or when the size of its first neglected term, M w^p / r^(p+2)
+// (r^2 = d^2 + eps^2), exceeds A: p = 2 for the plain walk (K > A r^4, the quadrupole is
+// neglected), p = 3 for WALK_QUAD (K / r > A r^4, the octupole is the first term it lacks).
+// est_out (estimate pass): writes alpha |a| per particle, in walk order, instead of out.
+template <int MODE, bool WIDE, bool REL>
 __global__ __launch_bounds__(256) void bh_walk_kernel(const NodeRec* __restrict__ nodes,
                                                       uint32_t total,
                                                       const float4* __restrict__ pts,
@@ -948,11 +991,14 @@ __global__ __launch_bounds__(256) void bh_walk_kernel(const NodeRec* __restrict_
                                                       uint32_t n, float theta2, float eps2,
                                                       float4* __restrict__ out,
                                                       BhInfo* __restrict__ info, WalkRows rows,
-                                                      const QuadH* __restrict__ qh) {
+                                                      const QuadH* __restrict__ qh,
+                                                      const float* __restrict__ est_in,
+                                                      float* __restrict__ est_out, float alpha) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63;
     const bool valid = i < n;
     const float4 p = pts[valid ? i : n - 1];
+    const float A = REL ? est_in[valid ? i : n - 1] : 0.f;
     float ax = 0.f, ay = 0.f, az = 0.f, cx = 0.f, cy = 0.f, cz = 0.f;
     using Off = std::conditional_t<WIDE, uint64_t, uint32_t>;
     Off resume = valid ? Off(0) : ~Off(0);
@@ -975,19 +1021,28 @@ __global__ __launch_bounds__(256) void bh_walk_kernel(const NodeRec* __restrict_
         const float dx = sub_rn(r.cx, p.x), dy = sub_rn(r.cy, p.y), dz = sub_rn(r.cz, p.z);
         const float d2 = norm2_ref(dx, dy, dz);
         const bool nz = d2 != 0.0f;
-        const bool crit = mul_rn(theta2, d2) < r.w2;
+        // REL with WALK_QUAD: the reciprocal distance is needed by every lane's criterion, not
+        // only by the accepting lanes' pair terms
+        constexpr bool REL_INV = REL && MODE == WALK_QUAD;
+        const float r2_rel = REL ? add_rn(d2, eps2) : 0.f;
+        const float inv_rel = REL_INV ? __builtin_amdgcn_rsqf(r2_rel) : 0.f;
+        const float K = __uint_as_float(r.begin);
+        const float lhs = REL ? (REL_INV ? mul_rn(K, inv_rel) : K) : 0.f;
+        const float rhs = REL ? mul_rn(mul_rn(A, r2_rel), r2_rel) : 0.f;
+        const bool crit = mul_rn(theta2, d2) < r.w2 || (REL && rhs < lhs);
         const bool open = active && nz && crit;
         const bool acc = active && nz && !crit;
-        // the wavefront's open mask from the three compares' own lane masks (a ballot of the
-        // combined bool would first copy it into a VGPR)
+        // the wavefront's open mask from the compares' own lane masks (a ballot of the combined
+        // bool would first copy it into a VGPR)
         const uint64_t live_mask = __builtin_amdgcn_ballot_w64(active) &
                                    __builtin_amdgcn_ballot_w64(nz);
-        const uint64_t crit_mask = __builtin_amdgcn_fcmpf(mul_rn(theta2, d2), r.w2, 4 /*olt*/);
+        uint64_t crit_mask = __builtin_amdgcn_fcmpf(mul_rn(theta2, d2), r.w2, 4 /*olt*/);
+        if constexpr (REL) crit_mask |= __builtin_amdgcn_fcmpf(rhs, lhs, 4 /*olt*/);
         const uint64_t open_mask = live_mask & crit_mask;
         const uint64_t acc_mask = live_mask & ~crit_mask;
         if (acc) {
-            const float r2 = add_rn(d2, eps2);
-            const float inv = __builtin_amdgcn_rsqf(r2);
+            const float r2 = REL ? r2_rel : add_rn(d2, eps2);
+            const float inv = REL_INV ? inv_rel : __builtin_amdgcn_rsqf(r2);
             const float s = (r.mass * inv) * (inv * inv);
             float yx, yy, yz;
             if (MODE == WALK_QUAD && __float_as_int(r.w2) >= 0) {  // internal (leaves: -inf)
@@ -1036,13 +1091,18 @@ __global__ __launch_bounds__(256) void bh_walk_kernel(const NodeRec* __restrict_
         if (!open) resume = max(resume, so);
         ++steps;
         // skip > k for every node, and skip = k + 1 for a leaf, so the two candidates agree there
-        advance<Off>(open_mask, ko + Off(32), so, ko, r, r_next, r_skip);
+        advance<Off, REL>(open_mask, ko + Off(32), so, ko, r, r_next, r_skip);
     }
     if (MODE == WALK_ROWS && lane == 0 && wave_slot * 64 < n) {  // wavefronts holding particles
         rows.count[wave_slot] = nrows;
         atomicMax(&info->max_rows, nrows);
     }
-    if (valid) out[perm ? perm[i] : i] = make_float4(ax, ay, az, 0.f);
+    if (valid) {
+        if (est_out)
+            est_out[i] = alpha * sqrtf(ax * ax + ay * ay + az * az);
+        else
+            out[perm ? perm[i] : i] = make_float4(ax, ay, az, 0.f);
+    }
     unsigned long long acc64 = accepted;
     for (int off = 32; off > 0; off >>= 1) acc64 += __shfl_xor(acc64, off, 64);
     if (lane == 0) {
@@ -1059,6 +1119,7 @@ static int build_tree(pnb_ctx* c, const float4* afg, size_t m, bool ref_com = fa
     BhState* s = c->bh;
     s->built = false;
     s->has_moments = false;
+    s->rrec_ok = false;
     s->m = m;
     s->nodes = 0;
     s->internal = 0;
@@ -1183,28 +1244,56 @@ static int build_tree(pnb_ctx* c, const float4* afg, size_t m, bool ref_com = fa
 
 const MlpWeights* mlp_weights(pnb_ctx* c);  // mlp.hip: nullptr until initialised
 
-template <int MODE>
+// Walk records for the relative criterion: node k's record with begin = the bits of K = M w^2
+// (cubic: M w^3; f32, each operation rounded; 0 for leaves, which are never opened) and end = 0,
+// for the total + 1 records (the last is the sentinel the walk prefetches).
+__global__ void bh_rel_rec_kernel(const NodeRec* __restrict__ rec, size_t total, int cubic,
+                                  NodeRec* __restrict__ rrec) {
+    const size_t k = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k > total) return;
+    NodeRec r = rec[k];
+    float K = 0.f;
+    if (k < total && __float_as_int(r.w2) >= 0) {
+        K = mul_rn(r.mass, r.w2);
+        if (cubic) K = mul_rn(K, __fsqrt_rn(r.w2));
+    }
+    r.begin = __float_as_uint(K);
+    r.end = 0;
+    rrec[k] = r;
+}
+
+template <int MODE, bool REL>
 static void launch_walk(bool wide, unsigned grid, hipStream_t st, const NodeRec* nr, uint32_t total,
                         const float4* pts, const uint32_t* perm, uint32_t n, float theta2,
                         float eps2, float4* out, BhInfo* info, const WalkRows& rows,
-                        const QuadH* qh) {
+                        const QuadH* qh, const float* est_in = nullptr, float* est_out = nullptr,
+                        float alpha = 0.f) {
     if (wide)
-        bh_walk_kernel<MODE, true><<<grid, 256, 0, st>>>(nr, total, pts, perm, n, theta2, eps2, out,
-                                                         info, rows, qh);
+        bh_walk_kernel<MODE, true, REL><<<grid, 256, 0, st>>>(
+            nr, total, pts, perm, n, theta2, eps2, out, info, rows, qh, est_in, est_out, alpha);
     else
-        bh_walk_kernel<MODE, false><<<grid, 256, 0, st>>>(nr, total, pts, perm, n, theta2, eps2,
-                                                          out, info, rows, qh);
+        bh_walk_kernel<MODE, false, REL><<<grid, 256, 0, st>>>(
+            nr, total, pts, perm, n, theta2, eps2, out, info, rows, qh, est_in, est_out, alpha);
 }
 
-// quad: add the learned correction (the per-node moments in s->qh).
+// quad: add the learned correction (the per-node moments in s->qh). rel: the relative criterion
+// in addition to the geometric one: an estimate pass (plain walk at s->rel_theta_est) gives each
+// particle alpha |a~|, then the walk opens a node also when M w^p / r^(p+2) > alpha |a~| (p = 2,
+// or 3 with quad). Both passes count in the walk's time and in the step/accepted counters.
 static int walk(pnb_ctx* c, const float4* pts, const uint32_t* perm, size_t n, float theta,
-                float softening, float4* out, const WalkRows* rows = nullptr, bool quad = false) {
+                float softening, float4* out, const WalkRows* rows = nullptr, bool quad = false,
+                bool rel = false) {
     BhState* s = c->bh;
     const float theta2 = theta * theta;  // (self.theta * self.theta), sequential.rs:491
     const float eps2 = softening * softening;
+    const uint32_t total = uint32_t(s->nodes);
+    rel = rel && total > 0 && !rows;
+    if (rel) {
+        PNB_HIP(c, s->rrec.ensure((size_t(total) + 1) * sizeof(NodeRec)));
+        PNB_HIP(c, s->est.ensure(n * sizeof(float)));
+    }
     return timed(c, PNB_PROF_BH_WALK, [&]() -> int {
         const unsigned grid = div_up(n, 256);
-        const uint32_t total = uint32_t(s->nodes);
         BhInfo* info = s->info.as<BhInfo>();
         const NodeRec* nr = s->rec.as<NodeRec>();
         // 64-bit record offsets once the records (and the one past the last) reach 4 GiB;
@@ -1212,15 +1301,39 @@ static int walk(pnb_ctx* c, const float4* pts, const uint32_t* perm, size_t n, f
         const char* force = std::getenv("PNB_WIDE_WALK");
         const bool wide = (uint64_t(total) + 1) * sizeof(NodeRec) >= (uint64_t(1) << 32) ||
                           (force && force[0] == '1');
-        if (quad)
-            launch_walk<WALK_QUAD>(wide, grid, c->stream, nr, total, pts, perm, uint32_t(n), theta2,
-                                   eps2, out, info, WalkRows{}, s->qh.as<QuadH>());
+        if (rel) {
+            const int cubic = quad ? 1 : 0;
+            if (!s->rrec_ok || s->rrec_cubic != cubic) {
+                bh_rel_rec_kernel<<<div_up(size_t(total) + 1, 256), 256, 0, c->stream>>>(
+                    nr, total, cubic, s->rrec.as<NodeRec>());
+                PNB_CHECK_LAUNCH(c);
+                s->rrec_ok = true;
+                s->rrec_cubic = cubic;
+            }
+            const float te = s->rel_theta_est;
+            float* est = s->est.as<float>();
+            launch_walk<WALK_PLAIN, false>(wide, grid, c->stream, nr, total, pts, perm, uint32_t(n),
+                                           te * te, eps2, out, info, WalkRows{}, nullptr, nullptr,
+                                           est, s->rel_alpha);
+            PNB_CHECK_LAUNCH(c);
+            const NodeRec* rr = s->rrec.as<NodeRec>();
+            if (quad)
+                launch_walk<WALK_QUAD, true>(wide, grid, c->stream, rr, total, pts, perm,
+                                             uint32_t(n), theta2, eps2, out, info, WalkRows{},
+                                             s->qh.as<QuadH>(), est);
+            else
+                launch_walk<WALK_PLAIN, true>(wide, grid, c->stream, rr, total, pts, perm,
+                                              uint32_t(n), theta2, eps2, out, info, WalkRows{},
+                                              nullptr, est);
+        } else if (quad)
+            launch_walk<WALK_QUAD, false>(wide, grid, c->stream, nr, total, pts, perm, uint32_t(n),
+                                          theta2, eps2, out, info, WalkRows{}, s->qh.as<QuadH>());
         else if (rows)
-            launch_walk<WALK_ROWS>(wide, grid, c->stream, nr, total, pts, perm, uint32_t(n), theta2,
-                                   eps2, out, info, *rows, nullptr);
+            launch_walk<WALK_ROWS, false>(wide, grid, c->stream, nr, total, pts, perm, uint32_t(n),
+                                          theta2, eps2, out, info, *rows, nullptr);
         else
-            launch_walk<WALK_PLAIN>(wide, grid, c->stream, nr, total, pts, perm, uint32_t(n),
-                                    theta2, eps2, out, info, WalkRows{}, nullptr);
+            launch_walk<WALK_PLAIN, false>(wide, grid, c->stream, nr, total, pts, perm, uint32_t(n),
+                                           theta2, eps2, out, info, WalkRows{}, nullptr);
         PNB_CHECK_LAUNCH(c);
         return PNB_OK;
     });
@@ -1253,7 +1366,7 @@ static int node_network(pnb_ctx* c) {
 // neighbours, walks, and scatters results back to the caller's order.
 static int accel_dev(pnb_ctx* c, const float4* affected, size_t n, const float4* afg, size_t m,
                      float theta, float softening, float4* out, bool mlp = false,
-                     bool ref_com = false) {
+                     bool ref_com = false, bool rel = false) {
     if (n == 0) return PNB_OK;
     if (n > 0x7fffffffu) return fail(c, PNB_E_ARG, "Barnes-Hut supports at most 2^31-1 particles");
     if (mlp && !mlp_weights(c))
@@ -1264,7 +1377,7 @@ static int accel_dev(pnb_ctx* c, const float4* affected, size_t n, const float4*
     if (quad)
         if (int rc = node_network(c)) return rc;
     if (affected == afg && n == m)
-        return walk(c, s->sp.as<float4>(), s->sidx, n, theta, softening, out, nullptr, quad);
+        return walk(c, s->sp.as<float4>(), s->sidx, n, theta, softening, out, nullptr, quad, rel);
     const unsigned g = div_up(n, 256);
     PNB_HIP(c, s->akeys.ensure(n * 8));
     PNB_HIP(c, s->akeys_alt.ensure(n * 8));
@@ -1286,7 +1399,7 @@ static int accel_dev(pnb_ctx* c, const float4* affected, size_t n, const float4*
         return rc;
     bh_gather_kernel<<<g, 256, 0, c->stream>>>(affected, n, si, s->apts.as<float4>());
     PNB_CHECK_LAUNCH(c);
-    return walk(c, s->apts.as<float4>(), si, n, theta, softening, out, nullptr, quad);
+    return walk(c, s->apts.as<float4>(), si, n, theta, softening, out, nullptr, quad, rel);
 }
 
 static int copy_h2d(pnb_ctx* c, DevBuf& b, const float* src, size_t count) {
@@ -1446,12 +1559,12 @@ int pnb_bh_accel_f32_dev(pnb_ctx* c, const float* affected, size_t n, const floa
                          float* out) {
     PNB_BH_ARGS_CHECK();
     (void)checked;  // the traversal skips zero-distance nodes for both (sequential.rs:485-487)
-    if (flags & ~(PNB_BH_MLP | PNB_BH_REF_COM)) return fail(c, PNB_E_ARG, "unknown Barnes-Hut flags");
+    if (flags & ~PNB_BH_ALL_FLAGS) return fail(c, PNB_E_ARG, "unknown Barnes-Hut flags");
     if (int rc = activate(c)) return rc;
     return accel_dev(c, reinterpret_cast<const float4*>(affected), n,
                      reinterpret_cast<const float4*>(affecting), m, theta, softening,
                      reinterpret_cast<float4*>(out), (flags & PNB_BH_MLP) != 0,
-                     (flags & PNB_BH_REF_COM) != 0);
+                     (flags & PNB_BH_REF_COM) != 0, (flags & PNB_BH_RELATIVE) != 0);
 }
 
 int pnb_bh_accel_f32(pnb_ctx* c, const float* affected, size_t n, const float* affecting,
@@ -1492,9 +1605,10 @@ int pnb_reordered_bh_accel_f32_dev(pnb_ctx* c, const float* particles, size_t n,
     const float4* afg = s->afg_tmp.as<float4>();
     if (m == n) afg = p;  // every particle is affecting: same storage, the tree order is reused
     (void)checked;
-    if (flags & ~(PNB_BH_MLP | PNB_BH_REF_COM)) return fail(c, PNB_E_ARG, "unknown Barnes-Hut flags");
+    if (flags & ~PNB_BH_ALL_FLAGS) return fail(c, PNB_E_ARG, "unknown Barnes-Hut flags");
     return accel_dev(c, p, n, afg, m, theta, softening, reinterpret_cast<float4*>(out),
-                     (flags & PNB_BH_MLP) != 0, (flags & PNB_BH_REF_COM) != 0);
+                     (flags & PNB_BH_MLP) != 0, (flags & PNB_BH_REF_COM) != 0,
+                     (flags & PNB_BH_RELATIVE) != 0);
 }
 
 int pnb_reordered_bh_accel_f32(pnb_ctx* c, const float* particles, size_t n, float theta,
@@ -1512,6 +1626,18 @@ int pnb_reordered_bh_accel_f32(pnb_ctx* c, const float* particles, size_t n, flo
         return rc;
     PNB_HIP(c, hipMemcpyAsync(out, s->h_out.ptr, n * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
     PNB_HIP(c, hipStreamSynchronize(c->stream));
+    return PNB_OK;
+}
+
+int pnb_bh_set_relative(pnb_ctx* c, float alpha, float estimate_theta) {
+    if (!c) return PNB_E_ARG;
+    if (!(alpha >= 0.0f) || !std::isfinite(alpha))
+        return fail(c, PNB_E_ARG, "alpha must be finite and >= 0");
+    if (!(estimate_theta > 0.0f) || !std::isfinite(estimate_theta))
+        return fail(c, PNB_E_ARG, "estimate_theta must be finite and > 0");
+    if (!c->bh) c->bh = new BhState();
+    c->bh->rel_alpha = alpha;
+    c->bh->rel_theta_est = estimate_theta;
     return PNB_OK;
 }
 

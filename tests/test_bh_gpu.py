@@ -226,3 +226,139 @@ def test_all_bodies_at_one_position(res, orc):
     np.testing.assert_allclose(got, ref, rtol=1e-6, atol=1e-7)
     self_acc = bh(res, src, 0.5)
     np.testing.assert_array_equal(self_acc, np.zeros((50, 3)))
+
+
+# ---- the relative opening criterion (PNB_BH_RELATIVE; no reference counterpart) ----
+
+
+def rel_bh(res, theta, alpha, mlp=False, estimate_theta=1.2, interaction=None):
+    return pa.BarnesHut(res, theta, interaction or pa.Acceleration.checked(), mlp=mlp,
+                        relative=alpha, estimate_theta=estimate_theta)
+
+
+def oracle_estimate(tree, particles, alpha, estimate_theta):
+    """alpha |a~| from the oracle's plain traversal at estimate_theta (the GPU's estimate pass;
+    the two differ only in the last bits of the summed accelerations)."""
+    a, vis = tree.barnes_hut(particles, estimate_theta, com64=True, accum64=True,
+                             return_visits=True)
+    return np.float32(alpha) * np.linalg.norm(a[:, :3], axis=1).astype(np.float32), vis
+
+
+@pytest.mark.parametrize("alpha", [1e-3, 1e-4])
+def test_relative_criterion_matches_oracle(res, orc, alpha):
+    """Plain walk with the relative rule (p = 2) against the oracle's restatement fed the oracle's
+    own estimate. A decision can flip where the two sides' last-bit rounding straddles the
+    threshold; each flip moves one particle by at most about alpha |a|, hence the alpha term in
+    the bar. The accepted-term counters (estimate pass + walk) must agree to 1e-4."""
+    p = workloads.direct_sum_case(20000, seed=41).particles
+    t = orc.Tree(p)
+    got = rel_bh(res, 1.0, alpha).compute(p)
+    _, accepted = pa.bh_stats(res)
+    est, est_vis = oracle_estimate(t, p, alpha, 1.2)
+    ref, vis = t.barnes_hut_rel(p, 1.0, est, order=2, com64=True, accum64=True,
+                                return_visits=True)
+    e = metrics.summary(got, ref)
+    print("alpha", alpha, "vs oracle", e, "accepted", accepted, int(est_vis.sum() + vis.sum()))
+    assert e["rms_rel"] < 5e-7 + 0.02 * alpha and e["max_rel"] < 1e-4 + 3 * alpha
+    assert abs(accepted - int(est_vis.sum() + vis.sum())) <= 1e-4 * accepted
+    exact = orc.direct_f64(p, p)
+    assert metrics.rms_rel(got, exact) < 5 * alpha
+
+
+def test_relative_criterion_limits_are_geometric_walks(res):
+    """alpha = 1e30 never opens by the relative rule: bit-identical to the geometric walk at the
+    same theta (plain and corrected); alpha = 0 opens every massive internal node: bit-identical
+    to theta = 0 (where the corrected walk, accepting only leaves, equals the plain one)."""
+    from particular_amd import learned
+
+    p = workloads.direct_sum_case(30000, seed=43).particles
+    learned.init(res, 0)
+    learned.set_params(res, learned.quadrupole_params())
+    chk = pa.Acceleration.checked()
+    for mlp in (False, True):
+        geo = pa.BarnesHut(res, 0.6, chk, mlp=mlp).compute(p)
+        np.testing.assert_array_equal(rel_bh(res, 0.6, 1e30, mlp=mlp).compute(p), geo)
+        np.testing.assert_array_equal(rel_bh(res, 0.6, 0.0, mlp=mlp).compute(p),
+                                      pa.BarnesHut(res, 0.0, chk).compute(p))
+
+
+def test_relative_corrected_walk(res, orc):
+    """With the learned quadrupole network the rule bounds the octupole (p = 3): at the same
+    alpha the corrected walk is more accurate than the plain relative walk and accepts fewer
+    terms; against the oracle's p = 3 decisions the plain-part sums agree."""
+    from particular_amd import learned
+
+    p = workloads.direct_sum_case(20000, seed=45).particles
+    learned.init(res, 0)
+    learned.set_params(res, learned.quadrupole_params())
+    exact = orc.direct_f64(p, p)
+    alpha = 1e-4
+    plain = rel_bh(res, 1.0, alpha).compute(p)
+    _, acc_plain = pa.bh_stats(res)
+    corr = rel_bh(res, 1.0, alpha, mlp=True).compute(p)
+    _, acc_corr = pa.bh_stats(res)
+    e_plain, e_corr = metrics.rms_rel(plain, exact), metrics.rms_rel(corr, exact)
+    print("plain", e_plain, acc_plain, "corrected", e_corr, acc_corr)
+    assert e_corr < e_plain and acc_corr < acc_plain
+    assert e_corr < 5 * alpha
+    # the decisions: the oracle's p = 3 traversal accepts the same number of terms
+    t = orc.Tree(p)
+    est, est_vis = oracle_estimate(t, p, alpha, 1.2)
+    _, vis = t.barnes_hut_rel(p, 1.0, est, order=3, com64=True, return_visits=True)
+    assert abs(acc_corr - int(est_vis.sum() + vis.sum())) <= 1e-4 * acc_corr
+
+
+def test_relative_wide_offsets_match_narrow(res, monkeypatch):
+    from particular_amd import learned
+
+    p = workloads.direct_sum_case(30000, seed=47).particles
+    learned.init(res, 0)
+    learned.set_params(res, learned.quadrupole_params())
+    runs = {}
+    for wide in ("0", "1"):
+        monkeypatch.setenv("PNB_WIDE_WALK", wide)
+        runs[wide] = [rel_bh(res, 1.0, 1e-3, mlp=m).compute(p) for m in (False, True)]
+    for a, b in zip(runs["0"], runs["1"]):
+        np.testing.assert_array_equal(a, b)
+
+
+def test_relative_storages_and_edge_cases(res, orc):
+    """Between distinct sets (softened, massless affected), Reordered, device storage, an empty
+    affecting set, a single affecting body, a whole affecting set at one point, bad parameters."""
+    import torch
+
+    s = workloads.massive_massless_case(n_massive=5000, n_total=30000, softening=1e-3)
+    inter = pa.AccelerationSoftened.checked(s.softening)
+    r = pa.Reordered.new(s.particles)
+    massive = r.affecting()
+    got = rel_bh(res, 1.0, 1e-4, interaction=inter).compute(r)
+    t = orc.Tree(massive)
+    est, _ = t.barnes_hut(s.particles, 1.2, softening=s.softening, com64=True, accum64=True,
+                          return_visits=True)
+    est = np.float32(1e-4) * np.linalg.norm(est[:, :3], axis=1).astype(np.float32)
+    ref = t.barnes_hut_rel(s.particles, 1.0, est, order=2, softening=s.softening, com64=True,
+                           accum64=True)
+    assert metrics.rms_rel(got, ref) < 5e-7 + 2e-6
+    got2 = rel_bh(res, 1.0, 1e-4, interaction=inter).compute(pa.Between(s.particles, massive))
+    np.testing.assert_array_equal(got, got2)
+    dev = rel_bh(res, 1.0, 1e-4, interaction=inter).compute(
+        pa.Reordered.new(torch.from_numpy(s.particles).cuda()))
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(dev.cpu().numpy()[:, :3], got)
+
+    a = workloads.direct_sum_case(100, seed=48).particles
+    empty = np.zeros((0, 4), np.float32)
+    np.testing.assert_array_equal(rel_bh(res, 1.0, 1e-4).compute(pa.Between(a, empty)),
+                                  np.zeros((100, 3)))
+    one = a[:1].copy()
+    np.testing.assert_allclose(rel_bh(res, 1.0, 1e-4).compute(pa.Between(a, one)),
+                               orc.barnes_hut(a, one, 0.7)[:, :3], rtol=1e-6)
+    src = np.zeros((50, 4), np.float32)
+    src[:, :3] = (0.25, -1.0, 3.0)
+    src[:, 3] = 1.0 / 50
+    np.testing.assert_allclose(rel_bh(res, 1.0, 1e-4).compute(pa.Between(a, src)),
+                               orc.Tree(src).barnes_hut(a, 0.5, com64=True, accum64=True)[:, :3],
+                               rtol=1e-6, atol=1e-7)
+    L = pa._lib.lib()
+    for alpha, te in ((-1.0, 1.2), (float("nan"), 1.2), (1e-4, 0.0), (1e-4, float("inf"))):
+        assert L.pnb_bh_set_relative(res.handle, alpha, te) == pa._lib.PNB_E_ARG

@@ -220,6 +220,93 @@ def test_bh_error_vs_f64_is_monopole_sized(orc):
 
 
 
+# ---- the relative opening criterion (the GPU backend's PNB_BH_RELATIVE; no reference
+# counterpart): open an internal node also when M w^p / r^(p+2) > est = alpha |a~| ----
+
+
+@pytest.mark.parametrize("order", [2, 3])
+def test_relative_criterion_limits(orc, order):
+    """est = +inf never opens by the relative rule (the geometric traversal, bit for bit); est = 0
+    opens every massive internal node (theta = 0: every leaf accepted)."""
+    s = workloads.direct_sum_case(1500, seed=31).particles
+    t = orc.Tree(s)
+    geo, vg = t.barnes_hut(s, 0.6, com64=True, return_visits=True)
+    rel, vr = t.barnes_hut_rel(s, 0.6, np.full(len(s), np.inf, np.float32), order=order,
+                               return_visits=True)
+    np.testing.assert_array_equal(rel, geo)
+    np.testing.assert_array_equal(vr, vg)
+    all_, va = t.barnes_hut(s, 0.0, com64=True, return_visits=True)
+    rel0, v0 = t.barnes_hut_rel(s, 0.6, np.zeros(len(s), np.float32), order=order,
+                                return_visits=True)
+    np.testing.assert_array_equal(rel0, all_)
+    np.testing.assert_array_equal(v0, va)
+
+
+@pytest.mark.parametrize("order", [2, 3])
+def test_relative_criterion_threshold_closed_form(orc, order):
+    """A compact cluster seen from outside along a ray: the root is accepted exactly beyond the
+    distance r* where M w^p / r*^(p+2) = alpha |a(r*)| (|a| from an FP64 direct sum, r* by
+    bisection), and opened inside it. A wrong power of w or r, or the wrong mass, moves r*."""
+    rng = np.random.default_rng(5)
+    cl = np.zeros((40, 4), np.float32)
+    cl[:, :3] = rng.uniform(-0.05, 0.05, (40, 3)).astype(np.float32)
+    cl[:, 3] = np.float32(1.0 / 40)
+    t = orc.Tree(cl)
+    root = t.nodes()[0]
+    M = float(np.float32(root["mass64"]))
+    w = float(root["bmax"][0] - root["bmin"][0])
+    c = root["com64"].astype(np.float32).astype(np.float64)
+    u = np.array([0.8, 0.36, 0.48])
+    alpha = 1e-3
+
+    def at(r):
+        return (c + r * u).astype(np.float32)
+
+    def amag(r):
+        p = np.zeros((1, 4), np.float32)
+        p[0, :3] = at(r)
+        return float(np.linalg.norm(numpy_direct_f64(p, cl)[0]))
+
+    def g(r):
+        x = at(r).astype(np.float64)
+        rr = float(np.linalg.norm(x - c))
+        return M * w ** order / rr ** (order + 2) - alpha * amag(r)
+
+    lo, hi = 2 * w, 100 * w  # g(lo) > 0 > g(hi)
+    assert g(lo) > 0 > g(hi)
+    for _ in range(60):
+        mid = 0.5 * (lo + hi)
+        lo, hi = (mid, hi) if g(mid) > 0 else (lo, mid)
+    rstar = 0.5 * (lo + hi)
+    for f, opened in ((0.98, True), (1.02, False)):
+        r = f * rstar
+        p = np.zeros((1, 4), np.float32)
+        p[0, :3] = at(r)
+        est = np.array([alpha * amag(r)], np.float32)
+        _, v = t.barnes_hut_rel(p, 1e6, est, order=order, return_visits=True)
+        assert (v[0] > 1) == opened, (f, v[0], rstar / w)
+
+
+@pytest.mark.parametrize("order", [2, 3])
+def test_relative_criterion_error_follows_alpha(orc, order):
+    """Against the FP64 direct sum the error falls with alpha. For order 2 every accepted node's
+    neglected quadrupole is bounded by alpha |a|, so the rms error stays within a few alpha; for
+    order 3 the plain monopole walk keeps its unbounded quadrupole error (the learned correction
+    supplies that term), so only the monotone decrease is asserted."""
+    s = workloads.direct_sum_case(2048, seed=33).particles
+    t = orc.Tree(s)
+    ref = orc.direct_f64(s, s)
+    amag = np.linalg.norm(ref, axis=1).astype(np.float32)
+    errs = []
+    for alpha in (1e-2, 1e-3, 1e-4):
+        got = t.barnes_hut_rel(s, 1e6, np.float32(alpha) * amag, order=order, com64=True,
+                               accum64=True)
+        errs.append(metrics.rms_rel(got, ref))
+        if order == 2:
+            assert errs[-1] < 5 * alpha, (alpha, errs[-1])
+    assert errs[0] > errs[1] > errs[2], errs
+
+
 @pytest.mark.parametrize("theta,tol", [(None, 1e-2), (0.0, 1e-2), (0.5, 1e-1)])
 def test_circular_orbit_oracle(orc, theta, tol):
     def accel(p):
