@@ -182,27 +182,39 @@ struct BruteForce {
     }
 };
 
-// parallel::BarnesHut replacement (particular/src/parallel.rs:298-367).
+// parallel::BarnesHut replacement (particular/src/parallel.rs:298-367). relative_alpha >= 0 adds
+// the relative opening criterion (PNB_BH_RELATIVE, no reference counterpart); the default keeps
+// the reference's criterion alone.
 template <class T>
 struct BarnesHut {
     HipResources& resources;
     float theta;
     T interaction;
+    float relative_alpha = -1.0f;
+    float estimate_theta = 1.2f;
+
+    unsigned flags() {
+        if (relative_alpha < 0.0f) return 0u;
+        resources.check(pnb_bh_set_relative(resources.get(), relative_alpha, estimate_theta));
+        return PNB_BH_RELATIVE;
+    }
 
     std::vector<Vec3> compute(Between<Slice<GravitationalField>, Slice<GravitationalField>> b) {
         std::vector<float> out(4 * b.affected.len);
+        const unsigned fl = flags();
         resources.check(pnb_bh_accel_f32(resources.get(), detail::f(b.affected), b.affected.len,
                                          detail::f(b.affecting), b.affecting.len, theta,
-                                         interaction.softening(), T::checked_v, 0u, out.data()));
+                                         interaction.softening(), T::checked_v, fl, out.data()));
         return detail::unpad(out);
     }
     std::vector<Vec3> compute(Slice<GravitationalField> s) { return compute({s, s}); }
     std::vector<Vec3> compute(const Ordered& o) { return compute({o.particles(), o.affecting()}); }
     std::vector<Vec3> compute(const Reordered& r) {
         std::vector<float> out(4 * r.unordered.len);
+        const unsigned fl = flags();
         resources.check(pnb_reordered_bh_accel_f32(resources.get(), detail::f(r.unordered),
                                                    r.unordered.len, theta,
-                                                   interaction.softening(), T::checked_v, 0u,
+                                                   interaction.softening(), T::checked_v, fl,
                                                    out.data()));
         return detail::unpad(out);
     }

@@ -99,6 +99,10 @@ int main(int argc, char** argv) {
         hip::BarnesHut<Acceleration<true>> bh5{res, 0.5f, Acceleration<true>::checked()};
         acceleration_error(bh5, 5e-1f, "hip::BarnesHut(0.5)");
         circular_orbit(bh5, orbits, 1e-1f, "hip::BarnesHut(0.5)");
+        // the relative opening criterion (no reference counterpart), theta = 1 as the floor
+        hip::BarnesHut<Acceleration<true>> bhr{res, 1.0f, Acceleration<true>::checked(), 1e-3f};
+        acceleration_error(bhr, 5e-1f, "hip::BarnesHut(relative 1e-3)");
+        circular_orbit(bhr, orbits, 1e-1f, "hip::BarnesHut(relative 1e-3)");
         // GpuCompute-style sugar over a Reordered storage
         const std::vector<GravitationalField> ps = {{{0, 0, 0}, 2}, {{1, 0, 0}, 0}};
         Reordered r{Slice<GravitationalField>(ps)};
