@@ -180,7 +180,8 @@ int pnb_bh_tree_size(pnb_ctx *ctx, size_t *nodes, int *max_level);
 /* Copies the nodes (cap >= nodes) and, if `order` is not NULL, the original index of every
  * affecting particle in tree order (m >= affecting count). */
 int pnb_bh_tree_export(pnb_ctx *ctx, pnb_tree_node *nodes, size_t cap, uint32_t *order, size_t m);
-/* Counters of the walks since the last build: wavefront node steps and accepted pair terms. */
+/* Counters of the walks since the last build: wavefront node steps and accepted pair terms (with
+ * PNB_BH_RELATIVE, the estimate pass's included). */
 int pnb_bh_stats(pnb_ctx *ctx, uint64_t *wave_steps, uint64_t *accepted);
 
 /* Two-dimensional particles (glam::Vec2 / f32; the reference's 2D GPU layout,
