@@ -131,8 +131,10 @@ int pnb_bf_clear_tuning(pnb_ctx *ctx);
  * each node's force error by a fraction of the particle's own acceleration, e.g. GADGET-2,
  * Springel 2005, section 3.1). An internal node is opened when the geometric rule above opens it
  * OR when M w^p / r^(p+2) > alpha |a~|: M the node's mass, w its width, r^2 = d^2 + eps^2, |a~|
- * the particle's acceleration from an estimate pass (a plain walk at estimate_theta over the same
- * tree; its time is part of the walk's), and p the order of the first term the walk leaves out:
+ * the acceleration from an estimate pass (a plain walk at estimate_theta over the same tree; its
+ * time is part of the walk's) at the particle itself or, by default, at the centroid of the 64
+ * consecutive particles of its wavefront in the walk's spatial order (1/64 of the cost, the same
+ * rms error), and p the order of the first term the walk leaves out:
  * 2 for the plain walk (the quadrupole), 3 with PNB_BH_MLP (whose learned moment stands in for
  * the quadrupole). Each node's error is then bounded relative to the particle's own acceleration;
  * `theta` stays a geometric floor (1 opens every node whose width exceeds the distance). alpha
@@ -141,8 +143,9 @@ int pnb_bf_clear_tuning(pnb_ctx *ctx);
 #define PNB_BH_RELATIVE 4u
 #define PNB_BH_ALL_FLAGS (PNB_BH_MLP | PNB_BH_REF_COM | PNB_BH_RELATIVE)
 /* Parameters of PNB_BH_RELATIVE for the context's later calls: alpha >= 0 (0 opens every internal
- * node with mass), estimate_theta > 0. */
-int pnb_bh_set_relative(pnb_ctx *ctx, float alpha, float estimate_theta);
+ * node with mass), estimate_theta > 0, per_particle != 0 for one estimate per particle instead of
+ * one per wavefront. */
+int pnb_bh_set_relative(pnb_ctx *ctx, float alpha, float estimate_theta, int per_particle);
 
 int pnb_bh_accel_f32(pnb_ctx *ctx, const float *affected, size_t n, const float *affecting,
                      size_t m, float theta, float softening, int checked, unsigned flags,

@@ -121,10 +121,11 @@ struct BhState {
     bool has_moments = false;
     // relative opening criterion (PNB_BH_RELATIVE): walk records carrying M w^3, per affected
     // particle alpha |a| from the estimate pass, and the criterion's parameters
-    DevBuf rrec, est;
+    DevBuf rrec, est, cen;
     bool rrec_ok = false;
     int rrec_cubic = 0;
     float rel_alpha = 1e-4f, rel_theta_est = 1.2f;
+    bool rel_per_particle = false;  // else one estimate per wavefront, at its centroid
     DevBuf info, bbox_part, hist, bits, afg_tmp;
     DevBuf akeys, akeys_alt, aidx, aidx_alt, apts, h_aff, h_afg, h_out;
     DevBuf c_starts, c_pts, c_bh, c_bf, c_list, c_count, c_target, c_wgt, c_rows, c_rwave,
@@ -138,6 +139,7 @@ void bh_free(BhState* s) {
     for (DevBuf* b : {&s->keys, &s->keys_alt, &s->idx, &s->idx_alt, &s->sp, &s->lcp, &s->cnt,
                       &s->ip, &s->key2, &s->rec, &s->sums, &s->level, &s->width,
                       &s->ilist, &s->lvl, &s->mom, &s->quad, &s->desc, &s->qh, &s->rrec, &s->est,
+                      &s->cen,
                       &s->info, &s->bbox_part, &s->hist, &s->bits, &s->afg_tmp, &s->akeys,
                       &s->akeys_alt, &s->aidx, &s->aidx_alt, &s->apts, &s->h_aff, &s->h_afg,
                       &s->h_out, &s->c_starts, &s->c_pts, &s->c_bh, &s->c_bf, &s->c_list,
@@ -977,7 +979,8 @@ __device__ __forceinline__ float half_hi(uint32_t v) {
 // learned moment of node k only when some lane accepts it as an internal node (prefetching it for
 // both candidates costs more scalar-cache misses than it hides).
 // WIDE: 64-bit byte offsets, for trees whose records span 4 GiB or more (2^27 nodes).
-// REL: the relative criterion. Each lane holds A = alpha |a~| (est_in, from an estimate pass) and
+// REL: the relative criterion. Each lane holds A = alpha |a~| (est_in[i >> est_shift], from an
+// estimate pass per particle, est_shift = 0, or per wavefront, est_shift = 6) and
 // the records carry K = M w^p in `begin` (0 for leaves); an internal node is opened when the
 // geometric rule opens it or when the size of its first neglected term, M w^p / r^(p+2)
 // (r^2 = d^2 + eps^2), exceeds A: p = 2 for the plain walk (K > A r^4, the quadrupole is
@@ -993,12 +996,13 @@ __global__ __launch_bounds__(256) void bh_walk_kernel(const NodeRec* __restrict_
                                                       BhInfo* __restrict__ info, WalkRows rows,
                                                       const QuadH* __restrict__ qh,
                                                       const float* __restrict__ est_in,
+                                                      uint32_t est_shift,
                                                       float* __restrict__ est_out, float alpha) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63;
     const bool valid = i < n;
     const float4 p = pts[valid ? i : n - 1];
-    const float A = REL ? est_in[valid ? i : n - 1] : 0.f;
+    const float A = REL ? est_in[(valid ? i : n - 1) >> est_shift] : 0.f;
     float ax = 0.f, ay = 0.f, az = 0.f, cx = 0.f, cy = 0.f, cz = 0.f;
     using Off = std::conditional_t<WIDE, uint64_t, uint32_t>;
     Off resume = valid ? Off(0) : ~Off(0);
@@ -1262,24 +1266,45 @@ __global__ void bh_rel_rec_kernel(const NodeRec* __restrict__ rec, size_t total,
     rrec[k] = r;
 }
 
+// Centroid of each wavefront's particles (64 consecutive particles in walk order; the last one
+// may hold fewer): a fixed-order butterfly sum per wavefront, {x, y, z, 0}.
+__global__ __launch_bounds__(256) void bh_wave_centroid_kernel(const float4* __restrict__ pts,
+                                                               uint32_t n,
+                                                               float4* __restrict__ cen) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = i < n;
+    const float4 p = valid ? pts[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    float x = p.x, y = p.y, z = p.z, c = valid ? 1.f : 0.f;
+    for (int off = 32; off > 0; off >>= 1) {
+        x += __shfl_xor(x, off, 64);
+        y += __shfl_xor(y, off, 64);
+        z += __shfl_xor(z, off, 64);
+        c += __shfl_xor(c, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0 && i < n) cen[i >> 6] = make_float4(x / c, y / c, z / c, 0.f);
+}
+
 template <int MODE, bool REL>
 static void launch_walk(bool wide, unsigned grid, hipStream_t st, const NodeRec* nr, uint32_t total,
                         const float4* pts, const uint32_t* perm, uint32_t n, float theta2,
                         float eps2, float4* out, BhInfo* info, const WalkRows& rows,
-                        const QuadH* qh, const float* est_in = nullptr, float* est_out = nullptr,
-                        float alpha = 0.f) {
+                        const QuadH* qh, const float* est_in = nullptr, uint32_t est_shift = 0,
+                        float* est_out = nullptr, float alpha = 0.f) {
     if (wide)
         bh_walk_kernel<MODE, true, REL><<<grid, 256, 0, st>>>(
-            nr, total, pts, perm, n, theta2, eps2, out, info, rows, qh, est_in, est_out, alpha);
+            nr, total, pts, perm, n, theta2, eps2, out, info, rows, qh, est_in, est_shift, est_out,
+            alpha);
     else
         bh_walk_kernel<MODE, false, REL><<<grid, 256, 0, st>>>(
-            nr, total, pts, perm, n, theta2, eps2, out, info, rows, qh, est_in, est_out, alpha);
+            nr, total, pts, perm, n, theta2, eps2, out, info, rows, qh, est_in, est_shift, est_out,
+            alpha);
 }
 
 // quad: add the learned correction (the per-node moments in s->qh). rel: the relative criterion
-// in addition to the geometric one: an estimate pass (plain walk at s->rel_theta_est) gives each
-// particle alpha |a~|, then the walk opens a node also when M w^p / r^(p+2) > alpha |a~| (p = 2,
-// or 3 with quad). Both passes count in the walk's time and in the step/accepted counters.
+// in addition to the geometric one: an estimate pass (plain walk at s->rel_theta_est) gives
+// alpha |a~| per particle or, by default, per wavefront at the centroid of its 64 particles; then
+// the walk opens a node also when M w^p / r^(p+2) > alpha |a~| (p = 2, or 3 with quad). Both
+// passes count in the walk's time and in the step/accepted counters.
 static int walk(pnb_ctx* c, const float4* pts, const uint32_t* perm, size_t n, float theta,
                 float softening, float4* out, const WalkRows* rows = nullptr, bool quad = false,
                 bool rel = false) {
@@ -1288,9 +1313,11 @@ static int walk(pnb_ctx* c, const float4* pts, const uint32_t* perm, size_t n, f
     const float eps2 = softening * softening;
     const uint32_t total = uint32_t(s->nodes);
     rel = rel && total > 0 && !rows;
+    const size_t waves = div_up(n, 64);
     if (rel) {
         PNB_HIP(c, s->rrec.ensure((size_t(total) + 1) * sizeof(NodeRec)));
-        PNB_HIP(c, s->est.ensure(n * sizeof(float)));
+        PNB_HIP(c, s->est.ensure((s->rel_per_particle ? n : waves) * sizeof(float)));
+        if (!s->rel_per_particle) PNB_HIP(c, s->cen.ensure(waves * sizeof(float4)));
     }
     return timed(c, PNB_PROF_BH_WALK, [&]() -> int {
         const unsigned grid = div_up(n, 256);
@@ -1312,19 +1339,30 @@ static int walk(pnb_ctx* c, const float4* pts, const uint32_t* perm, size_t n, f
             }
             const float te = s->rel_theta_est;
             float* est = s->est.as<float>();
-            launch_walk<WALK_PLAIN, false>(wide, grid, c->stream, nr, total, pts, perm, uint32_t(n),
-                                           te * te, eps2, out, info, WalkRows{}, nullptr, nullptr,
-                                           est, s->rel_alpha);
+            uint32_t shift = 0;
+            if (s->rel_per_particle) {
+                launch_walk<WALK_PLAIN, false>(wide, grid, c->stream, nr, total, pts, perm,
+                                               uint32_t(n), te * te, eps2, out, info, WalkRows{},
+                                               nullptr, nullptr, 0, est, s->rel_alpha);
+            } else {
+                float4* cen = s->cen.as<float4>();
+                bh_wave_centroid_kernel<<<grid, 256, 0, c->stream>>>(pts, uint32_t(n), cen);
+                PNB_CHECK_LAUNCH(c);
+                launch_walk<WALK_PLAIN, false>(wide, div_up(waves, 256), c->stream, nr, total, cen,
+                                               nullptr, uint32_t(waves), te * te, eps2, out, info,
+                                               WalkRows{}, nullptr, nullptr, 0, est, s->rel_alpha);
+                shift = 6;
+            }
             PNB_CHECK_LAUNCH(c);
             const NodeRec* rr = s->rrec.as<NodeRec>();
             if (quad)
                 launch_walk<WALK_QUAD, true>(wide, grid, c->stream, rr, total, pts, perm,
                                              uint32_t(n), theta2, eps2, out, info, WalkRows{},
-                                             s->qh.as<QuadH>(), est);
+                                             s->qh.as<QuadH>(), est, shift);
             else
                 launch_walk<WALK_PLAIN, true>(wide, grid, c->stream, rr, total, pts, perm,
                                               uint32_t(n), theta2, eps2, out, info, WalkRows{},
-                                              nullptr, est);
+                                              nullptr, est, shift);
         } else if (quad)
             launch_walk<WALK_QUAD, false>(wide, grid, c->stream, nr, total, pts, perm, uint32_t(n),
                                           theta2, eps2, out, info, WalkRows{}, s->qh.as<QuadH>());
@@ -1629,7 +1667,7 @@ int pnb_reordered_bh_accel_f32(pnb_ctx* c, const float* particles, size_t n, flo
     return PNB_OK;
 }
 
-int pnb_bh_set_relative(pnb_ctx* c, float alpha, float estimate_theta) {
+int pnb_bh_set_relative(pnb_ctx* c, float alpha, float estimate_theta, int per_particle) {
     if (!c) return PNB_E_ARG;
     if (!(alpha >= 0.0f) || !std::isfinite(alpha))
         return fail(c, PNB_E_ARG, "alpha must be finite and >= 0");
@@ -1638,6 +1676,7 @@ int pnb_bh_set_relative(pnb_ctx* c, float alpha, float estimate_theta) {
     if (!c->bh) c->bh = new BhState();
     c->bh->rel_alpha = alpha;
     c->bh->rel_theta_est = estimate_theta;
+    c->bh->rel_per_particle = per_particle != 0;
     return PNB_OK;
 }
 

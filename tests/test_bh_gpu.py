@@ -231,34 +231,56 @@ def test_all_bodies_at_one_position(res, orc):
 # ---- the relative opening criterion (PNB_BH_RELATIVE; no reference counterpart) ----
 
 
-def rel_bh(res, theta, alpha, mlp=False, estimate_theta=1.2, interaction=None):
+def rel_bh(res, theta, alpha, mlp=False, estimate_theta=1.2, interaction=None,
+           per_particle=False):
     return pa.BarnesHut(res, theta, interaction or pa.Acceleration.checked(), mlp=mlp,
-                        relative=alpha, estimate_theta=estimate_theta)
+                        relative=alpha, estimate_theta=estimate_theta,
+                        estimate_per_particle=per_particle)
 
 
 def oracle_estimate(tree, particles, alpha, estimate_theta):
-    """alpha |a~| from the oracle's plain traversal at estimate_theta (the GPU's estimate pass;
-    the two differ only in the last bits of the summed accelerations)."""
+    """alpha |a~| per particle from the oracle's plain traversal at estimate_theta (the GPU's
+    per-particle estimate pass; the two differ only in the last bits of the summed
+    accelerations)."""
     a, vis = tree.barnes_hut(particles, estimate_theta, com64=True, accum64=True,
                              return_visits=True)
     return np.float32(alpha) * np.linalg.norm(a[:, :3], axis=1).astype(np.float32), vis
 
 
-@pytest.mark.parametrize("alpha", [1e-3, 1e-4])
-def test_relative_criterion_matches_oracle(res, orc, alpha):
+def oracle_wave_estimate(tree, particles, alpha, estimate_theta):
+    """The GPU's default estimate for Between(p, p), whose walk order is the tree order: alpha |a~|
+    at the centroid of every 64 consecutive particles in tree order, given to each of them (in
+    the caller's order)."""
+    order = tree.order()
+    q = particles[order].astype(np.float64)
+    waves = (len(q) + 63) // 64
+    cen = np.zeros((waves, 4), np.float32)
+    for w in range(waves):
+        cen[w, :3] = q[64 * w:64 * w + 64, :3].mean(0)
+    a, vis = tree.barnes_hut(cen, estimate_theta, com64=True, accum64=True, return_visits=True)
+    ew = np.float32(alpha) * np.linalg.norm(a[:, :3], axis=1).astype(np.float32)
+    est = np.zeros(len(q), np.float32)
+    est[order] = ew[np.arange(len(q)) // 64]
+    return est, vis
+
+
+@pytest.mark.parametrize("alpha,per_particle", [(1e-3, False), (1e-4, False), (1e-4, True)])
+def test_relative_criterion_matches_oracle(res, orc, alpha, per_particle):
     """Plain walk with the relative rule (p = 2) against the oracle's restatement fed the oracle's
-    own estimate. A decision can flip where the two sides' last-bit rounding straddles the
-    threshold; each flip moves one particle by at most about alpha |a|, hence the alpha term in
-    the bar. The accepted-term counters (estimate pass + walk) must agree to 1e-4."""
+    own estimate (per wavefront centroid, the default, or per particle). A decision can flip where
+    the two sides' last-bit rounding (the GPU's v_rsq, its f32 centroids and compensated sums)
+    straddles the threshold; each flip moves one particle by at most about alpha |a|, hence the
+    alpha term in the bar. The accepted-term counters (estimate pass + walk) agree to 1e-4."""
     p = workloads.direct_sum_case(20000, seed=41).particles
     t = orc.Tree(p)
-    got = rel_bh(res, 1.0, alpha).compute(p)
+    got = rel_bh(res, 1.0, alpha, per_particle=per_particle).compute(p)
     _, accepted = pa.bh_stats(res)
-    est, est_vis = oracle_estimate(t, p, alpha, 1.2)
+    est, est_vis = (oracle_estimate if per_particle else oracle_wave_estimate)(t, p, alpha, 1.2)
     ref, vis = t.barnes_hut_rel(p, 1.0, est, order=2, com64=True, accum64=True,
                                 return_visits=True)
     e = metrics.summary(got, ref)
-    print("alpha", alpha, "vs oracle", e, "accepted", accepted, int(est_vis.sum() + vis.sum()))
+    print("alpha", alpha, "per particle", per_particle, "vs oracle", e, "accepted", accepted,
+          int(est_vis.sum() + vis.sum()))
     assert e["rms_rel"] < 5e-7 + 0.02 * alpha and e["max_rel"] < 1e-4 + 3 * alpha
     assert abs(accepted - int(est_vis.sum() + vis.sum())) <= 1e-4 * accepted
     exact = orc.direct_f64(p, p)
@@ -303,7 +325,7 @@ def test_relative_corrected_walk(res, orc):
     assert e_corr < 5 * alpha
     # the decisions: the oracle's p = 3 traversal accepts the same number of terms
     t = orc.Tree(p)
-    est, est_vis = oracle_estimate(t, p, alpha, 1.2)
+    est, est_vis = oracle_wave_estimate(t, p, alpha, 1.2)
     _, vis = t.barnes_hut_rel(p, 1.0, est, order=3, com64=True, return_visits=True)
     assert abs(acc_corr - int(est_vis.sum() + vis.sum())) <= 1e-4 * acc_corr
 
@@ -317,7 +339,8 @@ def test_relative_wide_offsets_match_narrow(res, monkeypatch):
     runs = {}
     for wide in ("0", "1"):
         monkeypatch.setenv("PNB_WIDE_WALK", wide)
-        runs[wide] = [rel_bh(res, 1.0, 1e-3, mlp=m).compute(p) for m in (False, True)]
+        runs[wide] = [rel_bh(res, 1.0, 1e-3, mlp=m, per_particle=pp).compute(p)
+                      for m in (False, True) for pp in (False, True)]
     for a, b in zip(runs["0"], runs["1"]):
         np.testing.assert_array_equal(a, b)
 
@@ -331,7 +354,8 @@ def test_relative_storages_and_edge_cases(res, orc):
     inter = pa.AccelerationSoftened.checked(s.softening)
     r = pa.Reordered.new(s.particles)
     massive = r.affecting()
-    got = rel_bh(res, 1.0, 1e-4, interaction=inter).compute(r)
+    # per-particle estimates: the walk order of a distinct affected set is its own key order
+    got = rel_bh(res, 1.0, 1e-4, interaction=inter, per_particle=True).compute(r)
     t = orc.Tree(massive)
     est, _ = t.barnes_hut(s.particles, 1.2, softening=s.softening, com64=True, accum64=True,
                           return_visits=True)
@@ -339,12 +363,21 @@ def test_relative_storages_and_edge_cases(res, orc):
     ref = t.barnes_hut_rel(s.particles, 1.0, est, order=2, softening=s.softening, com64=True,
                            accum64=True)
     assert metrics.rms_rel(got, ref) < 5e-7 + 2e-6
-    got2 = rel_bh(res, 1.0, 1e-4, interaction=inter).compute(pa.Between(s.particles, massive))
+    got2 = rel_bh(res, 1.0, 1e-4, interaction=inter, per_particle=True).compute(
+        pa.Between(s.particles, massive))
     np.testing.assert_array_equal(got, got2)
-    dev = rel_bh(res, 1.0, 1e-4, interaction=inter).compute(
+    dev = rel_bh(res, 1.0, 1e-4, interaction=inter, per_particle=True).compute(
         pa.Reordered.new(torch.from_numpy(s.particles).cuda()))
     torch.cuda.synchronize()
     np.testing.assert_array_equal(dev.cpu().numpy()[:, :3], got)
+    # per-wavefront estimates on the same storages: the same storage-independence, and the
+    # error against the FP64 direct sum of a plain relative walk
+    wav = rel_bh(res, 1.0, 1e-4, interaction=inter).compute(r)
+    np.testing.assert_array_equal(
+        rel_bh(res, 1.0, 1e-4, interaction=inter).compute(pa.Between(s.particles, massive)), wav)
+    idx = np.random.default_rng(3).choice(len(s.particles), 2000, replace=False)
+    exact = orc.direct_f64(s.particles[idx], massive, softening=s.softening)
+    assert metrics.rms_rel(wav[idx], exact) < 5e-4
 
     a = workloads.direct_sum_case(100, seed=48).particles
     empty = np.zeros((0, 4), np.float32)
@@ -361,4 +394,4 @@ def test_relative_storages_and_edge_cases(res, orc):
                                rtol=1e-6, atol=1e-7)
     L = pa._lib.lib()
     for alpha, te in ((-1.0, 1.2), (float("nan"), 1.2), (1e-4, 0.0), (1e-4, float("inf"))):
-        assert L.pnb_bh_set_relative(res.handle, alpha, te) == pa._lib.PNB_E_ARG
+        assert L.pnb_bh_set_relative(res.handle, alpha, te, 0) == pa._lib.PNB_E_ARG
