@@ -12,7 +12,9 @@
    checked against the FP64 direct sum on a seeded sample. Reports particles/s and error of each
    run and, by log-log interpolation of time against error on each curve, the speed-up of the
    corrected method over plain Barnes-Hut at equal rms error (at `--target-rms` and at the error
-   of each NN run).
+   of each NN run). Both are also run with the relative opening criterion (`--rel-alphas`,
+   `--nn-rel-alphas`; theta = 1 as the geometric floor) and compared at `--target-rms` with
+   plain Barnes-Hut under the reference's criterion and under the same relative one.
 Prints one JSON document.
 """
 from __future__ import annotations
@@ -45,12 +47,12 @@ def snapshot(n: int, seed: int, rot: int):
     return p
 
 
-def timed_bh(res, dev, theta, mlp, reps=2):
+def timed_bh(res, dev, theta, mlp, reps=2, alpha=None):
     import torch
 
     import particular_amd as pa
 
-    alg = pa.BarnesHut(res, theta, pa.Acceleration.checked(), mlp=mlp)
+    alg = pa.BarnesHut(res, theta, pa.Acceleration.checked(), mlp=mlp, relative=alpha)
     out = alg.compute(pa.Between(dev, dev))
     times = []
     for _ in range(reps):
@@ -98,6 +100,10 @@ def main():
     ap.add_argument("--sample", type=int, default=16384)
     ap.add_argument("--thetas", default="0.15,0.2,0.25,0.3,0.4,0.5,0.6,0.8")
     ap.add_argument("--nn-thetas", default="0.3,0.4,0.5,0.6,0.8")
+    ap.add_argument("--rel-alphas", default="3e-5,5e-5,1e-4",
+                    help="plain Barnes-Hut with the relative criterion (empty: none)")
+    ap.add_argument("--nn-rel-alphas", default="1e-4,2e-4,3e-4",
+                    help="the corrected walk with the relative criterion (empty: none)")
     ap.add_argument("--target-rms", type=float, default=1e-4)
     ap.add_argument("--init", choices=("quadrupole", "zero"), default="quadrupole",
                     help="starting network: the quadrupole term (learned.quadrupole_params) or a "
@@ -137,13 +143,14 @@ def main():
     idx = np.random.default_rng(5).choice(a.n, a.sample, replace=False)
     ref = fp64_reference(res, dev, torch.from_numpy(idx).cuda(), a.sample)
 
-    def run(theta, mlp):
-        out, (ms, (build, walk, corr)) = timed_bh(res, dev, theta, mlp)
+    def run(theta, mlp, alpha=None):
+        out, (ms, (build, walk, corr)) = timed_bh(res, dev, theta, mlp, alpha=alpha)
         err = metrics.summary(out.cpu().numpy()[idx], ref)
-        row = {"theta": theta, "ms": ms, "build_ms": build, "walk_ms": walk,
+        row = {"theta": theta, "alpha": alpha, "ms": ms, "build_ms": build, "walk_ms": walk,
                "correction_ms": corr, "mparticles_per_s": a.n / ms / 1e3, **err}
-        log(f"{'NN' if mlp else 'plain'} BH theta={theta}: {ms:.2f} ms (build {build:.2f}, walk "
-            f"{walk:.2f}, correction {corr:.2f}), rms {err['rms_rel']:.3e}")
+        log(f"{'NN' if mlp else 'plain'} BH theta={theta}"
+            f"{f' alpha={alpha}' if alpha is not None else ''}: {ms:.2f} ms (build {build:.2f}, "
+            f"walk {walk:.2f}, correction {corr:.2f}), rms {err['rms_rel']:.3e}")
         return row
 
     nn_thetas = [float(x) for x in a.nn_thetas.split(",")]
@@ -188,6 +195,14 @@ def main():
         q["error_vs_plain_same_theta"] = q["rms_rel"] / same["rms_rel"] if same else None
     t_plain = time_at_error(plain, a.target_rms)
     t_nn = time_at_error(nn, a.target_rms)
+    plain_rel = [run(1.0, False, float(x)) for x in a.rel_alphas.split(",") if x]
+    nn_rel = [run(1.0, True, float(x)) for x in a.nn_rel_alphas.split(",") if x]
+    t_prel = time_at_error(plain_rel, a.target_rms) if plain_rel else None
+    t_nrel = time_at_error(nn_rel, a.target_rms) if nn_rel else None
+
+    def ratio(x, y):
+        return (x / y) if (x and y) else None
+
     result = {
         "workload": f"two Plummer spheres, {a.n} particles, held-out seed; Acceleration::checked",
         "training": {"snapshots": k, "waves_per_snapshot": a.waves, "thetas": train_thetas,
@@ -204,6 +219,14 @@ def main():
         "plain_ms_at_target": t_plain,
         "nn_ms_at_target": t_nn,
         "speedup_at_target": (t_plain / t_nn) if (t_plain and t_nn) else None,
+        "relative_criterion": {
+            "plain": plain_rel,
+            "nn": nn_rel,
+            "plain_ms_at_target": t_prel,
+            "nn_ms_at_target": t_nrel,
+            "nn_speedup_vs_plain_geometric": ratio(t_plain, t_nrel),
+            "nn_speedup_vs_plain_relative": ratio(t_prel, t_nrel),
+        },
     }
     print(json.dumps(result, indent=1))
 
