@@ -70,6 +70,24 @@ def test_barnes_hut_2d_matches_3d_path_and_oracle(res, orc):
     assert errs[0] < 1e-6 and errs[0] < errs[1] < errs[2]
 
 
+def test_barnes_hut_2d_relative_criterion(res):
+    """The relative opening criterion through the 2D entry points: the same bits as the 3D path
+    on the z = 0 embedding (plain and with per-particle estimates), and a smaller error against the
+    2D direct sum as alpha falls."""
+    p = disc(4000, 3)
+    exact = direct_2d_f64(p, p)
+    errs = []
+    for alpha in (1e-2, 1e-3, 1e-4):
+        for pp in (False, True):
+            alg = pa.BarnesHut(res, 1.0, pa.Acceleration.checked(), relative=alpha,
+                               estimate_per_particle=pp)
+            b2 = alg.compute(p)
+            np.testing.assert_array_equal(b2, alg.compute(as3d(p))[:, :2])
+        errs.append(rel_rms(b2, exact))
+    print("2D relative criterion rms vs FP64", errs)
+    assert errs[0] > errs[1] > errs[2]
+
+
 def test_reordered_2d_split(res):
     p = disc(2000, 3, massless=1500)
     bf = pa.BruteForce(res, pa.AccelerationSoftened.checked(1e-3))
