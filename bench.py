@@ -410,9 +410,9 @@ def main():
     ap.add_argument("--bh-thetas", default="0.2,0.25,0.3,0.5,0.8", help="plain Barnes-Hut angles")
     ap.add_argument("--bh-nn-thetas", default="0.3,0.4",
                     help="angles with the learned correction (empty: none)")
-    ap.add_argument("--bh-rel-alphas", default="3e-5,1e-4",
+    ap.add_argument("--bh-rel-alphas", default="3e-5,1e-4,2e-4",
                     help="plain Barnes-Hut with the relative criterion (empty: none)")
-    ap.add_argument("--bh-nn-rel-alphas", default="1e-4,3e-4",
+    ap.add_argument("--bh-nn-rel-alphas", default="1e-4,3e-4,5e-4",
                     help="relative criterion with the learned correction (empty: none)")
     ap.add_argument("--bh-cpu-sample", type=int, default=16384)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,

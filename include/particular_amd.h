@@ -144,8 +144,11 @@ int pnb_bf_clear_tuning(pnb_ctx *ctx);
 #define PNB_BH_ALL_FLAGS (PNB_BH_MLP | PNB_BH_REF_COM | PNB_BH_RELATIVE)
 /* Parameters of PNB_BH_RELATIVE for the context's later calls: alpha >= 0 (0 opens every internal
  * node with mass), estimate_theta > 0, per_particle != 0 for one estimate per particle instead of
- * one per wavefront. */
-int pnb_bh_set_relative(pnb_ctx *ctx, float alpha, float estimate_theta, int per_particle);
+ * one per wavefront, and bucket (0 .. 31, default 24): an internal node of at most `bucket`
+ * bodies that a particle opens is summed body by body (in tree order) instead of being walked
+ * (0: never; nodes whose first body lies at or beyond 2^27 are always walked). */
+int pnb_bh_set_relative(pnb_ctx *ctx, float alpha, float estimate_theta, int per_particle,
+                        int bucket);
 
 int pnb_bh_accel_f32(pnb_ctx *ctx, const float *affected, size_t n, const float *affecting,
                      size_t m, float theta, float softening, int checked, unsigned flags,

@@ -193,11 +193,12 @@ struct BarnesHut {
     float relative_alpha = -1.0f;
     float estimate_theta = 1.2f;
     bool estimate_per_particle = false;
+    int bucket = 24;
 
     unsigned flags() {
         if (relative_alpha < 0.0f) return 0u;
         resources.check(pnb_bh_set_relative(resources.get(), relative_alpha, estimate_theta,
-                                            estimate_per_particle ? 1 : 0));
+                                            estimate_per_particle ? 1 : 0, bucket));
         return PNB_BH_RELATIVE;
     }
 

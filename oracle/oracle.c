@@ -359,11 +359,16 @@ int orc_bh_f32(const orc_tree *t, const float *affected, size_t n, float theta, 
  * lhs > (A * r2) * r2, with r2 = d2 + softening^2, w2 = width * width, A = est[i] (alpha |a~| of
  * affected particle i) and lhs = mass * w2 (order 2) or ((mass * w2) * sqrt(w2)) * (1 / sqrt(r2))
  * (order 3), every f32 operation rounded (IEEE sqrt and division). est = +inf gives orc_bh_f32's
- * decisions; est = 0 opens every internal node of positive mass. */
+ * decisions; est = 0 opens every internal node of positive mass. bucket > 0: an opened internal
+ * node of at most `bucket` particles whose first tree-order particle is below 2^27 is summed
+ * particle by particle in tree order (pairs at zero distance skipped, each counted in visits)
+ * instead of being descended; `affecting` is the array the tree was built from. */
 int orc_bh_rel_f32(const orc_tree *t, const float *affected, size_t n, float theta,
-                   float softening, const float *est, int order, int com64, int accum64,
-                   float *out, uint32_t *visits, int threads) {
+                   float softening, const float *est, int order, int bucket,
+                   const float *affecting, int com64, int accum64, float *out, uint32_t *visits,
+                   int threads) {
     if (!t || (n && (!affected || !out || !est)) || (order != 2 && order != 3)) return 1;
+    if (bucket < 0 || (bucket > 0 && !affecting)) return 1;
     set_threads(threads);
     const orc_node *nodes = t->nodes;
     size_t stack_cap = (size_t)(t->max_level + 2) * 8 + 8;
@@ -408,7 +413,25 @@ int orc_bh_rel_f32(const orc_tree *t, const float *affected, size_t n, float the
                     float rhs = (A * r2) * r2;
                     open = (th2 * d2) < w2 || rhs < lhs;
                 }
-                if (open) {
+                if (open && nd->count <= bucket && nd->begin < (1 << 27)) {
+                    for (int32_t q = nd->begin; q < nd->begin + nd->count; ++q) {
+                        const float *b = affecting + 4 * (size_t)t->order[q];
+                        float ex = b[0] - a[0], ey = b[1] - a[1], ez = b[2] - a[2];
+                        if (norm2f(ex, ey, ez) == 0.0f) continue;
+                        if (accum64) {
+                            double fx = (double)b[0] - a[0], fy = (double)b[1] - a[1],
+                                   fz = (double)b[2] - a[2];
+                            double q2 = fx * fx + fy * fy + fz * fz + (double)softening * softening;
+                            double s = (double)b[3] / (q2 * sqrt(q2));
+                            dx64 += fx * s; dy64 += fy * s; dz64 += fz * s;
+                        } else {
+                            float ox, oy, oz;
+                            pair(b[0], b[1], b[2], b[3], a[0], a[1], a[2], softening, 1, &ox, &oy, &oz);
+                            sx = sx + ox; sy = sy + oy; sz = sz + oz;
+                        }
+                        acc++;
+                    }
+                } else if (open) {
                     for (int c = 0; c < 8; ++c) stack[sp++] = nd->child[c];
                 } else {
                     if (accum64) {

@@ -75,11 +75,13 @@ int orc_bh_f32(const orc_tree *t, const float *affected, size_t n, float theta, 
 
 /* orc_bh_f32 with the GPU backend's relative opening criterion (PNB_BH_RELATIVE, no reference
  * counterpart): an internal node is also opened when M w^p / r^(p+2) > est[i], p = order (2 or
- * 3), r^2 = d2 + eps^2, est[i] = alpha |a~_i| supplied by the caller; see oracle.c for the f32
- * operation order. */
+ * 3), r^2 = d2 + eps^2, est[i] = alpha |a~_i| supplied by the caller; an opened node of at most
+ * `bucket` particles is summed particle by particle (`affecting`: the array the tree was built
+ * from); see oracle.c for the f32 operation order. */
 int orc_bh_rel_f32(const orc_tree *t, const float *affected, size_t n, float theta,
-                   float softening, const float *est, int order, int com64, int accum64,
-                   float *out, uint32_t *visits, int threads);
+                   float softening, const float *est, int order, int bucket,
+                   const float *affecting, int com64, int accum64, float *out, uint32_t *visits,
+                   int threads);
 
 #ifdef __cplusplus
 }

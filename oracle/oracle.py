@@ -90,7 +90,8 @@ def lib() -> ctypes.CDLL:
         ]
         L.orc_bh_rel_f32.argtypes = [
             ctypes.c_void_p, fp, sz, ctypes.c_float, ctypes.c_float, fp, ctypes.c_int,
-            ctypes.c_int, ctypes.c_int, fp, ctypes.POINTER(ctypes.c_uint32), ctypes.c_int,
+            ctypes.c_int, fp, ctypes.c_int, ctypes.c_int, fp, ctypes.POINTER(ctypes.c_uint32),
+            ctypes.c_int,
         ]
         _lib = L
     return _lib
@@ -199,10 +200,11 @@ class Tree:
         return (out, visits) if return_visits else out
 
     def barnes_hut_rel(self, affected, theta, est, order=2, softening=0.0, com64=True,
-                       threads=0, return_visits=False, accum64=False):
+                       threads=0, return_visits=False, accum64=False, bucket=0):
         """The traversal with the GPU backend's relative criterion (no reference counterpart):
         est[i] = alpha |a~_i| per affected particle, order 2 (plain walk) or 3 (with the learned
-        correction); see orc_bh_rel_f32."""
+        correction), opened nodes of at most `bucket` particles summed particle by particle; see
+        orc_bh_rel_f32."""
         a = _f32x4(affected)
         e = np.ascontiguousarray(est, dtype=np.float32)
         if e.shape != (a.shape[0],):
@@ -210,7 +212,8 @@ class Tree:
         out = np.zeros((a.shape[0], 4), np.float32)
         visits = np.zeros(a.shape[0], np.uint32)
         rc = lib().orc_bh_rel_f32(self._h, _fp(a), a.shape[0], theta, softening, _fp(e),
-                                  int(order), int(com64), int(accum64), _fp(out),
+                                  int(order), int(bucket), _fp(self._src), int(com64),
+                                  int(accum64), _fp(out),
                                   visits.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), threads)
         if rc:
             raise RuntimeError(f"orc_bh_rel_f32 failed ({rc})")

@@ -383,12 +383,14 @@ class BarnesHut(_Algorithm):
 
     def __init__(self, resources: HipResources, theta: float, interaction, mlp: bool = False,
                  reference_com: bool = False, relative: float | None = None,
-                 estimate_theta: float = 1.2, estimate_per_particle: bool = False):
+                 estimate_theta: float = 1.2, estimate_per_particle: bool = False,
+                 bucket: int = 24):
         """``relative=alpha`` adds the relative opening criterion (no reference counterpart): a
         node is also opened when its first neglected term, M w^p / r^(p+2) (p = 2, or 3 with
         ``mlp``), exceeds alpha times the acceleration from an estimate pass at
         ``estimate_theta`` (at the centroid of the particle's wavefront, or at the particle with
-        ``estimate_per_particle``); ``theta`` then acts as a geometric floor."""
+        ``estimate_per_particle``); ``theta`` then acts as a geometric floor, and an opened node
+        of at most ``bucket`` bodies is summed body by body."""
         super().__init__(resources, interaction)
         self.theta = float(theta)
         self.mlp = bool(mlp)
@@ -396,13 +398,14 @@ class BarnesHut(_Algorithm):
         self.relative = None if relative is None else float(relative)
         self.estimate_theta = float(estimate_theta)
         self.estimate_per_particle = bool(estimate_per_particle)
+        self.bucket = int(bucket)
 
     def _flags(self) -> int:
         f = (_lib.BH_MLP if self.mlp else 0) | (_lib.BH_REF_COM if self.reference_com else 0)
         if self.relative is not None:
             self.resources.check(_lib.lib().pnb_bh_set_relative(
                 self.resources.handle, self.relative, self.estimate_theta,
-                int(self.estimate_per_particle)))
+                int(self.estimate_per_particle), self.bucket))
             f |= _lib.BH_RELATIVE
         return f
 

@@ -76,7 +76,7 @@ SIGNATURES = {
     "pnb_bh_tree_size": (_i, [_vp, _c.POINTER(_sz), _pi]),
     "pnb_bh_tree_export": (_i, [_vp, _vp, _sz, _vp, _sz]),
     "pnb_bh_stats": (_i, [_vp, _pu64, _pu64]),
-    "pnb_bh_set_relative": (_i, [_vp, _f, _f, _i]),
+    "pnb_bh_set_relative": (_i, [_vp, _f, _f, _i, _i]),
     "pnb_batch_set_tuning": (_i, [_vp, _sz, _i]),
     "pnb_batch_init": (_i, [_vp, _vp, _vp, _vp, _vp, _sz, _f, _i]),
     "pnb_batch_run": (_i, [_vp, _sz, _d, _i]),

@@ -124,7 +124,8 @@ struct BhState {
     DevBuf rrec, est, cen;
     DevBuf wkey, wkey_alt, wval, wval_alt;  // wavefronts ordered by their estimate
     bool rrec_ok = false;
-    int rrec_cubic = 0;
+    int rrec_cubic = 0, rrec_bucket = 0;
+    int rel_bucket = 24;  // nodes of at most this many bodies are summed directly when opened
     float rel_alpha = 1e-4f, rel_theta_est = 1.2f;
     bool rel_per_particle = false;  // else one estimate per wavefront, at its centroid
     DevBuf info, bbox_part, hist, bits, afg_tmp;
@@ -986,6 +987,9 @@ __device__ __forceinline__ float half_hi(uint32_t v) {
 // geometric rule opens it or when the size of its first neglected term, M w^p / r^(p+2)
 // (r^2 = d^2 + eps^2), exceeds A: p = 2 for the plain walk (K > A r^4, the quadrupole is
 // neglected), p = 3 for WALK_QUAD (K / r > A r^4, the octupole is the first term it lacks).
+// REL buckets: a record whose `end` is non-zero packs the first body (27 bits) and the count
+// (5 bits) of a small internal node; lanes that open it add its bodies (`bodies`, the affecting
+// particles in tree order) directly and skip its subtree.
 // est_out (estimate pass): writes alpha |a| per particle, in walk order, instead of out.
 // wave_order (REL, optional): the launch's k-th wavefront walks the particles of wavefront
 // wave_order[k] (of `waves`), so that the heaviest wavefronts start first.
@@ -1001,7 +1005,8 @@ __global__ __launch_bounds__(256) void bh_walk_kernel(const NodeRec* __restrict_
                                                       const float* __restrict__ est_in,
                                                       uint32_t est_shift,
                                                       float* __restrict__ est_out, float alpha,
-                                                      const uint32_t* __restrict__ wave_order) {
+                                                      const uint32_t* __restrict__ wave_order,
+                                                      const float4* __restrict__ bodies) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (REL && wave_order) {
         const uint32_t k = i >> 6;  // this wavefront's place in the launch
@@ -1042,7 +1047,7 @@ __global__ __launch_bounds__(256) void bh_walk_kernel(const NodeRec* __restrict_
         const float lhs = REL ? (REL_INV ? mul_rn(K, inv_rel) : K) : 0.f;
         const float rhs = REL ? mul_rn(mul_rn(A, r2_rel), r2_rel) : 0.f;
         const bool crit = mul_rn(theta2, d2) < r.w2 || (REL && rhs < lhs);
-        const bool open = active && nz && crit;
+        bool open = active && nz && crit;
         const bool acc = active && nz && !crit;
         // the wavefront's open mask from the compares' own lane masks (a ballot of the combined
         // bool would first copy it into a VGPR)
@@ -1050,8 +1055,37 @@ __global__ __launch_bounds__(256) void bh_walk_kernel(const NodeRec* __restrict_
                                    __builtin_amdgcn_ballot_w64(nz);
         uint64_t crit_mask = __builtin_amdgcn_fcmpf(mul_rn(theta2, d2), r.w2, 4 /*olt*/);
         if constexpr (REL) crit_mask |= __builtin_amdgcn_fcmpf(rhs, lhs, 4 /*olt*/);
-        const uint64_t open_mask = live_mask & crit_mask;
+        uint64_t open_mask = live_mask & crit_mask;
         const uint64_t acc_mask = live_mask & ~crit_mask;
+        if constexpr (REL) {
+            const uint32_t bucket = r.end;
+            if (bucket != 0 && open_mask != 0) {  // a small node that some lanes open
+                if (open) {
+                    const uint32_t b0 = bucket & 0x7ffffffu, b1 = b0 + (bucket >> 27);
+                    for (uint32_t q = b0; q < b1; ++q) {
+                        const float4 b = bodies[q];  // wave-uniform: a scalar load
+                        const float ex = sub_rn(b.x, p.x), ey = sub_rn(b.y, p.y),
+                                    ez = sub_rn(b.z, p.z);
+                        const float e2 = norm2_ref(ex, ey, ez);
+                        if (e2 != 0.0f) {
+                            const float iv = __builtin_amdgcn_rsqf(add_rn(e2, eps2));
+                            const float sb = (b.w * iv) * (iv * iv);
+                            const float bx = fmaf(ex, sb, -cx), by = fmaf(ey, sb, -cy),
+                                        bz = fmaf(ez, sb, -cz);
+                            float t = ax + bx;
+                            cx = (t - ax) - bx; ax = t;
+                            t = ay + by;
+                            cy = (t - ay) - by; ay = t;
+                            t = az + bz;
+                            cz = (t - az) - bz; az = t;
+                            ++accepted;
+                        }
+                    }
+                }
+                open = false;  // nobody descends: every lane continues after the subtree
+                open_mask = 0;
+            }
+        }
         if (acc) {
             const float r2 = REL ? r2_rel : add_rn(d2, eps2);
             const float inv = REL_INV ? inv_rel : __builtin_amdgcn_rsqf(r2);
@@ -1257,20 +1291,25 @@ static int build_tree(pnb_ctx* c, const float4* afg, size_t m, bool ref_com = fa
 const MlpWeights* mlp_weights(pnb_ctx* c);  // mlp.hip: nullptr until initialised
 
 // Walk records for the relative criterion: node k's record with begin = the bits of K = M w^2
-// (cubic: M w^3; f32, each operation rounded; 0 for leaves, which are never opened) and end = 0,
-// for the total + 1 records (the last is the sentinel the walk prefetches).
+// (cubic: M w^3; f32, each operation rounded; 0 for leaves, which are never opened) and end =
+// first body | count << 27 for an internal node of at most `bucket` (<= 31) bodies whose first
+// body is below 2^27, else 0; for the total + 1 records (the last is the sentinel the walk
+// prefetches).
 __global__ void bh_rel_rec_kernel(const NodeRec* __restrict__ rec, size_t total, int cubic,
-                                  NodeRec* __restrict__ rrec) {
+                                  uint32_t bucket, NodeRec* __restrict__ rrec) {
     const size_t k = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (k > total) return;
     NodeRec r = rec[k];
     float K = 0.f;
+    uint32_t packed = 0;
     if (k < total && __float_as_int(r.w2) >= 0) {
         K = mul_rn(r.mass, r.w2);
         if (cubic) K = mul_rn(K, __fsqrt_rn(r.w2));
+        const uint32_t count = r.end - r.begin;
+        if (count <= bucket && r.begin < (1u << 27)) packed = r.begin | (count << 27);
     }
     r.begin = __float_as_uint(K);
-    r.end = 0;
+    r.end = packed;
     rrec[k] = r;
 }
 
@@ -1311,15 +1350,15 @@ static void launch_walk(bool wide, unsigned grid, hipStream_t st, const NodeRec*
                         float eps2, float4* out, BhInfo* info, const WalkRows& rows,
                         const QuadH* qh, const float* est_in = nullptr, uint32_t est_shift = 0,
                         float* est_out = nullptr, float alpha = 0.f,
-                        const uint32_t* wave_order = nullptr) {
+                        const uint32_t* wave_order = nullptr, const float4* bodies = nullptr) {
     if (wide)
         bh_walk_kernel<MODE, true, REL><<<grid, 256, 0, st>>>(
             nr, total, pts, perm, n, theta2, eps2, out, info, rows, qh, est_in, est_shift, est_out,
-            alpha, wave_order);
+            alpha, wave_order, bodies);
     else
         bh_walk_kernel<MODE, false, REL><<<grid, 256, 0, st>>>(
             nr, total, pts, perm, n, theta2, eps2, out, info, rows, qh, est_in, est_shift, est_out,
-            alpha, wave_order);
+            alpha, wave_order, bodies);
 }
 
 // quad: add the learned correction (the per-node moments in s->qh). rel: the relative criterion
@@ -1358,12 +1397,13 @@ static int walk(pnb_ctx* c, const float4* pts, const uint32_t* perm, size_t n, f
                           (force && force[0] == '1');
         if (rel) {
             const int cubic = quad ? 1 : 0;
-            if (!s->rrec_ok || s->rrec_cubic != cubic) {
+            if (!s->rrec_ok || s->rrec_cubic != cubic || s->rrec_bucket != s->rel_bucket) {
                 bh_rel_rec_kernel<<<div_up(size_t(total) + 1, 256), 256, 0, c->stream>>>(
-                    nr, total, cubic, s->rrec.as<NodeRec>());
+                    nr, total, cubic, uint32_t(s->rel_bucket), s->rrec.as<NodeRec>());
                 PNB_CHECK_LAUNCH(c);
                 s->rrec_ok = true;
                 s->rrec_cubic = cubic;
+                s->rrec_bucket = s->rel_bucket;
             }
             const float te = s->rel_theta_est;
             float* est = s->est.as<float>();
@@ -1403,14 +1443,16 @@ static int walk(pnb_ctx* c, const float4* pts, const uint32_t* perm, size_t n, f
                 order = sv;
             }
             const NodeRec* rr = s->rrec.as<NodeRec>();
+            const float4* bodies = s->sp.as<float4>();  // the affecting particles, tree order
             if (quad)
                 launch_walk<WALK_QUAD, true>(wide, grid, c->stream, rr, total, pts, perm,
                                              uint32_t(n), theta2, eps2, out, info, WalkRows{},
-                                             s->qh.as<QuadH>(), est, shift, nullptr, 0.f, order);
+                                             s->qh.as<QuadH>(), est, shift, nullptr, 0.f, order,
+                                             bodies);
             else
                 launch_walk<WALK_PLAIN, true>(wide, grid, c->stream, rr, total, pts, perm,
                                               uint32_t(n), theta2, eps2, out, info, WalkRows{},
-                                              nullptr, est, shift, nullptr, 0.f, order);
+                                              nullptr, est, shift, nullptr, 0.f, order, bodies);
         } else if (quad)
             launch_walk<WALK_QUAD, false>(wide, grid, c->stream, nr, total, pts, perm, uint32_t(n),
                                           theta2, eps2, out, info, WalkRows{}, s->qh.as<QuadH>());
@@ -1715,16 +1757,19 @@ int pnb_reordered_bh_accel_f32(pnb_ctx* c, const float* particles, size_t n, flo
     return PNB_OK;
 }
 
-int pnb_bh_set_relative(pnb_ctx* c, float alpha, float estimate_theta, int per_particle) {
+int pnb_bh_set_relative(pnb_ctx* c, float alpha, float estimate_theta, int per_particle,
+                        int bucket) {
     if (!c) return PNB_E_ARG;
     if (!(alpha >= 0.0f) || !std::isfinite(alpha))
         return fail(c, PNB_E_ARG, "alpha must be finite and >= 0");
     if (!(estimate_theta > 0.0f) || !std::isfinite(estimate_theta))
         return fail(c, PNB_E_ARG, "estimate_theta must be finite and > 0");
+    if (bucket < 0 || bucket > 31) return fail(c, PNB_E_ARG, "bucket must be in 0 .. 31");
     if (!c->bh) c->bh = new BhState();
     c->bh->rel_alpha = alpha;
     c->bh->rel_theta_est = estimate_theta;
     c->bh->rel_per_particle = per_particle != 0;
+    c->bh->rel_bucket = bucket;
     return PNB_OK;
 }
 

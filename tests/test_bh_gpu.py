@@ -232,10 +232,10 @@ def test_all_bodies_at_one_position(res, orc):
 
 
 def rel_bh(res, theta, alpha, mlp=False, estimate_theta=1.2, interaction=None,
-           per_particle=False):
+           per_particle=False, bucket=24):
     return pa.BarnesHut(res, theta, interaction or pa.Acceleration.checked(), mlp=mlp,
                         relative=alpha, estimate_theta=estimate_theta,
-                        estimate_per_particle=per_particle)
+                        estimate_per_particle=per_particle, bucket=bucket)
 
 
 def oracle_estimate(tree, particles, alpha, estimate_theta):
@@ -277,7 +277,7 @@ def test_relative_criterion_matches_oracle(res, orc, alpha, per_particle):
     _, accepted = pa.bh_stats(res)
     est, est_vis = (oracle_estimate if per_particle else oracle_wave_estimate)(t, p, alpha, 1.2)
     ref, vis = t.barnes_hut_rel(p, 1.0, est, order=2, com64=True, accum64=True,
-                                return_visits=True)
+                                return_visits=True, bucket=24)
     e = metrics.summary(got, ref)
     print("alpha", alpha, "per particle", per_particle, "vs oracle", e, "accepted", accepted,
           int(est_vis.sum() + vis.sum()))
@@ -288,20 +288,23 @@ def test_relative_criterion_matches_oracle(res, orc, alpha, per_particle):
 
 
 def test_relative_criterion_limits_are_geometric_walks(res):
-    """alpha = 1e30 never opens by the relative rule: bit-identical to the geometric walk at the
-    same theta (plain and corrected); alpha = 0 opens every massive internal node: bit-identical
-    to theta = 0 (where the corrected walk, accepting only leaves, equals the plain one)."""
+    """Without buckets, alpha = 1e30 never opens by the relative rule: bit-identical to the
+    geometric walk at the same theta (plain and corrected); alpha = 0 opens every massive internal
+    node: bit-identical to theta = 0 (where the corrected walk, accepting only leaves, equals the
+    plain one). With buckets, alpha = 0 sums the same pair terms as theta = 0 in another order."""
     from particular_amd import learned
 
     p = workloads.direct_sum_case(30000, seed=43).particles
     learned.init(res, 0)
     learned.set_params(res, learned.quadrupole_params())
     chk = pa.Acceleration.checked()
+    every = pa.BarnesHut(res, 0.0, chk).compute(p)
     for mlp in (False, True):
         geo = pa.BarnesHut(res, 0.6, chk, mlp=mlp).compute(p)
-        np.testing.assert_array_equal(rel_bh(res, 0.6, 1e30, mlp=mlp).compute(p), geo)
-        np.testing.assert_array_equal(rel_bh(res, 0.6, 0.0, mlp=mlp).compute(p),
-                                      pa.BarnesHut(res, 0.0, chk).compute(p))
+        np.testing.assert_array_equal(rel_bh(res, 0.6, 1e30, mlp=mlp, bucket=0).compute(p), geo)
+        np.testing.assert_array_equal(rel_bh(res, 0.6, 0.0, mlp=mlp, bucket=0).compute(p), every)
+        np.testing.assert_allclose(rel_bh(res, 0.6, 0.0, mlp=mlp).compute(p), every, rtol=1e-5,
+                                   atol=1e-7)
 
 
 def test_relative_corrected_walk(res, orc):
@@ -326,7 +329,7 @@ def test_relative_corrected_walk(res, orc):
     # the decisions: the oracle's p = 3 traversal accepts the same number of terms
     t = orc.Tree(p)
     est, est_vis = oracle_wave_estimate(t, p, alpha, 1.2)
-    _, vis = t.barnes_hut_rel(p, 1.0, est, order=3, com64=True, return_visits=True)
+    _, vis = t.barnes_hut_rel(p, 1.0, est, order=3, com64=True, return_visits=True, bucket=24)
     assert abs(acc_corr - int(est_vis.sum() + vis.sum())) <= 1e-4 * acc_corr
 
 
@@ -361,7 +364,7 @@ def test_relative_storages_and_edge_cases(res, orc):
                           return_visits=True)
     est = np.float32(1e-4) * np.linalg.norm(est[:, :3], axis=1).astype(np.float32)
     ref = t.barnes_hut_rel(s.particles, 1.0, est, order=2, softening=s.softening, com64=True,
-                           accum64=True)
+                           accum64=True, bucket=24)
     assert metrics.rms_rel(got, ref) < 5e-7 + 2e-6
     got2 = rel_bh(res, 1.0, 1e-4, interaction=inter, per_particle=True).compute(
         pa.Between(s.particles, massive))
@@ -393,5 +396,6 @@ def test_relative_storages_and_edge_cases(res, orc):
                                orc.Tree(src).barnes_hut(a, 0.5, com64=True, accum64=True)[:, :3],
                                rtol=1e-6, atol=1e-7)
     L = pa._lib.lib()
-    for alpha, te in ((-1.0, 1.2), (float("nan"), 1.2), (1e-4, 0.0), (1e-4, float("inf"))):
-        assert L.pnb_bh_set_relative(res.handle, alpha, te, 0) == pa._lib.PNB_E_ARG
+    for alpha, te, bucket in ((-1.0, 1.2, 16), (float("nan"), 1.2, 16), (1e-4, 0.0, 16),
+                              (1e-4, float("inf"), 16), (1e-4, 1.2, -1), (1e-4, 1.2, 32)):
+        assert L.pnb_bh_set_relative(res.handle, alpha, te, 0, bucket) == pa._lib.PNB_E_ARG

@@ -288,6 +288,31 @@ def test_relative_criterion_threshold_closed_form(orc, order):
 
 
 @pytest.mark.parametrize("order", [2, 3])
+def test_relative_buckets(orc, order):
+    """Opened nodes of at most `bucket` particles summed particle by particle: with est = 0 (every
+    massive internal node opened) the terms are exactly the theta = 0 traversal's leaves (distinct
+    positions: one particle per leaf), summed in another order; a bucket holding the whole set
+    opens the root into the direct sum; bucket = 0 is the plain relative traversal."""
+    s = workloads.direct_sum_case(1500, seed=35).particles
+    t = orc.Tree(s)
+    all_, va = t.barnes_hut(s, 0.0, com64=True, accum64=True, return_visits=True)
+    for bucket in (1, 4, 16, 31):
+        b, vb = t.barnes_hut_rel(s, 0.6, np.zeros(len(s), np.float32), order=order,
+                                 bucket=bucket, accum64=True, return_visits=True)
+        np.testing.assert_array_equal(vb, va)
+        np.testing.assert_allclose(b, all_, rtol=1e-6, atol=1e-12)
+    small = s[:20]
+    ts = orc.Tree(small)
+    got, v = ts.barnes_hut_rel(small, 0.6, np.zeros(20, np.float32), order=order, bucket=20,
+                               accum64=True, return_visits=True)
+    assert (v == 19).all()  # the root is one bucket: every other particle once
+    np.testing.assert_allclose(got[:, :3], numpy_direct_f64(small, small), rtol=1e-6)
+    est = np.full(len(s), 1e-4, np.float32)
+    np.testing.assert_array_equal(t.barnes_hut_rel(s, 0.6, est, order=order),
+                                  t.barnes_hut_rel(s, 0.6, est, order=order, bucket=0))
+
+
+@pytest.mark.parametrize("order", [2, 3])
 def test_relative_criterion_error_follows_alpha(orc, order):
     """Against the FP64 direct sum the error falls with alpha. For order 2 every accepted node's
     neglected quadrupole is bounded by alpha |a|, so the rms error stays within a few alpha; for
