@@ -32,6 +32,10 @@
 // opens node k lets the wave step to k+1; a lane that accepts or skips k waits until the cursor
 // reaches skip[k]. Each lane therefore applies exactly the reference's decisions; node records
 // are wave-uniform, so they arrive through scalar loads.
+// Optional relative criterion (PNB_BH_RELATIVE, no reference counterpart): a node also opens when
+// its first neglected term exceeds alpha times the particle's estimated acceleration; an
+// estimate pass over one centroid per wavefront gives that acceleration, walk records carrying
+// M w^p replace the plain ones, and small opened nodes are summed body by body (buckets).
 #include "pnb_internal.hpp"
 #include <hip/hip_fp16.h>
 #include "mlp_common.hpp"
@@ -991,8 +995,8 @@ __device__ __forceinline__ float half_hi(uint32_t v) {
 // (5 bits) of a small internal node; lanes that open it add its bodies (`bodies`, the affecting
 // particles in tree order) directly and skip its subtree.
 // est_out (estimate pass): writes alpha |a| per particle, in walk order, instead of out.
-// wave_order (REL, optional): the launch's k-th wavefront walks the particles of wavefront
-// wave_order[k] (of `waves`), so that the heaviest wavefronts start first.
+// wave_order (REL, optional): the launch's k-th wavefront (k < ceil(n / 64)) walks the particles
+// of wavefront wave_order[k], so that the heaviest wavefronts start first.
 template <int MODE, bool WIDE, bool REL>
 __global__ __launch_bounds__(256) void bh_walk_kernel(const NodeRec* __restrict__ nodes,
                                                       uint32_t total,
