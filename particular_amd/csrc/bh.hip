@@ -118,6 +118,7 @@ struct BhState {
     const uint64_t* skeys = nullptr;  // sorted keys (points into keys or keys_alt)
     const uint32_t* sidx = nullptr;   // tree order -> original index
     DevBuf keys, keys_alt, idx, idx_alt, sp, lcp, cnt, ip, key2, rec, sums, level, width;
+    DevBuf skipv;            // per node: the skip pointer alone (the bottom-up pass's child walk)
     DevBuf ilist;            // internal nodes grouped by level (bottom-up pass)
     DevBuf lvl;              // padded per-level counts and list cursors (2 x LVL_SLOTS lines)
     DevBuf mom, quad, desc;  // second moments (FP64), normalised quadrupoles, child shares
@@ -143,7 +144,7 @@ struct BhState {
 void bh_free(BhState* s) {
     if (!s) return;
     for (DevBuf* b : {&s->keys, &s->keys_alt, &s->idx, &s->idx_alt, &s->sp, &s->lcp, &s->cnt,
-                      &s->ip, &s->key2, &s->rec, &s->sums, &s->level, &s->width,
+                      &s->ip, &s->key2, &s->rec, &s->sums, &s->level, &s->width, &s->skipv,
                       &s->ilist, &s->lvl, &s->mom, &s->quad, &s->desc, &s->qh, &s->rrec, &s->est,
                       &s->cen, &s->wkey, &s->wkey_alt, &s->wval, &s->wval_alt,
                       &s->info, &s->bbox_part, &s->hist, &s->bits, &s->afg_tmp, &s->akeys,
@@ -582,7 +583,7 @@ __global__ __launch_bounds__(256) void bh_node_kernel(
     const int* __restrict__ ip, const uint32_t* __restrict__ S, size_t m,
     const BhInfo* __restrict__ info, NodeRec* __restrict__ rec, NodeSums* __restrict__ sums,
     uint8_t* __restrict__ level, float* __restrict__ width,
-    NodeMoments* __restrict__ mom, NodeQuad* __restrict__ quad) {
+    NodeMoments* __restrict__ mom, NodeQuad* __restrict__ quad, uint32_t* __restrict__ skipv) {
     const size_t p = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (p >= m) return;
     const int lp = lcp[p];
@@ -631,6 +632,7 @@ __global__ __launch_bounds__(256) void bh_node_kernel(
         const uint32_t k = base + uint32_t(l - lp - 1);
         rec[k].end = uint32_t(e);
         rec[k].skip = S[e];
+        skipv[k] = S[e];
         if (l == I + 1) {  // the leaf: a run of identical positions
             double tot = 0.0;
             for (size_t r = p; r < e; ++r) tot += double(sp[r].w);
@@ -660,12 +662,14 @@ __global__ __launch_bounds__(256) void bh_up_kernel(NodeRec* __restrict__ rec,
                                                     uint32_t count, NodeMoments* __restrict__ mom,
                                                     NodeQuad* __restrict__ quad,
                                                     NodeDesc* __restrict__ desc,
-                                                    const float* __restrict__ width) {
+                                                    const float* __restrict__ width,
+                                                    const uint32_t* __restrict__ skipv,
+                                                    const float4* __restrict__ sp) {
     for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < count;
          t += gridDim.x * blockDim.x) {
         const uint32_t k = nodes_at_level[t];  // an internal node of this level
         const NodeRec r = rec[k];
-        double4 a = make_double4(0, 0, 0, 0), b = make_double4(0, 0, 0, 0);
+        double4 a = make_double4(0, 0, 0, 0);
         // with moments, in the same pass over the children: second moments about o (the first
         // massive child's centre of mass) and first moments D = sum M_c (c_c - o), shifted to
         // the centre of mass below
@@ -675,10 +679,8 @@ __global__ __launch_bounds__(256) void bh_up_kernel(NodeRec* __restrict__ rec,
         int children = 0;
         size_t c = k + 1;
         while (c < r.skip) {
-            const NodeSums sc = sums[c];
-            const double4 s1 = sc.m, s2 = sc.p;
+            const double4 s1 = sums[c].m;
             a.x += s1.x; a.y += s1.y; a.z += s1.z; a.w += s1.w;
-            b.x += s2.x; b.y += s2.y; b.z += s2.z; b.w += s2.w;
             if (mom) {
                 ++children;
                 heaviest = fmax(heaviest, s1.x);
@@ -700,15 +702,38 @@ __global__ __launch_bounds__(256) void bh_up_kernel(NodeRec* __restrict__ rec,
                     mpos += s1.x;
                 }
             }
-            const uint32_t nx = rec[c].skip;
+            const uint32_t nx = skipv[c];
             c = nx > c ? nx : c + 1;
         }
         sums[k].m = a;
-        sums[k].p = b;
         float cx, cy, cz;
         if (a.x != 0.0) {
             cx = float(a.y / a.x); cy = float(a.z / a.x); cz = float(a.w / a.x);
         } else {
+            // Massless: the plain mean of the particles' positions (impls/mod.rs:127-128). Only
+            // nodes of zero mass keep count-weighted position sums in `p` (leaves always do), so
+            // with every child massless they add up; otherwise (masses of both signs summing to
+            // zero) the node's own particles are summed.
+            double4 b = make_double4(0, 0, 0, 0);
+            bool massless = true;
+            for (size_t q = k + 1; q < r.skip;) {
+                const NodeSums sc = sums[q];
+                if (sc.m.x != 0.0) {
+                    massless = false;
+                    break;
+                }
+                b.x += sc.p.x; b.y += sc.p.y; b.z += sc.p.z; b.w += sc.p.w;
+                const uint32_t nx = skipv[q];
+                q = nx > q ? nx : q + 1;
+            }
+            if (!massless) {
+                b = make_double4(0, 0, 0, 0);
+                for (uint32_t q = r.begin; q < r.end; ++q) {
+                    const float4 v = sp[q];
+                    b.x += v.x; b.y += v.y; b.z += v.z; b.w += 1.0;
+                }
+            }
+            sums[k].p = b;
             cx = float(b.x / b.w); cy = float(b.y / b.w); cz = float(b.z / b.w);
         }
         rec[k].cx = cx;
@@ -1258,10 +1283,12 @@ static int build_tree(pnb_ctx* c, const float4* afg, size_t m, bool ref_com = fa
             desc = s->desc.as<NodeDesc>();
         }
         NodeRec* rec = s->rec.as<NodeRec>();
+        PNB_HIP(c, s->skipv.ensure(size_t(total) * sizeof(uint32_t)));
+        uint32_t* skipv = s->skipv.as<uint32_t>();
         bh_node_kernel<<<g, 256, 0, c->stream>>>(sk, s->sp.as<float4>(), lcp, s->ip.as<int>(),
                                                  cnt, m, info, rec, s->sums.as<NodeSums>(),
                                                  s->level.as<uint8_t>(),
-                                                 s->width.as<float>(), mom, quad);
+                                                 s->width.as<float>(), mom, quad, skipv);
         PNB_CHECK_LAUNCH(c);
         // bottom-up sums, one launch per level over that level's internal nodes
         const int levels = std::max(h.max_level, 0);  // internal levels 0 .. max_level - 1
@@ -1278,7 +1305,7 @@ static int build_tree(pnb_ctx* c, const float4* afg, size_t m, bool ref_com = fa
             if (n_l == 0) continue;
             bh_up_kernel<<<div_up(n_l, 256), 256, 0, c->stream>>>(
                 rec, s->sums.as<NodeSums>(), ilist + off[L], n_l, mom, quad,
-                desc, s->width.as<float>());
+                desc, s->width.as<float>(), skipv, s->sp.as<float4>());
             PNB_CHECK_LAUNCH(c);
         }
         s->has_moments = moments;

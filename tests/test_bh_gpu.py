@@ -70,6 +70,26 @@ def test_tree_with_duplicates(res, orc):
     assert t.num_nodes == 1
 
 
+def test_tree_massless_and_cancelling_masses(res, orc):
+    """Nodes of zero total mass take the plain mean of their particles (impls/mod.rs:127-128):
+    a massless set (every node), and pairs of opposite masses whose nodes sum to exactly zero
+    while their children do not."""
+    p = workloads.direct_sum_case(600, seed=14).particles
+    massless = p.copy()
+    massless[:, 3] = 0.0
+    bh(res, massless, 0.5)
+    assert_same_tree(res, orc, massless)
+    pairs = p.copy()
+    pairs[:, 3] = np.float32(0.25)
+    pairs[1::2, 3] = np.float32(-0.25)
+    pairs[1::2, :3] = pairs[0::2, :3] + np.float32(1e-3)  # each pair shares its small nodes
+    bh(res, pairs, 0.5)
+    t = assert_same_tree(res, orc, pairs)
+    gn, _ = pa.tree_export(res, pairs.shape[0])
+    zero = (gn["mass64"] == 0.0) & (gn["internal"] == 1)
+    assert zero.sum() > 10, zero.sum()  # internal nodes whose masses cancel exactly
+
+
 @pytest.mark.parametrize("scale", [1e-7, 1e-16, 1e-30])
 def test_tree_deep_runs(res, orc, scale):
     """Distinct positions closer than 21 (and 42, 84) levels of subdivision resolve."""
