@@ -128,14 +128,63 @@ __global__ void rs_bits_init_kernel(unsigned long long* or_and) {
     or_and[1] = ~0ull;
 }
 
-// Sorts keys[0..n) with their values; the result ends in (*keys_out, *vals_out), which is either
-// the input pair or the alternate pair. Only bits [0, key_bits) are considered.
+// Small inputs in one workgroup: a bitonic network in LDS over (key bits [0, key_bits), value)
+// pairs, padded to a power of two with pairs that order last. Ordering equal keys by value is the
+// stable order when the values increase with the input position, as the callers' indices do.
+constexpr uint32_t SMALL_SORT = 4096;
+
+__global__ __launch_bounds__(1024) void small_sort_kernel(uint64_t* __restrict__ keys,
+                                                          uint32_t* __restrict__ vals, uint32_t n,
+                                                          uint64_t mask) {
+    __shared__ uint64_t sk[SMALL_SORT];
+    __shared__ uint32_t sv[SMALL_SORT];
+    uint32_t n2 = 2;
+    while (n2 < n) n2 <<= 1;
+    for (uint32_t i = threadIdx.x; i < n2; i += blockDim.x) {
+        sk[i] = i < n ? keys[i] : ~0ull;
+        sv[i] = i < n ? vals[i] : ~0u;
+    }
+    __syncthreads();
+    for (uint32_t size = 2; size <= n2; size <<= 1) {
+        for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+            for (uint32_t t = threadIdx.x; t < (n2 >> 1); t += blockDim.x) {
+                const uint32_t i = 2 * t - (t & (stride - 1)), j = i + stride;
+                const uint64_t ki = sk[i] & mask, kj = sk[j] & mask;
+                const bool gt = ki > kj || (ki == kj && sv[i] > sv[j]);
+                if (gt == ((i & size) == 0)) {  // out of order for this half's direction
+                    const uint64_t tk = sk[i];
+                    sk[i] = sk[j];
+                    sk[j] = tk;
+                    const uint32_t tv = sv[i];
+                    sv[i] = sv[j];
+                    sv[j] = tv;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+        keys[i] = sk[i];
+        vals[i] = sv[i];
+    }
+}
+
+// Sorts keys[0..n) with their values, stably; the result ends in (*keys_out, *vals_out), which is
+// either the input pair or the alternate pair. Only bits [0, key_bits) are considered. Up to
+// SMALL_SORT pairs one workgroup sorts in place (equal keys ordered by value: the values must
+// increase with the input position, as the index arrays of every caller do).
 int radix_sort_pairs(pnb_ctx* c, uint64_t* keys, uint32_t* vals, uint64_t* keys_alt,
                      uint32_t* vals_alt, size_t n, int key_bits, DevBuf& hist_buf,
                      DevBuf& bits_buf, uint64_t** keys_out, uint32_t** vals_out) {
     *keys_out = keys;
     *vals_out = vals;
     if (n <= 1) return PNB_OK;
+    if (n <= SMALL_SORT) {
+        const uint64_t mask = key_bits >= 64 ? ~0ull : ((1ull << key_bits) - 1);
+        small_sort_kernel<<<1, 1024, 0, c->stream>>>(keys, vals, uint32_t(n), mask);
+        PNB_CHECK_LAUNCH(c);
+        return PNB_OK;
+    }
     if (n > 0xffffffffu) return fail(c, PNB_E_ARG, "radix sort supports at most 2^32-1 keys");
     const uint32_t tiles = div_up(n, RS_TILE);
     PNB_HIP(c, hist_buf.ensure((size_t(256) * tiles + 1) * sizeof(uint32_t)));
