@@ -188,16 +188,20 @@ int radix_sort_pairs(pnb_ctx* c, uint64_t* keys, uint32_t* vals, uint64_t* keys_
     if (n > 0xffffffffu) return fail(c, PNB_E_ARG, "radix sort supports at most 2^32-1 keys");
     const uint32_t tiles = div_up(n, RS_TILE);
     PNB_HIP(c, hist_buf.ensure((size_t(256) * tiles + 1) * sizeof(uint32_t)));
-    PNB_HIP(c, bits_buf.ensure(2 * sizeof(unsigned long long)));
-    unsigned long long* oa = bits_buf.as<unsigned long long>();
-    rs_bits_init_kernel<<<1, 1, 0, c->stream>>>(oa);
-    PNB_CHECK_LAUNCH(c);
-    rs_bits_kernel<<<std::min<unsigned>(1024, div_up(n, 256)), 256, 0, c->stream>>>(keys, n, oa);
-    PNB_CHECK_LAUNCH(c);
-    unsigned long long h_oa[2];
-    PNB_HIP(c, hipMemcpyAsync(h_oa, oa, sizeof(h_oa), hipMemcpyDeviceToHost, c->stream));
-    PNB_HIP(c, hipStreamSynchronize(c->stream));
-    const uint64_t varying = h_oa[0] ^ h_oa[1];
+    uint64_t varying = 0xffull;  // one digit: sorted in one pass, no need to look for it first
+    if (key_bits > 8) {
+        // skip the digits no two keys differ in (found on the device, read back once)
+        PNB_HIP(c, bits_buf.ensure(2 * sizeof(unsigned long long)));
+        unsigned long long* oa = bits_buf.as<unsigned long long>();
+        rs_bits_init_kernel<<<1, 1, 0, c->stream>>>(oa);
+        PNB_CHECK_LAUNCH(c);
+        rs_bits_kernel<<<std::min<unsigned>(1024, div_up(n, 256)), 256, 0, c->stream>>>(keys, n, oa);
+        PNB_CHECK_LAUNCH(c);
+        unsigned long long h_oa[2];
+        PNB_HIP(c, hipMemcpyAsync(h_oa, oa, sizeof(h_oa), hipMemcpyDeviceToHost, c->stream));
+        PNB_HIP(c, hipStreamSynchronize(c->stream));
+        varying = h_oa[0] ^ h_oa[1];
+    }
     uint32_t* hist = hist_buf.as<uint32_t>();
     uint64_t *kin = keys, *kout = keys_alt;
     uint32_t *vin = vals, *vout = vals_alt;

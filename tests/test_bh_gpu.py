@@ -421,13 +421,15 @@ def test_relative_storages_and_edge_cases(res, orc):
         assert L.pnb_bh_set_relative(res.handle, alpha, te, 0, bucket) == pa._lib.PNB_E_ARG
 
 
-def test_relative_walk_large(res, orc):
-    """2.2M bodies: above 128 wavefronts per CU the relative walk runs in tree order (below it,
-    heaviest first); both walks bounded against the FP64 direct sum on a sample, repeatable bit
-    for bit, and the corrected one more accurate than the plain one at the same alpha."""
+@pytest.mark.parametrize("n", [400_000, 2_200_000])
+def test_relative_walk_large(res, orc, n):
+    """400K bodies (6,250 wavefronts: launched heaviest first, ordered by the multi-tile sort) and
+    2.2M (above 128 wavefronts per CU: tree order); both walks bounded against the FP64 direct sum
+    on a sample, repeatable bit for bit, and the corrected one more accurate than the plain one at
+    the same alpha."""
     from particular_amd import learned
 
-    s = workloads.two_plummer_case(2_200_000, seed=77)
+    s = workloads.two_plummer_case(n, seed=77)
     idx = np.random.default_rng(2).choice(s.n, 2048, replace=False)
     exact = orc.direct_f64(s.particles[idx], s.particles)
     learned.init(res, 0)
@@ -438,5 +440,5 @@ def test_relative_walk_large(res, orc):
         a = alg.compute(s.particles)
         np.testing.assert_array_equal(alg.compute(s.particles), a)
         errs[mlp] = metrics.rms_rel(a[idx], exact)
-    print("2.2M relative walk rms (plain, corrected)", errs)
+    print(n, "relative walk rms (plain, corrected)", errs)
     assert errs[False] < 5e-4 and errs[True] < errs[False]
