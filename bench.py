@@ -107,7 +107,7 @@ def bench_bf(args):
         res.synchronize()
         graph_ms = (time.perf_counter() - g0) * 1e3 / args.steps
 
-    # accuracy of the last accelerations against the FP64 judge on a sample of affected particles
+    # accuracy of the last accelerations against the FP64 direct sum on a sample of affected particles
     import torch
 
     sample = 16384

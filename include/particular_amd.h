@@ -215,7 +215,7 @@ int pnb_bf_accel_f64(pnb_ctx *ctx, const double *affected, size_t n, const doubl
 int pnb_bf_accel_f64_dev(pnb_ctx *ctx, const double *affected, size_t n, const double *affecting,
                          size_t m, double softening, int checked, double *out);
 
-/* ---- accuracy judge -------------------------------------------------------------------- */
+/* ---- accuracy reference ---------------------------------------------------------------- */
 
 /* FP64 direct sum on the GPU (device pointers): out3[3 i + k] = sum_j mu_j d / (|d|^2+eps^2)^1.5
  * with pairs at zero distance skipped. */
