@@ -31,7 +31,7 @@ void orc_accel_pair(const float *p2, const float *at, float softening, int check
 int orc_bf_f32(const float *affected, size_t n, const float *affecting, size_t m, float softening,
                int checked, float *out, int threads);
 
-/* FP64 direct sum, the accuracy judge: a_i = sum_j mu_j d_ij / (|d_ij|^2 + eps^2)^{3/2}, pairs
+/* FP64 direct sum, the accuracy reference: a_i = sum_j mu_j d_ij / (|d_ij|^2 + eps^2)^{3/2}, pairs
  * with |d_ij| == 0 skipped. Output 3 doubles per particle. */
 int orc_direct_f64(const float *affected, size_t n, const float *affecting, size_t m,
                    double softening, double *out3, int threads);

@@ -1,4 +1,4 @@
-// Brute-force gravitational acceleration on gfx950, and the FP64 direct-sum judge.
+// Brute-force gravitational acceleration on gfx950, and the FP64 direct sum used as the accuracy reference.
 //
 // Replaces the reference's wgpu brute force (particular/src/gpu/bruteforce_shared.wgsl:8-31 and
 // particular/src/gpu/mod.rs:179-208) with its own design for CDNA4:

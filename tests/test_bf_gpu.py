@@ -138,7 +138,7 @@ def test_massive_massless_at_scale_sampled(res, orc):
     assert metrics.rms_rel(got[idx], ref32) < 2 * metrics.rms_rel(ref32, ref64) + RMS_BAR
 
 
-def test_direct_f64_judge_matches_oracle(res, orc):
+def test_direct_f64_reference_matches_oracle(res, orc):
     import torch
 
     s = workloads.direct_sum_case(2000, seed=12)
