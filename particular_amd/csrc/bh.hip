@@ -126,7 +126,7 @@ struct BhState {
     bool has_moments = false;
     // relative opening criterion (PNB_BH_RELATIVE): walk records carrying M w^3, per affected
     // particle alpha |a| from the estimate pass, and the criterion's parameters
-    DevBuf rrec, est, cen;
+    DevBuf rrec, est, cen, cen_spread, est_spread;
     DevBuf wkey, wkey_alt, wval, wval_alt;  // wavefronts ordered by their estimate
     bool rrec_ok = false;
     int rrec_cubic = 0, rrec_bucket = 0;
@@ -146,7 +146,7 @@ void bh_free(BhState* s) {
     for (DevBuf* b : {&s->keys, &s->keys_alt, &s->idx, &s->idx_alt, &s->sp, &s->lcp, &s->cnt,
                       &s->ip, &s->key2, &s->rec, &s->sums, &s->level, &s->width, &s->skipv,
                       &s->ilist, &s->lvl, &s->mom, &s->quad, &s->desc, &s->qh, &s->rrec, &s->est,
-                      &s->cen, &s->wkey, &s->wkey_alt, &s->wval, &s->wval_alt,
+                      &s->cen, &s->cen_spread, &s->est_spread, &s->wkey, &s->wkey_alt, &s->wval, &s->wval_alt,
                       &s->info, &s->bbox_part, &s->hist, &s->bits, &s->afg_tmp, &s->akeys,
                       &s->akeys_alt, &s->aidx, &s->aidx_alt, &s->apts, &s->h_aff, &s->h_afg,
                       &s->h_out, &s->c_starts, &s->c_pts, &s->c_bh, &s->c_bf, &s->c_list,
@@ -1022,6 +1022,7 @@ __device__ __forceinline__ float half_hi(uint32_t v) {
 // est_out (estimate pass): writes alpha |a| per particle, in walk order, instead of out.
 // wave_order (REL, optional): the launch's k-th wavefront (k < ceil(n / 64)) walks the particles
 // of wavefront wave_order[k], so that the heaviest wavefronts start first.
+// lanes: only the first `lanes` lanes of each wavefront walk (64 except in the estimate pass).
 template <int MODE, bool WIDE, bool REL>
 __global__ __launch_bounds__(256) void bh_walk_kernel(const NodeRec* __restrict__ nodes,
                                                       uint32_t total,
@@ -1035,14 +1036,15 @@ __global__ __launch_bounds__(256) void bh_walk_kernel(const NodeRec* __restrict_
                                                       uint32_t est_shift,
                                                       float* __restrict__ est_out, float alpha,
                                                       const uint32_t* __restrict__ wave_order,
-                                                      const float4* __restrict__ bodies) {
+                                                      const float4* __restrict__ bodies,
+                                                      uint32_t lanes) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (REL && wave_order) {
         const uint32_t k = i >> 6;  // this wavefront's place in the launch
         if (k < ((n + 63) >> 6)) i = (wave_order[k] << 6) | (i & 63);
     }
     const int lane = threadIdx.x & 63;
-    const bool valid = i < n;
+    const bool valid = i < n && uint32_t(lane) < lanes;
     const float4 p = pts[valid ? i : n - 1];
     const float A = REL ? est_in[(valid ? i : n - 1) >> est_shift] : 0.f;
     float ax = 0.f, ay = 0.f, az = 0.f, cx = 0.f, cy = 0.f, cz = 0.f;
@@ -1375,21 +1377,42 @@ __global__ void bh_wave_keys_kernel(const float* __restrict__ est, uint32_t wave
     }
 }
 
+// The estimate pass walks EST_GROUP centroids per wavefront (its other lanes idle), so that more,
+// shorter wavefronts hide the walk's latency; every centroid's decisions and sums are its own,
+// so the estimates do not depend on the grouping.
+constexpr uint32_t EST_GROUP = 8;
+
+__global__ void bh_cen_spread_kernel(const float4* __restrict__ cen, uint32_t waves,
+                                     float4* __restrict__ spread, uint32_t groups) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < groups * 64) {
+        const uint32_t w = (t >> 6) * EST_GROUP + (t & 63) % EST_GROUP;  // idle lanes: a copy
+        spread[t] = cen[w < waves ? w : waves - 1];
+    }
+}
+
+__global__ void bh_est_gather_kernel(const float* __restrict__ spread, uint32_t waves,
+                                     float* __restrict__ est) {
+    const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w < waves) est[w] = spread[(w / EST_GROUP) * 64 + w % EST_GROUP];
+}
+
 template <int MODE, bool REL>
 static void launch_walk(bool wide, unsigned grid, hipStream_t st, const NodeRec* nr, uint32_t total,
                         const float4* pts, const uint32_t* perm, uint32_t n, float theta2,
                         float eps2, float4* out, BhInfo* info, const WalkRows& rows,
                         const QuadH* qh, const float* est_in = nullptr, uint32_t est_shift = 0,
                         float* est_out = nullptr, float alpha = 0.f,
-                        const uint32_t* wave_order = nullptr, const float4* bodies = nullptr) {
+                        const uint32_t* wave_order = nullptr, const float4* bodies = nullptr,
+                        uint32_t lanes = 64) {
     if (wide)
         bh_walk_kernel<MODE, true, REL><<<grid, 256, 0, st>>>(
             nr, total, pts, perm, n, theta2, eps2, out, info, rows, qh, est_in, est_shift, est_out,
-            alpha, wave_order, bodies);
+            alpha, wave_order, bodies, lanes);
     else
         bh_walk_kernel<MODE, false, REL><<<grid, 256, 0, st>>>(
             nr, total, pts, perm, n, theta2, eps2, out, info, rows, qh, est_in, est_shift, est_out,
-            alpha, wave_order, bodies);
+            alpha, wave_order, bodies, lanes);
 }
 
 // quad: add the learned correction (the per-node moments in s->qh). rel: the relative criterion
@@ -1410,7 +1433,10 @@ static int walk(pnb_ctx* c, const float4* pts, const uint32_t* perm, size_t n, f
         PNB_HIP(c, s->rrec.ensure((size_t(total) + 1) * sizeof(NodeRec)));
         PNB_HIP(c, s->est.ensure((s->rel_per_particle ? n : waves) * sizeof(float)));
         if (!s->rel_per_particle) {
+            const size_t groups = div_up(waves, size_t(EST_GROUP));
             PNB_HIP(c, s->cen.ensure(waves * sizeof(float4)));
+            PNB_HIP(c, s->cen_spread.ensure(groups * 64 * sizeof(float4)));
+            PNB_HIP(c, s->est_spread.ensure(groups * 64 * sizeof(float)));
             PNB_HIP(c, s->wkey.ensure(waves * sizeof(uint64_t)));
             PNB_HIP(c, s->wkey_alt.ensure(waves * sizeof(uint64_t)));
             PNB_HIP(c, s->wval.ensure(waves * sizeof(uint32_t)));
@@ -1449,9 +1475,23 @@ static int walk(pnb_ctx* c, const float4* pts, const uint32_t* perm, size_t n, f
                 float4* cen = s->cen.as<float4>();
                 bh_wave_centroid_kernel<<<grid, 256, 0, c->stream>>>(pts, uint32_t(n), cen);
                 PNB_CHECK_LAUNCH(c);
-                launch_walk<WALK_PLAIN, false>(wide, div_up(waves, 256), c->stream, nr, total, cen,
-                                               nullptr, uint32_t(waves), te * te, eps2, out, info,
-                                               WalkRows{}, nullptr, nullptr, 0, est, s->rel_alpha);
+                const uint32_t groups = uint32_t(div_up(waves, size_t(EST_GROUP)));
+                float4* spread = s->cen_spread.as<float4>();
+                float* est_spread = s->est_spread.as<float>();
+                bh_cen_spread_kernel<<<div_up(size_t(groups) * 64, 256), 256, 0, c->stream>>>(
+                    cen, uint32_t(waves), spread, groups);
+                PNB_CHECK_LAUNCH(c);
+                // lanes 0 .. EST_GROUP-1 of each wavefront walk; in the last one only those with a
+                // centroid (i < n excludes the rest)
+                const uint32_t n_est = (groups - 1) * 64 + (uint32_t(waves) - (groups - 1) * EST_GROUP);
+                launch_walk<WALK_PLAIN, false>(wide, div_up(size_t(groups) * 64, 256), c->stream,
+                                               nr, total, spread, nullptr, n_est, te * te,
+                                               eps2, out, info, WalkRows{}, nullptr, nullptr, 0,
+                                               est_spread, s->rel_alpha, nullptr, nullptr,
+                                               EST_GROUP);
+                PNB_CHECK_LAUNCH(c);
+                bh_est_gather_kernel<<<div_up(waves, 256), 256, 0, c->stream>>>(
+                    est_spread, uint32_t(waves), est);
                 PNB_CHECK_LAUNCH(c);
                 shift = 6;
             }
