@@ -656,18 +656,17 @@ __global__ __launch_bounds__(256) void bh_node_kernel(
 // (impls/mod.rs:126-131: mass-weighted mean, or the plain mean when the total mass is 0).
 // With `mom`, also the second moments by the parallel-axis rule
 // T = sum over children of (T_c + M_c (c_c - c)(c_c - c)^T), and the normalised quadrupole.
-__global__ __launch_bounds__(256) void bh_up_kernel(NodeRec* __restrict__ rec,
-                                                    NodeSums* __restrict__ sums,
-                                                    const uint32_t* __restrict__ nodes_at_level,
-                                                    uint32_t count, NodeMoments* __restrict__ mom,
-                                                    NodeQuad* __restrict__ quad,
-                                                    NodeDesc* __restrict__ desc,
-                                                    const float* __restrict__ width,
-                                                    const uint32_t* __restrict__ skipv,
-                                                    const float4* __restrict__ sp) {
-    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < count;
-         t += gridDim.x * blockDim.x) {
-        const uint32_t k = nodes_at_level[t];  // an internal node of this level
+// One internal node k: its sums from its children's (and, with moments, its second moments,
+// normalised quadrupole and child shares), its record's centre of mass and mass.
+__device__ __forceinline__ void up_node(uint32_t k, NodeRec* __restrict__ rec,
+                                        NodeSums* __restrict__ sums,
+                                        NodeMoments* __restrict__ mom,
+                                        NodeQuad* __restrict__ quad,
+                                        NodeDesc* __restrict__ desc,
+                                        const float* __restrict__ width,
+                                        const uint32_t* __restrict__ skipv,
+                                        const float4* __restrict__ sp) {
+    {
         const NodeRec r = rec[k];
         double4 a = make_double4(0, 0, 0, 0);
         // with moments, in the same pass over the children: second moments about o (the first
@@ -767,6 +766,46 @@ __global__ __launch_bounds__(256) void bh_up_kernel(NodeRec* __restrict__ rec,
             quad[k] = qd;
             desc[k] = ds;
         }
+    }
+}
+
+__global__ __launch_bounds__(256) void bh_up_kernel(NodeRec* __restrict__ rec,
+                                                    NodeSums* __restrict__ sums,
+                                                    const uint32_t* __restrict__ nodes_at_level,
+                                                    uint32_t count, NodeMoments* __restrict__ mom,
+                                                    NodeQuad* __restrict__ quad,
+                                                    NodeDesc* __restrict__ desc,
+                                                    const float* __restrict__ width,
+                                                    const uint32_t* __restrict__ skipv,
+                                                    const float4* __restrict__ sp) {
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < count;
+         t += gridDim.x * blockDim.x)
+        up_node(nodes_at_level[t], rec, sums, mom, quad, desc, width, skipv, sp);
+}
+
+// Consecutive small levels (deepest first) in one workgroup: a barrier between levels makes one
+// level's sums visible to the next (the waves of a workgroup share one CU and its cache), so a
+// run of small levels costs one launch instead of one per level.
+constexpr int UP_RUN_MAX = 32;
+constexpr uint32_t UP_RUN_NODES = 1024;  // a level joins a run when it has at most this many
+struct UpRun {
+    uint32_t off[UP_RUN_MAX], cnt[UP_RUN_MAX];
+    int levels;
+};
+
+__global__ __launch_bounds__(1024) void bh_up_run_kernel(NodeRec* __restrict__ rec,
+                                                         NodeSums* __restrict__ sums,
+                                                         const uint32_t* __restrict__ ilist,
+                                                         UpRun run, NodeMoments* __restrict__ mom,
+                                                         NodeQuad* __restrict__ quad,
+                                                         NodeDesc* __restrict__ desc,
+                                                         const float* __restrict__ width,
+                                                         const uint32_t* __restrict__ skipv,
+                                                         const float4* __restrict__ sp) {
+    for (int j = 0; j < run.levels; ++j) {
+        for (uint32_t t = threadIdx.x; t < run.cnt[j]; t += blockDim.x)
+            up_node(ilist[run.off[j] + t], rec, sums, mom, quad, desc, width, skipv, sp);
+        __syncthreads();
     }
 }
 
@@ -1302,12 +1341,25 @@ static int build_tree(pnb_ctx* c, const float4* afg, size_t m, bool ref_com = fa
         bh_level_list_kernel<<<lg, 256, 0, c->stream>>>(lcp, s->ip.as<int>(), cnt, m, lvl_cur,
                                                         ilist);
         PNB_CHECK_LAUNCH(c);
-        for (int L = levels - 1; L >= 0; --L) {
-            const uint32_t n_l = h.lvl_count[L];
-            if (n_l == 0) continue;
-            bh_up_kernel<<<div_up(n_l, 256), 256, 0, c->stream>>>(
-                rec, s->sums.as<NodeSums>(), ilist + off[L], n_l, mom, quad,
-                desc, s->width.as<float>(), skipv, s->sp.as<float4>());
+        for (int L = levels - 1; L >= 0;) {
+            if (h.lvl_count[L] > UP_RUN_NODES) {
+                const uint32_t n_l = h.lvl_count[L];
+                bh_up_kernel<<<div_up(n_l, 256), 256, 0, c->stream>>>(
+                    rec, s->sums.as<NodeSums>(), ilist + off[L], n_l, mom, quad,
+                    desc, s->width.as<float>(), skipv, s->sp.as<float4>());
+                PNB_CHECK_LAUNCH(c);
+                --L;
+                continue;
+            }
+            UpRun run{};
+            for (; L >= 0 && h.lvl_count[L] <= UP_RUN_NODES && run.levels < UP_RUN_MAX; --L) {
+                run.off[run.levels] = off[L];
+                run.cnt[run.levels] = h.lvl_count[L];
+                ++run.levels;
+            }
+            bh_up_run_kernel<<<1, 1024, 0, c->stream>>>(rec, s->sums.as<NodeSums>(), ilist, run,
+                                                        mom, quad, desc, s->width.as<float>(),
+                                                        skipv, s->sp.as<float4>());
             PNB_CHECK_LAUNCH(c);
         }
         s->has_moments = moments;
