@@ -35,12 +35,23 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_threads() -> int:
+# A one-GPU box grants this process a share of 16 host threads, whatever nproc reports: the CPU
+# baselines use that share and record both counts.
+HOST_THREAD_SHARE = 16
+
+
+def host_cores() -> dict:
     try:
-        n = len(os.sched_getaffinity(0))
+        affinity = len(os.sched_getaffinity(0))
     except AttributeError:
-        n = os.cpu_count() or 1
-    return max(1, min(n, 16))
+        affinity = os.cpu_count() or 1
+    return {"nproc": os.cpu_count(), "affinity": affinity,
+            "threads_used": max(1, min(affinity, HOST_THREAD_SHARE)),
+            "cpu_model": cpu_model()}
+
+
+def cpu_threads() -> int:
+    return host_cores()["threads_used"]
 
 
 def cpu_model() -> str:
@@ -180,6 +191,7 @@ def cpu_baseline_bf(s, sample: int):
         "unit": "G interactions/s",
         "cores": threads,
         "kind": "port",
+        "host": host_cores(),
         "sample": f"{aff.shape[0]} affected x {massive.shape[0]} massive, one evaluation "
                   f"({dt:.2f} s), C restatement of parallel::BruteForce; CPU: {cpu_model()}",
     }
@@ -195,6 +207,14 @@ def time_at_error(rows, rms):
     if not (e.min() <= x <= e.max()):
         return None
     return float(np.exp(np.interp(x, e[o], t[o])))
+
+
+def tree_size(res) -> int:
+    from particular_amd import _lib
+
+    nodes, level = ctypes.c_size_t(), ctypes.c_int()
+    res.check(_lib.lib().pnb_bh_tree_size(res.handle, ctypes.byref(nodes), ctypes.byref(level)))
+    return nodes.value
 
 
 def bench_bh(args):
@@ -250,7 +270,7 @@ def bench_bh(args):
         steps, accepted = pa.bh_stats(res)
         build_ms, walk_ms, sort_ms = build_ms / nb, walk_ms / nw, sort_ms / max(ns, 1)
         net_ms = net_ms / max(nn, 1)
-        nodes, _ = pa.tree_export(res)[0].shape[0], None
+        nodes = tree_size(res)
         err = metrics.summary(acc.cpu().numpy()[idx], ref)
         # the walk's counters accumulate over the reps since the last build: one build per call
         row = {
@@ -357,6 +377,7 @@ def cpu_baseline_bh(s, theta, sample):
         "unit": "M particles/s",
         "cores": threads,
         "kind": "port",
+        "host": host_cores(),
         "sample": f"theta={theta}: sequential build of all {s.n} bodies ({build:.2f} s) + threaded "
                   f"traversal of {sample} affected scaled to {s.n} ({walk:.2f} s)",
     }
@@ -417,7 +438,8 @@ def main():
     ap.add_argument("--bh-cpu-sample", type=int, default=16384)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="first copy a buffer of this size with torch (HBM counter calibration)")
-    ap.add_argument("--batch", action="store_true", default=False)
+    ap.add_argument("--batch", action="store_true", default=True)
+    ap.add_argument("--no-batch", dest="batch", action="store_false")
     ap.add_argument("--batch-steps", type=int, default=100)
     args = ap.parse_args()
     if args.gpus != 1:

@@ -288,6 +288,20 @@ void orc_tree_root_box(const orc_tree *t, float *bmin3, float *bmax3) {
     for (int a = 0; a < 3; ++a) { bmin3[a] = t->rmin[a]; bmax3[a] = t->rmax[a]; }
 }
 
+/* End of every node's subtree in the pre-order node array (the index of the first node after
+ * it), from the child links: a leaf ends at k + 1, an internal node where its last non-empty child
+ * ends. Children always follow their parent, so one backward pass settles every node. */
+void orc_tree_skips(const orc_tree *t, uint32_t *out) {
+    for (size_t k = t->len; k-- > 0;) {
+        uint32_t end = (uint32_t)k + 1;
+        for (int c = 0; c < 8; ++c) {
+            const int32_t ch = t->nodes[k].child[c];
+            if (ch >= 0 && out[ch] > end) end = out[ch];
+        }
+        out[k] = end;
+    }
+}
+
 /* sequential.rs:465-505. LIFO stack from the root; None children skipped; d2 = |com - x|^2
  * (acceleration.rs:121-126); skip when d2 == 0; open an Internal node when
  * (theta*theta)*d2 < width*width with width = bbox.max[0] - bbox.min[0] (partition.rs:183-188);

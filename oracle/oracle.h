@@ -64,6 +64,8 @@ const orc_node *orc_tree_nodes(const orc_tree *t);
 const uint32_t *orc_tree_order(const orc_tree *t);
 /* The cubic root box. */
 void orc_tree_root_box(const orc_tree *t, float *bmin3, float *bmax3);
+/* Subtree ends in pre-order: out[k] = index of the first node after node k's subtree. */
+void orc_tree_skips(const orc_tree *t, uint32_t *out);
 
 /* sequential::BarnesHut traversal (particular/src/sequential.rs:456-506) for every affected
  * particle. com64 = 0 uses the reference's f32 node data, 1 uses the FP64 reduction rounded to

@@ -84,6 +84,8 @@ def lib() -> ctypes.CDLL:
         L.orc_tree_order.argtypes = [ctypes.c_void_p]
         L.orc_tree_order.restype = ctypes.POINTER(ctypes.c_uint32)
         L.orc_tree_root_box.argtypes = [ctypes.c_void_p, fp, fp]
+        L.orc_tree_skips.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint32)]
+        L.orc_tree_skips.restype = None
         L.orc_bh_f32.argtypes = [
             ctypes.c_void_p, fp, sz, ctypes.c_float, ctypes.c_float, ctypes.c_int, ctypes.c_int,
             ctypes.c_int, fp, ctypes.POINTER(ctypes.c_uint32), ctypes.c_int,
@@ -170,7 +172,14 @@ class Tree:
         assert ctypes.sizeof(OrcNode) == NODE_DTYPE.itemsize
         ptr = lib().orc_tree_nodes(self._h)
         buf = (ctypes.c_char * (n * NODE_DTYPE.itemsize)).from_address(ctypes.addressof(ptr.contents))
-        return np.frombuffer(bytes(buf), dtype=NODE_DTYPE).copy()
+        return np.frombuffer(buf, dtype=NODE_DTYPE).copy()  # one copy out of the C array
+
+    def skips(self) -> np.ndarray:
+        """Per node, the index of the first node after its subtree (pre-order)."""
+        out = np.zeros(self.num_nodes, np.uint32)
+        if out.size:
+            lib().orc_tree_skips(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)))
+        return out
 
     def order(self) -> np.ndarray:
         m = self._src.shape[0]

@@ -128,21 +128,25 @@ __global__ void rs_bits_init_kernel(unsigned long long* or_and) {
     or_and[1] = ~0ull;
 }
 
-// Small inputs in one workgroup: a bitonic network in LDS over (key bits [0, key_bits), value)
-// pairs, padded to a power of two with pairs that order last. Ordering equal keys by value is the
-// stable order when the values increase with the input position, as the callers' indices do.
+// Small inputs in one workgroup: a bitonic network in LDS over (key bits [0, key_bits), input
+// position) with the values carried along, padded to a power of two with entries that order last
+// (their positions are past every input's). Ties on the key are broken by the input position, so
+// equal keys keep their input order whatever the values: the same stable order as the radix path.
 constexpr uint32_t SMALL_SORT = 4096;
+static_assert(SMALL_SORT <= 65536, "input positions are held as 16-bit values");
 
 __global__ __launch_bounds__(1024) void small_sort_kernel(uint64_t* __restrict__ keys,
                                                           uint32_t* __restrict__ vals, uint32_t n,
                                                           uint64_t mask) {
     __shared__ uint64_t sk[SMALL_SORT];
     __shared__ uint32_t sv[SMALL_SORT];
+    __shared__ uint16_t sp[SMALL_SORT];
     uint32_t n2 = 2;
     while (n2 < n) n2 <<= 1;
     for (uint32_t i = threadIdx.x; i < n2; i += blockDim.x) {
         sk[i] = i < n ? keys[i] : ~0ull;
         sv[i] = i < n ? vals[i] : ~0u;
+        sp[i] = uint16_t(i);
     }
     __syncthreads();
     for (uint32_t size = 2; size <= n2; size <<= 1) {
@@ -150,7 +154,7 @@ __global__ __launch_bounds__(1024) void small_sort_kernel(uint64_t* __restrict__
             for (uint32_t t = threadIdx.x; t < (n2 >> 1); t += blockDim.x) {
                 const uint32_t i = 2 * t - (t & (stride - 1)), j = i + stride;
                 const uint64_t ki = sk[i] & mask, kj = sk[j] & mask;
-                const bool gt = ki > kj || (ki == kj && sv[i] > sv[j]);
+                const bool gt = ki > kj || (ki == kj && sp[i] > sp[j]);
                 if (gt == ((i & size) == 0)) {  // out of order for this half's direction
                     const uint64_t tk = sk[i];
                     sk[i] = sk[j];
@@ -158,6 +162,9 @@ __global__ __launch_bounds__(1024) void small_sort_kernel(uint64_t* __restrict__
                     const uint32_t tv = sv[i];
                     sv[i] = sv[j];
                     sv[j] = tv;
+                    const uint16_t tp = sp[i];
+                    sp[i] = sp[j];
+                    sp[j] = tp;
                 }
             }
             __syncthreads();
@@ -171,8 +178,7 @@ __global__ __launch_bounds__(1024) void small_sort_kernel(uint64_t* __restrict__
 
 // Sorts keys[0..n) with their values, stably; the result ends in (*keys_out, *vals_out), which is
 // either the input pair or the alternate pair. Only bits [0, key_bits) are considered. Up to
-// SMALL_SORT pairs one workgroup sorts in place (equal keys ordered by value: the values must
-// increase with the input position, as the index arrays of every caller do).
+// SMALL_SORT pairs one workgroup sorts in place; either way equal keys keep their input order.
 int radix_sort_pairs(pnb_ctx* c, uint64_t* keys, uint32_t* vals, uint64_t* keys_alt,
                      uint32_t* vals_alt, size_t n, int key_bits, DevBuf& hist_buf,
                      DevBuf& bits_buf, uint64_t** keys_out, uint32_t** vals_out) {

@@ -144,6 +144,11 @@ def quadrupole_params() -> np.ndarray:
     return dict_to_params(d)
 
 
+def default_params() -> np.ndarray:
+    """The parameters the corrected walk uses unless told otherwise."""
+    return quadrupole_params()
+
+
 def fit(res, iterations: int, lrs=(1e-3, 3e-4, 1e-4, 3e-5, 1e-5), probe: int = 40) -> dict:
     """Full-batch training from the current parameters with the learning rate chosen by probes.
 

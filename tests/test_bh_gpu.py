@@ -11,36 +11,9 @@ import pytest
 import particular_amd as pa
 from particular_amd import metrics, workloads
 
+from .treecheck import assert_same_tree, bh, check_against_oracle
+
 pytestmark = pytest.mark.gpu
-
-
-def bh(res, storage, theta, interaction=None):
-    return pa.BarnesHut(res, theta, interaction or pa.Acceleration.checked()).compute(storage)
-
-
-def assert_same_tree(res, orc, particles):
-    t = orc.Tree(particles)
-    on = t.nodes()
-    gn, gorder = pa.tree_export(res, particles.shape[0])
-    assert gn.shape[0] == on.shape[0], (gn.shape[0], on.shape[0])
-    np.testing.assert_array_equal(gorder, t.order())
-    np.testing.assert_array_equal(gn["level"], on["level"])
-    np.testing.assert_array_equal(gn["begin"], on["begin"])
-    np.testing.assert_array_equal(gn["count"], on["count"])
-    np.testing.assert_array_equal(gn["internal"], on["internal"])
-    ow = on["bmax"][:, 0] - on["bmin"][:, 0]
-    internal = on["internal"] == 1
-    np.testing.assert_array_equal(gn["width"][internal], ow[internal])
-    np.testing.assert_allclose(gn["mass64"], on["mass64"], rtol=1e-12)
-    np.testing.assert_allclose(gn["com"], on["com64"].astype(np.float32), rtol=3e-7, atol=1e-30)
-    # skip pointers: pre-order subtree ends
-    ends = np.arange(len(on)) + 1
-    for k in range(len(on) - 1, -1, -1):
-        kids = [c for c in on["child"][k] if c >= 0]
-        if kids:
-            ends[k] = ends[max(kids)]
-    np.testing.assert_array_equal(gn["skip"], ends)
-    return t
 
 
 @pytest.mark.parametrize("n,seed", [(1, 1), (2, 2), (3, 3), (100, 4), (5000, 5)])
@@ -107,29 +80,51 @@ def test_tree_deep_runs(res, orc, scale):
     print("max level", t.max_level)
 
 
-def check_against_oracle(res, tree, particles, theta, idx=None, label=""):
-    """Three comparisons on the same inputs and theta:
-    * the default walk (FP64 centres of mass) against the oracle with the same node data whose
-      accepted terms are summed in FP64 -- isolates our traversal and summation: rms <= 5e-7;
-    * the walk with reference-exact f32 centres of mass (PNB_BH_REF_COM) against the
-      reference-exact restatement of parallel::BarnesHut -- the parity bar: rms <= 1e-5;
-    * the default walk against the reference: its distance is the reference's own f32
-      centre-of-mass rounding (measured here as reference vs exact-COM oracle), plus <= 2e-6."""
-    sel = slice(None) if idx is None else idx
-    aff = particles[sel]
-    got = bh(res, particles, theta)[sel]
-    got_ref = pa.BarnesHut(res, theta, pa.Acceleration.checked(), reference_com=True).compute(particles)[sel]
-    exact = tree.barnes_hut(aff, theta, com64=True, accum64=True)
-    ref = tree.barnes_hut(aff, theta, com64=False)
-    e_alg = metrics.summary(got, exact)
-    e_par = metrics.summary(got_ref, ref)
-    e_dev = metrics.summary(got, ref)
-    e_refcom = metrics.rms_rel(ref, exact)
-    print(label, theta, "vs exact-COM oracle", e_alg, "ref-COM mode vs reference", e_par,
-          "default vs reference", e_dev, "reference COM rounding", e_refcom)
-    assert e_alg["rms_rel"] < 5e-7 and e_alg["max_rel"] < 1e-4
-    assert e_par["rms_rel"] < 1e-5
-    assert e_dev["rms_rel"] < e_refcom + 2e-6
+# Sorts of at most 4,096 keys run in one workgroup; the trees below hold more bodies, so their
+# keys take the multi-tile LSD radix sort that builds every large tree (stable order of equal keys,
+# deep-run refinement after it).
+RADIX_BASE = 6000
+
+
+def test_tree_with_duplicates_radix_path(res, orc):
+    p = workloads.direct_sum_case(RADIX_BASE, seed=16).particles
+    p = np.concatenate([p, p[:400], p[10:12], p[10:11], p[::7][:300], p[5:6], p[5:6]])
+    assert p.shape[0] > 4096
+    bh(res, p, 0.5)
+    assert_same_tree(res, orc, p)
+
+
+def test_tree_massless_and_cancelling_masses_radix_path(res, orc):
+    p = workloads.direct_sum_case(RADIX_BASE, seed=17).particles
+    massless = p.copy()
+    massless[:, 3] = 0.0
+    bh(res, massless, 0.5)
+    assert_same_tree(res, orc, massless)
+    pairs = p.copy()
+    pairs[:, 3] = np.float32(0.25)
+    pairs[1::2, 3] = np.float32(-0.25)
+    pairs[1::2, :3] = pairs[0::2, :3] + np.float32(1e-3)
+    bh(res, pairs, 0.5)
+    assert_same_tree(res, orc, pairs)
+    gn, _ = pa.tree_export(res, pairs.shape[0])
+    assert ((gn["mass64"] == 0.0) & (gn["internal"] == 1)).sum() > 10
+
+
+@pytest.mark.parametrize("scale", [1e-7, 1e-16, 1e-30])
+def test_tree_deep_runs_radix_path(res, orc, scale):
+    rng = np.random.default_rng(70)
+    base = workloads.direct_sum_case(RADIX_BASE, seed=18).particles
+    cluster = np.zeros((40, 4), np.float32)
+    cluster[:, :3] = np.float32(0.25) + (rng.integers(0, 4, (40, 3)) * scale).astype(np.float32)
+    cluster[:, 3] = 1.0 / RADIX_BASE
+    tiny = np.zeros((9, 4), np.float32)
+    tiny[:, 0] = (np.arange(9) * scale).astype(np.float32)
+    tiny[:, 3] = 1.0 / RADIX_BASE
+    # the deep structures interleaved with the base, so equal 21-level keys arrive out of order
+    p = np.concatenate([base[:2500], cluster, base[2500:], tiny, cluster[:5]])
+    bh(res, p, 0.5)
+    t = assert_same_tree(res, orc, p)
+    print("max level", t.max_level)
 
 
 @pytest.mark.parametrize("theta", [0.0, 0.3, 0.5, 0.8])

@@ -1,0 +1,162 @@
+"""The Barnes-Hut and batched-ensemble workloads at their full sizes, through the C ABI, against
+the oracle.
+
+* Two Plummer spheres of 2,097,152 bodies each (4,194,304 in all): the GPU octree node for node
+  against the restated RootedOrthtree (this also pins the multi-tile radix sort on 4M keys and the
+  6.2M-node pre-order), then theta in {0.3, 0.5, 0.8} on a seeded sample against the restated
+  parallel::BarnesHut (particular/src/parallel.rs:342-367, traversal sequential.rs:456-506, tree
+  tree/mod.rs:91-139) and against the FP64 direct sum.
+* The same system with the learned correction at theta = 0.8: the corrected walk minus the plain
+  walk against the oracle's per-lane prediction on sampled wavefronts, and a smaller error than
+  plain Barnes-Hut against the FP64 direct sum.
+* 2,048 Plummer systems of 64 ... 65,536 bodies (18.4M in all): initial per-system forces against
+  the oracle's sequential::BruteForce (sequential.rs:181-194) on a seeded sample of systems that
+  includes the largest, and graph-replayed steps bit-identical to eager ones.
+"""
+import numpy as np
+import pytest
+
+import particular_amd as pa
+from oracle import mlp_oracle as M
+from particular_amd import _lib, learned, metrics, workloads
+
+from .treecheck import assert_same_tree, bh, check_against_oracle
+
+pytestmark = pytest.mark.gpu
+
+N_TWO_PLUMMER = 4194304
+
+
+@pytest.fixture(scope="module")
+def two_plummer():
+    return workloads.two_plummer_case(N_TWO_PLUMMER)
+
+
+@pytest.fixture(scope="module")
+def two_plummer_tree(orc, two_plummer):
+    return orc.Tree(two_plummer.particles)
+
+
+def test_two_plummer_4m_tree_matches_oracle(res, orc, two_plummer, two_plummer_tree):
+    p = two_plummer.particles
+    bh(res, pa.Between(p[:64], p), 0.5)  # the tree of all 4M bodies; a one-wavefront walk
+    t = assert_same_tree(res, orc, p, two_plummer_tree)
+    print("nodes", t.num_nodes, "max level", t.max_level, flush=True)
+    assert t.num_nodes > p.shape[0]
+
+
+@pytest.mark.parametrize("theta", [0.3, 0.5, 0.8])
+def test_two_plummer_4m_matches_reference(res, orc, two_plummer, two_plummer_tree, theta):
+    p = two_plummer.particles
+    idx = np.random.default_rng(20 + int(theta * 10)).choice(p.shape[0], 4096, replace=False)
+    check_against_oracle(res, two_plummer_tree, p, theta, idx, "two-plummer 4M")
+    # accuracy of the method against the FP64 direct sum (the error is theta's, not the walk's)
+    sub = idx[:2048]
+    got = bh(res, pa.Between(p[sub], p), theta)
+    e64 = metrics.summary(got, orc.direct_f64(p[sub], p))
+    print(f"theta {theta}: vs FP64 direct sum on {sub.size} bodies {e64}", flush=True)
+    assert e64["rms_rel"] < {0.3: 5e-4, 0.5: 2e-3, 0.8: 1e-2}[theta]
+
+
+def _wave_index(sorted_particles, lanes):
+    """Wavefront number of each sampled wavefront, from its first lane's position."""
+    first = {tuple(sorted_particles[64 * w, :3].tolist()): w
+             for w in range(sorted_particles.shape[0] // 64)}
+    return np.array([first[tuple(l[0, :3].tolist())] for l in lanes])
+
+
+def test_two_plummer_4m_corrected_walk(res, orc, two_plummer):
+    import torch
+
+    p = two_plummer.particles
+    theta = 0.8
+    params = learned.default_params()
+    learned.set_params(res, params)
+    held = learned.params_to_dict(learned.get_params(res).astype(np.float64))
+    dev = torch.from_numpy(p).cuda()
+    learned.clear(res)
+    learned.collect(res, dev, theta, waves=24, seed=11)
+    d = learned.dataset(res)
+    plain = bh(res, p, theta)
+    nn = bh(res, p, theta, mlp=True)
+    nodes, order = pa.tree_export(res, p.shape[0])
+    waves = _wave_index(p[order], d["lanes"])
+    y, _ = M.forward(held, d["feats"][:, :M.IN].astype(np.float64), bf16=True)
+    pred = M.predict(y, d["row_wave"].astype(np.int64), d["mask"], d["geom"].astype(np.float64),
+                     d["lanes"].astype(np.float64), len(waves))
+    lanes_idx = (64 * waves[:, None] + np.arange(64)[None, :]).ravel()
+    corr = (nn - plain)[order][lanes_idx]
+    err = np.abs(corr - pred.reshape(-1, 3))
+    scale = np.abs(corr).max()
+    print(f"correction scale {scale:.3e}, max err {err.max():.3e}, median {np.median(err):.3e}",
+          flush=True)
+    assert scale > 0
+    assert err.max() < 3e-2 * scale and np.median(err) < 1e-3 * scale
+    # the corrected walk is closer to the FP64 direct sum than plain Barnes-Hut, on the same lanes
+    orig = order[lanes_idx]
+    ref = orc.direct_f64(p[orig], p)
+    e_plain = metrics.summary(plain[orig], ref)
+    e_nn = metrics.summary(nn[orig], ref)
+    print(f"theta {theta} on {orig.size} bodies: plain {e_plain}, corrected {e_nn}", flush=True)
+    assert e_nn["rms_rel"] < 0.75 * e_plain["rms_rel"]
+
+
+@pytest.fixture(scope="module")
+def ensemble():
+    return workloads.ensemble_case()
+
+
+def _batch_init(res, e):
+    pos = np.ascontiguousarray(e.pos)
+    vel = np.ascontiguousarray(e.vel)
+    mu = np.ascontiguousarray(e.particles[:, 3])
+    off = np.ascontiguousarray(e.offsets, dtype=np.int64)
+    res.check(_lib.lib().pnb_batch_init(res.handle, pos.ctypes.data, vel.ctypes.data,
+                                        mu.ctypes.data, off.ctypes.data, e.num_systems,
+                                        e.softening, 1))
+
+
+def _batch_read(res, n):
+    x = np.zeros((n, 3))
+    v = np.zeros((n, 3))
+    a = np.zeros((n, 4), np.float32)
+    res.check(_lib.lib().pnb_batch_read(res.handle, x.ctypes.data, v.ctypes.data, a.ctypes.data))
+    return x, v, a
+
+
+def test_ensemble_2048_initial_forces(res, orc, ensemble):
+    e = ensemble
+    assert e.num_systems == 2048 and e.particles.shape[0] > 16_000_000
+    res.reset_stream()
+    _batch_init(res, e)
+    _, _, a = _batch_read(res, e.particles.shape[0])
+    sizes = np.diff(e.offsets)
+    rng = np.random.default_rng(5)
+    sample = np.unique(np.concatenate([[int(np.argmax(sizes)), int(np.argmin(sizes))],
+                                       rng.choice(e.num_systems, 10, replace=False)]))
+    for k in sample:
+        lo, hi = e.offsets[k], e.offsets[k + 1]
+        sys_p = e.particles[lo:hi]
+        ref = orc.brute_force(sys_p, sys_p, softening=e.softening)
+        err = metrics.summary(a[lo:hi], ref)
+        print(f"system {k} ({hi - lo} bodies): {err}", flush=True)
+        assert err["rms_rel"] < 1e-5 and err["max_rel"] < 1e-4
+
+
+def test_ensemble_2048_graph_replays_match_eager(res, ensemble):
+    e = ensemble
+    n = e.particles.shape[0]
+    L = _lib.lib()
+    res.reset_stream()
+    _batch_init(res, e)
+    eager = []
+    for _ in range(2):
+        res.check(L.pnb_batch_run(res.handle, 1, 1e-3, 0))
+        res.synchronize()
+        eager.append(_batch_read(res, n))
+    _batch_init(res, e)
+    for i in range(2):
+        res.check(L.pnb_batch_run(res.handle, 1, 1e-3, 1))
+        res.synchronize()
+        for got, want in zip(_batch_read(res, n), eager[i]):
+            np.testing.assert_array_equal(got, want)
