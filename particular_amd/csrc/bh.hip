@@ -134,6 +134,9 @@ struct BhState {
     float rel_alpha = 1e-4f, rel_theta_est = 1.2f;
     bool rel_per_particle = false;  // else one estimate per wavefront, at its centroid
     DevBuf info, bbox_part, hist, bits, afg_tmp;
+    // reference-exact centres of mass (PNB_BH_REF_COM): per body its node at the current level,
+    // (group, input index) keys and the bodies regrouped in that order
+    DevBuf rc_cur, rc_keys, rc_keys_alt, rc_vals, rc_vals_alt, rc_ord;
     DevBuf akeys, akeys_alt, aidx, aidx_alt, apts, h_aff, h_afg, h_out;
     DevBuf c_starts, c_pts, c_bh, c_bf, c_list, c_count, c_target, c_wgt, c_rows, c_rwave,
         c_cfeats, c_cgeom, c_cmask, c_lanes;
@@ -809,39 +812,95 @@ __global__ __launch_bounds__(1024) void bh_up_run_kernel(NodeRec* __restrict__ r
     }
 }
 
-// Reference-exact node data (PNB_BH_REF_COM): the f32 folds of impls/mod.rs:126-131 in the
-// node's particle order (tree order restricted to the node), one thread per node. The folds are
-// sequential by definition, so a node of k bodies costs k dependent steps on one lane.
-__global__ __launch_bounds__(256) void bh_ref_com_kernel(NodeRec* __restrict__ rec,
-                                                         const float4* __restrict__ sp,
-                                                         size_t total) {
-    for (size_t k = size_t(blockIdx.x) * blockDim.x + threadIdx.x; k < total;
-         k += size_t(gridDim.x) * blockDim.x) {
-        const uint32_t b = rec[k].begin, e = rec[k].end;
-        float tot = 0.0f;
-        for (uint32_t q = b; q < e; ++q) tot = add_rn(tot, sp[q].w);
-        float cx = 0.0f, cy = 0.0f, cz = 0.0f;
-        if (tot == 0.0f) {
-            for (uint32_t q = b; q < e; ++q) {
-                const float4 v = sp[q];
-                cx = add_rn(cx, v.x); cy = add_rn(cy, v.y); cz = add_rn(cz, v.z);
-            }
-            const float len = float(e - b);
-            cx = __fdiv_rn(cx, len); cy = __fdiv_rn(cy, len); cz = __fdiv_rn(cz, len);
-        } else {
-            for (uint32_t q = b; q < e; ++q) {
-                const float4 v = sp[q];
-                const float f = __fdiv_rn(v.w, tot);
-                cx = add_rn(cx, mul_rn(v.x, f));
-                cy = add_rn(cy, mul_rn(v.y, f));
-                cz = add_rn(cz, mul_rn(v.z, f));
-            }
+// Reference-exact node data (PNB_BH_REF_COM): the f32 folds of impls/mod.rs:126-131 over each
+// node's slice in the order the reference holds it. The reference fills a child's slice by pushing
+// the parent's bodies in order (tree/mod.rs:116-125), so every node folds its bodies in input
+// order. Tree order agrees with that inside a leaf (its bodies share one key, and the sort keeps
+// equal keys in input order) but not across the children of an internal node. Leaves therefore
+// fold tree order; the internal nodes of level L fold `ord`, the bodies regrouped by the level-L
+// node that holds them and sorted by input index inside each group. One thread per node: the
+// folds are sequential by definition, so a node of k bodies costs k dependent steps on one lane.
+__device__ __forceinline__ void ref_com_fold(const float4* __restrict__ src, uint32_t b,
+                                             uint32_t e, NodeRec* __restrict__ r) {
+    float tot = 0.0f;
+    for (uint32_t q = b; q < e; ++q) tot = add_rn(tot, src[q].w);
+    float cx = 0.0f, cy = 0.0f, cz = 0.0f;
+    if (tot == 0.0f) {
+        for (uint32_t q = b; q < e; ++q) {
+            const float4 v = src[q];
+            cx = add_rn(cx, v.x); cy = add_rn(cy, v.y); cz = add_rn(cz, v.z);
         }
-        rec[k].cx = cx;
-        rec[k].cy = cy;
-        rec[k].cz = cz;
-        rec[k].mass = tot;
+        const float len = float(e - b);
+        cx = __fdiv_rn(cx, len); cy = __fdiv_rn(cy, len); cz = __fdiv_rn(cz, len);
+    } else {
+        for (uint32_t q = b; q < e; ++q) {
+            const float4 v = src[q];
+            const float f = __fdiv_rn(v.w, tot);
+            cx = add_rn(cx, mul_rn(v.x, f));
+            cy = add_rn(cy, mul_rn(v.y, f));
+            cz = add_rn(cz, mul_rn(v.z, f));
+        }
     }
+    r->cx = cx;
+    r->cy = cy;
+    r->cz = cz;
+    r->mass = tot;
+}
+
+__global__ __launch_bounds__(256) void bh_ref_com_leaf_kernel(NodeRec* __restrict__ rec,
+                                                              const float4* __restrict__ sp,
+                                                              size_t total) {
+    const size_t k = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= total || __float_as_int(rec[k].w2) >= 0) return;  // internal nodes: by level
+    ref_com_fold(sp, rec[k].begin, rec[k].end, &rec[k]);
+}
+
+// Per body (tree position q): cur[q] moves to the node of level L that holds q (the child of
+// cur[q] whose range covers q, when cur[q] is internal; a leaf above level L keeps it), and the
+// sort key is (that node's first tree position, input index). Every body of an internal node of
+// level L then shares its group; groups tile [0, m) in tree order, so after the sort each such
+// node's bodies occupy its own range [begin, end), in input order.
+__global__ __launch_bounds__(256) void bh_ref_com_group_kernel(const NodeRec* __restrict__ rec,
+                                                               const uint32_t* __restrict__ skipv,
+                                                               const uint32_t* __restrict__ sidx,
+                                                               uint32_t m, int descend,
+                                                               uint32_t* __restrict__ cur,
+                                                               uint64_t* __restrict__ keys,
+                                                               uint32_t* __restrict__ vals) {
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= m) return;
+    uint32_t k = descend ? cur[q] : 0u;
+    if (descend && __float_as_int(rec[k].w2) >= 0) {
+        const uint32_t last = skipv[k];  // children are k + 1, skip[k + 1], ... before skip[k]
+        uint32_t ch = k + 1;
+        while (rec[ch].end <= q) {
+            const uint32_t nx = skipv[ch];
+            if (nx >= last) break;
+            ch = nx;
+        }
+        k = ch;
+    }
+    cur[q] = k;
+    keys[q] = (uint64_t(rec[k].begin) << 32) | sidx[q];
+    vals[q] = q;
+}
+
+__global__ __launch_bounds__(256) void bh_gather_keyed_kernel(const float4* __restrict__ afg,
+                                                              const uint64_t* __restrict__ keys,
+                                                              uint32_t m,
+                                                              float4* __restrict__ ord) {
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < m) ord[q] = afg[uint32_t(keys[q])];
+}
+
+__global__ __launch_bounds__(256) void bh_ref_com_internal_kernel(NodeRec* __restrict__ rec,
+                                                                  const uint32_t* __restrict__ list,
+                                                                  uint32_t n,
+                                                                  const float4* __restrict__ ord) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t k = list[i];
+    ref_com_fold(ord, rec[k].begin, rec[k].end, &rec[k]);
 }
 
 // ---- learned correction: node features and the node network ------------------------------------
@@ -1364,8 +1423,35 @@ static int build_tree(pnb_ctx* c, const float4* afg, size_t m, bool ref_com = fa
         }
         s->has_moments = moments;
         if (ref_com) {
-            bh_ref_com_kernel<<<div_up(total, 256), 256, 0, c->stream>>>(rec, s->sp.as<float4>(), total);
+            bh_ref_com_leaf_kernel<<<div_up(total, 256), 256, 0, c->stream>>>(
+                rec, s->sp.as<float4>(), total);
             PNB_CHECK_LAUNCH(c);
+            PNB_HIP(c, s->rc_cur.ensure(m * 4));
+            PNB_HIP(c, s->rc_keys.ensure(m * 8));
+            PNB_HIP(c, s->rc_keys_alt.ensure(m * 8));
+            PNB_HIP(c, s->rc_vals.ensure(m * 4));
+            PNB_HIP(c, s->rc_vals_alt.ensure(m * 4));
+            PNB_HIP(c, s->rc_ord.ensure(m * sizeof(float4)));
+            for (int L = 0; L < levels; ++L) {
+                bh_ref_com_group_kernel<<<g, 256, 0, c->stream>>>(
+                    rec, skipv, si, uint32_t(m), L > 0, s->rc_cur.as<uint32_t>(),
+                    s->rc_keys.as<uint64_t>(), s->rc_vals.as<uint32_t>());
+                PNB_CHECK_LAUNCH(c);
+                uint64_t* ko;
+                uint32_t* vo;
+                if ((rc = radix_sort_pairs(c, s->rc_keys.as<uint64_t>(), s->rc_vals.as<uint32_t>(),
+                                           s->rc_keys_alt.as<uint64_t>(),
+                                           s->rc_vals_alt.as<uint32_t>(), m, 63, s->hist, s->bits,
+                                           &ko, &vo)))
+                    return rc;
+                bh_gather_keyed_kernel<<<g, 256, 0, c->stream>>>(afg, ko, uint32_t(m),
+                                                                 s->rc_ord.as<float4>());
+                PNB_CHECK_LAUNCH(c);
+                if (h.lvl_count[L] == 0) continue;
+                bh_ref_com_internal_kernel<<<div_up(h.lvl_count[L], 256), 256, 0, c->stream>>>(
+                    rec, ilist + off[L], h.lvl_count[L], s->rc_ord.as<float4>());
+                PNB_CHECK_LAUNCH(c);
+            }
         }
 
         s->built = true;

@@ -35,6 +35,34 @@ def test_python_signatures_cover_header():
     assert set(declared_symbols()) <= set(_lib.SIGNATURES)
 
 
+def _c_params(decl):
+    """Parameter count of each C declaration `int pnb_x(...)` in the header."""
+    txt = re.sub(r"/\*.*?\*/", "", decl, flags=re.S)
+    out = {}
+    for name, params in re.findall(r"\b(pnb_[a-z0-9_]+)\s*\(([^)]*)\)", txt):
+        params = params.strip()
+        out[name] = 0 if params in ("", "void") else params.count(",") + 1
+    return out
+
+
+def test_rust_bindings_match_header():
+    """Every function the Rust crate declares (particular-hip/src/ffi.rs) is declared in the
+    header with the same number of parameters and exported by the library."""
+    header = {}
+    for fn in os.listdir(os.path.join(ROOT, "include")):
+        if fn.endswith(".h"):
+            header.update(_c_params(open(os.path.join(ROOT, "include", fn)).read()))
+    ffi = open(os.path.join(ROOT, "particular-hip", "src", "ffi.rs")).read()
+    rust = {name: (0 if not params.strip() else params.count(",") + 1 - params.rstrip().endswith(","))
+            for name, params in re.findall(r"pub fn (pnb_[a-z0-9_]+)\(([^)]*)\)", ffi)}
+    assert len(rust) >= 10
+    L = _lib.lib()
+    for name, count in rust.items():
+        assert name in header, name
+        assert header[name] == count, (name, header[name], count)
+        assert hasattr(L, name), name
+
+
 def gpu_present():
     return pa.device_count() > 0
 

@@ -11,7 +11,7 @@ import pytest
 import particular_amd as pa
 from particular_amd import metrics, workloads
 
-from .treecheck import assert_same_tree, bh, check_against_oracle
+from .treecheck import assert_reference_com, assert_same_tree, bh, check_against_oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -91,7 +91,8 @@ def test_tree_with_duplicates_radix_path(res, orc):
     p = np.concatenate([p, p[:400], p[10:12], p[10:11], p[::7][:300], p[5:6], p[5:6]])
     assert p.shape[0] > 4096
     bh(res, p, 0.5)
-    assert_same_tree(res, orc, p)
+    t = assert_same_tree(res, orc, p)
+    assert_reference_com(res, p, t)
 
 
 def test_tree_massless_and_cancelling_masses_radix_path(res, orc):
@@ -99,15 +100,17 @@ def test_tree_massless_and_cancelling_masses_radix_path(res, orc):
     massless = p.copy()
     massless[:, 3] = 0.0
     bh(res, massless, 0.5)
-    assert_same_tree(res, orc, massless)
+    t = assert_same_tree(res, orc, massless)
+    assert_reference_com(res, massless, t)
     pairs = p.copy()
     pairs[:, 3] = np.float32(0.25)
     pairs[1::2, 3] = np.float32(-0.25)
     pairs[1::2, :3] = pairs[0::2, :3] + np.float32(1e-3)
     bh(res, pairs, 0.5)
-    assert_same_tree(res, orc, pairs)
+    t = assert_same_tree(res, orc, pairs)
     gn, _ = pa.tree_export(res, pairs.shape[0])
     assert ((gn["mass64"] == 0.0) & (gn["internal"] == 1)).sum() > 10
+    assert_reference_com(res, pairs, t)
 
 
 @pytest.mark.parametrize("scale", [1e-7, 1e-16, 1e-30])
@@ -125,6 +128,7 @@ def test_tree_deep_runs_radix_path(res, orc, scale):
     bh(res, p, 0.5)
     t = assert_same_tree(res, orc, p)
     print("max level", t.max_level)
+    assert_reference_com(res, p, t)
 
 
 @pytest.mark.parametrize("theta", [0.0, 0.3, 0.5, 0.8])

@@ -20,7 +20,7 @@ import particular_amd as pa
 from oracle import mlp_oracle as M
 from particular_amd import _lib, learned, metrics, workloads
 
-from .treecheck import assert_same_tree, bh, check_against_oracle
+from .treecheck import assert_reference_com, assert_same_tree, bh, check_against_oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -43,6 +43,7 @@ def test_two_plummer_4m_tree_matches_oracle(res, orc, two_plummer, two_plummer_t
     t = assert_same_tree(res, orc, p, two_plummer_tree)
     print("nodes", t.num_nodes, "max level", t.max_level, flush=True)
     assert t.num_nodes > p.shape[0]
+    assert_reference_com(res, p, two_plummer_tree)
 
 
 @pytest.mark.parametrize("theta", [0.3, 0.5, 0.8])

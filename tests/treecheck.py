@@ -28,6 +28,18 @@ def assert_same_tree(res, orc, particles, tree=None):
     return t
 
 
+def assert_reference_com(res, particles, tree):
+    """A build with reference-exact centres of mass (PNB_BH_REF_COM) gives every node the f32
+    mass and centre of mass of the reference's folds (impls/mod.rs:120-134 over each node's slice
+    in the reference's order, tree/mod.rs:108-125), as the oracle restates them, bit for bit."""
+    bh(res, pa.Between(particles[:1], particles), 0.5, reference_com=True)
+    gn, _ = pa.tree_export(res, particles.shape[0])
+    on = tree.nodes()
+    assert gn.shape[0] == on.shape[0]
+    np.testing.assert_array_equal(gn["mass"], on["mass"])
+    np.testing.assert_array_equal(gn["com"], on["com"])
+
+
 def bh(res, storage, theta, interaction=None, **kw):
     return pa.BarnesHut(res, theta, interaction or pa.Acceleration.checked(), **kw).compute(storage)
 
