@@ -416,6 +416,59 @@ def bench_batch(args):
     }
 
 
+def bench_all_massive(args):
+    """All-massive brute force at N = args.massive_n (one Plummer sphere, mu = 1/N,
+    AccelerationSoftened::checked(1e-3), every body affecting every other): the one-sided kernel
+    (N^2 pair terms, sequential::BruteForce's sums) and the pair kernel (N (N - 1) / 2 pair terms,
+    both accelerations from each, sequential::BruteForcePairs' sums). Device time per evaluation
+    from HIP events on the library stream; interactions counted as N (N - 1) for both, so the
+    rates compare directly; each against the FP64 direct sum on 4,096 sampled bodies."""
+    import torch
+
+    import particular_amd as pa
+    from particular_amd import _lib, metrics, workloads
+
+    n = args.massive_n
+    log(f"generating all-massive Plummer workload ({n})")
+    s = workloads.direct_sum_case(n, seed=2024)
+    dev = torch.from_numpy(s.particles).cuda()
+    res = pa.HipResources(0)
+    res.use_torch_stream()
+    eps = 1e-3
+    sample = min(4096, n)
+    idx = np.random.default_rng(9).choice(n, sample, replace=False)
+    pts = dev[torch.from_numpy(idx).cuda()].contiguous()
+    ref = torch.empty((sample, 3), dtype=torch.float64, device="cuda")
+    res.check(_lib.lib().pnb_direct_f64_dev(res.handle, pts.data_ptr(), sample, dev.data_ptr(), n,
+                                            eps, ref.data_ptr()))
+    torch.cuda.synchronize()
+    ref = ref.cpu().numpy()
+    out = {"workload": f"{n} bodies, one Plummer sphere (scale 1), mu = 1/N, "
+                       "AccelerationSoftened::checked(1e-3), all massive",
+           "interactions_per_eval": n * (n - 1)}
+    inter = n * (n - 1)
+    for name, alg in (("one_sided", pa.BruteForce(res, pa.AccelerationSoftened.checked(eps))),
+                      ("pairs", pa.BruteForcePairs(res, pa.AccelerationSoftened.checked(eps)))):
+        acc = alg.compute(dev)
+        torch.cuda.synchronize()
+        res.profile(True)
+        for _ in range(args.massive_reps):
+            acc = alg.compute(dev)
+        torch.cuda.synchronize()
+        ms, launches = res.profile_read(_lib.PROF_BF)
+        res.profile(False)
+        ms /= max(launches, 1)
+        err = metrics.summary(acc.cpu().numpy()[idx], ref)
+        out[name] = {"ms_per_eval": ms, "g_interactions_per_s": inter / (ms * 1e-3) / 1e9,
+                     "pair_terms_per_eval": n * n if name == "one_sided" else n * (n - 1) // 2,
+                     **err}
+        log(f"all-massive {name}: {ms:.2f} ms per evaluation "
+            f"({out[name]['g_interactions_per_s']:.0f} G interactions/s), rms {err['rms_rel']:.2e}")
+    out["pairs_speedup"] = out["one_sided"]["ms_per_eval"] / out["pairs"]["ms_per_eval"]
+    res.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -441,6 +494,11 @@ def main():
     ap.add_argument("--batch", action="store_true", default=True)
     ap.add_argument("--no-batch", dest="batch", action="store_false")
     ap.add_argument("--batch-steps", type=int, default=100)
+    ap.add_argument("--massive", action="store_true", default=True,
+                    help="all-massive brute force, one-sided and over pairs")
+    ap.add_argument("--no-massive", dest="massive", action="store_false")
+    ap.add_argument("--massive-n", type=int, default=1048576)
+    ap.add_argument("--massive-reps", type=int, default=2)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -468,6 +526,8 @@ def main():
         }
     if args.batch:
         result["batch"] = bench_batch(args)
+    if args.massive:
+        result["all_massive"] = bench_all_massive(args)
     print(json.dumps(result), flush=True)
 
 

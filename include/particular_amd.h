@@ -101,6 +101,21 @@ int pnb_reordered_bf_accel_f32(pnb_ctx *ctx, const float *particles, size_t n, f
 int pnb_reordered_bf_accel_f32_dev(pnb_ctx *ctx, const float *particles, size_t n,
                                    float softening, int checked, float *out);
 
+/* Brute force over pairs for one particle set (every particle affects and is affected): each
+ * unordered pair is evaluated once and gives both accelerations (Newton's third law). Replaces
+ * sequential::BruteForcePairs (particular/src/sequential.rs:275-305, RestrictedBruteForce
+ * :215-263) with Acceleration's pair term (particular/src/gravity/impls/mod.rs:185-205):
+ * out_i = sum_{j != i} mu_j d_ij / (|d_ij|^2 + eps^2)^1.5. Its Ordered and Reordered forms
+ * (sequential.rs:288-305, 368-385) give the same sums over the whole set (pairs of two massless
+ * particles add zero). checked = 0 keeps the reference's unchecked pair form (a pair at zero
+ * distance is non-finite); checked = 1 drops such pairs as pnb_bf_accel_f32 does. Sums are
+ * compensated, in a fixed order. n up to 2^22; device scratch of 16 * ceil(n / 4096)^2 * 4096
+ * bytes (one partial sum per body and block of 4,096 bodies: 4.3 GB at n = 2^20). */
+int pnb_bf_pairs_accel_f32(pnb_ctx *ctx, const float *particles, size_t n, float softening,
+                           int checked, float *out);
+int pnb_bf_pairs_accel_f32_dev(pnb_ctx *ctx, const float *particles, size_t n, float softening,
+                               int checked, float *out);
+
 /* Brute-force kernel shape. variant: 0 = affecting tile staged in LDS, 1 = affecting particles
  * read as wave-uniform scalar loads. per_lane: affected particles per lane (1, 2, 4, 8).
  * j_split: affecting ranges per affected block (1..64, partial sums reduced in fixed order).

@@ -5,6 +5,7 @@
  *   pair kernel  particular/src/gravity/impls/mod.rs:151-166, impls/glam.rs:232
  *   norm         glam Vec3::length_squared = (x*x + y*y) + z*z (impls/glam.rs:180)
  *   BruteForce   particular/src/sequential.rs:181-194
+ *   BruteForcePairs particular/src/sequential.rs:215-305, impls/mod.rs:185-205
  *   COM          particular/src/gravity/impls/mod.rs:120-134
  *   root box     particular/src/tree/partition.rs:121-188
  *   tree build   particular/src/tree/mod.rs:73-139, partition.rs:193-220
@@ -73,6 +74,39 @@ int orc_bf_f32(const float *affected, size_t n, const float *affecting, size_t m
             sx = sx + ox; sy = sy + oy; sz = sz + oz;
         }
         out[4 * i + 0] = sx; out[4 * i + 1] = sy; out[4 * i + 2] = sz; out[4 * i + 3] = 0.0f;
+    }
+    return 0;
+}
+
+/* sequential.rs:232-263 (RestrictedBruteForce::compute) with Acceleration's pair term
+ * (newtonian/acceleration.rs:95-109 -> impls/mod.rs:185-205): for i < restrict_len and
+ * j = i+1 .. n-1, dir = acceleration_at::<false>(GravitationalField(x_j, 1), x_i, eps), i.e.
+ * d * (1 / (r2s * sqrt(r2s))) with no zero-distance check; output_i += dir * mu_j (a separate
+ * fold, added to out[i] after the j loop) and out[j] += -dir * mu_i. checked = 1 (not the
+ * reference's behaviour, which has no check on this path) drops pairs at zero unsoftened
+ * distance instead. Sequential by definition. */
+int orc_bf_pairs_f32(const float *p, size_t n, size_t restrict_len, float softening, int checked,
+                     float *out) {
+    if ((n && (!p || !out)) || restrict_len > n) return 1;
+    for (size_t k = 0; k < 4 * n; ++k) out[k] = 0.0f;
+    for (size_t i = 0; i < restrict_len; ++i) {
+        const float *a = p + 4 * i;
+        float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+        for (size_t j = i + 1; j < n; ++j) {
+            const float *b = p + 4 * j;
+            float dx = b[0] - a[0], dy = b[1] - a[1], dz = b[2] - a[2];
+            float r2 = norm2f(dx, dy, dz);
+            if (checked && r2 == 0.0f) continue;
+            float ns = r2 + softening * softening;
+            float f = 1.0f / (ns * sqrtf(ns));
+            float ux = dx * f, uy = dy * f, uz = dz * f;
+            sx = sx + ux * b[3]; sy = sy + uy * b[3]; sz = sz + uz * b[3];
+            float *o = out + 4 * j;
+            o[0] = o[0] + (-ux) * a[3]; o[1] = o[1] + (-uy) * a[3]; o[2] = o[2] + (-uz) * a[3];
+        }
+        out[4 * i] = out[4 * i] + sx;
+        out[4 * i + 1] = out[4 * i + 1] + sy;
+        out[4 * i + 2] = out[4 * i + 2] + sz;
     }
     return 0;
 }

@@ -31,6 +31,14 @@ void orc_accel_pair(const float *p2, const float *at, float softening, int check
 int orc_bf_f32(const float *affected, size_t n, const float *affecting, size_t m, float softening,
                int checked, float *out, int threads);
 
+/* sequential::BruteForcePairs over a slice (restrict_len = n) or an Ordered storage (restrict_len =
+ * affecting count, massive particles first): RestrictedBruteForce (particular/src/sequential.rs:
+ * 215-263) with the pair term of particular/src/gravity/impls/mod.rs:185-205, in the reference's
+ * order. checked = 0 is the reference (no zero-distance check on this path); checked = 1 drops
+ * pairs at zero unsoftened distance. Single-threaded. */
+int orc_bf_pairs_f32(const float *p, size_t n, size_t restrict_len, float softening, int checked,
+                     float *out);
+
 /* FP64 direct sum, the accuracy reference: a_i = sum_j mu_j d_ij / (|d_ij|^2 + eps^2)^{3/2}, pairs
  * with |d_ij| == 0 skipped. Output 3 doubles per particle. */
 int orc_direct_f64(const float *affected, size_t n, const float *affecting, size_t m,

@@ -72,6 +72,7 @@ def lib() -> ctypes.CDLL:
         sz = ctypes.c_size_t
         L.orc_accel_pair.argtypes = [fp, fp, ctypes.c_float, ctypes.c_int, fp]
         L.orc_bf_f32.argtypes = [fp, sz, fp, sz, ctypes.c_float, ctypes.c_int, fp, ctypes.c_int]
+        L.orc_bf_pairs_f32.argtypes = [fp, sz, sz, ctypes.c_float, ctypes.c_int, fp]
         L.orc_direct_f64.argtypes = [fp, sz, fp, sz, ctypes.c_double, dp, ctypes.c_int]
         L.orc_tree_build.argtypes = [fp, sz, ctypes.c_int]
         L.orc_tree_build.restype = ctypes.c_void_p
@@ -127,6 +128,19 @@ def brute_force(affected, affecting, softening=0.0, checked=True, threads=0) -> 
                           _fp(out), threads)
     if rc:
         raise RuntimeError(f"orc_bf_f32 failed ({rc})")
+    return out
+
+
+def brute_force_pairs(particles, restrict_len=None, softening=0.0, checked=False) -> np.ndarray:
+    """f32 sequential::BruteForcePairs (RestrictedBruteForce over the first restrict_len
+    particles, default all); returns (n, 4) float32 with w = 0. checked=False is the reference."""
+    a = _f32x4(particles)
+    n = a.shape[0]
+    out = np.zeros((n, 4), np.float32)
+    rc = lib().orc_bf_pairs_f32(_fp(a), n, n if restrict_len is None else restrict_len,
+                                softening, int(checked), _fp(out))
+    if rc:
+        raise RuntimeError(f"orc_bf_pairs_f32 failed ({rc})")
     return out
 
 

@@ -25,6 +25,9 @@ extern "C" {
     pub fn pnb_reordered_bf_accel_f32(ctx: *mut PnbCtx, particles: *const c_float, n: usize,
                                       softening: c_float, checked: c_int,
                                       out: *mut c_float) -> c_int;
+    pub fn pnb_bf_pairs_accel_f32(ctx: *mut PnbCtx, particles: *const c_float, n: usize,
+                                  softening: c_float, checked: c_int,
+                                  out: *mut c_float) -> c_int;
     pub fn pnb_bh_accel_f32(ctx: *mut PnbCtx, affected: *const c_float, n: usize,
                             affecting: *const c_float, m: usize, theta: c_float,
                             softening: c_float, checked: c_int, flags: c_uint,

@@ -12,6 +12,7 @@ reference (particular 0.8)       here
 ``Ordered`` / ``Reordered``       :class:`Ordered` / :class:`Reordered` (storage.rs:51-205)
 ``GpuResources``                  :class:`HipResources`       (particular/src/gpu/mod.rs:84-143)
 ``gpu::BruteForce``               :class:`BruteForce`         (particular/src/gpu/mod.rs:149-208)
+``sequential::BruteForcePairs``   :class:`BruteForcePairs`    (particular/src/sequential.rs:275-305)
 ``parallel::BarnesHut``           :class:`BarnesHut`          (particular/src/parallel.rs:298-367)
 ``GpuCompute`` sugar              :func:`hip_brute_force`, :func:`hip_barnes_hut` (gpu/mod.rs:13-37)
 ==============================  ==============================================================
@@ -37,7 +38,8 @@ from ._lib import PnbError, check
 
 __all__ = [
     "Between", "Acceleration", "AccelerationSoftened", "Ordered", "Reordered", "HipResources",
-    "BruteForce", "BarnesHut", "hip_brute_force", "hip_barnes_hut", "PnbError", "device_count",
+    "BruteForce", "BruteForcePairs", "BarnesHut", "hip_brute_force", "hip_barnes_hut", "PnbError",
+    "device_count",
 ]
 
 
@@ -371,6 +373,40 @@ class BruteForce(_Algorithm):
         eps, chk = self._params()
         fn = _lib.lib().pnb_bf_accel_f32_2d_dev if dev else _lib.lib().pnb_bf_accel_f32_2d
         self.resources.check(fn(self.resources.handle, a, n, b, m, eps, chk, out))
+
+
+class BruteForcePairs(_Algorithm):
+    """Brute force over pairs on the GPU: sequential::BruteForcePairs
+    (particular/src/sequential.rs:275-305). One particle set, every pair evaluated once for both
+    accelerations (Newton's third law). Storages as the reference takes them: a slice (an (n, 4)
+    array or CUDA tensor), Ordered and Reordered, whose sums are those over the whole set (pairs
+    of two massless particles add zero). Between has no pair form in the reference."""
+
+    def compute(self, storage):
+        if isinstance(storage, Between):
+            raise TypeError("BruteForcePairs computes one particle set (a slice, Ordered or "
+                            "Reordered), not Between")
+        if isinstance(storage, Reordered):
+            particles = storage.unordered
+        elif isinstance(storage, Ordered):
+            particles = storage.particles()
+        else:
+            particles = storage
+        eps, chk = self._params()
+        if _is_torch(particles) and particles.is_cuda:
+            import torch
+
+            a = _check_dev(particles)
+            out = torch.empty((a.shape[0], 4), dtype=torch.float32, device=a.device)
+            self.resources.use_torch_stream()
+            self.resources.check(_lib.lib().pnb_bf_pairs_accel_f32_dev(
+                self.resources.handle, a.data_ptr(), a.shape[0], eps, chk, out.data_ptr()))
+            return out
+        a = _as_host(particles)
+        out = np.zeros((a.shape[0], 4), np.float32)
+        self.resources.check(_lib.lib().pnb_bf_pairs_accel_f32(
+            self.resources.handle, a.ctypes.data, a.shape[0], eps, chk, out.ctypes.data))
+        return out[:, :3]
 
 
 class BarnesHut(_Algorithm):

@@ -59,6 +59,8 @@ SIGNATURES = {
     "pnb_bf_accel_f32": (_i, [_vp, _vp, _sz, _vp, _sz, _f, _i, _vp]),
     "pnb_bf_accel_f32_dev": (_i, [_vp, _vp, _sz, _vp, _sz, _f, _i, _vp]),
     "pnb_reordered_bf_accel_f32": (_i, [_vp, _vp, _sz, _f, _i, _vp]),
+    "pnb_bf_pairs_accel_f32": (_i, [_vp, _vp, _sz, _f, _i, _vp]),
+    "pnb_bf_pairs_accel_f32_dev": (_i, [_vp, _vp, _sz, _f, _i, _vp]),
     "pnb_reordered_bf_accel_f32_dev": (_i, [_vp, _vp, _sz, _f, _i, _vp]),
     "pnb_bf_set_tuning": (_i, [_vp, _sz, _i, _i, _i]),
     "pnb_bf_clear_tuning": (_i, [_vp]),

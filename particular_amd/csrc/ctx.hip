@@ -253,7 +253,7 @@ void pnb_destroy(pnb_ctx* c) {
     pnb::batch_free(c->batch);
     pnb::mlp_free(c->mlp);
     for (pnb::DevBuf* b : {&c->h_affected, &c->h_affecting, &c->h_out, &c->split_tmp,
-                           &c->split_flags, &c->bf_partial, &c->scan_tmp, &c->d2_raw_a,
+                           &c->split_flags, &c->bf_partial, &c->scan_tmp, &c->pairs_part, &c->d2_raw_a,
                            &c->d2_raw_b, &c->d2_a, &c->d2_b, &c->d2_out4, &c->d2_out2})
         b->release();
     for (int k = 0; k < pnb_ctx::EV_POOL; ++k) {

@@ -44,6 +44,7 @@ struct pnb_ctx {
     // scratch for host-pointer entry points and the massive/massless split
     pnb::DevBuf h_affected, h_affecting, h_out, split_tmp, split_flags, bf_partial;
     pnb::DevBuf scan_tmp;
+    pnb::DevBuf pairs_part;  // pair brute force: one partial per body and super-tile
     // two-dimensional entry points: {x, y, mu} records staged, widened to float4, results narrowed
     pnb::DevBuf d2_raw_a, d2_raw_b, d2_a, d2_b, d2_out4, d2_out2;
 

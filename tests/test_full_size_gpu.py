@@ -161,3 +161,25 @@ def test_ensemble_2048_graph_replays_match_eager(res, ensemble):
         res.synchronize()
         for got, want in zip(_batch_read(res, n), eager[i]):
             np.testing.assert_array_equal(got, want)
+
+
+def test_all_massive_1m_pairs(res, orc):
+    """The all-massive N = 1,048,576 Plummer sphere (eps = 1e-3) over pairs: a seeded sample of
+    bodies against the oracle's sequential::BruteForce sums and the FP64 direct sum, and every body
+    against the one-sided GPU brute force."""
+    import torch
+
+    s = workloads.direct_sum_case(1 << 20, seed=2024)
+    p = s.particles
+    dev = torch.from_numpy(p).cuda()
+    inter = pa.AccelerationSoftened.checked(1e-3)
+    got = pa.BruteForcePairs(res, inter).compute(dev).cpu().numpy()[:, :3]
+    one = pa.BruteForce(res, inter).compute(dev).cpu().numpy()[:, :3]
+    e_one = metrics.summary(got, one)
+    idx = np.random.default_rng(12).choice(p.shape[0], 1024, replace=False)
+    e = metrics.summary(got[idx], orc.brute_force(p[idx], p, softening=1e-3))
+    e64 = metrics.summary(got[idx], orc.direct_f64(p[idx], p, 1e-3))
+    print(f"pairs at 1M: vs one-sided GPU {e_one}, vs oracle {e}, vs FP64 {e64}", flush=True)
+    assert e_one["rms_rel"] < 1e-6
+    assert e["rms_rel"] < 1e-5 and e["max_rel"] < 1e-4
+    assert e64["rms_rel"] < 1e-5 and e64["max_rel"] < 1e-4

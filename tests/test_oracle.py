@@ -108,6 +108,80 @@ def test_large_softening_limit(orc):
     np.testing.assert_allclose(got, exp, rtol=1e-4, atol=1e-12 * np.abs(exp).max())
 
 
+# ---- BruteForcePairs (sequential.rs:215-305) ----------------------------------------------------
+
+def test_pairs_kat_reordered(orc):
+    """The reference's acceleration_error! runs BruteForcePairs on the Reordered storage: pairs
+    over the reordered particles (massive first) restricted to the massive ones, outputs mapped
+    back to the original order (sequential.rs:368-385). Against the test's own FP64 expected
+    value, at the reference's tolerance and at rounding level."""
+    p = kat_particles()
+    aff = p[:, 3] != 0
+    order = np.concatenate([np.nonzero(aff)[0], np.nonzero(~aff)[0]])
+    got_re = orc.brute_force_pairs(p[order], restrict_len=int(aff.sum()))
+    got = np.empty_like(got_re)
+    got[order] = got_re
+    massive = p[aff].astype(np.float64)
+    exp = np.zeros((6, 3))
+    for i, p1 in enumerate(p.astype(np.float64)):
+        for p2 in massive:
+            d = p2[:3] - p1[:3]
+            m2 = d @ d
+            if m2 != 0:
+                exp[i] += d * p2[3] * np.sqrt(1.0 / m2) / m2
+    assert np.all(np.linalg.norm(1.0 - got[:, :3] / exp, axis=1) <= 1e-2)
+    assert metrics.max_rel(got, exp) < 1e-6
+
+
+@pytest.mark.parametrize("n,eps", [(2, 0.0), (9, 0.0), (64, 0.0), (37, 0.05)])
+def test_pairs_vs_exhaustive_f64(orc, n, eps):
+    """Unequal masses (a swapped mu_i / mu_j or a wrong reaction sign fails this)."""
+    rng = np.random.default_rng(100 + n)
+    p = rng.normal(size=(n, 4)).astype(np.float32)
+    p[:, 3] = rng.uniform(0.1, 3.0, n)
+    got = orc.brute_force_pairs(p, softening=eps)
+    exp = numpy_direct_f64(p, p, eps)
+    assert metrics.max_rel(got, exp) < 2e-5
+
+
+def test_pairs_restricted_is_ordered_brute_force(orc):
+    """RestrictedBruteForce over massive-first particles with restrict_len = massive count is
+    Between(particles, massive): the pairs of two massless particles are never formed."""
+    s = workloads.direct_sum_case(300, seed=21)
+    p = s.particles.copy()
+    p[200:, 3] = 0.0
+    got = orc.brute_force_pairs(p, restrict_len=200)
+    exp = numpy_direct_f64(p, p[:200])
+    assert metrics.max_rel(got, exp) < 2e-5
+    # coincident massless particles: the restricted form never pairs them, so no NaN
+    q = np.concatenate([p, p[250:252]])
+    with np.errstate(invalid="ignore", divide="ignore"):
+        got = orc.brute_force_pairs(q, restrict_len=200)
+    assert np.all(np.isfinite(got))
+
+
+def test_pairs_checked_and_unchecked(orc):
+    """The reference's pair form has no zero-distance check: a coincident pair is non-finite
+    (1 / 0 then 0 * inf); checked drops the pair as the one-sided brute force does."""
+    p = np.array([[1, 2, 3, 5], [1, 2, 3, 2], [2, 2, 3, 1]], np.float32)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        u = orc.brute_force_pairs(p)
+    assert not np.all(np.isfinite(u[:2, :3]))
+    c = orc.brute_force_pairs(p, checked=True)
+    np.testing.assert_allclose(c, orc.brute_force(p, p, checked=True), rtol=1e-6)
+
+
+def test_pairs_reaction_order(orc):
+    """Particle k receives the reactions of i < k in order i = 0 .. k-1, then its own j > k fold
+    (sequential.rs:249-258): +1e8, +1, -1e8 reactions sum to 0 only in that order."""
+    # particle 3 at the origin; particles 0, 1, 2 at x = +1, +1, -1 with masses 1e8, 1, 1e8
+    p = np.array([[1, 0, 0, 1e8], [1, 0, 0, 1], [-1, 0, 0, 1e8], [0, 0, 0, 1]], np.float32)
+    got = orc.brute_force_pairs(p)
+    assert got[3, 0] == 0.0
+    got = orc.brute_force_pairs(p[[0, 2, 1, 3]])
+    assert got[3, 0] == 1.0
+
+
 # ---- octree ---------------------------------------------------------------------------------
 
 def test_tree_hand_worked_two_bodies(orc):
