@@ -93,6 +93,18 @@ int pnb_bf_accel_f32(pnb_ctx *ctx, const float *affected, size_t n, const float 
 int pnb_bf_accel_f32_dev(pnb_ctx *ctx, const float *affected, size_t n, const float *affecting,
                          size_t m, float softening, int checked, float *out);
 
+/* pnb_bf_accel_f32_dev without any device allocation. No counterpart in the reference: for
+ * callers that cannot allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). The same kernels and shapes as pnb_bf_accel_f32_dev, so the same results; the
+ * partial sums of the affecting split go to a static 256 MiB array per GPU (splits * n records of
+ * 16 bytes, at most 16,777,216; the default shapes stay within it on 256 CUs) instead of the
+ * context's buffers. A shape that needs more (a pnb_bf_set_tuning entry with a large split) is
+ * rejected with PNB_E_ARG. Every call on one GPU shares that array, whatever the context: calls
+ * that overlap on one GPU (several streams or threads) race. */
+int pnb_bf_accel_f32_dev_noalloc(pnb_ctx *ctx, const float *affected, size_t n,
+                                 const float *affecting, size_t m, float softening, int checked,
+                                 float *out);
+
 /* Reordered lowering (particular/src/storage.rs:219-229): the affecting set is the particles
  * with mu != 0, in their original relative order (Ordered::new, storage.rs:85-95); every particle
  * is affected and out follows the original order. */

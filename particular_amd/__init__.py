@@ -189,6 +189,10 @@ class HipResources:
     def check(self, status: int) -> None:
         check(status, self.handle)
 
+    def last_error(self) -> str:
+        """Message of the last failure on this context (pnb_last_error)."""
+        return _lib.lib().pnb_last_error(self.handle).decode()
+
     def synchronize(self) -> None:
         self.check(_lib.lib().pnb_synchronize(self.handle))
 

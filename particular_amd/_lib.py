@@ -58,6 +58,7 @@ SIGNATURES = {
     "pnb_profile_read": (_i, [_vp, _i, _pd, _pu64]),
     "pnb_bf_accel_f32": (_i, [_vp, _vp, _sz, _vp, _sz, _f, _i, _vp]),
     "pnb_bf_accel_f32_dev": (_i, [_vp, _vp, _sz, _vp, _sz, _f, _i, _vp]),
+    "pnb_bf_accel_f32_dev_noalloc": (_i, [_vp, _vp, _sz, _vp, _sz, _f, _i, _vp]),
     "pnb_reordered_bf_accel_f32": (_i, [_vp, _vp, _sz, _f, _i, _vp]),
     "pnb_bf_pairs_accel_f32": (_i, [_vp, _vp, _sz, _f, _i, _vp]),
     "pnb_bf_pairs_accel_f32_dev": (_i, [_vp, _vp, _sz, _f, _i, _vp]),

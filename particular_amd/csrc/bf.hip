@@ -19,6 +19,7 @@
 #include "pnb_internal.hpp"
 
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 namespace pnb {
@@ -208,6 +209,54 @@ int bf_prepare(pnb_ctx* c, size_t n, size_t m) {
     return PNB_OK;
 }
 
+// Scratch of the allocation-free entry point (pnb_bf_accel_f32_dev_noalloc): the partial sums of
+// the affecting split, splits * n records. 256 MiB: the default shapes need at most
+// 64 * CUs * BF_BLOCK * 4 records (16,777,216 on 256 CUs). Every GPU the library runs on holds its
+// own copy until the process ends; every call on that GPU shares it.
+constexpr size_t BF_SCRATCH_RECORDS = size_t(1) << 24;
+static __device__ __align__(256) float4 scratch[BF_SCRATCH_RECORDS];
+constexpr int BF_SCRATCH_DEVICES = 64;
+static float4* g_scratch_dev[BF_SCRATCH_DEVICES];  // device address of `scratch` per device
+static std::mutex g_scratch_mu;
+
+// Once per device, with that device current (called by pnb_create): the device address of
+// `scratch`. In host code the array's name is a host-side stand-in, not a device address.
+int bf_noalloc_init(pnb_ctx* c) {
+    if (c->device < 0 || c->device >= BF_SCRATCH_DEVICES) return PNB_OK;  // entry point refuses
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[c->device]) {
+        void* ptr = nullptr;
+        PNB_HIP(c, hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        g_scratch_dev[c->device] = static_cast<float4*>(ptr);
+    }
+    c->noalloc_scratch = g_scratch_dev[c->device];
+    return PNB_OK;
+}
+
+// The launch itself, with the split's partial sums in `partial` (at least splits * n records;
+// unused when the plan has one split). No allocation.
+static int bf_launch_with(pnb_ctx* c, const BfPlan& p, const float4* affected, size_t n,
+                          const float4* affecting, size_t m, float softening, int checked,
+                          float4* out, float4* partial) {
+    const BfShape sh = p.shape;
+    const bool mask = checked && softening == 0.0f;
+    BfKernel k = pick_kernel(sh.variant, sh.per_lane, mask);
+    const float eps2 = softening * softening;
+    const uint32_t splits = p.splits, chunk = p.chunk;
+    dim3 grid(div_up(n, size_t(BF_BLOCK) * sh.per_lane), splits);
+    float4* dst = splits > 1 ? partial : out;
+    return timed(c, PNB_PROF_BF, [&]() -> int {
+        k<<<grid, BF_BLOCK, 0, c->stream>>>(affected, uint32_t(n), affecting, uint32_t(m), chunk,
+                                            eps2, dst);
+        PNB_CHECK_LAUNCH(c);
+        if (splits > 1) {
+            bf_reduce_kernel<<<div_up(n, 256), 256, 0, c->stream>>>(dst, uint32_t(n), splits, out);
+            PNB_CHECK_LAUNCH(c);
+        }
+        return PNB_OK;
+    });
+}
+
 int bf_launch(pnb_ctx* c, const float4* affected, size_t n, const float4* affecting, size_t m,
               float softening, int checked, float4* out) {
     if (n == 0) return PNB_OK;
@@ -219,30 +268,14 @@ int bf_launch(pnb_ctx* c, const float4* affected, size_t n, const float4* affect
         return PNB_OK;
     }
     const BfPlan p = plan(c, n, m);
-    const BfShape sh = p.shape;
-    const bool mask = checked && softening == 0.0f;
-    BfKernel k = pick_kernel(sh.variant, sh.per_lane, mask);
-    const float eps2 = softening * softening;
-    const uint32_t splits = p.splits, chunk = p.chunk;
-    dim3 grid(div_up(n, size_t(BF_BLOCK) * sh.per_lane), splits);
-    float4* dst = out;
-    if (splits > 1) {
-        const size_t need = size_t(splits) * n * sizeof(float4);
+    if (p.splits > 1) {
+        const size_t need = size_t(p.splits) * n * sizeof(float4);
         if (c->capturing && c->bf_partial.cap < need)
             return fail(c, PNB_E_STATE, "brute-force scratch not prepared before graph capture");
         PNB_HIP(c, c->bf_partial.ensure(need));
-        dst = c->bf_partial.as<float4>();
     }
-    return timed(c, PNB_PROF_BF, [&]() -> int {
-        k<<<grid, BF_BLOCK, 0, c->stream>>>(affected, uint32_t(n), affecting, uint32_t(m), chunk,
-                                            eps2, dst);
-        PNB_CHECK_LAUNCH(c);
-        if (splits > 1) {
-            bf_reduce_kernel<<<div_up(n, 256), 256, 0, c->stream>>>(dst, uint32_t(n), splits, out);
-            PNB_CHECK_LAUNCH(c);
-        }
-        return PNB_OK;
-    });
+    return bf_launch_with(c, p, affected, n, affecting, m, softening, checked, out,
+                          c->bf_partial.as<float4>());
 }
 
 // ---- FP64 direct sum: one workgroup per affected particle, fixed-order tree reduction --------
@@ -346,6 +379,36 @@ int pnb_bf_accel_f32_dev(pnb_ctx* c, const float* affected, size_t n, const floa
     return bf_launch(c, reinterpret_cast<const float4*>(affected), n,
                      reinterpret_cast<const float4*>(affecting), m, softening, checked,
                      reinterpret_cast<float4*>(out));
+}
+
+int pnb_bf_accel_f32_dev_noalloc(pnb_ctx* c, const float* affected, size_t n,
+                                 const float* affecting, size_t m, float softening, int checked,
+                                 float* out) {
+    if (!c) return PNB_E_ARG;
+    if ((n && (!affected || !out)) || (m && !affecting))
+        return fail(c, PNB_E_ARG, "null pointer with non-zero length");
+    if (!(softening >= 0.0f)) return fail(c, PNB_E_ARG, "softening must be >= 0");
+    if (n == 0) return PNB_OK;
+    if (n > 0x7fffffffu || m > 0x7fffffffu)
+        return fail(c, PNB_E_ARG, "brute force supports at most 2^31-1 particles per side");
+    if (!c->noalloc_scratch)
+        return fail(c, PNB_E_STATE, "no allocation-free scratch on device %d", c->device);
+    if (int rc = activate(c)) return rc;
+    float4* o = reinterpret_cast<float4*>(out);
+    if (m == 0) {
+        fill_zero_kernel<<<div_up(n, 256), 256, 0, c->stream>>>(o, n);
+        PNB_CHECK_LAUNCH(c);
+        return PNB_OK;
+    }
+    const BfPlan p = plan(c, n, m);
+    if (p.splits > 1 && size_t(p.splits) * n > BF_SCRATCH_RECORDS)
+        return fail(c, PNB_E_ARG,
+                    "the allocation-free brute force holds %zu partial sums; %u affecting ranges "
+                    "x %zu affected need %zu",
+                    BF_SCRATCH_RECORDS, p.splits, n, size_t(p.splits) * n);
+    return bf_launch_with(c, p, reinterpret_cast<const float4*>(affected), n,
+                          reinterpret_cast<const float4*>(affecting), m, softening, checked, o,
+                          c->noalloc_scratch);
 }
 
 int pnb_bf_accel_f32(pnb_ctx* c, const float* affected, size_t n, const float* affecting,

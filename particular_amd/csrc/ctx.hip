@@ -238,6 +238,11 @@ int pnb_create(int device, pnb_ctx** out) {
         return PNB_E_NODEV;
     }
     c->stream = c->own_stream;
+    if (pnb::bf_noalloc_init(c) != PNB_OK) {
+        (void)hipGetLastError();
+        pnb_destroy(c);
+        return PNB_E_HIP;
+    }
     *out = c;
     return PNB_OK;
 }

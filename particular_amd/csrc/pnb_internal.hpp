@@ -45,6 +45,8 @@ struct pnb_ctx {
     pnb::DevBuf h_affected, h_affecting, h_out, split_tmp, split_flags, bf_partial;
     pnb::DevBuf scan_tmp;
     pnb::DevBuf pairs_part;  // pair brute force: one partial per body and super-tile
+    // device address of the allocation-free entry point's static scratch on this device
+    float4* noalloc_scratch = nullptr;
     // two-dimensional entry points: {x, y, mu} records staged, widened to float4, results narrowed
     pnb::DevBuf d2_raw_a, d2_raw_b, d2_a, d2_b, d2_out4, d2_out2;
 
@@ -108,6 +110,9 @@ int timed(pnb_ctx* c, int kernel_id, F&& launch) {
     if (rc != PNB_OK) c->ev_used = k;  // drop this slot and any nested ones
     return rc;
 }
+
+// Looks up the allocation-free brute force's static scratch on c->device (once per device).
+int bf_noalloc_init(pnb_ctx* c);
 
 void bf_tune_free(BfTune*);
 void bh_free(BhState*);
