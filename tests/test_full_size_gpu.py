@@ -177,9 +177,15 @@ def test_all_massive_1m_pairs(res, orc):
     one = pa.BruteForce(res, inter).compute(dev).cpu().numpy()[:, :3]
     e_one = metrics.summary(got, one)
     idx = np.random.default_rng(12).choice(p.shape[0], 1024, replace=False)
-    e = metrics.summary(got[idx], orc.brute_force(p[idx], p, softening=1e-3))
-    e64 = metrics.summary(got[idx], orc.direct_f64(p[idx], p, 1e-3))
-    print(f"pairs at 1M: vs one-sided GPU {e_one}, vs oracle {e}, vs FP64 {e64}", flush=True)
+    ref = orc.brute_force(p[idx], p, softening=1e-3)
+    ref64 = orc.direct_f64(p[idx], p, 1e-3)
+    e = metrics.summary(got[idx], ref)
+    e64 = metrics.summary(got[idx], ref64)
+    e_ref = metrics.summary(ref, ref64)  # the reference's own f32 fold over 1M terms
+    print(f"pairs at 1M: vs one-sided GPU {e_one}, vs oracle {e}, vs FP64 {e64}; "
+          f"oracle vs FP64 {e_ref}", flush=True)
     assert e_one["rms_rel"] < 1e-6
-    assert e["rms_rel"] < 1e-5 and e["max_rel"] < 1e-4
-    assert e64["rms_rel"] < 1e-5 and e64["max_rel"] < 1e-4
+    assert e64["rms_rel"] < 1e-6 and e64["max_rel"] < 1e-5
+    # the distance to the restated reference is the reference's own rounding (its sequential
+    # f32 fold of 1M terms is ~2e-5 rms from FP64), not this path's
+    assert e["rms_rel"] < e_ref["rms_rel"] + 1e-6 and e["max_rel"] < e_ref["max_rel"] + 1e-5
