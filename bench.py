@@ -11,6 +11,12 @@ entry uses the brute-force kernel's average launch time from HIP events recorded
 the timed region, 20 flop per interaction, against the 157.3 TFLOP/s FP32 peak. The CPU baseline
 times the oracle's restatement of parallel::BruteForce (same per-particle fold, all host threads
 up to 16) on a bounded sample of affected particles against the full massive set.
+
+Further entries of the same line: `barnes_hut` (two Plummer spheres of 4,194,304 bodies: plain and
+learned-correction walks across angles and the relative criterion, the equal-error comparison,
+the walk's roofline against the brute-force pair rate with counters from profiles/, the CPU
+baseline of parallel::BarnesHut), `batch` (2,048 systems, 100 KDK steps in one hipGraph) and
+`all_massive` (1,048,576 bodies, one-sided brute force against BruteForcePairs).
 """
 from __future__ import annotations
 
@@ -65,11 +71,61 @@ def cpu_model() -> str:
 
 
 def load_traffic(name: str):
-    path = os.path.join(ROOT, "profiles", f"{name}_traffic.json")
+    return load_profile(f"{name}_traffic")
+
+
+def load_profile(name: str):
+    """A machine-readable profile summary from profiles/ (counters measured with rocprofv3)."""
+    path = os.path.join(ROOT, "profiles", f"{name}.json")
     if os.path.exists(path):
         with open(path) as f:
             return json.load(f)
     return None
+
+
+def bh_roofline(rows, bf_result):
+    """The plain walk at the angle where its error is closest to 1e-4 rms: accepted pair terms
+    per second (live) against the brute-force kernel's pair rate on the same chip (live), with the
+    walk's counters (VALU per wavefront step, cache hit rates, HBM traffic) from the rocprofv3
+    summary of the same build in profiles/."""
+    plain = [r for r in rows if r["criterion"] == "geometric" and not r["learned_correction"]]
+    if not plain:
+        return None
+    r = min(plain, key=lambda r: abs(np.log(r["rms_rel"] / 1e-4)))
+    bf_rate = bf_result["config"]["interactions_per_step"] / (
+        bf_result["roofline"]["avg_kernel_ms"] * 1e-3) / 1e9
+    out = {
+        "bound": "issue",
+        "kernel": "bh_walk_kernel<0> (plain walk)",
+        "theta": r["theta"],
+        "rms_rel": r["rms_rel"],
+        "achieved": r["accepted_G_terms_per_s"],
+        "peak": bf_rate,
+        "unit": "G pair terms/s (peak: the brute-force kernel's pair rate, live)",
+        "frac": r["accepted_G_terms_per_s"] / bf_rate,
+        "walk_ms": r["walk_ms"],
+        "wave_steps_per_wave": r["wave_steps_per_wave"],
+        "accepted_terms_per_particle": r["accepted_terms_per_particle"],
+    }
+    prof = load_profile("bh_walk_two_plummer_4M")
+    if prof:
+        key = min(prof["theta"], key=lambda k: abs(float(k) - r["theta"]))
+        c = prof["theta"][key]
+        out.update({
+            "counters_theta": float(key),
+            "valu_per_wave_step": c["valu_per_wave_step"],
+            "salu_per_wave_step": c["salu_per_wave_step"],
+            "valu_lane_ops_per_accepted_term": c["valu"] * 64 / (
+                r["accepted_terms_per_particle"] * 4194304) if float(key) == r["theta"] else None,
+            "valu_per_cu_cycle": c["valu_per_cu_cycle"],
+            "sqc_hit_rate": c["sqc_hit_rate"],
+            "l2_hit_rate": c["l2_hit_rate"],
+            "traffic": c["hbm_read_bytes"] + c["hbm_write_bytes"],
+            "hbm_GBps": c["hbm_GBps"],
+            "hbm_frac": c["hbm_GBps"] / HBM_PEAK_GBS,
+            "profile": prof["source"],
+        })
+    return out
 
 
 def bench_bf(args):
@@ -465,6 +521,11 @@ def bench_all_massive(args):
         log(f"all-massive {name}: {ms:.2f} ms per evaluation "
             f"({out[name]['g_interactions_per_s']:.0f} G interactions/s), rms {err['rms_rel']:.2e}")
     out["pairs_speedup"] = out["one_sided"]["ms_per_eval"] / out["pairs"]["ms_per_eval"]
+    prof = load_profile("pairs_kernel_all_massive_1M")
+    if prof and n == 1 << 20:
+        out["counters"] = {k: prof[k] for k in (
+            "valu_lane_ops_per_interaction", "valu_per_cu_cycle",
+            "one_sided_valu_lane_ops_per_interaction", "one_sided_valu_per_cu_cycle", "source")}
     res.close()
     return out
 
@@ -524,6 +585,7 @@ def main():
             "corrected_vs_plain": comparison,
             "cpu_baseline": None if args.no_cpu else cpu_baseline_bh(s3, 0.5, args.bh_cpu_sample),
         }
+        result["barnes_hut"]["roofline"] = bh_roofline(bh_rows, result)
     if args.batch:
         result["batch"] = bench_batch(args)
     if args.massive:
