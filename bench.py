@@ -15,8 +15,10 @@ up to 16) on a bounded sample of affected particles against the full massive set
 Further entries of the same line: `barnes_hut` (two Plummer spheres of 4,194,304 bodies: plain and
 learned-correction walks across angles and the relative criterion, the equal-error comparison,
 the walk's roofline against the brute-force pair rate with counters from profiles/, the CPU
-baseline of parallel::BarnesHut), `batch` (2,048 systems, 100 KDK steps in one hipGraph) and
-`all_massive` (1,048,576 bodies, one-sided brute force against BruteForcePairs).
+baseline of parallel::BarnesHut), `batch` (2,048 systems, 100 KDK steps in one hipGraph),
+`all_massive` (1,048,576 bodies, one-sided brute force against BruteForcePairs) and
+`small_barnes_hut` (per-call latency on 64 and 4,096 bodies, ordinary, capturable and replayed
+from a hipGraph).
 """
 from __future__ import annotations
 
@@ -472,6 +474,53 @@ def bench_batch(args):
     }
 
 
+def bench_small_bh(args):
+    """Latency of one Barnes-Hut evaluation (theta = 0.5, tree rebuilt every call) on small
+    Plummer spheres, device tensors, median of `--small-calls` calls each synchronised: the
+    ordinary build (two host round trips), the capturable build (none) called eagerly, and the
+    capturable build replayed from a hipGraph."""
+    import torch
+
+    import particular_amd as pa
+    from particular_amd import workloads
+
+    out = {}
+    for n in (64, 4096):
+        s = workloads.direct_sum_case(n, seed=11)
+        dev = torch.from_numpy(s.particles).cuda()
+        both = pa.Between(dev, dev)  # one device pointer twice: pnb_bh_accel_f32_dev
+        row = {}
+        for mode in ("ordinary", "capturable", "graph"):
+            res = pa.HipResources(0)
+            alg = pa.BarnesHut(res, 0.5, pa.Acceleration.checked())
+            if mode != "ordinary":
+                res.set_bh_capturable(True)
+            alg.compute(both)
+            alg.compute(both)
+            torch.cuda.synchronize()
+            call = lambda: alg.compute(both)
+            if mode == "graph":
+                g = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(g):
+                    alg.compute(both)
+                res.reset_stream()
+                call = g.replay
+            times = []
+            for _ in range(args.small_calls):
+                t0 = time.perf_counter()
+                call()
+                torch.cuda.synchronize()
+                times.append(time.perf_counter() - t0)
+            if mode != "ordinary":
+                res.bh_check()
+            res.close()
+            row[f"{mode}_us"] = float(np.median(times) * 1e6)
+        out[str(n)] = row
+        log(f"Barnes-Hut per call at {n} bodies: " +
+            ", ".join(f"{k} {v:.0f}" for k, v in row.items()))
+    return out
+
+
 def bench_all_massive(args):
     """All-massive brute force at N = args.massive_n (one Plummer sphere, mu = 1/N,
     AccelerationSoftened::checked(1e-3), every body affecting every other): the one-sided kernel
@@ -560,6 +609,10 @@ def main():
     ap.add_argument("--no-massive", dest="massive", action="store_false")
     ap.add_argument("--massive-n", type=int, default=1048576)
     ap.add_argument("--massive-reps", type=int, default=2)
+    ap.add_argument("--small", action="store_true", default=True,
+                    help="per-call latency of Barnes-Hut on 64 and 4,096 bodies")
+    ap.add_argument("--no-small", dest="small", action="store_false")
+    ap.add_argument("--small-calls", type=int, default=200)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -590,6 +643,8 @@ def main():
         result["batch"] = bench_batch(args)
     if args.massive:
         result["all_massive"] = bench_all_massive(args)
+    if args.small:
+        result["small_barnes_hut"] = bench_small_bh(args)
     print(json.dumps(result), flush=True)
 
 

@@ -192,6 +192,23 @@ int pnb_reordered_bh_accel_f32(pnb_ctx *ctx, const float *particles, size_t n, f
 int pnb_reordered_bh_accel_f32_dev(pnb_ctx *ctx, const float *particles, size_t n, float theta,
                                    float softening, int checked, unsigned flags, float *out);
 
+/* Capturable Barnes-Hut (no reference counterpart; for step loops captured in a hipGraph and
+ * for callers that cannot wait on the host). With enable != 0, pnb_bh_accel_f32_dev (flags 0,
+ * Between on device pointers) builds and walks the same tree, with the same results, without
+ * any host round trip or allocation once sized: the first call at given particle counts is an
+ * ordinary call that sizes the buffers (a quarter more nodes and 8 more levels than its tree
+ * needs) and must not be captured; later calls at the same counts can be captured on the
+ * context's stream (pnb_set_stream) and replayed with new particle data. A later tree that
+ * needs more nodes or levels than were sized is flagged on the device: its accelerations are NaN
+ * and pnb_bh_check then returns PNB_E_STATE (call again outside a capture to size anew). The other
+ * flags (PNB_BH_MLP, PNB_BH_REF_COM, PNB_BH_RELATIVE) are refused in this mode; the Reordered
+ * entry points run in it but read the massive count on the host, so they are refused inside a
+ * capture. Changing the mode forgets the sizes. */
+int pnb_bh_set_capturable(pnb_ctx *ctx, int enable);
+/* Waits for the context's stream and reports the status of the last capturable build: PNB_OK,
+ * PNB_E_STATE (did not fit its sizes), PNB_E_DEPTH or PNB_E_ARG (non-finite positions). */
+int pnb_bh_check(pnb_ctx *ctx);
+
 /* The last octree built on a context, for inspection (the reference exposes its tree through
  * RootedOrthtree::get, particular/src/storage.rs:41-45). Nodes are in the reference's order
  * (depth-first pre-order, children in orthant order). */

@@ -189,6 +189,15 @@ class HipResources:
     def check(self, status: int) -> None:
         check(status, self.handle)
 
+    def set_bh_capturable(self, enable: bool = True) -> None:
+        """Barnes-Hut without host round trips (pnb_bh_set_capturable): after one ordinary call
+        at given sizes, device-pointer calls can be captured in a graph and replayed."""
+        self.check(_lib.lib().pnb_bh_set_capturable(self.handle, int(enable)))
+
+    def bh_check(self) -> None:
+        """Raises when the last capturable Barnes-Hut build did not fit its sizes (pnb_bh_check)."""
+        self.check(_lib.lib().pnb_bh_check(self.handle))
+
     def last_error(self) -> str:
         """Message of the last failure on this context (pnb_last_error)."""
         return _lib.lib().pnb_last_error(self.handle).decode()

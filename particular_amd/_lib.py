@@ -73,6 +73,8 @@ SIGNATURES = {
     "pnb_bh_accel_f32_2d": (_i, [_vp, _vp, _sz, _vp, _sz, _f, _f, _i, _u32, _vp]),
     "pnb_bh_accel_f32_2d_dev": (_i, [_vp, _vp, _sz, _vp, _sz, _f, _f, _i, _u32, _vp]),
     "pnb_bh_accel_f32": (_i, [_vp, _vp, _sz, _vp, _sz, _f, _f, _i, _u32, _vp]),
+    "pnb_bh_set_capturable": (_i, [_vp, _i]),
+    "pnb_bh_check": (_i, [_vp]),
     "pnb_bh_accel_f32_dev": (_i, [_vp, _vp, _sz, _vp, _sz, _f, _f, _i, _u32, _vp]),
     "pnb_reordered_bh_accel_f32": (_i, [_vp, _vp, _sz, _f, _f, _i, _u32, _vp]),
     "pnb_reordered_bh_accel_f32_dev": (_i, [_vp, _vp, _sz, _f, _f, _i, _u32, _vp]),

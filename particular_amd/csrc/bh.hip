@@ -50,7 +50,8 @@ namespace pnb {
 
 int radix_sort_pairs(pnb_ctx* c, uint64_t* keys, uint32_t* vals, uint64_t* keys_alt,
                      uint32_t* vals_alt, size_t n, int key_bits, DevBuf& hist_buf,
-                     DevBuf& bits_buf, uint64_t** keys_out, uint32_t** vals_out);
+                     DevBuf& bits_buf, uint64_t** keys_out, uint32_t** vals_out,
+                     bool all_digits = false);
 int split_affecting(pnb_ctx* c, const float4* particles, size_t n, float4* dst, uint32_t* m_host);
 
 constexpr int KEY_LEVELS = 21;
@@ -60,6 +61,7 @@ constexpr int DEEP_STACK = 24;
 
 constexpr uint32_t FLAG_NONFINITE = 1u;
 constexpr uint32_t FLAG_DEPTH = 2u;
+constexpr uint32_t FLAG_CAPACITY = 4u;  // capturable build: more nodes or levels than sized for
 
 struct __align__(32) NodeRec {
     float cx, cy, cz, mass;  // centre of mass (f32) and total mu
@@ -102,6 +104,7 @@ struct BhInfo {
     uint32_t max_rows;  // most rows one wavefront listed in the last row-listing walk
     unsigned long long steps, accepted;  // traversal counters of the last walk
     uint32_t lvl_count[MAX_LEVEL + 2];   // internal nodes per level
+    uint32_t lvl_off[MAX_LEVEL + 2];     // their first slot in the level-grouped list
 };
 
 // Per-level counters that many workgroups update: one 128-byte line each, so that atomics on
@@ -134,6 +137,11 @@ struct BhState {
     float rel_alpha = 1e-4f, rel_theta_est = 1.2f;
     bool rel_per_particle = false;  // else one estimate per wavefront, at its centroid
     DevBuf info, bbox_part, hist, bits, afg_tmp;
+    // capturable builds (pnb_bh_set_capturable): sized once by an ordinary build, then no host
+    // round trip; dd_last: the last build's node count and status are only on the device
+    bool dd_mode = false, dd_last = false;
+    size_t dd_m = 0, dd_n = 0, dd_cap = 0;
+    int dd_levels = 0;
     // reference-exact centres of mass (PNB_BH_REF_COM): per body its node at the current level,
     // (group, input index) keys and the bodies regrouped in that order
     DevBuf rc_cur, rc_keys, rc_keys_alt, rc_vals, rc_vals_alt, rc_ord;
@@ -498,6 +506,7 @@ __global__ __launch_bounds__(256) void bh_level_offsets_kernel(BhInfo* __restric
             const uint32_t n_l = v[l];
             v[l] = acc;
             info->lvl_count[l] = n_l;
+            info->lvl_off[l] = acc;
             acc += n_l;
         }
     }
@@ -515,7 +524,9 @@ __global__ __launch_bounds__(256) void bh_level_list_kernel(const int* __restric
                                                             const uint32_t* __restrict__ S,
                                                             size_t m,
                                                             uint32_t* __restrict__ lvl_cur,
-                                                            uint32_t* __restrict__ ilist) {
+                                                            uint32_t* __restrict__ ilist,
+                                                            const BhInfo* __restrict__ info) {
+    if (info->flags) return;  // a capturable build that did not fit: nothing to list
     __shared__ uint32_t cnt[LVL_SLOTS];
     for (int l = threadIdx.x; l < LVL_SLOTS; l += blockDim.x) cnt[l] = 0;
     __syncthreads();
@@ -588,7 +599,7 @@ __global__ __launch_bounds__(256) void bh_node_kernel(
     uint8_t* __restrict__ level, float* __restrict__ width,
     NodeMoments* __restrict__ mom, NodeQuad* __restrict__ quad, uint32_t* __restrict__ skipv) {
     const size_t p = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (p >= m) return;
+    if (p >= m || info->flags) return;  // flags: a capturable build that did not fit
     const int lp = lcp[p];
     if (lp == LCP_SAME) return;
     const int I = ip[p];
@@ -808,6 +819,47 @@ __global__ __launch_bounds__(1024) void bh_up_run_kernel(NodeRec* __restrict__ r
     for (int j = 0; j < run.levels; ++j) {
         for (uint32_t t = threadIdx.x; t < run.cnt[j]; t += blockDim.x)
             up_node(ilist[run.off[j] + t], rec, sums, mom, quad, desc, width, skipv, sp);
+        __syncthreads();
+    }
+}
+
+// ---- capturable builds: node count and status stay on the device ----------------------------
+// The scan's total against the capacities sized by an earlier ordinary build; a tree that does
+// not fit flags the build (every later kernel of it then returns at once and the walk writes NaN).
+__global__ void bh_dd_check_kernel(const uint32_t* __restrict__ S, size_t m, uint32_t cap,
+                                   int levels, BhInfo* __restrict__ info) {
+    const uint32_t total = S[m];
+    info->total = total;
+    if (total > cap || info->max_level > levels) atomicOr(&info->flags, FLAG_CAPACITY);
+}
+
+// The bottom-up sums of one level, its count and place in the level-grouped list read from the
+// device (the host launches every level up to the sized depth; levels the tree lacks are empty).
+__global__ __launch_bounds__(256) void bh_up_level_dd_kernel(NodeRec* __restrict__ rec,
+                                                             NodeSums* __restrict__ sums,
+                                                             const uint32_t* __restrict__ ilist,
+                                                             const BhInfo* __restrict__ info,
+                                                             int level,
+                                                             const uint32_t* __restrict__ skipv,
+                                                             const float4* __restrict__ sp) {
+    if (info->flags) return;
+    const uint32_t cnt = info->lvl_count[level], off = info->lvl_off[level];
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < cnt; t += gridDim.x * blockDim.x)
+        up_node(ilist[off + t], rec, sums, nullptr, nullptr, nullptr, nullptr, skipv, sp);
+}
+
+// Small trees: every level in one workgroup, deepest first, a barrier between levels.
+__global__ __launch_bounds__(1024) void bh_up_all_dd_kernel(NodeRec* __restrict__ rec,
+                                                            NodeSums* __restrict__ sums,
+                                                            const uint32_t* __restrict__ ilist,
+                                                            const BhInfo* __restrict__ info,
+                                                            const uint32_t* __restrict__ skipv,
+                                                            const float4* __restrict__ sp) {
+    if (info->flags) return;  // uniform: every thread returns, or none
+    for (int L = info->max_level - 1; L >= 0; --L) {  // internal levels 0 .. max_level - 1
+        const uint32_t cnt = info->lvl_count[L], off = info->lvl_off[L];
+        for (uint32_t t = threadIdx.x; t < cnt; t += blockDim.x)
+            up_node(ilist[off + t], rec, sums, nullptr, nullptr, nullptr, nullptr, skipv, sp);
         __syncthreads();
     }
 }
@@ -1121,6 +1173,7 @@ __device__ __forceinline__ float half_hi(uint32_t v) {
 // wave_order (REL, optional): the launch's k-th wavefront (k < ceil(n / 64)) walks the particles
 // of wavefront wave_order[k], so that the heaviest wavefronts start first.
 // lanes: only the first `lanes` lanes of each wavefront walk (64 except in the estimate pass).
+// dd: a capturable build's tree (`total` is then its capacity; the count is read from `info`).
 template <int MODE, bool WIDE, bool REL>
 __global__ __launch_bounds__(256) void bh_walk_kernel(const NodeRec* __restrict__ nodes,
                                                       uint32_t total,
@@ -1135,8 +1188,18 @@ __global__ __launch_bounds__(256) void bh_walk_kernel(const NodeRec* __restrict_
                                                       float* __restrict__ est_out, float alpha,
                                                       const uint32_t* __restrict__ wave_order,
                                                       const float4* __restrict__ bodies,
-                                                      uint32_t lanes) {
+                                                      uint32_t lanes, int dd) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (dd) {
+        // capturable build: the node count and the build's status are on the device; a build
+        // that did not fit gives NaN (pnb_bh_check reports why)
+        if (info->flags) {
+            if (i < n && uint32_t(threadIdx.x & 63) < lanes)
+                out[perm ? perm[i] : i] = make_float4(NAN, NAN, NAN, 0.f);
+            return;
+        }
+        total = info->total;
+    }
     if (REL && wave_order) {
         const uint32_t k = i >> 6;  // this wavefront's place in the launch
         if (k < ((n + 63) >> 6)) i = (wave_order[k] << 6) | (i & 63);
@@ -1295,6 +1358,7 @@ static int build_tree(pnb_ctx* c, const float4* afg, size_t m, bool ref_com = fa
     s->built = false;
     s->has_moments = false;
     s->rrec_ok = false;
+    s->dd_last = false;
     s->m = m;
     s->nodes = 0;
     s->internal = 0;
@@ -1398,7 +1462,7 @@ static int build_tree(pnb_ctx* c, const float4* afg, size_t m, bool ref_com = fa
         s->internal = off[levels];
         uint32_t* ilist = s->ilist.as<uint32_t>();
         bh_level_list_kernel<<<lg, 256, 0, c->stream>>>(lcp, s->ip.as<int>(), cnt, m, lvl_cur,
-                                                        ilist);
+                                                        ilist, info);
         PNB_CHECK_LAUNCH(c);
         for (int L = levels - 1; L >= 0;) {
             if (h.lvl_count[L] > UP_RUN_NODES) {
@@ -1457,6 +1521,147 @@ static int build_tree(pnb_ctx* c, const float4* afg, size_t m, bool ref_com = fa
         s->built = true;
         return PNB_OK;
     });
+}
+
+// Whether work is being captured on the context's stream into a graph.
+static bool stream_capturing(pnb_ctx* c) {
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(c->stream, &st) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return st != hipStreamCaptureStatusNone;
+}
+
+// Capacities of a capturable build from an ordinary build of the same size (its node count and
+// depth known on the host): a quarter more nodes and 8 more levels, so that later trees of
+// similar inputs fit.
+static int dd_size(pnb_ctx* c, size_t m) {
+    BhState* s = c->bh;
+    const size_t cap = s->nodes + s->nodes / 4 + 1024;
+    if (cap + 1 >= (size_t(1) << 27))
+        return fail(c, PNB_E_ARG, "capturable Barnes-Hut trees are limited to 2^27 nodes");
+    PNB_HIP(c, s->rec.ensure((cap + 1) * sizeof(NodeRec)));
+    PNB_HIP(c, s->sums.ensure(cap * sizeof(NodeSums)));
+    PNB_HIP(c, s->level.ensure(cap));
+    PNB_HIP(c, s->width.ensure(cap * sizeof(float)));
+    PNB_HIP(c, s->skipv.ensure(cap * sizeof(uint32_t)));
+    PNB_HIP(c, s->ilist.ensure(cap * sizeof(uint32_t)));
+    s->dd_cap = cap;
+    s->dd_levels = std::min(std::max(s->max_level, 0) + 8, MAX_LEVEL + 1);
+    s->dd_m = m;
+    return PNB_OK;
+}
+
+// The octree of a capturable build: the ordinary build's kernels with no host round trip (every
+// digit of the sort, the node count checked and used on the device, the bottom-up sums launched
+// for every level up to the sized depth). Buffers were sized by dd_size for these m.
+static int build_tree_dd(pnb_ctx* c, const float4* afg, size_t m) {
+    BhState* s = c->bh;
+    s->built = false;
+    s->has_moments = false;
+    s->rrec_ok = false;
+    s->m = m;
+    BhInfo* info = s->info.as<BhInfo>();
+    uint32_t* lvl_count = s->lvl.as<uint32_t>();
+    uint32_t* lvl_cur = lvl_count + LVL_SLOTS * LVL_STRIDE;
+    bh_info_init_kernel<<<1, 256, 0, c->stream>>>(info, lvl_count);
+    PNB_CHECK_LAUNCH(c);
+    s->nodes = s->dd_cap;  // an upper bound; the count itself is info->total
+    s->max_level = s->dd_levels;
+    s->internal = 0;
+    s->dd_last = true;
+    if (m == 0) {
+        s->built = true;
+        return PNB_OK;
+    }
+    const unsigned g = div_up(m, 256);
+    return timed(c, PNB_PROF_BH_BUILD, [&]() -> int {
+        const int bb = int(std::min<size_t>(BB_BLOCKS, div_up(m, 256)));
+        bh_bbox_kernel<<<bb, 256, 0, c->stream>>>(afg, m, s->bbox_part.as<float>(), info);
+        PNB_CHECK_LAUNCH(c);
+        bh_root_kernel<<<1, 64, 0, c->stream>>>(s->bbox_part.as<float>(), bb, info);
+        PNB_CHECK_LAUNCH(c);
+        bh_keys_kernel<<<g, 256, 0, c->stream>>>(afg, m, info, s->keys.as<uint64_t>(), s->idx.as<uint32_t>());
+        PNB_CHECK_LAUNCH(c);
+        uint64_t* sk;
+        uint32_t* si;
+        int rc = timed(c, PNB_PROF_SORT, [&]() -> int {
+            return radix_sort_pairs(c, s->keys.as<uint64_t>(), s->idx.as<uint32_t>(),
+                                    s->keys_alt.as<uint64_t>(), s->idx_alt.as<uint32_t>(), m, 63,
+                                    s->hist, s->bits, &sk, &si, true);
+        });
+        if (rc) return rc;
+        s->skeys = sk;
+        s->sidx = si;
+        float4* sp = s->sp.as<float4>();
+        bh_gather_kernel<<<g, 256, 0, c->stream>>>(afg, m, si, sp);
+        PNB_CHECK_LAUNCH(c);
+        int* lcp = s->lcp.as<int>();
+        bh_lcp_kernel<<<g, 256, 0, c->stream>>>(sk, sp, m, lcp);
+        PNB_CHECK_LAUNCH(c);
+        bh_deep_kernel<<<g, 256, 0, c->stream>>>(sk, m, sp, si, lcp, s->key2.as<uint64_t>(), info);
+        PNB_CHECK_LAUNCH(c);
+        uint32_t* cnt = s->cnt.as<uint32_t>();
+        const unsigned lg = std::min<unsigned>(g, 4 * std::max(c->cu_count, 1));
+        bh_count_kernel<<<lg, 256, 0, c->stream>>>(lcp, m, cnt, s->ip.as<int>(), info, lvl_count);
+        PNB_CHECK_LAUNCH(c);
+        bh_level_offsets_kernel<<<1, 256, 0, c->stream>>>(info, lvl_count, lvl_cur);
+        PNB_CHECK_LAUNCH(c);
+        if ((rc = scan_u32(c, cnt, cnt, m, true))) return rc;
+        bh_dd_check_kernel<<<1, 1, 0, c->stream>>>(cnt, m, uint32_t(s->dd_cap), s->dd_levels, info);
+        PNB_CHECK_LAUNCH(c);
+        NodeRec* rec = s->rec.as<NodeRec>();
+        uint32_t* skipv = s->skipv.as<uint32_t>();
+        bh_node_kernel<<<g, 256, 0, c->stream>>>(sk, sp, lcp, s->ip.as<int>(), cnt, m, info, rec,
+                                                 s->sums.as<NodeSums>(), s->level.as<uint8_t>(),
+                                                 s->width.as<float>(), nullptr, nullptr, skipv);
+        PNB_CHECK_LAUNCH(c);
+        uint32_t* ilist = s->ilist.as<uint32_t>();
+        bh_level_list_kernel<<<lg, 256, 0, c->stream>>>(lcp, s->ip.as<int>(), cnt, m, lvl_cur,
+                                                        ilist, info);
+        PNB_CHECK_LAUNCH(c);
+        if (s->dd_cap <= 4 * UP_RUN_NODES) {
+            bh_up_all_dd_kernel<<<1, 1024, 0, c->stream>>>(rec, s->sums.as<NodeSums>(), ilist,
+                                                           info, skipv, sp);
+            PNB_CHECK_LAUNCH(c);
+        } else {
+            const unsigned ug = std::min<unsigned>(div_up(s->dd_cap, 256),
+                                                   4 * std::max(c->cu_count, 1));
+            for (int L = s->dd_levels - 1; L >= 0; --L) {
+                bh_up_level_dd_kernel<<<ug, 256, 0, c->stream>>>(rec, s->sums.as<NodeSums>(),
+                                                                 ilist, info, L, skipv, sp);
+                PNB_CHECK_LAUNCH(c);
+            }
+        }
+        s->built = true;
+        return PNB_OK;
+    });
+}
+
+// After a capturable build: waits for it and takes its node count, depth and status to the host.
+static int dd_resolve(pnb_ctx* c) {
+    BhState* s = c->bh;
+    if (!s || !s->dd_last) return PNB_OK;
+    BhInfo h;
+    PNB_HIP(c, hipMemcpyAsync(&h, s->info.ptr, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    PNB_HIP(c, hipStreamSynchronize(c->stream));
+    s->dd_last = false;
+    if (h.flags) {
+        s->built = false;
+        if (h.flags & FLAG_NONFINITE) return fail(c, PNB_E_ARG, "non-finite affecting position");
+        if (h.flags & FLAG_DEPTH) return fail(c, PNB_E_DEPTH, "octree deeper than %d levels", MAX_LEVEL);
+        return fail(c, PNB_E_STATE,
+                    "the capturable Barnes-Hut build was sized for %zu nodes and %d levels; this "
+                    "tree needs more (the accelerations are NaN): call again outside a capture to "
+                    "size it anew", s->dd_cap, s->dd_levels);
+    }
+    s->nodes = s->m ? h.total : 0;
+    s->max_level = s->m ? h.max_level : -1;
+    size_t internal = 0;
+    for (int L = 0; L < std::max(s->max_level, 0); ++L) internal += h.lvl_count[L];
+    s->internal = internal;
+    return PNB_OK;
 }
 
 const MlpWeights* mlp_weights(pnb_ctx* c);  // mlp.hip: nullptr until initialised
@@ -1542,15 +1747,15 @@ static void launch_walk(bool wide, unsigned grid, hipStream_t st, const NodeRec*
                         const QuadH* qh, const float* est_in = nullptr, uint32_t est_shift = 0,
                         float* est_out = nullptr, float alpha = 0.f,
                         const uint32_t* wave_order = nullptr, const float4* bodies = nullptr,
-                        uint32_t lanes = 64) {
+                        uint32_t lanes = 64, int dd = 0) {
     if (wide)
         bh_walk_kernel<MODE, true, REL><<<grid, 256, 0, st>>>(
             nr, total, pts, perm, n, theta2, eps2, out, info, rows, qh, est_in, est_shift, est_out,
-            alpha, wave_order, bodies, lanes);
+            alpha, wave_order, bodies, lanes, dd);
     else
         bh_walk_kernel<MODE, false, REL><<<grid, 256, 0, st>>>(
             nr, total, pts, perm, n, theta2, eps2, out, info, rows, qh, est_in, est_shift, est_out,
-            alpha, wave_order, bodies, lanes);
+            alpha, wave_order, bodies, lanes, dd);
 }
 
 // quad: add the learned correction (the per-node moments in s->qh). rel: the relative criterion
@@ -1560,7 +1765,7 @@ static void launch_walk(bool wide, unsigned grid, hipStream_t st, const NodeRec*
 // passes count in the walk's time and in the step/accepted counters.
 static int walk(pnb_ctx* c, const float4* pts, const uint32_t* perm, size_t n, float theta,
                 float softening, float4* out, const WalkRows* rows = nullptr, bool quad = false,
-                bool rel = false) {
+                bool rel = false, bool dd = false) {
     BhState* s = c->bh;
     const float theta2 = theta * theta;  // (self.theta * self.theta), sequential.rs:491
     const float eps2 = softening * softening;
@@ -1670,7 +1875,8 @@ static int walk(pnb_ctx* c, const float4* pts, const uint32_t* perm, size_t n, f
                                           theta2, eps2, out, info, *rows, nullptr);
         else
             launch_walk<WALK_PLAIN, false>(wide, grid, c->stream, nr, total, pts, perm, uint32_t(n),
-                                           theta2, eps2, out, info, WalkRows{}, nullptr);
+                                           theta2, eps2, out, info, WalkRows{}, nullptr, nullptr, 0,
+                                           nullptr, 0.f, nullptr, nullptr, 64, dd ? 1 : 0);
         PNB_CHECK_LAUNCH(c);
         return PNB_OK;
     });
@@ -1708,13 +1914,50 @@ static int accel_dev(pnb_ctx* c, const float4* affected, size_t n, const float4*
     if (n > 0x7fffffffu) return fail(c, PNB_E_ARG, "Barnes-Hut supports at most 2^31-1 particles");
     if (mlp && !mlp_weights(c))
         return fail(c, PNB_E_STATE, "PNB_BH_MLP needs an initialised MLP (pnb_mlp_init)");
-    if (int rc = build_tree(c, afg, m, ref_com, mlp)) return rc;
+    if (!c->bh) c->bh = new BhState();
+    bool dd = false;
+    if (c->bh->dd_mode) {
+        if (mlp || ref_com || rel)
+            return fail(c, PNB_E_ARG, "capturable Barnes-Hut takes none of PNB_BH_MLP, "
+                                      "PNB_BH_REF_COM and PNB_BH_RELATIVE");
+        BhState* s = c->bh;
+        const bool sized = s->dd_cap > 0 && s->dd_m == m && ((affected == afg && n == m) || s->dd_n == n);
+        if (sized) {
+            dd = true;
+        } else if (stream_capturing(c)) {
+            return fail(c, PNB_E_STATE, "capturable Barnes-Hut: call once outside the capture "
+                                        "with these sizes first (it sizes the tree's buffers)");
+        }
+    }
+    // inside an outside capture: no events on the stream
+    const bool was_capturing = c->capturing;
+    if (dd) c->capturing = c->capturing || stream_capturing(c);
+    struct Restore {
+        pnb_ctx* c;
+        bool v;
+        ~Restore() { c->capturing = v; }
+    } restore{c, was_capturing};
+    if (dd) {
+        if (int rc = build_tree_dd(c, afg, m)) return rc;
+    } else {
+        if (int rc = build_tree(c, afg, m, ref_com, mlp)) return rc;
+        if (c->bh->dd_mode) {
+            // The sizing call of a capturable build: capacities from this tree, then the tree
+            // again in the sized buffers (growing a buffer discards its contents, so the tree
+            // just built may be gone), built the capturable way.
+            if (int rc = dd_size(c, m)) return rc;
+            c->bh->dd_n = n;
+            dd = true;
+            if (int rc = build_tree_dd(c, afg, m)) return rc;
+        }
+    }
     BhState* s = c->bh;
     const bool quad = mlp && m;
     if (quad)
         if (int rc = node_network(c)) return rc;
     if (affected == afg && n == m)
-        return walk(c, s->sp.as<float4>(), s->sidx, n, theta, softening, out, nullptr, quad, rel);
+        return walk(c, s->sp.as<float4>(), s->sidx, n, theta, softening, out, nullptr, quad, rel,
+                    dd);
     const unsigned g = div_up(n, 256);
     PNB_HIP(c, s->akeys.ensure(n * 8));
     PNB_HIP(c, s->akeys_alt.ensure(n * 8));
@@ -1723,7 +1966,7 @@ static int accel_dev(pnb_ctx* c, const float4* affected, size_t n, const float4*
     PNB_HIP(c, s->apts.ensure(n * sizeof(float4)));
     if (m == 0) {
         // no tree: every particle gets zero (the reference's root is None)
-        return walk(c, affected, nullptr, n, theta, softening, out);
+        return walk(c, affected, nullptr, n, theta, softening, out, nullptr, false, false, dd);
     }
     bh_keys_kernel<<<g, 256, 0, c->stream>>>(affected, n, s->info.as<BhInfo>(),
                                              s->akeys.as<uint64_t>(), s->aidx.as<uint32_t>());
@@ -1732,11 +1975,11 @@ static int accel_dev(pnb_ctx* c, const float4* affected, size_t n, const float4*
     uint32_t* si;
     if (int rc = radix_sort_pairs(c, s->akeys.as<uint64_t>(), s->aidx.as<uint32_t>(),
                                   s->akeys_alt.as<uint64_t>(), s->aidx_alt.as<uint32_t>(), n, 63,
-                                  s->hist, s->bits, &sk, &si))
+                                  s->hist, s->bits, &sk, &si, dd))
         return rc;
     bh_gather_kernel<<<g, 256, 0, c->stream>>>(affected, n, si, s->apts.as<float4>());
     PNB_CHECK_LAUNCH(c);
-    return walk(c, s->apts.as<float4>(), si, n, theta, softening, out, nullptr, quad, rel);
+    return walk(c, s->apts.as<float4>(), si, n, theta, softening, out, nullptr, quad, rel, dd);
 }
 
 static int copy_h2d(pnb_ctx* c, DevBuf& b, const float* src, size_t count) {
@@ -1935,6 +2178,9 @@ int pnb_reordered_bh_accel_f32_dev(pnb_ctx* c, const float* particles, size_t n,
     if (int rc = activate(c)) return rc;
     if (!c->bh) c->bh = new BhState();
     BhState* s = c->bh;
+    if (s->dd_mode && stream_capturing(c))
+        return fail(c, PNB_E_STATE, "the massive/massless split reads its count on the host: "
+                                    "capture pnb_bh_accel_f32_dev instead");
     PNB_HIP(c, s->afg_tmp.ensure(n * sizeof(float4)));
     uint32_t m = 0;
     const float4* p = reinterpret_cast<const float4*>(particles);
@@ -1982,8 +2228,31 @@ int pnb_bh_set_relative(pnb_ctx* c, float alpha, float estimate_theta, int per_p
     return PNB_OK;
 }
 
+int pnb_bh_set_capturable(pnb_ctx* c, int enable) {
+    if (!c) return PNB_E_ARG;
+    if (!c->bh) c->bh = new BhState();
+    BhState* s = c->bh;
+    if (int rc = activate(c)) return rc;
+    (void)dd_resolve(c);  // takes the last build's status off the device; its failure is not this call's
+    s->dd_mode = enable != 0;
+    s->dd_cap = 0;  // the next call sizes anew
+    s->dd_m = s->dd_n = 0;
+    return PNB_OK;
+}
+
+int pnb_bh_check(pnb_ctx* c) {
+    if (!c) return PNB_E_ARG;
+    if (!c->bh) return PNB_OK;
+    if (int rc = activate(c)) return rc;
+    return dd_resolve(c);
+}
+
 int pnb_bh_tree_size(pnb_ctx* c, size_t* nodes, int* max_level) {
     if (!c) return PNB_E_ARG;
+    if (c->bh && c->bh->dd_last) {
+        if (int rc = activate(c)) return rc;
+        if (int rc = dd_resolve(c)) return rc;
+    }
     if (!c->bh || !c->bh->built) return fail(c, PNB_E_STATE, "no octree has been built");
     if (nodes) *nodes = c->bh->nodes;
     if (max_level) *max_level = c->bh->max_level;
@@ -1992,6 +2261,10 @@ int pnb_bh_tree_size(pnb_ctx* c, size_t* nodes, int* max_level) {
 
 int pnb_bh_tree_export(pnb_ctx* c, pnb_tree_node* out, size_t cap, uint32_t* order, size_t m) {
     if (!c) return PNB_E_ARG;
+    if (c->bh && c->bh->dd_last) {
+        if (int rc = activate(c)) return rc;
+        if (int rc = dd_resolve(c)) return rc;
+    }
     BhState* s = c->bh;
     if (!s || !s->built) return fail(c, PNB_E_STATE, "no octree has been built");
     if (cap < s->nodes || (order && m < s->m)) return fail(c, PNB_E_ARG, "output too small");

@@ -179,9 +179,11 @@ __global__ __launch_bounds__(1024) void small_sort_kernel(uint64_t* __restrict__
 // Sorts keys[0..n) with their values, stably; the result ends in (*keys_out, *vals_out), which is
 // either the input pair or the alternate pair. Only bits [0, key_bits) are considered. Up to
 // SMALL_SORT pairs one workgroup sorts in place; either way equal keys keep their input order.
+// Digits no two keys differ in are skipped after one read-back to the host, unless all_digits.
 int radix_sort_pairs(pnb_ctx* c, uint64_t* keys, uint32_t* vals, uint64_t* keys_alt,
                      uint32_t* vals_alt, size_t n, int key_bits, DevBuf& hist_buf,
-                     DevBuf& bits_buf, uint64_t** keys_out, uint32_t** vals_out) {
+                     DevBuf& bits_buf, uint64_t** keys_out, uint32_t** vals_out,
+                     bool all_digits) {
     *keys_out = keys;
     *vals_out = vals;
     if (n <= 1) return PNB_OK;
@@ -194,8 +196,10 @@ int radix_sort_pairs(pnb_ctx* c, uint64_t* keys, uint32_t* vals, uint64_t* keys_
     if (n > 0xffffffffu) return fail(c, PNB_E_ARG, "radix sort supports at most 2^32-1 keys");
     const uint32_t tiles = div_up(n, RS_TILE);
     PNB_HIP(c, hist_buf.ensure((size_t(256) * tiles + 1) * sizeof(uint32_t)));
-    uint64_t varying = 0xffull;  // one digit: sorted in one pass, no need to look for it first
-    if (key_bits > 8) {
+    // one digit: sorted in one pass, no need to look for it first; all_digits: every pass runs
+    // (no read-back, so the sort can be captured in a graph)
+    uint64_t varying = all_digits ? ~0ull : 0xffull;
+    if (key_bits > 8 && !all_digits) {
         // skip the digits no two keys differ in (found on the device, read back once)
         PNB_HIP(c, bits_buf.ensure(2 * sizeof(unsigned long long)));
         unsigned long long* oa = bits_buf.as<unsigned long long>();
