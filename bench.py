@@ -485,7 +485,7 @@ def bench_small_bh(args):
     from particular_amd import workloads
 
     out = {}
-    for n in (64, 4096):
+    for n in (int(v) for v in args.small_sizes.split(",")):
         s = workloads.direct_sum_case(n, seed=11)
         dev = torch.from_numpy(s.particles).cuda()
         both = pa.Between(dev, dev)  # one device pointer twice: pnb_bh_accel_f32_dev
@@ -613,6 +613,7 @@ def main():
                     help="per-call latency of Barnes-Hut on 64 and 4,096 bodies")
     ap.add_argument("--no-small", dest="small", action="store_false")
     ap.add_argument("--small-calls", type=int, default=200)
+    ap.add_argument("--small-sizes", default="64,4096", help="body counts of the latency leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "

@@ -142,6 +142,8 @@ struct BhState {
     bool dd_mode = false, dd_last = false;
     size_t dd_m = 0, dd_n = 0, dd_cap = 0;
     int dd_levels = 0;
+    std::vector<uint32_t> last_lvl;  // internal nodes per level of the last ordinary build
+    std::vector<uint32_t> dd_plan;   // the same, of the build that sized the capturable one
     // reference-exact centres of mass (PNB_BH_REF_COM): per body its node at the current level,
     // (group, input index) keys and the bodies regrouped in that order
     DevBuf rc_cur, rc_keys, rc_keys_alt, rc_vals, rc_vals_alt, rc_ord;
@@ -848,15 +850,17 @@ __global__ __launch_bounds__(256) void bh_up_level_dd_kernel(NodeRec* __restrict
         up_node(ilist[off + t], rec, sums, nullptr, nullptr, nullptr, nullptr, skipv, sp);
 }
 
-// Small trees: every level in one workgroup, deepest first, a barrier between levels.
-__global__ __launch_bounds__(1024) void bh_up_all_dd_kernel(NodeRec* __restrict__ rec,
-                                                            NodeSums* __restrict__ sums,
-                                                            const uint32_t* __restrict__ ilist,
-                                                            const BhInfo* __restrict__ info,
-                                                            const uint32_t* __restrict__ skipv,
-                                                            const float4* __restrict__ sp) {
+// A run of levels hi, hi - 1, ..., lo in one workgroup, a barrier between levels (levels the tree
+// lacks are empty).
+__global__ __launch_bounds__(1024) void bh_up_levels_dd_kernel(NodeRec* __restrict__ rec,
+                                                               NodeSums* __restrict__ sums,
+                                                               const uint32_t* __restrict__ ilist,
+                                                               const BhInfo* __restrict__ info,
+                                                               int hi, int lo,
+                                                               const uint32_t* __restrict__ skipv,
+                                                               const float4* __restrict__ sp) {
     if (info->flags) return;  // uniform: every thread returns, or none
-    for (int L = info->max_level - 1; L >= 0; --L) {  // internal levels 0 .. max_level - 1
+    for (int L = hi; L >= lo; --L) {
         const uint32_t cnt = info->lvl_count[L], off = info->lvl_off[L];
         for (uint32_t t = threadIdx.x; t < cnt; t += blockDim.x)
             up_node(ilist[off + t], rec, sums, nullptr, nullptr, nullptr, nullptr, skipv, sp);
@@ -1173,6 +1177,11 @@ __device__ __forceinline__ float half_hi(uint32_t v) {
 // wave_order (REL, optional): the launch's k-th wavefront (k < ceil(n / 64)) walks the particles
 // of wavefront wave_order[k], so that the heaviest wavefronts start first.
 // lanes: only the first `lanes` lanes of each wavefront walk (64 except in the estimate pass).
+// packed (not with REL): the wavefront's particles are consecutive, `lanes` of them from
+// (launch wavefront index) * lanes, instead of 64 from (launch wavefront index) * 64. Narrow
+// wavefronts walk smaller unions of their particles' traversals, so that a small problem runs as
+// many short walks instead of a few long ones; every lane's result is the same either way (it
+// adds exactly the nodes of its own pre-order traversal, in that order).
 // dd: a capturable build's tree (`total` is then its capacity; the count is read from `info`).
 template <int MODE, bool WIDE, bool REL>
 __global__ __launch_bounds__(256) void bh_walk_kernel(const NodeRec* __restrict__ nodes,
@@ -1188,8 +1197,9 @@ __global__ __launch_bounds__(256) void bh_walk_kernel(const NodeRec* __restrict_
                                                       float* __restrict__ est_out, float alpha,
                                                       const uint32_t* __restrict__ wave_order,
                                                       const float4* __restrict__ bodies,
-                                                      uint32_t lanes, int dd) {
+                                                      uint32_t lanes, int dd, int packed) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (packed) i = (i >> 6) * lanes + (threadIdx.x & 63);
     if (dd) {
         // capturable build: the node count and the build's status are on the device; a build
         // that did not fit gives NaN (pnb_bh_check reports why)
@@ -1363,6 +1373,7 @@ static int build_tree(pnb_ctx* c, const float4* afg, size_t m, bool ref_com = fa
     s->nodes = 0;
     s->internal = 0;
     s->max_level = -1;
+    s->last_lvl.clear();
     PNB_HIP(c, s->info.ensure(sizeof(BhInfo)));
     BhInfo* info = s->info.as<BhInfo>();
     PNB_HIP(c, s->lvl.ensure(size_t(2) * LVL_SLOTS * LVL_STRIDE * sizeof(uint32_t)));
@@ -1460,6 +1471,7 @@ static int build_tree(pnb_ctx* c, const float4* afg, size_t m, bool ref_com = fa
         for (int L = 0; L < levels; ++L) off[L + 1] = off[L] + h.lvl_count[L];
         PNB_HIP(c, s->ilist.ensure(std::max<size_t>(off[levels], 1) * sizeof(uint32_t)));
         s->internal = off[levels];
+        s->last_lvl.assign(h.lvl_count, h.lvl_count + levels);
         uint32_t* ilist = s->ilist.as<uint32_t>();
         bh_level_list_kernel<<<lg, 256, 0, c->stream>>>(lcp, s->ip.as<int>(), cnt, m, lvl_cur,
                                                         ilist, info);
@@ -1549,6 +1561,7 @@ static int dd_size(pnb_ctx* c, size_t m) {
     PNB_HIP(c, s->ilist.ensure(cap * sizeof(uint32_t)));
     s->dd_cap = cap;
     s->dd_levels = std::min(std::max(s->max_level, 0) + 8, MAX_LEVEL + 1);
+    s->dd_plan = s->last_lvl;
     s->dd_m = m;
     return PNB_OK;
 }
@@ -1621,18 +1634,29 @@ static int build_tree_dd(pnb_ctx* c, const float4* afg, size_t m) {
         bh_level_list_kernel<<<lg, 256, 0, c->stream>>>(lcp, s->ip.as<int>(), cnt, m, lvl_cur,
                                                         ilist, info);
         PNB_CHECK_LAUNCH(c);
-        if (s->dd_cap <= 4 * UP_RUN_NODES) {
-            bh_up_all_dd_kernel<<<1, 1024, 0, c->stream>>>(rec, s->sums.as<NodeSums>(), ilist,
-                                                           info, skipv, sp);
-            PNB_CHECK_LAUNCH(c);
-        } else {
-            const unsigned ug = std::min<unsigned>(div_up(s->dd_cap, 256),
-                                                   4 * std::max(c->cu_count, 1));
-            for (int L = s->dd_levels - 1; L >= 0; --L) {
+        // Bottom-up sums on the sizing build's plan: a level that held more than half a run's
+        // nodes then has a launch of its own, runs of the others one single-workgroup launch.
+        // The counts are read on the device, so a level that outgrew its plan is still summed
+        // (a run only takes longer, a launch strides over more nodes).
+        const auto planned = [&](int L) -> uint32_t {
+            return size_t(L) < s->dd_plan.size() ? s->dd_plan[size_t(L)] : 0u;
+        };
+        const unsigned max_grid = 4 * unsigned(std::max(c->cu_count, 1));
+        for (int L = s->dd_levels - 1; L >= 0;) {
+            if (planned(L) > UP_RUN_NODES / 2) {
+                const uint32_t n_l = planned(L);
+                const unsigned ug = std::min<unsigned>(div_up(n_l + n_l / 4, 256u), max_grid);
                 bh_up_level_dd_kernel<<<ug, 256, 0, c->stream>>>(rec, s->sums.as<NodeSums>(),
                                                                  ilist, info, L, skipv, sp);
                 PNB_CHECK_LAUNCH(c);
+                --L;
+                continue;
             }
+            const int hi = L;
+            while (L >= 0 && planned(L) <= UP_RUN_NODES / 2) --L;
+            bh_up_levels_dd_kernel<<<1, 1024, 0, c->stream>>>(rec, s->sums.as<NodeSums>(), ilist,
+                                                              info, hi, L + 1, skipv, sp);
+            PNB_CHECK_LAUNCH(c);
         }
         s->built = true;
         return PNB_OK;
@@ -1747,15 +1771,34 @@ static void launch_walk(bool wide, unsigned grid, hipStream_t st, const NodeRec*
                         const QuadH* qh, const float* est_in = nullptr, uint32_t est_shift = 0,
                         float* est_out = nullptr, float alpha = 0.f,
                         const uint32_t* wave_order = nullptr, const float4* bodies = nullptr,
-                        uint32_t lanes = 64, int dd = 0) {
+                        uint32_t lanes = 64, int dd = 0, int packed = 0) {
     if (wide)
         bh_walk_kernel<MODE, true, REL><<<grid, 256, 0, st>>>(
             nr, total, pts, perm, n, theta2, eps2, out, info, rows, qh, est_in, est_shift, est_out,
-            alpha, wave_order, bodies, lanes, dd);
+            alpha, wave_order, bodies, lanes, dd, packed);
     else
         bh_walk_kernel<MODE, false, REL><<<grid, 256, 0, st>>>(
             nr, total, pts, perm, n, theta2, eps2, out, info, rows, qh, est_in, est_shift, est_out,
-            alpha, wave_order, bodies, lanes, dd);
+            alpha, wave_order, bodies, lanes, dd, packed);
+}
+
+// Particles per wavefront of the plain and corrected walks: the largest power of two (at most
+// 64, at least 1) that still gives WALK_WAVES_PER_CU wavefronts per CU; 64 for a problem that
+// fits one wavefront. Measured at theta = 0.5 on Plummer spheres (walk time, 256 CUs): 1,024
+// bodies 173 us at 64 lanes, 97 us at 1; 4,096: 352 us at 64, 225-238 us at 2-8; 16,384:
+// 577 us at 64, 382 us at 8; 65,536: 695 us at 64, 630 us at 32; from 262,144 on, 64 is best.
+// PNB_WALK_LANES (1..64) overrides it.
+constexpr size_t WALK_WAVES_PER_CU = 8;
+static uint32_t walk_lanes(const pnb_ctx* c, size_t n) {
+    if (const char* e = std::getenv("PNB_WALK_LANES")) {
+        const long v = std::strtol(e, nullptr, 10);
+        if (v >= 1 && v <= 64) return uint32_t(v);
+    }
+    if (n <= 64) return 64;
+    const size_t target = WALK_WAVES_PER_CU * size_t(std::max(c->cu_count, 1));
+    uint32_t lanes = 64;
+    while (lanes > 1 && div_up(n, size_t(lanes)) < target) lanes >>= 1;
+    return lanes;
 }
 
 // quad: add the learned correction (the per-node moments in s->qh). rel: the relative criterion
@@ -1867,16 +1910,23 @@ static int walk(pnb_ctx* c, const float4* pts, const uint32_t* perm, size_t n, f
                 launch_walk<WALK_PLAIN, true>(wide, grid, c->stream, rr, total, pts, perm,
                                               uint32_t(n), theta2, eps2, out, info, WalkRows{},
                                               nullptr, est, shift, nullptr, 0.f, order, bodies);
-        } else if (quad)
-            launch_walk<WALK_QUAD, false>(wide, grid, c->stream, nr, total, pts, perm, uint32_t(n),
-                                          theta2, eps2, out, info, WalkRows{}, s->qh.as<QuadH>());
-        else if (rows)
+        } else if (rows) {
             launch_walk<WALK_ROWS, false>(wide, grid, c->stream, nr, total, pts, perm, uint32_t(n),
                                           theta2, eps2, out, info, *rows, nullptr);
-        else
-            launch_walk<WALK_PLAIN, false>(wide, grid, c->stream, nr, total, pts, perm, uint32_t(n),
-                                           theta2, eps2, out, info, WalkRows{}, nullptr, nullptr, 0,
-                                           nullptr, 0.f, nullptr, nullptr, 64, dd ? 1 : 0);
+        } else {
+            const uint32_t lanes = walk_lanes(c, n);
+            const unsigned pgrid = unsigned(div_up(div_up(n, size_t(lanes)) * 64, size_t(256)));
+            if (quad)
+                launch_walk<WALK_QUAD, false>(wide, pgrid, c->stream, nr, total, pts, perm,
+                                              uint32_t(n), theta2, eps2, out, info, WalkRows{},
+                                              s->qh.as<QuadH>(), nullptr, 0, nullptr, 0.f, nullptr,
+                                              nullptr, lanes, dd ? 1 : 0, 1);
+            else
+                launch_walk<WALK_PLAIN, false>(wide, pgrid, c->stream, nr, total, pts, perm,
+                                               uint32_t(n), theta2, eps2, out, info, WalkRows{},
+                                               nullptr, nullptr, 0, nullptr, 0.f, nullptr, nullptr,
+                                               lanes, dd ? 1 : 0, 1);
+        }
         PNB_CHECK_LAUNCH(c);
         return PNB_OK;
     });

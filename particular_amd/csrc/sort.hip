@@ -128,57 +128,88 @@ __global__ void rs_bits_init_kernel(unsigned long long* or_and) {
     or_and[1] = ~0ull;
 }
 
-// Small inputs in one workgroup: a bitonic network in LDS over (key bits [0, key_bits), input
-// position) with the values carried along, padded to a power of two with entries that order last
-// (their positions are past every input's). Ties on the key are broken by the input position, so
-// equal keys keep their input order whatever the values: the same stable order as the radix path.
-constexpr uint32_t SMALL_SORT = 4096;
-static_assert(SMALL_SORT <= 65536, "input positions are held as 16-bit values");
+// Small inputs: every key's place is the number of keys below it plus the number of equal keys
+// before it in the input, which is exactly the stable order of the radix path. One thread per
+// key; a wavefront reads the keys it counts against 64 at a time (one per lane, coalesced) and
+// broadcasts them lane by lane with v_readlane into scalar registers, so each compare is a
+// vector-scalar compare with no memory access. A workgroup counts against at most RANK_SPAN keys:
+// inputs above it split the count over blockIdx.y and add the parts with atomics before a
+// scatter launch.
+constexpr uint32_t SMALL_SORT = 16384;
+constexpr uint32_t RANK_SPAN = 1024;
 
-__global__ __launch_bounds__(1024) void small_sort_kernel(uint64_t* __restrict__ keys,
-                                                          uint32_t* __restrict__ vals, uint32_t n,
-                                                          uint64_t mask) {
-    __shared__ uint64_t sk[SMALL_SORT];
-    __shared__ uint32_t sv[SMALL_SORT];
-    __shared__ uint16_t sp[SMALL_SORT];
-    uint32_t n2 = 2;
-    while (n2 < n) n2 <<= 1;
-    for (uint32_t i = threadIdx.x; i < n2; i += blockDim.x) {
-        sk[i] = i < n ? keys[i] : ~0ull;
-        sv[i] = i < n ? vals[i] : ~0u;
-        sp[i] = uint16_t(i);
-    }
-    __syncthreads();
-    for (uint32_t size = 2; size <= n2; size <<= 1) {
-        for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
-            for (uint32_t t = threadIdx.x; t < (n2 >> 1); t += blockDim.x) {
-                const uint32_t i = 2 * t - (t & (stride - 1)), j = i + stride;
-                const uint64_t ki = sk[i] & mask, kj = sk[j] & mask;
-                const bool gt = ki > kj || (ki == kj && sp[i] > sp[j]);
-                if (gt == ((i & size) == 0)) {  // out of order for this half's direction
-                    const uint64_t tk = sk[i];
-                    sk[i] = sk[j];
-                    sk[j] = tk;
-                    const uint32_t tv = sv[i];
-                    sv[i] = sv[j];
-                    sv[j] = tv;
-                    const uint16_t tp = sp[i];
-                    sp[i] = sp[j];
-                    sp[j] = tp;
-                }
+__device__ __forceinline__ uint64_t lane_key(uint64_t v, int k) {
+    const uint32_t lo = __builtin_amdgcn_readlane(uint32_t(v), k);
+    const uint32_t hi = __builtin_amdgcn_readlane(uint32_t(v >> 32), k);
+    return (uint64_t(hi) << 32) | lo;
+}
+
+// Keys of at most 63 bits: "kj <= ki" is "kj < ki + 1", so keys before position i count below
+// ki + 1 and the others below ki; positions past the span read as ~0, which counts for neither.
+__global__ __launch_bounds__(256) void rank_count_kernel(const uint64_t* __restrict__ keys,
+                                                         const uint32_t* __restrict__ vals,
+                                                         uint32_t n, uint64_t mask,
+                                                         uint32_t* __restrict__ rank,
+                                                         uint64_t* __restrict__ keys_out,
+                                                         uint32_t* __restrict__ vals_out) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t wbase = i - lane;  // the wavefront's first position
+    const uint64_t ki = i < n ? (keys[i] & mask) : 0ull;
+    const uint32_t j0 = blockIdx.y * RANK_SPAN, j1 = min(n, j0 + RANK_SPAN);
+    uint32_t cc[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // independent counts: no carry chain
+    uint64_t next = j0 + lane < j1 ? (keys[j0 + lane] & mask) : ~0ull;
+    for (uint32_t t0 = j0; t0 < j1; t0 += 64) {
+        const uint64_t kj = next;
+        const uint32_t tn = t0 + 64 + lane;  // the next 64 keys load while these are counted
+        next = tn < j1 ? (keys[tn] & mask) : ~0ull;
+        if (t0 == wbase) {  // the wavefront's own keys: position decides per lane
+#pragma unroll
+            for (int k = 0; k < 64; ++k)
+                cc[k & 7] += lane_key(kj, k) < (uint32_t(k) < lane ? ki + 1 : ki) ? 1u : 0u;
+        } else {
+            const uint64_t thr = t0 < wbase ? ki + 1 : ki;
+#pragma unroll
+            for (int k = 0; k < 64; k += 8) {  // eight broadcasts, then eight compares
+                uint64_t b[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) b[u] = lane_key(kj, k + u);
+#pragma unroll
+                for (int u = 0; u < 8; ++u) cc[u] += b[u] < thr ? 1u : 0u;
             }
-            __syncthreads();
         }
     }
-    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
-        keys[i] = sk[i];
-        vals[i] = sv[i];
+    const uint32_t c = ((cc[0] + cc[1]) + (cc[2] + cc[3])) + ((cc[4] + cc[5]) + (cc[6] + cc[7]));
+    if (i >= n) return;
+    if (gridDim.y > 1) {
+        atomicAdd(&rank[i], c);
+        return;
     }
+    keys_out[c] = keys[i];
+    vals_out[c] = vals[i];
+}
+
+__global__ __launch_bounds__(256) void rank_zero_kernel(uint32_t* __restrict__ rank, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) rank[i] = 0u;
+}
+
+__global__ __launch_bounds__(256) void rank_scatter_kernel(const uint64_t* __restrict__ keys,
+                                                           const uint32_t* __restrict__ vals,
+                                                           uint32_t n,
+                                                           const uint32_t* __restrict__ rank,
+                                                           uint64_t* __restrict__ keys_out,
+                                                           uint32_t* __restrict__ vals_out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    keys_out[rank[i]] = keys[i];
+    vals_out[rank[i]] = vals[i];
 }
 
 // Sorts keys[0..n) with their values, stably; the result ends in (*keys_out, *vals_out), which is
 // either the input pair or the alternate pair. Only bits [0, key_bits) are considered. Up to
-// SMALL_SORT pairs one workgroup sorts in place; either way equal keys keep their input order.
+// SMALL_SORT pairs (keys of at most 63 bits) are ranked by counting, into the alternate pair;
+// either way equal keys keep their input order.
 // Digits no two keys differ in are skipped after one read-back to the host, unless all_digits.
 int radix_sort_pairs(pnb_ctx* c, uint64_t* keys, uint32_t* vals, uint64_t* keys_alt,
                      uint32_t* vals_alt, size_t n, int key_bits, DevBuf& hist_buf,
@@ -187,10 +218,25 @@ int radix_sort_pairs(pnb_ctx* c, uint64_t* keys, uint32_t* vals, uint64_t* keys_
     *keys_out = keys;
     *vals_out = vals;
     if (n <= 1) return PNB_OK;
-    if (n <= SMALL_SORT) {
+    if (n <= SMALL_SORT && key_bits <= 63) {
         const uint64_t mask = key_bits >= 64 ? ~0ull : ((1ull << key_bits) - 1);
-        small_sort_kernel<<<1, 1024, 0, c->stream>>>(keys, vals, uint32_t(n), mask);
+        const uint32_t nn = uint32_t(n), g = div_up(nn, 256u), parts = div_up(nn, RANK_SPAN);
+        uint32_t* rank = nullptr;
+        if (parts > 1) {
+            PNB_HIP(c, hist_buf.ensure(size_t(nn) * sizeof(uint32_t)));
+            rank = hist_buf.as<uint32_t>();
+            rank_zero_kernel<<<g, 256, 0, c->stream>>>(rank, nn);
+            PNB_CHECK_LAUNCH(c);
+        }
+        rank_count_kernel<<<dim3(g, parts), 256, 0, c->stream>>>(keys, vals, nn, mask, rank,
+                                                                keys_alt, vals_alt);
         PNB_CHECK_LAUNCH(c);
+        if (parts > 1) {
+            rank_scatter_kernel<<<g, 256, 0, c->stream>>>(keys, vals, nn, rank, keys_alt, vals_alt);
+            PNB_CHECK_LAUNCH(c);
+        }
+        *keys_out = keys_alt;
+        *vals_out = vals_alt;
         return PNB_OK;
     }
     if (n > 0xffffffffu) return fail(c, PNB_E_ARG, "radix sort supports at most 2^32-1 keys");

@@ -80,16 +80,16 @@ def test_tree_deep_runs(res, orc, scale):
     print("max level", t.max_level)
 
 
-# Sorts of at most 4,096 keys run in one workgroup; the trees below hold more bodies, so their
-# keys take the multi-tile LSD radix sort that builds every large tree (stable order of equal keys,
+# Sorts of at most 16,384 keys rank by counting; the trees below hold more bodies, so their keys
+# take the multi-tile LSD radix sort that builds every large tree (stable order of equal keys,
 # deep-run refinement after it).
-RADIX_BASE = 6000
+RADIX_BASE = 17000
 
 
 def test_tree_with_duplicates_radix_path(res, orc):
     p = workloads.direct_sum_case(RADIX_BASE, seed=16).particles
     p = np.concatenate([p, p[:400], p[10:12], p[10:11], p[::7][:300], p[5:6], p[5:6]])
-    assert p.shape[0] > 4096
+    assert p.shape[0] > 16384
     bh(res, p, 0.5)
     t = assert_same_tree(res, orc, p)
     assert_reference_com(res, p, t)
@@ -124,7 +124,7 @@ def test_tree_deep_runs_radix_path(res, orc, scale):
     tiny[:, 0] = (np.arange(9) * scale).astype(np.float32)
     tiny[:, 3] = 1.0 / RADIX_BASE
     # the deep structures interleaved with the base, so equal 21-level keys arrive out of order
-    p = np.concatenate([base[:2500], cluster, base[2500:], tiny, cluster[:5]])
+    p = np.concatenate([base[:8500], cluster, base[8500:], tiny, cluster[:5]])
     bh(res, p, 0.5)
     t = assert_same_tree(res, orc, p)
     print("max level", t.max_level)
@@ -218,6 +218,54 @@ def test_wide_record_offsets_match_narrow(res, monkeypatch):
     for a, b in zip(runs["0"], runs["1"]):
         np.testing.assert_array_equal(a, b)
     assert np.abs(runs["0"][2] - runs["0"][0]).max() > 0  # the correction did something
+
+
+@pytest.mark.parametrize("m", [4096, 4097, 9000, 16384, 16385])
+def test_sort_size_classes_with_ties(res, orc, m):
+    """Tree order around the sort's size classes (ranking by counting up to 16,384 keys, the count
+    split over workgroups above 4,096; radix passes above), with most bodies sharing a position
+    with others (equal keys, which keep their input order), and an affected set of another size
+    class sorted along the tree."""
+    rng = np.random.default_rng(m)
+    p = np.zeros((m, 4), np.float32)
+    p[:, :3] = rng.integers(0, 12, size=(m, 3)).astype(np.float32) * 0.25  # 1,728 points
+    p[:, 3] = rng.uniform(0.5, 1.5, m).astype(np.float32)
+    bh(res, p, 0.5)
+    assert_same_tree(res, orc, p)
+    a = workloads.direct_sum_case(m // 2 + 3, seed=m).particles
+    got = bh(res, pa.Between(a, p), 0.5)
+    ref = orc.Tree(p).barnes_hut(a, 0.5, com64=True, accum64=True)[:, :3]
+    # per particle, relative to its acceleration's magnitude (a lattice of coincident bodies gives
+    # components that cancel to a small fraction of it)
+    row = np.linalg.norm(got - ref, axis=1) / np.linalg.norm(ref, axis=1)
+    assert metrics.rms_rel(got, ref) < 5e-7 and row.max() < 1e-5, (metrics.rms_rel(got, ref),
+                                                                     row.max())
+
+
+@pytest.mark.parametrize("n", [64, 4096, 70000])
+def test_wavefront_width_changes_no_result(res, monkeypatch, n):
+    """Each lane adds exactly the nodes of its own traversal, in pre-order, whatever particles
+    share its wavefront: walks with 1 to 64 particles per wavefront (PNB_WALK_LANES; small
+    problems take narrow ones by default) give bit-identical results, plain and corrected, for one
+    set and for two."""
+    from particular_amd import learned
+
+    s = workloads.direct_sum_case(n, seed=62)
+    other = workloads.direct_sum_case(max(n // 3, 5), seed=63).particles
+    learned.init(res, 0)
+    learned.set_params(res, learned.quadrupole_params())
+    runs = {}
+    for lanes in ("64", "16", "3", "1"):
+        monkeypatch.setenv("PNB_WALK_LANES", lanes)
+        runs[lanes] = [pa.BarnesHut(res, th, pa.Acceleration.checked(), mlp=m).compute(st)
+                       for th, m, st in ((0.5, False, s.particles), (0.5, True, s.particles),
+                                         (0.7, False, pa.Between(other, s.particles)))]
+    for lanes in ("16", "3", "1"):
+        for a, b in zip(runs["64"], runs[lanes]):
+            np.testing.assert_array_equal(a, b, err_msg=f"{lanes} lanes")
+    monkeypatch.delenv("PNB_WALK_LANES")
+    default = pa.BarnesHut(res, 0.5, pa.Acceleration.checked()).compute(s.particles)
+    np.testing.assert_array_equal(default, runs["64"][0])
 
 
 @pytest.mark.parametrize("m", [5, 64, 65, 257])
