@@ -310,21 +310,27 @@ int pnb_batch_info(pnb_ctx *ctx, size_t *particles, size_t *items,
                    unsigned long long *interactions_per_step);
 
 /* ---- learned far-field correction (Barnes-Hut + MLP) -----------------------------------------
- * No reference counterpart. An 11 -> 64 -> 64 -> 6 ReLU MLP (BF16 weights on MFMA, FP32
+ * No reference counterpart. A 16 -> 64 -> 64 -> 13 ReLU MLP (BF16 weights on MFMA, FP32
  * accumulate) corrects the monopole term of every internal node a lane accepts in the Barnes-Hut
- * walk (pnb_bh_accel_f32* with PNB_BH_MLP). Its input describes the node: the normalised
- * quadrupole Q = (3T - tr(T) I) / (M w^2) (6, see pnb_tree_node), 1 + log2(mass / root mass) / 32,
- * log2(bodies) / 32, depth / 64, the heaviest child's mass share and children / 8. Its output is an
- * effective second moment in units of M w^2; every lane that accepts the node adds that moment's
- * field at its own position, a = -Q d |d|^-5 + 2.5 (d.Q.d) d |d|^-7 (d = centre of mass - lane).
- * The network runs once per node after the octree build. Training data are generated by the
- * method itself: per-lane targets are brute force minus Barnes-Hut. */
+ * walk (pnb_bh_accel_f32* with PNB_BH_MLP). Its input describes the node (pnb_bh_node_features):
+ * f0..5 the normalised quadrupole Q = (3T - tr(T) I) / (M w^2) (see pnb_tree_node),
+ * f6 1 + log2(mass / root mass) / 32, f7 log2(bodies) / 32, f8 depth / 64, f9..15 the normalised
+ * octupole (5/2) U~ / (M w^3), U~ the traceless part of the third moments about the centre of mass,
+ * as {xxx, yyy, zzz, xxy, xxz, xyy, xyz}. Its outputs are two fields' coefficients: y[0..5] =
+ * f0..5 + MLP(f)[0..5] (a residual on the quadrupole) is a second moment Q in units of M w^2 and
+ * y[6..12] = MLP(f)[6..12] (learned from zero) a traceless third moment O in units of M w^3 (same
+ * seven components). Every lane that accepts the node adds, with d = centre of mass - lane,
+ * a = -Q d |d|^-5 + 2.5 (d.Q.d) d |d|^-7 + 3 F |d|^-7 - 7 (d.F) d |d|^-9 with F_i = O_ijk d_j d_k;
+ * with y[6..12] = f9..15 the last two terms are the octupole term of the node's multipole
+ * expansion. The network runs once per node after the octree build. Training data are generated
+ * by the method itself: per-lane targets are brute force minus Barnes-Hut. */
 
 int pnb_mlp_num_params(void);
 /* He-normal hidden layers from a seed (SplitMix64 + Box-Muller), zero biases and a zero output
- * layer (a fresh network corrects nothing); resets the Adam state. */
+ * layer (a fresh network adds nothing to its residual: the correction is the quadrupole term);
+ * resets the Adam state. */
 int pnb_mlp_init(pnb_ctx *ctx, uint64_t seed);
-/* Parameters as one FP32 vector: W1[11][64], b1[64], W2[64][64], b2[64], W3[64][6], b3[6]. */
+/* Parameters as one FP32 vector: W1[16][64], b1[64], W2[64][64], b2[64], W3[64][13], b3[13]. */
 int pnb_mlp_set_params(pnb_ctx *ctx, const float *params, size_t n);
 int pnb_mlp_get_params(pnb_ctx *ctx, float *params, size_t n);
 /* Builds the octree of `particles` (device, n x 4, all affecting), walks `waves` seeded
@@ -336,8 +342,8 @@ int pnb_mlp_collect(pnb_ctx *ctx, const float *particles, size_t n, float theta,
                     size_t waves, uint64_t seed, size_t row_cap);
 int pnb_mlp_clear_dataset(pnb_ctx *ctx);
 int pnb_mlp_dataset_info(pnb_ctx *ctx, size_t *rows, size_t *waves, uint64_t *dropped);
-/* Training set on the host: feats rows x 16 (11 used), geom rows x 4 (node centre of mass,
- * M w^2), mask rows (64-bit lane masks), row_wave rows, wave_rows waves + 1 (row offsets), lanes
+/* Training set on the host: feats rows x 16, geom rows x 8 (node centre of mass, M w^2, M w^3,
+ * three zeros), mask rows (64-bit lane masks), row_wave rows, wave_rows waves + 1 (row offsets), lanes
  * waves x 64 x 4 (lane positions, w = softening^2), target waves x 64 x 4, wgt waves x 64. */
 int pnb_mlp_dataset_read(pnb_ctx *ctx, float *feats, float *geom, uint64_t *mask,
                          uint32_t *row_wave, uint32_t *wave_rows, float *lanes, float *target,
@@ -350,8 +356,13 @@ int pnb_mlp_dataset_load(pnb_ctx *ctx, const float *feats, const float *geom,
 int pnb_mlp_grad(pnb_ctx *ctx, float *loss, float *grads, size_t n);
 /* Full-batch Adam steps; loss_history (optional, `iterations` floats) gets the loss before each. */
 int pnb_mlp_train(pnb_ctx *ctx, size_t iterations, float lr, float *loss_history);
-/* Inference on device rows: y (rows x 8, 6 used) = MLP(feats (rows x 16, 11 used)). */
+/* Inference on device rows: y (rows x 16, 13 used) = MLP(feats (rows x 16)), plus feats[0..5]
+ * on y[0..5]. */
 int pnb_mlp_forward_dev(pnb_ctx *ctx, const float *feats, size_t rows, float *y);
+/* The network's input of every node of the last octree built with PNB_BH_MLP, for inspection:
+ * out (cap >= nodes, host) gets 16 floats per node in pre-order, as the node network reads them
+ * (zeros for leaves and massless nodes). PNB_E_STATE when the last build had no moments. */
+int pnb_bh_node_features(pnb_ctx *ctx, float *out, size_t cap);
 
 #ifdef __cplusplus
 }

@@ -4,23 +4,28 @@ No reference counterpart (particular has no learned component). The network and 
 defined here in plain numpy and pinned by finite differences (tests/test_mlp_oracle.py); the GPU
 forward/backward kernels are checked against it.
 
-Network: 11 -> 64 -> 64 -> 6, ReLU, y = relu(relu(x W1 + b1) W2 + b2) W3 + b3. When
-``bf16=True`` the inputs of every matrix product (x, h1, h2, g_y, g_z2, g_z1 and W1..W3) are
-rounded to bfloat16 (round to nearest even) as the MFMA kernels round them; sums are FP64.
+Network: 16 -> 64 -> 64 -> 13, ReLU, y = relu(relu(x W1 + b1) W2 + b2) W3 + b3 plus x[:, :6] on
+y[:, :6] (a residual on the node's normalised quadrupole, inputs 0..5). When ``bf16=True`` the
+inputs of every matrix product (x, h1, h2, g_y, g_z2, g_z1 and W1..W3) are rounded to bfloat16
+(round to nearest even) as the MFMA kernels round them; the residual and all sums are FP64. The
+residual does not depend on the parameters, so it leaves every gradient formula as it is.
 
 A row is an internal node accepted by some lanes of a wavefront: row r belongs to wave
 ``wave[r]``, applies to the lanes set in ``mask[r]`` (64-bit), and has geometry ``geom[r]`` =
-(centre of mass, M w^2). Its output y[r] is a second moment Q = M w^2 y[r] ({xx, yy, zz, xy, xz,
-yz}); lane l of wave w at ``lanes[w, l, :3]`` (softening^2 in ``lanes[w, l, 3]``) predicts
-``pred[w, l] = sum_{r in w, l in mask[r]} field(Q_r, d)`` with d = centre - lane and
-field(Q, d) = -Q d |d|^-5 + 2.5 (d.Q.d) d |d|^-7, for target ``t[w, l]`` with weight
-``wgt[w, l]``; loss = sum wgt * |pred - t|^2 / (number of lanes with wgt > 0).
+(centre of mass, M w^2, M w^3, ...). Its output y[r] holds a second moment Q = M w^2 y[r, :6]
+({xx, yy, zz, xy, xz, yz}) and a traceless third moment O = M w^3 y[r, 6:13] ({xxx, yyy, zzz,
+xxy, xxz, xyy, xyz}); lane l of wave w at ``lanes[w, l, :3]`` (softening^2 in ``lanes[w, l, 3]``)
+predicts ``pred[w, l] = sum_{r in w, l in mask[r]} field(Q_r, O_r, d)`` with d = centre - lane,
+field = -Q d |d|^-5 + 2.5 (d.Q.d) d |d|^-7 + 3 F |d|^-7 - 7 (d.F) d |d|^-9, F_i = O_ijk d_j d_k,
+for target ``t[w, l]`` with weight ``wgt[w, l]``; loss = sum wgt * |pred - t|^2 / (number of
+lanes with wgt > 0).
 """
 from __future__ import annotations
 
 import numpy as np
 
-IN, HID, OUT = 11, 64, 6
+IN, HID, OUT = 16, 64, 13
+QOUT = 6  # quadrupole outputs; the rest are octupole coefficients
 
 
 def bf16_round(a) -> np.ndarray:
@@ -52,6 +57,7 @@ def forward(p: dict, x: np.ndarray, bf16: bool = False):
     z2 = r(h1) @ r(p["W2"]) + p["b2"]
     h2 = np.maximum(z2, 0.0)
     y = r(h2) @ r(p["W3"]) + p["b3"]
+    y[:, :QOUT] += np.asarray(x, np.float64)[:, :QOUT]
     return y, (z1, h1, z2, h2)
 
 
@@ -75,13 +81,41 @@ def field_basis(d: np.ndarray, inv: np.ndarray) -> np.ndarray:
     return inv5[..., None, None] * (2.5 * inv2[..., None, None] * d[..., :, None] * c[..., None, :] - m)
 
 
+def oct_field_basis(d: np.ndarray, inv: np.ndarray) -> np.ndarray:
+    """B (..., 3, 7) with the octupole field 3 F |d|^-7 - 7 (d.F) d |d|^-9 = B @ o for the seven
+    coefficients o = {xxx, yyy, zzz, xxy, xxz, xyy, xyz} of a traceless symmetric O (the others
+    from O_iik = 0), F_i = O_ijk d_j d_k; inv = 1 / |d| (softened). Built from the full tensor."""
+    idx = {(0, 0, 0): 0, (1, 1, 1): 1, (2, 2, 2): 2, (0, 0, 1): 3, (0, 0, 2): 4, (0, 1, 1): 5,
+           (0, 1, 2): 6}
+    cols = []
+    for j in range(7):
+        o = np.zeros(7)
+        o[j] = 1.0
+        O = np.zeros((3, 3, 3))
+        full = {**{k: o[v] for k, v in idx.items()},
+                (0, 2, 2): -o[0] - o[5], (1, 2, 2): -o[3] - o[1], (1, 1, 2): -o[4] - o[2]}
+        for i in range(3):
+            for a in range(3):
+                for b in range(3):
+                    O[i, a, b] = full[tuple(sorted((i, a, b)))]
+        F = np.einsum("iab,...a,...b->...i", O, d, d)
+        G = (F * d).sum(-1)
+        inv2 = inv * inv
+        inv7 = inv2 ** 3 * inv
+        cols.append(inv7[..., None] * (3.0 * F - 7.0 * (G * inv2)[..., None] * d))
+    return np.stack(cols, -1)
+
+
 def _row_lane_basis(wave, mask, geom, lanes):
-    """Per (row, lane): the basis scaled by M w^2 (rows x 64 x 3 x 6) and the lane bits."""
+    """Per (row, lane): the quadrupole basis scaled by M w^2 and the octupole basis scaled by M w^3
+    (rows x 64 x 3 x 13) and the lane bits."""
     bits = lane_bits(mask)
-    d = np.asarray(geom, np.float64)[:, None, :3] - np.asarray(lanes, np.float64)[wave][:, :, :3]
+    geom = np.asarray(geom, np.float64)
+    d = geom[:, None, :3] - np.asarray(lanes, np.float64)[wave][:, :, :3]
     r2 = (d * d).sum(-1) + np.asarray(lanes, np.float64)[wave][:, :, 3]
     inv = np.where(bits, 1.0 / np.sqrt(np.where(r2 > 0, r2, 1.0)), 0.0)
-    B = field_basis(d, inv) * np.asarray(geom, np.float64)[:, 3, None, None, None]
+    B = np.concatenate([field_basis(d, inv) * geom[:, 3, None, None, None],
+                        oct_field_basis(d, inv) * geom[:, 4, None, None, None]], -1)
     return B * bits[:, :, None, None], bits
 
 

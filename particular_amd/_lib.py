@@ -80,6 +80,7 @@ SIGNATURES = {
     "pnb_reordered_bh_accel_f32_dev": (_i, [_vp, _vp, _sz, _f, _f, _i, _u32, _vp]),
     "pnb_bh_tree_size": (_i, [_vp, _c.POINTER(_sz), _pi]),
     "pnb_bh_tree_export": (_i, [_vp, _vp, _sz, _vp, _sz]),
+    "pnb_bh_node_features": (_i, [_vp, _vp, _sz]),
     "pnb_bh_stats": (_i, [_vp, _pu64, _pu64]),
     "pnb_bh_set_relative": (_i, [_vp, _f, _f, _i, _i]),
     "pnb_batch_set_tuning": (_i, [_vp, _sz, _i]),

@@ -83,6 +83,9 @@ struct __align__(64) NodeSums {
 
 struct NodeMoments {
     double xx = 0, yy = 0, zz = 0, xy = 0, xz = 0, yz = 0;
+    // third moments U = sum m (x - c)(x - c)(x - c) about the same centre:
+    // {xxx, yyy, zzz, xxy, xxz, xyy, yyz, xzz, yzz, xyz}
+    double u[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
 };
 // Traceless quadrupole normalised by mass and width: Q = (3 T - tr(T) I) / (M w^2), f32,
 // a = {xx, yy, zz, xy}, b = {xz, yz, 0, 0}.
@@ -90,10 +93,13 @@ struct __align__(32) NodeQuad {
     float4 a, b;
 };
 
-// Per internal node, from its children (built with the moments).
-struct NodeDesc {
-    float max_child;  // heaviest child mass / node mass
-    float children;   // children / 8
+// Per internal node (built with the moments): its normalised octupole (5/2) U~ / (M w^3), U~ the
+// traceless part of its third moments about its centre of mass, as the seven coefficients
+// {xxx, yyy, zzz, xxy, xxz, xyy, xyz} (oct_basis); the octupole term of its multipole expansion
+// is oct_field of these with scale M w^3.
+struct __align__(32) NodeDesc {
+    float oct[7];
+    float pad;
 };
 
 struct BhInfo {
@@ -150,6 +156,7 @@ struct BhState {
     DevBuf akeys, akeys_alt, aidx, aidx_alt, apts, h_aff, h_afg, h_out;
     DevBuf c_starts, c_pts, c_bh, c_bf, c_list, c_count, c_target, c_wgt, c_rows, c_rwave,
         c_cfeats, c_cgeom, c_cmask, c_lanes;
+    DevBuf feat_out;  // pnb_bh_node_features
     float last_theta = 0.f;
     unsigned long long last_steps = 0, last_accepted = 0;
 };
@@ -164,7 +171,7 @@ void bh_free(BhState* s) {
                       &s->akeys_alt, &s->aidx, &s->aidx_alt, &s->apts, &s->h_aff, &s->h_afg,
                       &s->h_out, &s->c_starts, &s->c_pts, &s->c_bh, &s->c_bf, &s->c_list,
                       &s->c_count, &s->c_target, &s->c_wgt, &s->c_rows, &s->c_rwave, &s->c_cfeats,
-                      &s->c_cgeom, &s->c_cmask, &s->c_lanes})
+                      &s->c_cgeom, &s->c_cmask, &s->c_lanes, &s->feat_out})
         b->release();
     delete s;
 }
@@ -672,8 +679,8 @@ __global__ __launch_bounds__(256) void bh_node_kernel(
 // (impls/mod.rs:126-131: mass-weighted mean, or the plain mean when the total mass is 0).
 // With `mom`, also the second moments by the parallel-axis rule
 // T = sum over children of (T_c + M_c (c_c - c)(c_c - c)^T), and the normalised quadrupole.
-// One internal node k: its sums from its children's (and, with moments, its second moments,
-// normalised quadrupole and child shares), its record's centre of mass and mass.
+// One internal node k: its sums from its children's (and, with moments, its second and third
+// moments, normalised quadrupole and octupole), its record's centre of mass and mass.
 __device__ __forceinline__ void up_node(uint32_t k, NodeRec* __restrict__ rec,
                                         NodeSums* __restrict__ sums,
                                         NodeMoments* __restrict__ mom,
@@ -689,16 +696,13 @@ __device__ __forceinline__ void up_node(uint32_t k, NodeRec* __restrict__ rec,
         // massive child's centre of mass) and first moments D = sum M_c (c_c - o), shifted to
         // the centre of mass below
         NodeMoments so;
-        double ox = 0, oy = 0, oz = 0, dx_ = 0, dy_ = 0, dz_ = 0, mpos = 0, heaviest = 0;
+        double ox = 0, oy = 0, oz = 0, dx_ = 0, dy_ = 0, dz_ = 0, mpos = 0;
         bool have_o = false;
-        int children = 0;
         size_t c = k + 1;
         while (c < r.skip) {
             const double4 s1 = sums[c].m;
             a.x += s1.x; a.y += s1.y; a.z += s1.z; a.w += s1.w;
             if (mom) {
-                ++children;
-                heaviest = fmax(heaviest, s1.x);
                 if (s1.x > 0.0) {
                     const NodeMoments tc = mom[c];
                     const double cx = s1.y / s1.x, cy = s1.z / s1.x, cz = s1.w / s1.x;
@@ -758,7 +762,7 @@ __device__ __forceinline__ void up_node(uint32_t k, NodeRec* __restrict__ rec,
         if (mom) {
             NodeMoments t;
             NodeQuad qd{make_float4(0.f, 0.f, 0.f, 0.f), make_float4(0.f, 0.f, 0.f, 0.f)};
-            NodeDesc ds{0.f, 0.f};
+            NodeDesc ds{{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, 0.f};
             if (a.x > 0.0 && width[k] > 0.0f) {
                 // T = sum over massive children of T_c + M_c (c_c - p)(c_c - p)^T with p the
                 // centre of mass: with u_c = c_c - o and v = p - o,
@@ -775,8 +779,46 @@ __device__ __forceinline__ void up_node(uint32_t k, NodeRec* __restrict__ rec,
                 qd.a = make_float4(float((3.0 * t.xx - tr) * f), float((3.0 * t.yy - tr) * f),
                                    float((3.0 * t.zz - tr) * f), float(3.0 * t.xy * f));
                 qd.b = make_float4(float(3.0 * t.xz * f), float(3.0 * t.yz * f), 0.f, 0.f);
-                ds.max_child = float(heaviest / a.x);
-                ds.children = float(children) * 0.125f;
+            }
+            if (a.x > 0.0 && width[k] > 0.0f) {
+                // third moments about the centre of mass p by the parallel-axis rule,
+                // U = sum over massive children of U_c + sym(u_c, T_c) + M_c u_c u_c u_c with
+                // u_c = c_c - p, then the normalised octupole from their traceless part
+                const double px = a.y / a.x, py = a.z / a.x, pz = a.w / a.x;
+                for (size_t q = k + 1; q < r.skip;) {
+                    const double4 s1 = sums[q].m;
+                    if (s1.x > 0.0) {
+                        const NodeMoments& mc = mom[q];
+                        const double ux = s1.y / s1.x - px, uy = s1.z / s1.x - py,
+                                     uz = s1.w / s1.x - pz;
+                        const double M = s1.x;
+                        t.u[0] += mc.u[0] + 3.0 * ux * mc.xx + M * ux * ux * ux;
+                        t.u[1] += mc.u[1] + 3.0 * uy * mc.yy + M * uy * uy * uy;
+                        t.u[2] += mc.u[2] + 3.0 * uz * mc.zz + M * uz * uz * uz;
+                        t.u[3] += mc.u[3] + 2.0 * ux * mc.xy + uy * mc.xx + M * ux * ux * uy;
+                        t.u[4] += mc.u[4] + 2.0 * ux * mc.xz + uz * mc.xx + M * ux * ux * uz;
+                        t.u[5] += mc.u[5] + ux * mc.yy + 2.0 * uy * mc.xy + M * ux * uy * uy;
+                        t.u[6] += mc.u[6] + 2.0 * uy * mc.yz + uz * mc.yy + M * uy * uy * uz;
+                        t.u[7] += mc.u[7] + ux * mc.zz + 2.0 * uz * mc.xz + M * ux * uz * uz;
+                        t.u[8] += mc.u[8] + uy * mc.zz + 2.0 * uz * mc.yz + M * uy * uz * uz;
+                        t.u[9] += mc.u[9] + ux * mc.yz + uy * mc.xz + uz * mc.xy + M * ux * uy * uz;
+                    }
+                    const uint32_t nx = skipv[q];
+                    q = nx > q ? nx : q + 1;
+                }
+                // traceless part: U~_ijk = U_ijk - (d_ij V_k + d_ik V_j + d_jk V_i) / 5, V_k = U_llk
+                const double* u = t.u;
+                const double vx = u[0] + u[5] + u[7], vy = u[3] + u[1] + u[8],
+                             vz = u[4] + u[6] + u[2];
+                const double w = width[k];
+                const double f = 2.5 / (a.x * w * w * w);
+                ds.oct[0] = float((u[0] - 0.6 * vx) * f);
+                ds.oct[1] = float((u[1] - 0.6 * vy) * f);
+                ds.oct[2] = float((u[2] - 0.6 * vz) * f);
+                ds.oct[3] = float((u[3] - 0.2 * vy) * f);
+                ds.oct[4] = float((u[4] - 0.2 * vz) * f);
+                ds.oct[5] = float((u[5] - 0.2 * vx) * f);
+                ds.oct[6] = float(u[9] * f);
             }
             mom[k] = t;
             quad[k] = qd;
@@ -965,11 +1007,14 @@ __global__ __launch_bounds__(256) void bh_ref_com_internal_kernel(NodeRec* __res
 //   f6     1 + log2(node mass / root mass) / 32
 //   f7     log2(bodies in the node) / 32
 //   f8     depth / 64
-//   f9     heaviest child's share of the node's mass
-//   f10    non-empty children / 8
-// The network's output y is an effective second moment in units of M w^2: every lane that
-// accepts the node adds quad_field(M w^2 y) at its own position. With y = f0..5 this is the
-// quadrupole term of the node's multipole expansion (learned.quadrupole_params sets that network).
+//   f9..15 normalised octupole (5/2) U~ / (M w^3) (NodeDesc::oct)
+// The network's outputs are two correction fields: y[0..5] = f0..5 + MLP(f)[0..5] is an
+// effective second moment in units of M w^2 (a residual on the quadrupole) and y[6..12] =
+// MLP(f)[6..12] an effective traceless third moment in units of M w^3, learned from zero; every
+// lane that accepts the node adds quad_field(M w^2 y[0..5]) + oct_field(M w^3 y[6..12]) at its
+// own position. With a zero output layer (a fresh network) this is the quadrupole term of the
+// node's multipole expansion; the octupole term (y[6..12] = f9..15) and whatever else lowers the
+// brute-force residual is what training adds.
 
 __device__ __forceinline__ void node_features(const NodeRec& r, const NodeQuad& q, uint8_t level,
                                               const NodeDesc& dsc, float root_mass,
@@ -987,16 +1032,17 @@ __device__ __forceinline__ void node_features(const NodeRec& r, const NodeQuad& 
                      (1.f / 32.f);
     f[7] = __builtin_amdgcn_logf(float(r.end - r.begin)) * (1.f / 32.f);
     f[8] = float(level) * (1.f / 64.f);
-    f[9] = dsc.max_child;
-    f[10] = dsc.children;
+#pragma unroll
+    for (int i = 0; i < 7; ++i) f[9 + i] = dsc.oct[i];
 }
 
-// The learned moment of a node, y = MLP(node) in units of the node's M w^2, as six halves and
-// two zero halves of padding (16 bytes, one scalar load when a lane accepts the node). y is O(1)
-// and the correction it feeds is itself a small fraction of the force, so 11-bit mantissas cost
-// nothing measurable, while the walk's scalar loads grow by half of what floats would take.
-struct __align__(16) QuadH {
-    __half q[8];
+// The learned moments of a node, y[0..12] (quadrupole in units of M w^2, octupole in units of
+// M w^3), as halves with three zero halves of padding (32 bytes, one scalar load when a lane
+// accepts the node). y is O(1) and the correction it feeds is itself a small fraction of the
+// force, so 11-bit mantissas cost nothing measurable, while the walk's scalar loads take half of
+// what floats would.
+struct __align__(32) QuadH {
+    __half q[16];
 };
 
 // qh[k] = MLP(features of node k) for the internal nodes listed in inodes (zero for massless
@@ -1039,16 +1085,32 @@ __global__ __launch_bounds__(256) void mlp_node_kernel(const NodeRec* __restrict
             const float v = g == 0 ? f[q] : g == 1 ? f[4 + q] : g == 2 ? f[8 + q] : f[12 + q];
             bx[q] = short(f2bf(v));
         }
-        const f32x4_t yv = mlp_forward16_regs(w, bx);  // lanes of group g: outputs 4 g .. 4 g + 3
+        f32x4_t yv = mlp_forward16_regs(w, bx);  // lanes of group g: outputs 4 g .. 4 g + 3
+        // the residual: the node's own normalised quadrupole, f0..5 in FP32
+#pragma unroll
+        for (int q = 0; q < 4; ++q) yv[q] += g == 0 ? f[q] : (g == 1 && q < 2) ? f[4 + q] : 0.f;
         const float keep = (r.w2 >= 0.0f && r.mass > 0.0f) ? 1.f : 0.f;
-        if (listed && g < 2) {
+        if (listed) {
             __half* dst = qh[k].q + 4 * g;
-            dst[0] = __float2half(keep * yv[0]);
-            dst[1] = __float2half(keep * yv[1]);
-            dst[2] = __float2half(g == 0 ? keep * yv[2] : 0.f);  // q[6], q[7]: padding
-            dst[3] = __float2half(g == 0 ? keep * yv[3] : 0.f);
+#pragma unroll
+            for (int q = 0; q < 4; ++q)  // outputs 13..15: padding
+                dst[q] = __float2half(4 * g + q < MLP_OUT ? keep * yv[q] : 0.f);
         }
     }
+}
+
+// Every node's features (pnb_bh_node_features): MLP_FEAT floats per node, pre-order.
+__global__ __launch_bounds__(256) void bh_features_kernel(const NodeRec* __restrict__ rec,
+                                                          const NodeQuad* __restrict__ quad,
+                                                          const uint8_t* __restrict__ level,
+                                                          const NodeDesc* __restrict__ desc,
+                                                          size_t total, float* __restrict__ out) {
+    const size_t k = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= total) return;
+    float f[MLP_KIN];
+    node_features(rec[k], quad[k], level[k], desc[k], rec[0].mass, f);
+#pragma unroll
+    for (int i = 0; i < MLP_FEAT; ++i) out[k * MLP_FEAT + i] = f[i];
 }
 
 // ---- traversal -----------------------------------------------------------------------------------
@@ -1294,23 +1356,40 @@ __global__ __launch_bounds__(256) void bh_walk_kernel(const NodeRec* __restrict_
             const float s = (r.mass * inv) * (inv * inv);
             float yx, yy, yz;
             if (MODE == WALK_QUAD && __float_as_int(r.w2) >= 0) {  // internal (leaves: -inf)
-                // internal node: the monopole plus the field of Q = M w^2 y (quad_field's
-                // formula, w2 = width^2), summed as d (s + s5 t) - s5 Q d in one compensated term
-                const uint4 h = *reinterpret_cast<const uint4*>(
-                    reinterpret_cast<const char*>(qh) + (ko >> 1));  // 16-byte entries
+                // internal node: the monopole plus the field of Q = M w^2 y[0..5] (quad_field's
+                // formula, w2 = width^2) and of O = M w^3 y[6..12] (oct_field's), summed as
+                // d (s + s5 t - 7 s7 G |d|^-2) - s5 Q d + 3 s7 F in one compensated term
+                const char* e = reinterpret_cast<const char*>(qh) + ko;  // 32-byte entries
+                const uint4 h = *reinterpret_cast<const uint4*>(e);
+                const uint4 h2 = *reinterpret_cast<const uint4*>(e + 16);
                 const float q[6] = {half_lo(h.x), half_hi(h.x), half_lo(h.y),
                                     half_hi(h.y), half_lo(h.z), half_hi(h.z)};
+                const float o[7] = {half_lo(h.w), half_hi(h.w), half_lo(h2.x), half_hi(h2.x),
+                                    half_lo(h2.y), half_hi(h2.y), half_lo(h2.z)};
                 // fma chains from +0 so that every half-precision product is one v_fma_mix
                 const float qx = fmaf(q[4], dz, fmaf(q[3], dy, fmaf(q[0], dx, 0.f)));
                 const float qy = fmaf(q[5], dz, fmaf(q[1], dy, fmaf(q[3], dx, 0.f)));
                 const float qz = fmaf(q[2], dz, fmaf(q[5], dy, fmaf(q[4], dx, 0.f)));
                 const float dqd = dx * qx + dy * qy + dz * qz;
+                float ob_x[7], ob_y[7], ob_z[7];
+                oct_basis(dx, dy, dz, ob_x, ob_y, ob_z);
+                float fx = 0.f, fy = 0.f, fz = 0.f;
+#pragma unroll
+                for (int j = 0; j < 7; ++j) {
+                    fx = fmaf(o[j], ob_x[j], fx);
+                    fy = fmaf(o[j], ob_y[j], fy);
+                    fz = fmaf(o[j], ob_z[j], fz);
+                }
+                const float gdd = dx * fx + dy * fy + dz * fz;
                 const float inv2 = inv * inv;
                 const float s5 = s * (r.w2 * inv2);  // M w^2 |d|^-5 from the monopole's M |d|^-3
-                const float st = fmaf(s5, 2.5f * dqd * inv2, s);
-                yx = fmaf(-s5, qx, fmaf(dx, st, -cx));
-                yy = fmaf(-s5, qy, fmaf(dy, st, -cy));
-                yz = fmaf(-s5, qz, fmaf(dz, st, -cz));
+                // M w^3 |d|^-7 from M w^2 |d|^-5 (w = sqrt(w2))
+                const float s7 = s5 * (__builtin_amdgcn_sqrtf(r.w2) * inv2);
+                const float st = fmaf(-7.f * s7, gdd * inv2, fmaf(s5, 2.5f * dqd * inv2, s));
+                const float s7x3 = 3.f * s7;
+                yx = fmaf(s7x3, fx, fmaf(-s5, qx, fmaf(dx, st, -cx)));
+                yy = fmaf(s7x3, fy, fmaf(-s5, qy, fmaf(dy, st, -cy)));
+                yz = fmaf(s7x3, fz, fmaf(-s5, qz, fmaf(dz, st, -cz)));
             } else {
                 yx = fmaf(dx, s, -cx);
                 yy = fmaf(dy, s, -cy);
@@ -2067,7 +2146,8 @@ __global__ void mlp_target_kernel(const float4* __restrict__ bf, const float4* _
 }
 
 // One thread per stored row: the node's features (node_features, as mlp_node_kernel computes
-// them) and geometry {centre of mass, M w^2}, written compactly at wave_rows[w] + r.
+// them) and geometry {centre of mass, M w^2, M w^3, 0, 0, 0} (two float4 per row), written
+// compactly at wave_rows[w] + r.
 __global__ void mlp_row_feats_kernel(const NodeRec* __restrict__ nodes,
                                      const float* __restrict__ widths,
                                      const NodeQuad* __restrict__ quad,
@@ -2091,7 +2171,8 @@ __global__ void mlp_row_feats_kernel(const NodeRec* __restrict__ nodes,
 #pragma unroll
         for (int q = 0; q < MLP_FEAT; ++q) feats[dst * MLP_FEAT + q] = f[q];
         const float wd = widths[row.x];
-        geom[dst] = make_float4(nr.cx, nr.cy, nr.cz, nr.mass * wd * wd);
+        geom[2 * dst] = make_float4(nr.cx, nr.cy, nr.cz, nr.mass * wd * wd);
+        geom[2 * dst + 1] = make_float4(nr.mass * wd * wd * wd, 0.f, 0.f, 0.f);
         mask[dst] = (unsigned long long)row.z | ((unsigned long long)row.w << 32);
         rwave[dst] = w;
     }
@@ -2153,7 +2234,7 @@ static int collect(pnb_ctx* c, const float4* p, size_t n, float theta, float sof
     }
     const size_t R = offs[W];
     PNB_HIP(c, s->c_cfeats.ensure(std::max<size_t>(R, 1) * MLP_FEAT * 4));
-    PNB_HIP(c, s->c_cgeom.ensure(std::max<size_t>(R, 1) * 16));
+    PNB_HIP(c, s->c_cgeom.ensure(std::max<size_t>(R, 1) * MLP_GEOM * sizeof(float)));
     PNB_HIP(c, s->c_cmask.ensure(std::max<size_t>(R, 1) * 8));
     PNB_HIP(c, s->c_rwave.ensure(std::max<size_t>(R, 1) * 4));
     PNB_HIP(c, hipMemcpyAsync(s->c_rows.ptr, offs.data(), (W + 1) * 4, hipMemcpyHostToDevice, c->stream));
@@ -2306,6 +2387,29 @@ int pnb_bh_tree_size(pnb_ctx* c, size_t* nodes, int* max_level) {
     if (!c->bh || !c->bh->built) return fail(c, PNB_E_STATE, "no octree has been built");
     if (nodes) *nodes = c->bh->nodes;
     if (max_level) *max_level = c->bh->max_level;
+    return PNB_OK;
+}
+
+int pnb_bh_node_features(pnb_ctx* c, float* out, size_t cap) {
+    if (!c) return PNB_E_ARG;
+    if (c->bh && c->bh->dd_last) {
+        if (int rc = activate(c)) return rc;
+        if (int rc = dd_resolve(c)) return rc;
+    }
+    BhState* s = c->bh;
+    if (!s || !s->built) return fail(c, PNB_E_STATE, "no octree has been built");
+    if (!s->has_moments) return fail(c, PNB_E_STATE, "the last octree was built without moments");
+    if (cap < s->nodes || (s->nodes && !out)) return fail(c, PNB_E_ARG, "output too small");
+    if (int rc = activate(c)) return rc;
+    if (s->nodes == 0) return PNB_OK;
+    const size_t bytes = s->nodes * MLP_FEAT * sizeof(float);
+    PNB_HIP(c, s->feat_out.ensure(bytes));
+    bh_features_kernel<<<div_up(s->nodes, size_t(256)), 256, 0, c->stream>>>(
+        s->rec.as<NodeRec>(), s->quad.as<NodeQuad>(), s->level.as<uint8_t>(),
+        s->desc.as<NodeDesc>(), s->nodes, s->feat_out.as<float>());
+    PNB_CHECK_LAUNCH(c);
+    PNB_HIP(c, hipMemcpyAsync(out, s->feat_out.ptr, bytes, hipMemcpyDeviceToHost, c->stream));
+    PNB_HIP(c, hipStreamSynchronize(c->stream));
     return PNB_OK;
 }
 

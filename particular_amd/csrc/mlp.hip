@@ -2,15 +2,17 @@
 //
 // Parameters live as an FP32 master vector (Adam state beside it) and are packed into BF16 MFMA
 // operand copies (MlpWeights) after every update. A training set holds rows (node features, the
-// node's centre of mass and M w^2, accepting-lane mask, wavefront id), per wavefront the lanes'
+// node's centre of mass, M w^2 and M w^3, accepting-lane mask, wavefront id), per wavefront the lanes'
 // positions (w = softening^2), and per-lane targets (brute force minus Barnes-Hut acceleration)
 // with weights 1 / |a_bf|^2. One full-batch step:
-//   mlp_fwd_kernel    y = MLP(x) for every row (MFMA, 16-row tiles per wavefront)
+//   mlp_fwd_kernel    y = MLP(x) plus x[0..5] on y[0..5], for every row (MFMA, 16-row tiles
+//                     per wavefront)
 //   mlp_pred_kernel   per (128-row chunk of a wavefront, lane l): the partial sum over the chunk's
-//                     rows holding l of the field of Q = M w^2 y at the lane (quad_field)
+//                     rows holding l of the fields of Q = M w^2 y[0..5] (quad_field) and of
+//                     O = M w^3 y[6..12] (oct_field) at the lane
 //   mlp_resid_kernel  per (wavefront, lane): pred_l from the partials, loss, g_pred
-//   mlp_rowgrad_kernel per row: g_y = sum over its lanes of M w^2 (d field / d Q)^T g_pred
-//                     (quad_field_grad)
+//   mlp_rowgrad_kernel per row: g_y = sum over its lanes of the scaled (d field / d y)^T g_pred
+//                     (quad_field_grad, oct_field_grad)
 //   mlp_bwd_kernel    recomputes the forward per 16-row tile and back-propagates with MFMA;
 //                     weight gradients accumulate in registers, one partial per wavefront
 //   mlp_reduce_kernel fixed-order sum of the partials (deterministic)
@@ -104,7 +106,9 @@ __device__ __forceinline__ void load_x16(MlpTile& t, const float* __restrict__ f
     wave_lds_sync();
 }
 
-// y (rows x MLP_YS, the first MLP_OUT used) = MLP(feats), register-resident forward.
+// y (rows x MLP_YS, the first MLP_OUT used) = MLP(feats), plus feats[0..5] on y[0..5],
+// register-resident forward: the quadrupole outputs are a residual on the node's normalised
+// quadrupole (features 0..5).
 __global__ __launch_bounds__(256) void mlp_fwd_kernel(const float* __restrict__ feats, size_t rows,
                                                       const MlpWeights* __restrict__ wg,
                                                       float* __restrict__ y) {
@@ -124,8 +128,13 @@ __global__ __launch_bounds__(256) void mlp_fwd_kernel(const float* __restrict__ 
         }
         const f32x4_t yv = mlp_forward16_regs(w, bx);
         if (g < MLP_YS / 4 && r < rows) {  // lanes of group g hold outputs 4 g .. 4 g + 3
+            // plus the residual on 0..5: the row's normalised quadrupole, features 0..5 in FP32
 #pragma unroll
-            for (int i = 0; i < 4; ++i) y[r * MLP_YS + 4 * g + i] = (4 * g + i < MLP_OUT) ? yv[i] : 0.f;
+            for (int i = 0; i < 4; ++i) {
+                const int o = 4 * g + i;
+                const float res = o < MLP_QOUT ? feats[r * MLP_FEAT + o] : 0.f;
+                y[r * MLP_YS + o] = o < MLP_OUT ? yv[i] + res : 0.f;
+            }
         }
     }
 }
@@ -133,8 +142,8 @@ __global__ __launch_bounds__(256) void mlp_fwd_kernel(const float* __restrict__ 
 // ---- per-lane aggregation, loss, gradient of y ---------------------------------------------------
 
 // The loss over the lanes, L = sum_lanes w |pred - target|^2 / count with
-// pred = sum over the lane's accepted rows of quad_field(y_row), and its gradient with respect to
-// each row's six outputs, in three passes that each keep every lane of the GPU busy:
+// pred = sum over the lane's accepted rows of quad_field(y_row[0..5]) + oct_field(y_row[6..12]),
+// and its gradient with respect to each row's thirteen outputs, in three passes that each keep every lane of the GPU busy:
 //  1. mlp_pred_kernel: per (chunk of MLP_CHUNK rows of one wavefront, lane), the partial pred;
 //  2. mlp_resid_kernel: per (wavefront, lane), the chunks' partials summed in chunk order, the
 //     residual, the loss share and g = 2 w (pred - target) / count;
@@ -159,13 +168,17 @@ __global__ __launch_bounds__(64) void mlp_pred_kernel(const float* __restrict__ 
     float px = 0.f, py = 0.f, pz = 0.f;
     for (uint32_t r = rb; r < re; ++r) {
         if ((mask[r] >> lane) & 1ull) {
-            const float4 gm = geom[r];
+            const float4 gm = geom[2 * size_t(r)];
+            const float mw3 = geom[2 * size_t(r) + 1].x;
             const float dx = gm.x - p.x, dy = gm.y - p.y, dz = gm.z - p.z;
             const float inv = rsqrtf(dx * dx + dy * dy + dz * dz + p.w);
-            float q[6];
+            float q[6], o[7];
 #pragma unroll
             for (int j = 0; j < 6; ++j) q[j] = y[size_t(r) * MLP_YS + j];
+#pragma unroll
+            for (int j = 0; j < 7; ++j) o[j] = y[size_t(r) * MLP_YS + MLP_QOUT + j];
             quad_field(q, dx, dy, dz, inv, gm.w, px, py, pz);
+            oct_field(o, dx, dy, dz, inv, mw3, px, py, pz);
         }
     }
     part[size_t(c) * 64 + lane] = make_float4(px, py, pz, 0.f);
@@ -208,8 +221,10 @@ __global__ __launch_bounds__(256) void mlp_rowgrad_kernel(size_t rows,
     if (r >= rows) return;
     const size_t w = rwave[r];
     unsigned long long m = mask[r];
-    const float4 gm = geom[r];
+    const float4 gm = geom[2 * r];
+    const float mw3 = geom[2 * r + 1].x;
     float g6[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    float g7[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     while (m) {  // the row's lanes in ascending order
         const int l = __builtin_ctzll(m);
         m &= m - 1;
@@ -218,9 +233,12 @@ __global__ __launch_bounds__(256) void mlp_rowgrad_kernel(size_t rows,
         const float ex = gm.x - p.x, ey = gm.y - p.y, ez = gm.z - p.z;
         const float inv = rsqrtf(ex * ex + ey * ey + ez * ez + p.w);
         quad_field_grad(ex, ey, ez, inv, g.x, g.y, g.z, g6);
+        oct_field_grad(ex, ey, ez, inv, g.x, g.y, g.z, g7);
     }
 #pragma unroll
-    for (int j = 0; j < MLP_YS; ++j) gy[r * MLP_YS + j] = j < 6 ? gm.w * g6[j] : 0.f;
+    for (int j = 0; j < MLP_YS; ++j)
+        gy[r * MLP_YS + j] = j < MLP_QOUT ? gm.w * g6[j]
+                             : j < MLP_OUT ? mw3 * g7[j - MLP_QOUT] : 0.f;
 }
 
 // ---- backward ------------------------------------------------------------------------------------
@@ -477,7 +495,7 @@ int mlp_append(pnb_ctx* c, const float* feats, const float4* geom, const unsigne
     auto cap = [](size_t need, size_t have) { return need <= have ? have : std::max(need, 2 * have); };
     const size_t cr = cap(nr, s->cap_rows), cw = cap(nwv, s->cap_waves);
     if (int rc = grow(c, s->feats, s->rows * MLP_FEAT * 4, cr * MLP_FEAT * 4)) return rc;
-    if (int rc = grow(c, s->geom, s->rows * 16, cr * 16)) return rc;
+    if (int rc = grow(c, s->geom, s->rows * MLP_GEOM * 4, cr * MLP_GEOM * 4)) return rc;
     if (int rc = grow(c, s->mask, s->rows * 8, cr * 8)) return rc;
     if (int rc = grow(c, s->rwave, s->rows * 4, cr * 4)) return rc;
     if (int rc = grow(c, s->lanes, s->waves * 64 * 16, cw * 64 * 16)) return rc;
@@ -489,7 +507,7 @@ int mlp_append(pnb_ctx* c, const float* feats, const float4* geom, const unsigne
     hipStream_t st = c->stream;
     if (rows) {
         PNB_HIP(c, hipMemcpyAsync(s->feats.as<float>() + s->rows * MLP_FEAT, feats, rows * MLP_FEAT * 4, hipMemcpyDeviceToDevice, st));
-        PNB_HIP(c, hipMemcpyAsync(s->geom.as<float4>() + s->rows, geom, rows * 16, hipMemcpyDeviceToDevice, st));
+        PNB_HIP(c, hipMemcpyAsync(s->geom.as<float>() + s->rows * MLP_GEOM, geom, rows * MLP_GEOM * 4, hipMemcpyDeviceToDevice, st));
         PNB_HIP(c, hipMemcpyAsync(s->mask.as<unsigned long long>() + s->rows, mask, rows * 8, hipMemcpyDeviceToDevice, st));
         mlp_offset_kernel<<<div_up(rows, 256), 256, 0, st>>>(row_wave_local, rows, uint32_t(s->waves),
                                                              s->rwave.as<uint32_t>() + s->rows);
@@ -599,7 +617,7 @@ int pnb_mlp_init(pnb_ctx* c, uint64_t seed) {
     if (int rc = mlp_ensure(c)) return rc;
     MlpState* s = c->mlp;
     // He-normal hidden layers (Box-Muller over SplitMix64), zero biases, and a zero output layer:
-    // a fresh network predicts no correction, i.e. starts exactly at plain Barnes-Hut
+    // a fresh network adds nothing to its residual, i.e. starts exactly at the quadrupole term
     std::vector<float> p(MLP_NPARAM, 0.f);
     uint64_t st = seed;
     auto normal = [&]() {
@@ -675,7 +693,7 @@ int pnb_mlp_dataset_read(pnb_ctx* c, float* feats, float* geom, uint64_t* mask, 
     if (int rc = activate(c)) return rc;
     hipStream_t st = c->stream;
     if (feats && s->rows) PNB_HIP(c, hipMemcpyAsync(feats, s->feats.ptr, s->rows * MLP_FEAT * 4, hipMemcpyDeviceToHost, st));
-    if (geom && s->rows) PNB_HIP(c, hipMemcpyAsync(geom, s->geom.ptr, s->rows * 16, hipMemcpyDeviceToHost, st));
+    if (geom && s->rows) PNB_HIP(c, hipMemcpyAsync(geom, s->geom.ptr, s->rows * MLP_GEOM * 4, hipMemcpyDeviceToHost, st));
     if (mask && s->rows) PNB_HIP(c, hipMemcpyAsync(mask, s->mask.ptr, s->rows * 8, hipMemcpyDeviceToHost, st));
     if (row_wave && s->rows) PNB_HIP(c, hipMemcpyAsync(row_wave, s->rwave.ptr, s->rows * 4, hipMemcpyDeviceToHost, st));
     if (wave_rows) PNB_HIP(c, hipMemcpyAsync(wave_rows, s->wave_rows.ptr, (s->waves + 1) * 4, hipMemcpyDeviceToHost, st));
@@ -695,10 +713,10 @@ int pnb_mlp_dataset_load(pnb_ctx* c, const float* feats, const float* geom, cons
         return fail(c, PNB_E_ARG, "incomplete training set");
     if (int rc = activate(c)) return rc;
     DevBuf tmp;
-    const size_t bytes = rows * (MLP_FEAT * 4 + 16 + 8 + 4) + (waves + 1) * 4 + waves * 64 * 36;
+    const size_t bytes = rows * (MLP_FEAT * 4 + MLP_GEOM * 4 + 8 + 4) + (waves + 1) * 4 + waves * 64 * 36;
     PNB_HIP(c, tmp.ensure(bytes));
     char* p = tmp.as<char>();
-    float4* dg = reinterpret_cast<float4*>(p); p += rows * 16;  // 16-byte records first
+    float4* dg = reinterpret_cast<float4*>(p); p += rows * MLP_GEOM * 4;  // 16-byte records first
     float4* dl = reinterpret_cast<float4*>(p); p += waves * 64 * 16;
     float* dt = reinterpret_cast<float*>(p); p += waves * 64 * 16;
     float* df = reinterpret_cast<float*>(p); p += rows * MLP_FEAT * 4;
@@ -710,7 +728,7 @@ int pnb_mlp_dataset_load(pnb_ctx* c, const float* feats, const float* geom, cons
     hipError_t e = hipSuccess;
     if (rows) {
         if (e == hipSuccess) e = hipMemcpyAsync(df, feats, rows * MLP_FEAT * 4, hipMemcpyHostToDevice, st);
-        if (e == hipSuccess) e = hipMemcpyAsync(dg, geom, rows * 16, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(dg, geom, rows * MLP_GEOM * 4, hipMemcpyHostToDevice, st);
         if (e == hipSuccess) e = hipMemcpyAsync(dm, mask, rows * 8, hipMemcpyHostToDevice, st);
         if (e == hipSuccess) e = hipMemcpyAsync(drw, row_wave, rows * 4, hipMemcpyHostToDevice, st);
     }

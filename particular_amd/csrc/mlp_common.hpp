@@ -1,12 +1,15 @@
 // Far-field correction MLP: shared device definitions (bh.hip build and walk, mlp.hip training).
 //
-// Network: 11 inputs -> 64 -> 64 -> 6, ReLU. BF16 weights and activations feed MFMA
+// Network: 16 inputs -> 64 -> 64 -> 13, ReLU. BF16 weights and activations feed MFMA
 // (v_mfma_f32_16x16x16_bf16, FP32 accumulate); biases and everything outside the products are
 // FP32. The network maps a description of an internal octree node (its normalised quadrupole,
-// mass fraction, size in bodies, depth and how its mass splits over its children) to an
-// effective second moment Q (6 components, in units of the node's M w^2). Every lane that
-// accepts the node in the Barnes-Hut walk adds the field of Q at its own position (quad_field)
-// to the node's monopole term. Leaves are never corrected (their monopole is exact).
+// mass fraction, size in bodies, depth and normalised octupole) to the coefficients of two
+// correction fields: y[0..5] is added to the node's normalised quadrupole (a residual: a zero
+// output layer leaves the quadrupole term) to give an effective second moment Q in units of
+// M w^2 (quad_field), and y[6..12] are an effective traceless third moment in units of M w^3
+// (oct_field), learned from zero. Every lane that accepts the node in the Barnes-Hut walk adds
+// both fields at its own position to the node's monopole term. Leaves are never corrected (their
+// monopole is exact).
 //
 // MFMA 16x16x16 bf16 operand layout (lane l of 64): A[m][k] with m = l % 16, k = 4 (l / 16) + i;
 // B[k][n] with n = l % 16, k = 4 (l / 16) + i; D[m][n] with n = l % 16, m = 4 (l / 16) + i,
@@ -20,9 +23,10 @@
 
 namespace pnb {
 
-constexpr int MLP_IN = 11;
+constexpr int MLP_IN = 16;
 constexpr int MLP_H = 64;
-constexpr int MLP_OUT = 6;
+constexpr int MLP_OUT = 13;
+constexpr int MLP_QOUT = 6;  // outputs 0..5: quadrupole; 6..12: octupole
 constexpr int MLP_KIN = 16;   // inputs padded to one K step
 constexpr int MLP_NOUT = 16;  // outputs padded to one N tile
 constexpr int MLP_OFF_W1 = 0;
@@ -31,9 +35,10 @@ constexpr int MLP_OFF_W2 = MLP_OFF_B1 + MLP_H;
 constexpr int MLP_OFF_B2 = MLP_OFF_W2 + MLP_H * MLP_H;
 constexpr int MLP_OFF_W3 = MLP_OFF_B2 + MLP_H;
 constexpr int MLP_OFF_B3 = MLP_OFF_W3 + MLP_H * MLP_OUT;
-constexpr int MLP_NPARAM = MLP_OFF_B3 + MLP_OUT;  // 5318
-constexpr int MLP_FEAT = 16;                      // stored feature stride (11 used)
-constexpr int MLP_YS = 8;                         // stored output stride (6 used)
+constexpr int MLP_NPARAM = MLP_OFF_B3 + MLP_OUT;  // 6093
+constexpr int MLP_FEAT = 16;                      // stored feature stride (all used)
+constexpr int MLP_YS = 16;                        // stored output stride (13 used)
+constexpr int MLP_GEOM = 8;  // per training row: centre of mass, M w^2, M w^3, 3 zeros
 
 typedef short bf16x4_t __attribute__((ext_vector_type(4)));
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
@@ -43,7 +48,7 @@ typedef float f32x4_t __attribute__((ext_vector_type(4)));
 struct MlpForward {
     uint16_t w1t[MLP_H][MLP_KIN];   // [n][k] = W1[k][n], k >= MLP_IN zero
     uint16_t w2t[MLP_H][MLP_H];     // [n][k] = W2[k][n]
-    uint16_t w3t[MLP_NOUT][MLP_H];  // [n][k] = W3[k][n], n >= 3 zero
+    uint16_t w3t[MLP_NOUT][MLP_H];  // [n][k] = W3[k][n], n >= MLP_OUT zero
     float b1[MLP_H], b2[MLP_H], b3[MLP_NOUT];
 };
 struct MlpWeights : MlpForward {
@@ -202,6 +207,63 @@ __device__ __forceinline__ void quad_field_grad(float dx, float dy, float dz, fl
     g6[3] += inv5 * (2.f * gd * dx * dy - (dy * gx + dx * gy));
     g6[4] += inv5 * (2.f * gd * dx * dz - (dz * gx + dx * gz));
     g6[5] += inv5 * (2.f * gd * dy * dz - (dz * gy + dy * gz));
+}
+
+// The seven coefficients of a traceless symmetric third-order tensor O, {xxx, yyy, zzz, xxy, xxz,
+// xyy, xyz} (the rest follow from O_iik = 0: xzz = -xxx - xyy, yzz = -xxy - yyy,
+// yyz = -xxz - zzz). With d = centre of mass - point, F_i = O_ijk d_j d_k and G = d.F:
+// oct_basis gives F's derivative with respect to each coefficient (B_j, a 3-vector).
+__device__ __forceinline__ void oct_basis(float dx, float dy, float dz, float (&bx)[7],
+                                          float (&by)[7], float (&bz)[7]) {
+    const float xx = dx * dx, yy = dy * dy, zz = dz * dz;
+    const float xy = 2.f * dx * dy, xz = 2.f * dx * dz, yz = 2.f * dy * dz;
+    bx[0] = xx - zz; by[0] = 0.f;     bz[0] = -xz;
+    bx[1] = 0.f;     by[1] = yy - zz; bz[1] = -yz;
+    bx[2] = 0.f;     by[2] = -yz;     bz[2] = zz - yy;
+    bx[3] = xy;      by[3] = xx - zz; bz[3] = -yz;
+    bx[4] = xz;      by[4] = -yz;     bz[4] = xx - yy;
+    bx[5] = yy - zz; by[5] = xy;      bz[5] = -xz;
+    bx[6] = yz;      by[6] = xz;      bz[6] = xy;
+}
+
+// Field at a point of the octupole term of a third moment O = scale * o (o as in oct_basis):
+// the gradient of O_ijk r_i r_j r_k / |r|^7 (r = point - centre of mass), in terms of d = -r,
+// a = scale (3 F |d|^-7 - 7 G d |d|^-9), added to (ax, ay, az). inv = 1 / |d| (softened like the
+// monopole term). With o = (5/2) U~ / (M w^3), U~ the traceless part of the node's third moments
+// about its centre of mass and scale = M w^3, this is the multipole expansion's octupole term.
+__device__ __forceinline__ void oct_field(const float (&o)[7], float dx, float dy, float dz,
+                                          float inv, float scale, float& ax, float& ay, float& az) {
+    float bx[7], by[7], bz[7];
+    oct_basis(dx, dy, dz, bx, by, bz);
+    float fx = 0.f, fy = 0.f, fz = 0.f;
+#pragma unroll
+    for (int j = 0; j < 7; ++j) {
+        fx += o[j] * bx[j];
+        fy += o[j] * by[j];
+        fz += o[j] * bz[j];
+    }
+    const float g = dx * fx + dy * fy + dz * fz;
+    const float inv2 = inv * inv;
+    const float s7 = scale * (inv2 * inv2) * (inv2 * inv);
+    const float t = 7.f * g * inv2;
+    ax += s7 * (3.f * fx - t * dx);
+    ay += s7 * (3.f * fy - t * dy);
+    az += s7 * (3.f * fz - t * dz);
+}
+
+// g . (d a / d o_j) for the seven coefficients of oct_field's o (scale 1), added to g7.
+__device__ __forceinline__ void oct_field_grad(float dx, float dy, float dz, float inv, float gx,
+                                               float gy, float gz, float (&g7)[7]) {
+    float bx[7], by[7], bz[7];
+    oct_basis(dx, dy, dz, bx, by, bz);
+    const float inv2 = inv * inv;
+    const float inv7 = (inv2 * inv2) * (inv2 * inv);
+    const float gd = 7.f * (gx * dx + gy * dy + gz * dz) * inv2;  // 7 (g.d) / |d|^2
+#pragma unroll
+    for (int j = 0; j < 7; ++j) {
+        const float c = dx * bx[j] + dy * by[j] + dz * bz[j];  // dG / do_j
+        g7[j] += inv7 * (3.f * (gx * bx[j] + gy * by[j] + gz * bz[j]) - gd * c);
+    }
 }
 
 }  // namespace pnb

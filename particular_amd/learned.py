@@ -7,15 +7,18 @@ training set for checks against the CPU oracle.
 from __future__ import annotations
 
 import ctypes
+import os
 
 import numpy as np
 
 from . import _lib
 
 FEAT = 16  # stored feature stride
-IN = 11    # features the network reads
-OUT = 6    # outputs: an effective second moment {xx, yy, zz, xy, xz, yz} in units of M w^2
-YS = 8     # stored output stride
+IN = 16    # features the network reads
+OUT = 13   # outputs: quadrupole {xx, yy, zz, xy, xz, yz} (a residual on features 0..5), then
+QOUT = 6   # octupole {xxx, yyy, zzz, xxy, xxz, xyy, xyz} (features 9..15 hold the node's own)
+YS = 16    # stored output stride
+GEOM = 8   # per row: centre of mass, M w^2, M w^3, three zeros
 
 
 def num_params() -> int:
@@ -79,7 +82,7 @@ def info(res) -> tuple[int, int, int]:
 def dataset(res) -> dict:
     rows, waves, _ = info(res)
     d = {
-        "feats": np.zeros((rows, FEAT), np.float32), "geom": np.zeros((rows, 4), np.float32),
+        "feats": np.zeros((rows, FEAT), np.float32), "geom": np.zeros((rows, GEOM), np.float32),
         "mask": np.zeros(rows, np.uint64), "row_wave": np.zeros(rows, np.uint32),
         "wave_rows": np.zeros(waves + 1, np.uint32), "lanes": np.zeros((waves, 64, 4), np.float32),
         "target": np.zeros((waves, 64, 4), np.float32), "wgt": np.zeros((waves, 64), np.float32),
@@ -106,7 +109,7 @@ def load(res, d: dict) -> None:
             or f["wgt"].shape[1:] != (64,)):
         raise ValueError("lanes and target must be (waves, 64, 4) and wgt (waves, 64)")
     rows, waves = f["feats"].shape[0], f["wave_rows"].shape[0] - 1
-    if not (f["geom"].shape == (rows, 4) and f["mask"].size == f["row_wave"].size == rows
+    if not (f["geom"].shape == (rows, GEOM) and f["mask"].size == f["row_wave"].size == rows
             and f["target"].shape[0] == f["wgt"].shape[0] == f["lanes"].shape[0] == waves):
         raise ValueError("inconsistent training-set sizes")
     res.check(_lib.lib().pnb_mlp_dataset_load(
@@ -129,24 +132,48 @@ def train(res, iterations: int, lr: float = 1e-3) -> np.ndarray:
 
 
 def quadrupole_params() -> np.ndarray:
-    """Parameters that make the network output the node's own normalised quadrupole (features
-    0..5), so every accepting lane adds the quadrupole term of the node's multipole expansion:
-    the six features pass through 12 hidden units as relu(+f) and relu(-f) (every weight is 0 or
-    +-1, exact in bf16). A physics-informed starting point for training."""
+    """Parameters with a zero output layer: the network adds nothing to its residual, the node's
+    own normalised quadrupole (features 0..5), so every accepting lane adds the quadrupole term of
+    the node's multipole expansion. pnb_mlp_init gives the same outputs (random hidden layers,
+    zero output layer) and is where training starts."""
+    return np.zeros(num_params(), np.float32)
+
+
+def octupole_params() -> np.ndarray:
+    """Parameters that make the network's octupole outputs (6..12) equal the node's own normalised
+    octupole (features 9..15) and add nothing to the quadrupole residual: each of the seven features
+    passes through two hidden units as relu(+f) and relu(-f) (weights 0 or +-1, exact in bf16).
+    The walk then adds the multipole expansion's quadrupole and octupole terms: the analytic
+    reference a trained network is measured against."""
     d = params_to_dict(np.zeros(num_params(), np.float32))
-    for j in range(OUT):
-        d["W1"][j, 2 * j] = 1.0
-        d["W1"][j, 2 * j + 1] = -1.0
+    for j in range(OUT - QOUT):
+        d["W1"][9 + j, 2 * j] = 1.0
+        d["W1"][9 + j, 2 * j + 1] = -1.0
         d["W2"][2 * j, 2 * j] = 1.0
         d["W2"][2 * j + 1, 2 * j + 1] = 1.0
-        d["W3"][2 * j, j] = 1.0
-        d["W3"][2 * j + 1, j] = -1.0
+        d["W3"][2 * j, QOUT + j] = 1.0
+        d["W3"][2 * j + 1, QOUT + j] = -1.0
     return dict_to_params(d)
 
 
+TRAINED_PARAMS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "params",
+                              "two_plummer_4M.npy")
+
+
+def trained_params() -> np.ndarray | None:
+    """The network trained by particular_amd.nn_bh on two-Plummer snapshots (the file written by
+    its --params-out), or None when that file is absent or holds another network's size."""
+    if not os.path.exists(TRAINED_PARAMS):
+        return None
+    p = np.load(TRAINED_PARAMS, allow_pickle=False).astype(np.float32).ravel()
+    return p if p.size == num_params() else None
+
+
 def default_params() -> np.ndarray:
-    """The parameters the corrected walk uses unless told otherwise."""
-    return quadrupole_params()
+    """The parameters the corrected walk uses unless told otherwise: the trained network when
+    it is present, else the quadrupole term."""
+    p = trained_params()
+    return p if p is not None else quadrupole_params()
 
 
 def fit(res, iterations: int, lrs=(1e-3, 3e-4, 1e-4, 3e-5, 1e-5), probe: int = 40) -> dict:
@@ -155,7 +182,7 @@ def fit(res, iterations: int, lrs=(1e-3, 3e-4, 1e-4, 3e-5, 1e-5), probe: int = 4
     Every candidate in `lrs` runs `probe` Adam steps from the same start; the one with the
     lowest loss then runs `iterations` steps in three phases (lr for half of them, lr/10 and
     lr/100 for a quarter each). If the result fits the training set worse than the starting
-    parameters, those are restored (a fresh network is plain Barnes-Hut, the fallback).
+    parameters, those are restored (a fresh network is the quadrupole term, the fallback).
     Returns the loss history, the chosen rate and the start / end losses."""
     p0 = get_params(res)
     loss0, _ = grad(res)
@@ -182,8 +209,20 @@ def fit(res, iterations: int, lrs=(1e-3, 3e-4, 1e-4, 3e-5, 1e-5), probe: int = 4
             "kept": kept}
 
 
+def node_features(res) -> np.ndarray:
+    """(nodes, FEAT) network inputs of every node of the last octree built with moments
+    (pnb_bh_node_features), pre-order; zeros for leaves and massless nodes."""
+    L = _lib.lib()
+    n = ctypes.c_size_t()
+    lvl = ctypes.c_int()
+    res.check(L.pnb_bh_tree_size(res.handle, ctypes.byref(n), ctypes.byref(lvl)))
+    out = np.zeros((n.value, FEAT), np.float32)
+    res.check(L.pnb_bh_node_features(res.handle, out.ctypes.data if n.value else None, n.value))
+    return out
+
+
 def forward(res, feats_dev, y_dev) -> None:
-    """y_dev (rows x YS) = MLP(feats_dev (rows x FEAT)); device tensors."""
+    """y_dev (rows x YS) = feats_dev[:, :6] + MLP(feats_dev (rows x FEAT)); device tensors."""
     res.use_torch_stream()
     res.check(_lib.lib().pnb_mlp_forward_dev(res.handle, feats_dev.data_ptr(), feats_dev.shape[0],
                                              y_dev.data_ptr()))

@@ -4,18 +4,20 @@
    rotations, which changes how the octree cuts it), `--waves` wavefronts each, rows and
    brute-force-minus-Barnes-Hut targets generated on the GPU, at opening angles cycling through
    `--train-thetas`.
-2. Full-batch Adam on the GPU (`--iters`), learning rate picked by short probes and stepped
-   down by 10x twice (particular_amd.learned.fit); the trained network replaces the starting one
-   only if it also lowers the error on a separate validation snapshot.
-3. A held-out snapshot: plain Barnes-Hut at `--thetas` and NN-corrected Barnes-Hut at
-   `--nn-thetas` (with the starting network as well when it is the quadrupole one), each timed (tree build + walk [+ correction], device time from HIP events) and
-   checked against the FP64 direct sum on a seeded sample. Reports particles/s and error of each
-   run and, by log-log interpolation of time against error on each curve, the speed-up of the
-   corrected method over plain Barnes-Hut at equal rms error (at `--target-rms` and at the error
-   of each NN run). Both are also run with the relative opening criterion (`--rel-alphas`,
-   `--nn-rel-alphas`; theta = 1 as the geometric floor) and compared at `--target-rms` with
-   plain Barnes-Hut under the reference's criterion and under the same relative one.
-Prints one JSON document.
+2. Full-batch Adam on the GPU (`--iters`), from a fresh network (random hidden layers, zero
+   output layer: the residual network then outputs each node's own quadrupole), learning rate
+   picked by short probes and stepped down by 10x twice (particular_amd.learned.fit); the trained
+   network replaces the starting one only if it also lowers the error on a separate validation
+   snapshot.
+3. A held-out snapshot: plain Barnes-Hut at `--thetas`, the starting network (the quadrupole
+   term) and the trained network at `--nn-thetas`, each timed (tree build + walk [+ node network],
+   device time from HIP events) and checked against the FP64 direct sum on a seeded sample.
+   Reports particles/s and error of each run and, by log-log interpolation of time against error
+   on each curve, at `--target-rms` and at the error of each trained run: the trained network
+   against the quadrupole term (what training adds) and both against plain Barnes-Hut. All three
+   are also run with the relative opening criterion (`--rel-alphas`, `--nn-rel-alphas`; theta = 1
+   as the geometric floor) and compared at `--target-rms` under that same criterion.
+Prints one JSON document; `--params-out` saves the network the corrected runs used.
 """
 from __future__ import annotations
 
@@ -105,9 +107,7 @@ def main():
     ap.add_argument("--nn-rel-alphas", default="1e-4,2e-4,3e-4",
                     help="the corrected walk with the relative criterion (empty: none)")
     ap.add_argument("--target-rms", type=float, default=1e-4)
-    ap.add_argument("--init", choices=("quadrupole", "zero"), default="quadrupole",
-                    help="starting network: the quadrupole term (learned.quadrupole_params) or a "
-                         "zero output layer")
+    ap.add_argument("--seed", type=int, default=1808, help="hidden-layer initialisation")
     ap.add_argument("--params-out", default="")
     a = ap.parse_args()
     import torch
@@ -115,9 +115,7 @@ def main():
     import particular_amd as pa
 
     res = pa.HipResources(0)
-    learned.init(res, 1808)
-    if a.init == "quadrupole":
-        learned.set_params(res, learned.quadrupole_params())
+    learned.init(res, a.seed)
     learned.clear(res)
     t0 = time.time()
     train_thetas = [float(x) for x in a.train_thetas.split(",")]
@@ -154,7 +152,9 @@ def main():
         return row
 
     nn_thetas = [float(x) for x in a.nn_thetas.split(",")]
-    start = [run(th, True) for th in nn_thetas] if a.init == "quadrupole" else []
+    start = [run(th, True) for th in nn_thetas]
+    nn_rel_alphas = [float(x) for x in a.nn_rel_alphas.split(",") if x]
+    start_rel = [run(1.0, True, al) for al in nn_rel_alphas]
 
     # validation snapshot (its own seed): the trained network is kept only if it beats the
     # starting one there, at the middle training angle
@@ -193,12 +193,20 @@ def main():
         q["speedup_at_same_error"] = t_plain / q["ms"] if t_plain else None
         same = next((r for r in plain if abs(r["theta"] - q["theta"]) < 1e-9), None)
         q["error_vs_plain_same_theta"] = q["rms_rel"] / same["rms_rel"] if same else None
+    for q in nn:
+        t_q = time_at_error(start, q["rms_rel"])
+        q["quadrupole_ms_at_same_error"] = t_q
+        q["speedup_vs_quadrupole_at_same_error"] = t_q / q["ms"] if t_q else None
+        same = next((r for r in start if abs(r["theta"] - q["theta"]) < 1e-9), None)
+        q["error_vs_quadrupole_same_theta"] = q["rms_rel"] / same["rms_rel"] if same else None
     t_plain = time_at_error(plain, a.target_rms)
     t_nn = time_at_error(nn, a.target_rms)
+    t_start = time_at_error(start, a.target_rms)
     plain_rel = [run(1.0, False, float(x)) for x in a.rel_alphas.split(",") if x]
-    nn_rel = [run(1.0, True, float(x)) for x in a.nn_rel_alphas.split(",") if x]
+    nn_rel = [run(1.0, True, al) for al in nn_rel_alphas]
     t_prel = time_at_error(plain_rel, a.target_rms) if plain_rel else None
     t_nrel = time_at_error(nn_rel, a.target_rms) if nn_rel else None
+    t_srel = time_at_error(start_rel, a.target_rms) if start_rel else None
 
     def ratio(x, y):
         return (x / y) if (x and y) else None
@@ -211,21 +219,27 @@ def main():
                      "loss_end": float(fit["loss_end"]), "validation_theta": th_val,
                      "validation_rms_start": val_start, "validation_rms_trained": val_end,
                      "trained_network_kept": kept, "train_ms": train_ms},
-        "init": a.init,
+        "init_seed": a.seed,
         "plain": plain,
         "nn_start": start,
         "nn": nn,
         "target_rms": a.target_rms,
         "plain_ms_at_target": t_plain,
+        "quadrupole_ms_at_target": t_start,
         "nn_ms_at_target": t_nn,
-        "speedup_at_target": (t_plain / t_nn) if (t_plain and t_nn) else None,
+        "speedup_at_target": ratio(t_plain, t_nn),
+        "quadrupole_speedup_at_target": ratio(t_plain, t_start),
+        "trained_vs_quadrupole_at_target": ratio(t_start, t_nn),
         "relative_criterion": {
             "plain": plain_rel,
+            "nn_start": start_rel,
             "nn": nn_rel,
             "plain_ms_at_target": t_prel,
+            "quadrupole_ms_at_target": t_srel,
             "nn_ms_at_target": t_nrel,
-            "nn_speedup_vs_plain_geometric": ratio(t_plain, t_nrel),
             "nn_speedup_vs_plain_relative": ratio(t_prel, t_nrel),
+            "trained_vs_quadrupole_relative": ratio(t_srel, t_nrel),
+            "nn_speedup_vs_plain_geometric": ratio(t_plain, t_nrel),
         },
     }
     print(json.dumps(result, indent=1))

@@ -140,9 +140,10 @@ def test_mlp_parameter_layout_matches_library():
 
     L = _lib.lib()
     assert L.pnb_mlp_num_params() == learned.IN * 64 + 64 + 64 * 64 + 64 + 64 * learned.OUT + learned.OUT
-    flat = learned.quadrupole_params()
+    flat = np.random.default_rng(0).normal(size=learned.num_params()).astype(np.float32)
     back = learned.dict_to_params(learned.params_to_dict(flat))
     np.testing.assert_array_equal(back, flat)
+    assert not learned.quadrupole_params().any()  # zero output layer: the residual alone
 
 
 def test_tree_node_record_size():

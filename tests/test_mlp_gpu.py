@@ -34,9 +34,10 @@ def random_dataset(seed, waves=40, rows_per_wave=50):
     feats = np.zeros((rows, learned.FEAT), np.float32)
     feats[:, :NF] = rng.normal(size=(rows, NF))
     u = rng.normal(size=(rows, 3))
-    geom = np.zeros((rows, 4), np.float32)
+    geom = np.zeros((rows, learned.GEOM), np.float32)
     geom[:, :3] = 3.0 * u / np.linalg.norm(u, axis=1, keepdims=True)
-    geom[:, 3] = rng.uniform(0.5, 2.0, rows)
+    geom[:, 3] = rng.uniform(0.5, 2.0, rows)  # M w^2
+    geom[:, 4] = geom[:, 3] * rng.uniform(0.5, 1.5, rows)  # M w^3
     lanes = np.zeros((waves, 64, 4), np.float32)
     lanes[..., :3] = rng.normal(size=(waves, 64, 3)) * 0.3
     lanes[..., 3] = 1e-4
@@ -130,9 +131,10 @@ def test_collect_rows_and_targets(res):
     np.testing.assert_allclose(f[:, 0] + f[:, 1] + f[:, 2], 0.0, atol=1e-5)  # traceless
     assert np.all((f[:, 6] > 0) & (f[:, 6] <= 1.0 + 1e-6))  # mass fraction
     assert np.all(f[:, 7] >= 1.0 / 32.0 - 1e-6)             # at least two bodies
-    assert np.all((f[:, 9] > 0) & (f[:, 9] <= 1.0 + 1e-6))  # heaviest child's share
-    assert np.all((f[:, 10] >= 0.125 - 1e-6) & (f[:, 10] <= 1.0))  # 1..8 children
-    assert not f[:, NF:].any()
+    # normalised octupole (5/2) U~ / (M w^3): every body is within sqrt(3) w of the centre of
+    # mass, so |U_ijk| <= 3^1.5 M w^3, |V_k| <= 3^2.5 M w^3 and |U~_ijk| <= 3^1.5 + 3^2.5 * 3 / 5
+    assert np.all(np.abs(f[:, 9:16]) < 2.5 * (3**1.5 + 0.6 * 3**2.5))
+    assert np.abs(f[:, 9:16]).max() > 0
     np.testing.assert_allclose(d["lanes"][..., 3], 0.0)  # softening^2
     # targets are brute force minus Barnes-Hut: |target| relative to |a| is a monopole-sized error
     a_ref = 1.0 / np.sqrt(d["wgt"][d["wgt"] > 0])
@@ -189,10 +191,11 @@ def test_collected_rows_match_cpu_walk(res):
         assert [int(m) for m in d["mask"][b:e]] == [m for _, m in want], w
         for r, (k, m) in zip(range(b, e), want):
             n = nodes[k]
-            np.testing.assert_allclose(d["feats"][r, :NF], F.node_features(nodes, k),
+            np.testing.assert_allclose(d["feats"][r, :NF], F.node_features(nodes, k, sp),
                                        rtol=2e-5, atol=2e-6)
             np.testing.assert_array_equal(d["geom"][r, :3], n["com"])
             assert abs(d["geom"][r, 3] / (float(n["mass"]) * float(n["width"]) ** 2) - 1) < 1e-6
+            assert abs(d["geom"][r, 4] / (float(n["mass"]) * float(n["width"]) ** 3) - 1) < 1e-6
 
 
 def test_node_quadrupoles_match_particles(res):
@@ -216,6 +219,41 @@ def test_node_quadrupoles_match_particles(res):
         assert err < 1e-5 + 1e-5 * np.abs(want).max(), (k, n["quad"], want)
     assert not nodes["quad"][nodes["internal"] == 0].any()
     print("worst quadrupole error", worst)
+
+
+@pytest.mark.parametrize("case", ["two_plummer", "uniform_with_massless"])
+def test_node_features_match_bodies(res, case):
+    """Every input of the node network (pnb_bh_node_features) against FP64 sums over the node's
+    own bodies (oracle/features.py): mass share, quadrupole, and the normalised octupole from
+    third moments summed body by body -- independent of the bottom-up moments the library builds
+    them from. Structure (ranges, levels) comes from the export, which the tree tests pin."""
+    import torch
+
+    if case == "two_plummer":
+        p = workloads.two_plummer_case(8192, seed=48).particles
+    else:
+        p = workloads.direct_sum_case(6000, seed=49).particles.copy()
+        p[::4, 3] = 0.0  # massless bodies inside massive nodes, and some massless nodes
+    dev = torch.from_numpy(p).cuda()
+    learned.init(res, 1)
+    pa.BarnesHut(res, 0.5, pa.Acceleration.checked(), mlp=True).compute(pa.Between(dev, dev))
+    nodes, order = pa.tree_export(res, p.shape[0])
+    got = learned.node_features(res)
+    assert got.shape == (nodes.shape[0], learned.FEAT)
+    assert not got[nodes["internal"] == 0].any()
+    sp = p[order]
+    internal = np.flatnonzero(nodes["internal"] == 1)
+    rng = np.random.default_rng(0)
+    pick = np.unique(np.concatenate([internal[:40], internal[-40:],
+                                     rng.choice(internal, 300, replace=False)]))
+    worst = np.zeros(16)
+    for k in pick:
+        want = F.node_features(nodes, int(k), sp)
+        err = np.abs(got[k] - want)
+        worst = np.maximum(worst, err)
+        np.testing.assert_allclose(got[k], want, rtol=1e-5, atol=2e-6, err_msg=str(k))
+    print("worst feature error", worst)
+    assert np.abs(got[pick, 9:16]).max() > 1e-2  # the octupole features are not all zero
 
 
 def test_walk_correction_matches_rows(res):
@@ -264,13 +302,40 @@ def test_quadrupole_network_reduces_error(res, orc):
         assert rms(quad) < 0.5 * rms(plain)
 
 
-def test_untrained_network_leaves_walk_unchanged(res):
-    """A fresh network has a zero output layer: the corrected walk is plain Barnes-Hut, bit for bit."""
+def test_octupole_network_reduces_error_below_the_quadrupole(res, orc):
+    """The network set to pass each node's own normalised octupole to its octupole outputs
+    (learned.octupole_params) adds the multipole expansion's octupole term: at the same theta its
+    error against FP64 must fall well below the quadrupole term's."""
+    s = workloads.two_plummer_case(32768, seed=47)
+    idx = np.random.default_rng(1).choice(s.n, 512, replace=False)
+    ref = orc.direct_f64(s.particles[idx], s.particles)
+
+    def rms(a):
+        e = np.linalg.norm(a[idx, :3] - ref, axis=1) / np.linalg.norm(ref, axis=1)
+        return float(np.sqrt(np.mean(e**2)))
+
+    # the terms beyond the octupole grow with theta: measured 0.42 at 0.5, 0.67 at 0.8
+    for theta, bar in ((0.5, 0.5), (0.8, 0.75)):
+        learned.set_params(res, learned.quadrupole_params())
+        quad = pa.BarnesHut(res, theta, pa.Acceleration.checked(), mlp=True).compute(s.particles)
+        learned.set_params(res, learned.octupole_params())
+        octu = pa.BarnesHut(res, theta, pa.Acceleration.checked(), mlp=True).compute(s.particles)
+        print(f"theta {theta}: quadrupole rms {rms(quad):.3e}, octupole rms {rms(octu):.3e}")
+        assert rms(octu) < bar * rms(quad)
+
+
+def test_untrained_network_is_the_quadrupole_term(res):
+    """A fresh network has a zero output layer, so it adds nothing to its residual: the corrected
+    walk is quadrupole Barnes-Hut (the all-zero learned.quadrupole_params), bit for bit, and not
+    plain Barnes-Hut."""
     s = workloads.direct_sum_case(8192, seed=44)
     learned.init(res, 3)
-    plain = pa.BarnesHut(res, 0.7, pa.Acceleration.checked()).compute(s.particles)
     nn0 = pa.BarnesHut(res, 0.7, pa.Acceleration.checked(), mlp=True).compute(s.particles)
-    np.testing.assert_array_equal(nn0, plain)
+    learned.set_params(res, learned.quadrupole_params())
+    quad = pa.BarnesHut(res, 0.7, pa.Acceleration.checked(), mlp=True).compute(s.particles)
+    plain = pa.BarnesHut(res, 0.7, pa.Acceleration.checked()).compute(s.particles)
+    np.testing.assert_array_equal(nn0, quad)
+    assert np.abs(nn0 - plain).max() > 0
 
 
 def test_mlp_flag_needs_initialised_network():
@@ -284,12 +349,14 @@ def test_mlp_flag_needs_initialised_network():
         r.close()
 
 
-def test_trained_network_beats_plain_barnes_hut(res, orc):
-    """The whole learned path from scratch: a fresh network (zero output layer, i.e. plain
-    Barnes-Hut) trained with the GPU forward/backward/Adam kernels on the brute-force minus
-    Barnes-Hut residuals the walk itself collects, then judged on a held-out snapshot against the
-    FP64 direct sum. It must cut the error of the method without it well down, towards what the
-    node's own quadrupole gives."""
+def test_trained_network_from_the_quadrupole_start(res, orc):
+    """The whole learned path: a fresh network (zero output layer, i.e. the quadrupole term)
+    trained with the GPU forward/backward/Adam kernels on the brute-force minus Barnes-Hut
+    residuals the walk itself collects, then judged on a held-out snapshot against the FP64 direct
+    sum. Its octupole outputs start at zero, so whatever it gains over the quadrupole term is
+    learned from the residuals: measured 0.63x (theta 0.5) and 0.70x (theta 0.8) of the
+    quadrupole term's error after 300 steps on three snapshots (the 4M-body comparison is
+    nn_bh's)."""
     import torch
 
     n = 16384
@@ -301,7 +368,8 @@ def test_trained_network_beats_plain_barnes_hut(res, orc):
         for theta in (0.5, 0.8):
             learned.collect(res, dev, theta, waves=n // 64, seed=seed)
     out = learned.fit(res, 300)
-    assert out["kept"] and out["loss_end"] < 0.5 * out["loss_start"], out["loss_end"]
+    print("training loss", out["loss_start"], "->", out["loss_end"], "kept", out["kept"])
+    assert out["kept"] and out["loss_end"] < 0.7 * out["loss_start"]
     trained = learned.get_params(res)
 
     h = workloads.two_plummer_case(n, seed=80)
@@ -320,4 +388,4 @@ def test_trained_network_beats_plain_barnes_hut(res, orc):
         quad = pa.BarnesHut(res, theta, pa.Acceleration.checked(), mlp=True).compute(h.particles)
         print(f"theta {theta}: plain {rms(plain):.3e}, trained {rms(nn):.3e}, "
               f"quadrupole {rms(quad):.3e}")
-        assert rms(nn) < bar * rms(plain)
+        assert rms(nn) < bar * rms(plain) and rms(nn) < 0.85 * rms(quad)

@@ -13,10 +13,11 @@ def problem(seed=0, waves=3, rows_per_wave=9):
     x = rng.normal(size=(rows, M.IN))
     wave = np.repeat(np.arange(waves), rows_per_wave)
     mask = rng.integers(0, 2**63, rows, dtype=np.int64).astype(np.uint64) | np.uint64(1)
-    geom = np.zeros((rows, 4))
+    geom = np.zeros((rows, 8))
     u = rng.normal(size=(rows, 3))
     geom[:, :3] = 3.0 * u / np.linalg.norm(u, axis=1, keepdims=True)
     geom[:, 3] = rng.uniform(0.5, 2.0, rows)  # M w^2
+    geom[:, 4] = geom[:, 3] * rng.uniform(0.5, 1.5, rows)  # M w^3
     lanes = np.zeros((waves, 64, 4))
     lanes[..., :3] = rng.normal(size=(waves, 64, 3)) * 0.3
     lanes[..., 3] = 1e-4
@@ -48,7 +49,9 @@ def test_aggregation_matches_loop():
         for lane in range(64):
             if (int(mask[r]) >> lane) & 1:
                 lp = lanes[wave[r], lane]
-                ref[wave[r], lane] += F.quad_field(geom[r, 3] * y[r], geom[r, :3] - lp[:3], lp[3])
+                d = geom[r, :3] - lp[:3]
+                ref[wave[r], lane] += (F.quad_field(geom[r, 3] * y[r, :6], d, lp[3])
+                                       + F.oct_field(geom[r, 4] * y[r, 6:], d, lp[3]))
     assert np.abs(ref).max() > 0
     np.testing.assert_allclose(pred, ref, rtol=1e-12, atol=1e-13)
 
