@@ -12,8 +12,9 @@ the timed region, 20 flop per interaction, against the 157.3 TFLOP/s FP32 peak. 
 times the oracle's restatement of parallel::BruteForce (same per-particle fold, all host threads
 up to 16) on a bounded sample of affected particles against the full massive set.
 
-Further entries of the same line: `barnes_hut` (two Plummer spheres of 4,194,304 bodies: plain and
-learned-correction walks across angles and the relative criterion, the equal-error comparison,
+Further entries of the same line: `barnes_hut` (two Plummer spheres of 4,194,304 bodies: plain
+walks and corrected walks with three networks -- the quadrupole term, the trained network and the
+analytic octupole term -- across angles and the relative criterion, the equal-error comparison,
 the walk's roofline against the brute-force pair rate with counters from profiles/, the CPU
 baseline of parallel::BarnesHut), `batch` (2,048 systems, 100 KDK steps in one hipGraph),
 `all_massive` (1,048,576 bodies, one-sided brute force against BruteForcePairs) and
@@ -277,11 +278,12 @@ def tree_size(res) -> int:
 
 def bench_bh(args):
     """Barnes-Hut on two Plummer spheres (4,194,304 bodies, mu = 1/N): plain at theta 0.2 ... 0.8
-    and with the learned correction at theta 0.3 / 0.4 (the network that outputs each node's own
-    quadrupole, learned.quadrupole_params), both with the reference's geometric criterion; then
-    both with the relative criterion (a node also opens when its first neglected term exceeds
-    alpha times the particle's estimated acceleration; theta = 1 as geometric floor) at several
-    alpha. Tree build + walk (+ node network) device time from HIP events (the relative walk's
+    and with the learned correction under three networks -- the quadrupole term (the zero-output
+    start of training, learned.quadrupole_params), the trained network (learned.trained_params,
+    from particular_amd.nn_bh) and the analytic octupole term (learned.octupole_params) -- with
+    the reference's geometric criterion; then each with the relative criterion (a node also opens
+    when its first neglected term exceeds alpha times the particle's estimated acceleration;
+    theta = 1 as geometric floor) at several alpha. Tree build + walk (+ node network) device time from HIP events (the relative walk's
     time includes its estimate pass), accepted pair terms and wavefront node steps from the walk's
     counters, accuracy against the FP64 direct sum on a seeded sample, and the particles/s ratios
     at equal rms error (1e-4): each curve is also run at the knob (theta or alpha) where its error
@@ -306,7 +308,16 @@ def bench_bh(args):
     torch.cuda.synchronize()
     ref = ref.cpu().numpy()
     learned.init(res, 0)
-    learned.set_params(res, learned.quadrupole_params())
+    trained = learned.trained_params()
+    networks = {"quadrupole": learned.quadrupole_params(), "octupole_reference": learned.octupole_params()}
+    if trained is not None:
+        networks["trained"] = trained
+    current = {"name": None}
+
+    def use(name):
+        if current["name"] != name:
+            learned.set_params(res, networks[name])
+            current["name"] = name
 
     def run(knob, mlp, rel=False):
         theta, alpha = (1.0, knob) if rel else (knob, None)
@@ -337,6 +348,7 @@ def bench_bh(args):
             "alpha": alpha,
             "knob": knob,
             "learned_correction": mlp,
+            "network": current["name"] if mlp else None,
             "mparticles_per_s": s.n / wall / 1e6,
             "ms_per_eval": wall * 1e3,
             "build_ms": build_ms,
@@ -350,7 +362,7 @@ def bench_bh(args):
             "accepted_G_terms_per_s": accepted / (walk_ms * 1e-3) / 1e9,
             **err,
         }
-        log(f"BH{' + correction' if mlp else ''} theta={theta}"
+        log(f"BH{(' + ' + current['name']) if mlp else ''} theta={theta}"
             f"{f' alpha={alpha}' if rel else ''}: {row['ms_per_eval']:.2f} ms "
             f"({row['mparticles_per_s']:.1f} Mp/s), build {build_ms:.2f} ms, walk {walk_ms:.2f} ms, "
             f"rms {err['rms_rel']:.2e}")
@@ -358,10 +370,12 @@ def bench_bh(args):
 
     target = 1e-4
 
-    def refine(rows, mlp, rel=False):
+    def refine(rows, mlp, rel=False, net=None):
         """Adds runs at the knob (theta, or alpha for the relative criterion) where the rms
         error meets the target: secant steps of log(rms) against log(knob) between the closest
         bracketing rows (at most two)."""
+        if net:
+            use(net)
         for _ in range(2):
             lo = [r for r in rows if r["rms_rel"] <= target]
             hi = [r for r in rows if r["rms_rel"] > target]
@@ -379,12 +393,21 @@ def bench_bh(args):
         return [float(v) for v in text.split(",") if v]
 
     plain = [run(th, False) for th in knobs(args.bh_thetas)]
-    nn = [run(th, True) for th in knobs(args.bh_nn_thetas)]
     plain_rel = [run(a, False, True) for a in knobs(args.bh_rel_alphas)]
-    nn_rel = [run(a, True, True) for a in knobs(args.bh_nn_rel_alphas)]
-    for rows, mlp, rel in ((plain, False, False), (nn, True, False), (plain_rel, False, True),
-                           (nn_rel, True, True)):
-        refine(rows, mlp, rel)
+    refine(plain, False)
+    refine(plain_rel, False, True)
+    curves = {}
+    for net, th_arg, al_arg in (("quadrupole", args.bh_quad_thetas, args.bh_quad_rel_alphas),
+                                ("trained", args.bh_nn_thetas, args.bh_nn_rel_alphas),
+                                ("octupole_reference", args.bh_oct_thetas, args.bh_oct_rel_alphas)):
+        if net not in networks:
+            continue
+        use(net)
+        geo = [run(th, True) for th in knobs(th_arg)]
+        rel = [run(a, True, True) for a in knobs(al_arg)]
+        refine(geo, True, net=net)
+        refine(rel, True, True, net=net)
+        curves[net] = (geo, rel)
     res.close()
 
     def at(rows):
@@ -393,28 +416,44 @@ def bench_bh(args):
     def ratio(a, b):
         return (a / b) if (a and b) else None
 
-    t_plain, t_nn, t_prel, t_nrel = at(plain), at(nn), at(plain_rel), at(nn_rel)
+    t_plain, t_prel = at(plain), at(plain_rel)
+    per_net = {}
+    for net, (geo, rel) in curves.items():
+        t_g, t_r = at(geo), at(rel)
+        per_net[net] = {
+            "ms_at_target": t_g,
+            "mparticles_per_s_at_target": s.n / t_g / 1e3 if t_g else None,
+            "speedup_vs_plain": ratio(t_plain, t_g),
+            "relative_criterion_ms_at_target": t_r,
+            "relative_criterion_speedup_vs_plain_relative": ratio(t_prel, t_r),
+            "relative_criterion_speedup_vs_plain_geometric": ratio(t_plain, t_r),
+        }
+    q, t = per_net.get("quadrupole", {}), per_net.get("trained", {})
     comparison = {
         "target_rms": target,
         "plain_ms_at_target": t_plain,
-        "corrected_ms_at_target": t_nn,
         "plain_mparticles_per_s_at_target": s.n / t_plain / 1e3 if t_plain else None,
-        "corrected_mparticles_per_s_at_target": s.n / t_nn / 1e3 if t_nn else None,
-        "speedup_at_equal_error": ratio(t_plain, t_nn),
-        "relative_criterion": {
-            "plain_ms_at_target": t_prel,
-            "corrected_ms_at_target": t_nrel,
-            "corrected_mparticles_per_s_at_target": s.n / t_nrel / 1e3 if t_nrel else None,
-            "speedup_vs_plain_geometric": ratio(t_plain, t_nrel),
-            "speedup_vs_plain_relative": ratio(t_prel, t_nrel),
-            "plain_relative_vs_plain_geometric": ratio(t_plain, t_prel),
-        },
-        "network": "learned.quadrupole_params (outputs each node's own quadrupole; on held-out "
-                   "validation the trained networks did not beat it, see particular_amd/nn_bh.py)",
+        "plain_relative_ms_at_target": t_prel,
+        "plain_relative_vs_plain_geometric": ratio(t_plain, t_prel),
+        "networks": per_net,
+        # the learned network against its own starting point, each at the same error and with the
+        # same criterion
+        "trained_vs_quadrupole": ratio(q.get("ms_at_target"), t.get("ms_at_target")),
+        "trained_vs_quadrupole_relative_criterion": ratio(q.get("relative_criterion_ms_at_target"),
+                                                          t.get("relative_criterion_ms_at_target")),
+        "speedup_at_equal_error": t.get("speedup_vs_plain"),
+        "network": ("trained: particular_amd/params/two_plummer_4M.npy (particular_amd.nn_bh, 64 "
+                    "snapshots; record beside it); quadrupole: its zero-output start "
+                    "(learned.quadrupole_params); octupole_reference: the analytic octupole term "
+                    "through the same network (learned.octupole_params)"),
     }
-    log(f"equal error {target}: plain {t_plain} ms, corrected {t_nn} ms; relative criterion: "
-        f"plain {t_prel} ms, corrected {t_nrel} ms")
-    return plain + nn + plain_rel + nn_rel, comparison, s
+    log(f"equal error {target}: plain {t_plain} ms; " + "; ".join(
+        f"{k} {v['ms_at_target']} ms (relative {v['relative_criterion_ms_at_target']} ms)"
+        for k, v in per_net.items()))
+    rows = plain + plain_rel
+    for geo, rel in curves.values():
+        rows += geo + rel
+    return rows, comparison, s
 
 
 def cpu_baseline_bh(s, theta, sample):
@@ -592,12 +631,20 @@ def main():
     ap.add_argument("--no-bh", dest="bh", action="store_false")
     ap.add_argument("--bh-n", type=int, default=4194304)
     ap.add_argument("--bh-thetas", default="0.2,0.25,0.3,0.5,0.8", help="plain Barnes-Hut angles")
-    ap.add_argument("--bh-nn-thetas", default="0.3,0.4",
-                    help="angles with the learned correction (empty: none)")
+    ap.add_argument("--bh-quad-thetas", default="0.3,0.4",
+                    help="angles with the quadrupole network (empty: none)")
+    ap.add_argument("--bh-quad-rel-alphas", default="2e-4,4e-4",
+                    help="relative criterion with the quadrupole network (empty: none)")
+    ap.add_argument("--bh-oct-thetas", default="0.4,0.5",
+                    help="angles with the analytic octupole network (empty: none)")
+    ap.add_argument("--bh-oct-rel-alphas", default="3e-4,6e-4",
+                    help="relative criterion with the analytic octupole network (empty: none)")
+    ap.add_argument("--bh-nn-thetas", default="0.4,0.5",
+                    help="angles with the trained network (empty: none)")
     ap.add_argument("--bh-rel-alphas", default="3e-5,1e-4,2e-4",
                     help="plain Barnes-Hut with the relative criterion (empty: none)")
-    ap.add_argument("--bh-nn-rel-alphas", default="1e-4,3e-4,5e-4",
-                    help="relative criterion with the learned correction (empty: none)")
+    ap.add_argument("--bh-nn-rel-alphas", default="2e-4,4e-4",
+                    help="relative criterion with the trained network (empty: none)")
     ap.add_argument("--bh-cpu-sample", type=int, default=16384)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="first copy a buffer of this size with torch (HBM counter calibration)")

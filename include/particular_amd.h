@@ -166,7 +166,9 @@ int pnb_bf_clear_tuning(pnb_ctx *ctx);
  * consecutive particles of its wavefront in the walk's spatial order (1/64 of the cost, the same
  * rms error), and p the order of the first term the walk leaves out:
  * 2 for the plain walk (the quadrupole), 3 with PNB_BH_MLP (whose learned moment stands in for
- * the quadrupole). Each node's error is then bounded relative to the particle's own acceleration;
+ * the quadrupole) and 4 with PNB_BH_MLP when the network has octupole outputs (a non-zero weight
+ * or bias of outputs 6..12, or any training). Each node's error is then bounded relative to the
+ * particle's own acceleration;
  * `theta` stays a geometric floor (1 opens every node whose width exceeds the distance). alpha
  * and estimate_theta come from pnb_bh_set_relative (defaults 1e-4 and 1.2). Combines with the
  * other flags. */
@@ -314,14 +316,14 @@ int pnb_batch_info(pnb_ctx *ctx, size_t *particles, size_t *items,
  * accumulate) corrects the monopole term of every internal node a lane accepts in the Barnes-Hut
  * walk (pnb_bh_accel_f32* with PNB_BH_MLP). Its input describes the node (pnb_bh_node_features):
  * f0..5 the normalised quadrupole Q = (3T - tr(T) I) / (M w^2) (see pnb_tree_node),
- * f6 1 + log2(mass / root mass) / 32, f7 log2(bodies) / 32, f8 depth / 64, f9..15 the normalised
- * octupole (5/2) U~ / (M w^3), U~ the traceless part of the third moments about the centre of mass,
- * as {xxx, yyy, zzz, xxy, xxz, xyy, xyz}. Its outputs are two fields' coefficients: y[0..5] =
+ * f6 1 + log2(mass / root mass) / 32, f7 log2(bodies) / 32, f8 depth / 64, f9..15 32 times the
+ * normalised octupole (5/2) U~ / (M w^3), U~ the traceless part of the third moments about the
+ * centre of mass, as {xxx, yyy, zzz, xxy, xxz, xyy, xyz}. Its outputs are two fields' coefficients: y[0..5] =
  * f0..5 + MLP(f)[0..5] (a residual on the quadrupole) is a second moment Q in units of M w^2 and
  * y[6..12] = MLP(f)[6..12] (learned from zero) a traceless third moment O in units of M w^3 (same
  * seven components). Every lane that accepts the node adds, with d = centre of mass - lane,
  * a = -Q d |d|^-5 + 2.5 (d.Q.d) d |d|^-7 + 3 F |d|^-7 - 7 (d.F) d |d|^-9 with F_i = O_ijk d_j d_k;
- * with y[6..12] = f9..15 the last two terms are the octupole term of the node's multipole
+ * with y[6..12] = f9..15 / 32 the last two terms are the octupole term of the node's multipole
  * expansion. The network runs once per node after the octree build. Training data are generated
  * by the method itself: per-lane targets are brute force minus Barnes-Hut. */
 

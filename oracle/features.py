@@ -102,14 +102,17 @@ def oct_field(o7, d, eps2: float = 0.0) -> np.ndarray:
     return 3.0 * F / r**7 - 7.0 * (d @ F) * d / r**9
 
 
+OCT_FEATURE_SCALE = 32.0  # the network reads 32 x the normalised octupole as f9..15
+
+
 def node_features(nodes, k: int, sp: np.ndarray | None = None) -> np.ndarray:
     """Features f[0..15] of node k of an exported tree (zeros for a leaf or a massless node).
 
     Without `sp`, f0..8 from the exported node data (f9..15 left zero). With `sp` (the bodies in
     tree order, x y z mu), every feature from FP64 sums over the node's own bodies, the tree's
     structure (range, level) taken from the export: the mass share, the quadrupole and the
-    normalised octupole (f9..15, octupole7 of third_moments) are then computed independently of
-    the library's bottom-up moments."""
+    normalised octupole (f9..15 = 32 x octupole7 of third_moments) are then computed independently
+    of the library's bottom-up moments."""
     n = nodes[k]
     f = np.zeros(16)
     if not n["internal"] or not n["mass"] > 0:
@@ -125,7 +128,7 @@ def node_features(nodes, k: int, sp: np.ndarray | None = None) -> np.ndarray:
         w = float(n["width"])
         f[0:6] = quadrupole(own[:, :3], own[:, 3], w)
         f[6] = 1.0 + np.log2(max(M / root_mass, 1e-30)) / 32.0
-        f[9:16] = octupole7(third_moments(own[:, :3], own[:, 3]), M, w)
+        f[9:16] = OCT_FEATURE_SCALE * octupole7(third_moments(own[:, :3], own[:, 3]), M, w)
     f[7] = np.log2(float(n["count"])) / 32.0
     f[8] = float(n["level"]) / 64.0
     return f

@@ -435,8 +435,8 @@ class BarnesHut(_Algorithm):
                  estimate_theta: float = 1.2, estimate_per_particle: bool = False,
                  bucket: int = 24):
         """``relative=alpha`` adds the relative opening criterion (no reference counterpart): a
-        node is also opened when its first neglected term, M w^p / r^(p+2) (p = 2, or 3 with
-        ``mlp``), exceeds alpha times the acceleration from an estimate pass at
+        node is also opened when its first neglected term, M w^p / r^(p+2) (p = 2; with ``mlp``
+        3, or 4 when the network has octupole outputs), exceeds alpha times the acceleration from an estimate pass at
         ``estimate_theta`` (at the centroid of the particle's wavefront, or at the particle with
         ``estimate_per_particle``); ``theta`` then acts as a geometric floor, and an opened node
         of at most ``bucket`` bodies is summed body by body."""

@@ -101,6 +101,7 @@ struct __align__(32) NodeDesc {
     float oct[7];
     float pad;
 };
+constexpr float OCT_FEATURE_SCALE = 32.f;  // network input f9..15 = 32 x NodeDesc::oct
 
 struct BhInfo {
     float mn[3], mx[3];  // root box
@@ -131,14 +132,14 @@ struct BhState {
     DevBuf ilist;            // internal nodes grouped by level (bottom-up pass)
     DevBuf lvl;              // padded per-level counts and list cursors (2 x LVL_SLOTS lines)
     DevBuf mom, quad, desc;  // second moments (FP64), normalised quadrupoles, child shares
-    DevBuf qh;               // per node: MLP(node) as halves, the learned correction's moment
+    DevBuf qh, oh;           // per node: MLP(node) as halves, the learned correction's moments
     bool has_moments = false;
     // relative opening criterion (PNB_BH_RELATIVE): walk records carrying M w^3, per affected
     // particle alpha |a| from the estimate pass, and the criterion's parameters
     DevBuf rrec, est, cen, cen_spread, est_spread;
     DevBuf wkey, wkey_alt, wval, wval_alt;  // wavefronts ordered by their estimate
     bool rrec_ok = false;
-    int rrec_cubic = 0, rrec_bucket = 0;
+    int rrec_power = 0, rrec_bucket = 0;
     int rel_bucket = 24;  // nodes of at most this many bodies are summed directly when opened
     float rel_alpha = 1e-4f, rel_theta_est = 1.2f;
     bool rel_per_particle = false;  // else one estimate per wavefront, at its centroid
@@ -165,7 +166,7 @@ void bh_free(BhState* s) {
     if (!s) return;
     for (DevBuf* b : {&s->keys, &s->keys_alt, &s->idx, &s->idx_alt, &s->sp, &s->lcp, &s->cnt,
                       &s->ip, &s->key2, &s->rec, &s->sums, &s->level, &s->width, &s->skipv,
-                      &s->ilist, &s->lvl, &s->mom, &s->quad, &s->desc, &s->qh, &s->rrec, &s->est,
+                      &s->ilist, &s->lvl, &s->mom, &s->quad, &s->desc, &s->qh, &s->oh, &s->rrec, &s->est,
                       &s->cen, &s->cen_spread, &s->est_spread, &s->wkey, &s->wkey_alt, &s->wval, &s->wval_alt,
                       &s->info, &s->bbox_part, &s->hist, &s->bits, &s->afg_tmp, &s->akeys,
                       &s->akeys_alt, &s->aidx, &s->aidx_alt, &s->apts, &s->h_aff, &s->h_afg,
@@ -1007,14 +1008,15 @@ __global__ __launch_bounds__(256) void bh_ref_com_internal_kernel(NodeRec* __res
 //   f6     1 + log2(node mass / root mass) / 32
 //   f7     log2(bodies in the node) / 32
 //   f8     depth / 64
-//   f9..15 normalised octupole (5/2) U~ / (M w^3) (NodeDesc::oct)
+//   f9..15 normalised octupole (5/2) U~ / (M w^3) (NodeDesc::oct) times 32 (its entries are
+//          about 0.01, the other features' about 0.1 to 1)
 // The network's outputs are two correction fields: y[0..5] = f0..5 + MLP(f)[0..5] is an
 // effective second moment in units of M w^2 (a residual on the quadrupole) and y[6..12] =
 // MLP(f)[6..12] an effective traceless third moment in units of M w^3, learned from zero; every
 // lane that accepts the node adds quad_field(M w^2 y[0..5]) + oct_field(M w^3 y[6..12]) at its
 // own position. With a zero output layer (a fresh network) this is the quadrupole term of the
-// node's multipole expansion; the octupole term (y[6..12] = f9..15) and whatever else lowers the
-// brute-force residual is what training adds.
+// node's multipole expansion; the octupole term (y[6..12] = f9..15 / 32) and whatever else lowers
+// the brute-force residual is what training adds.
 
 __device__ __forceinline__ void node_features(const NodeRec& r, const NodeQuad& q, uint8_t level,
                                               const NodeDesc& dsc, float root_mass,
@@ -1033,17 +1035,27 @@ __device__ __forceinline__ void node_features(const NodeRec& r, const NodeQuad& 
     f[7] = __builtin_amdgcn_logf(float(r.end - r.begin)) * (1.f / 32.f);
     f[8] = float(level) * (1.f / 64.f);
 #pragma unroll
-    for (int i = 0; i < 7; ++i) f[9 + i] = dsc.oct[i];
+    for (int i = 0; i < 7; ++i) f[9 + i] = OCT_FEATURE_SCALE * dsc.oct[i];
 }
 
-// The learned moments of a node, y[0..12] (quadrupole in units of M w^2, octupole in units of
-// M w^3), as halves with three zero halves of padding (32 bytes, one scalar load when a lane
-// accepts the node). y is O(1) and the correction it feeds is itself a small fraction of the
-// force, so 11-bit mantissas cost nothing measurable, while the walk's scalar loads take half of
-// what floats would.
-struct __align__(32) QuadH {
-    __half q[16];
+// The learned moments of a node, in two entries read with scalar loads when a lane accepts it.
+// QuadH (16 bytes): the quadrupole y[0..5] (units of M w^2) as halves and a flag, 1 when the
+// node's octupole (OctH) is worth adding: its largest coefficient is at least OCT_MIN, which
+// leaves out nodes whose octupole vanishes (two equal masses) or nearly does, about a third of
+// the accepted ones on Plummer spheres, at no measurable cost in accuracy. y is O(1) and the
+// correction it feeds is itself a small fraction of the force, so 11-bit mantissas cost nothing
+// measurable, while the walk's scalar loads take half of what floats would.
+// OctH (48 bytes): the octupole y[6..12] (units of M w^3) times the node's width w, as all ten
+// components of the traceless tensor {xxx, yyy, zzz, xxy, xxz, xyy, xyz, xzz, yzz, yyz} in FP32,
+// so that the walk multiplies each by a product of d with one FMA (a vector instruction reads one
+// scalar operand here) and needs no square root for w^3.
+struct __align__(16) QuadH {
+    __half q[8];  // y[0..5], flag, 0
 };
+struct __align__(16) OctH {
+    float o[12];  // w * octupole, ten components, 0, 0
+};
+constexpr float OCT_MIN = 3e-3f;
 
 // qh[k] = MLP(features of node k) for the internal nodes listed in inodes (zero for massless
 // ones; the walk reads qh only for internal nodes): 16 nodes per MFMA tile, one tile per
@@ -1055,7 +1067,8 @@ __global__ __launch_bounds__(256) void mlp_node_kernel(const NodeRec* __restrict
                                                        const uint32_t* __restrict__ inodes,
                                                        uint32_t count,
                                                        const MlpWeights* __restrict__ wg,
-                                                       QuadH* __restrict__ qh) {
+                                                       QuadH* __restrict__ qh,
+                                                       OctH* __restrict__ oh) {
     __shared__ MlpForward w;
     {
         const uint32_t* src = reinterpret_cast<const uint32_t*>(wg);
@@ -1090,11 +1103,47 @@ __global__ __launch_bounds__(256) void mlp_node_kernel(const NodeRec* __restrict
 #pragma unroll
         for (int q = 0; q < 4; ++q) yv[q] += g == 0 ? f[q] : (g == 1 && q < 2) ? f[4 + q] : 0.f;
         const float keep = (r.w2 >= 0.0f && r.mass > 0.0f) ? 1.f : 0.f;
-        if (listed) {
-            __half* dst = qh[k].q + 4 * g;
+        // the octupole outputs 6..12 of node r16 gathered in its group-0 lane: 6, 7 from lane
+        // r16 + 16, 8..11 from r16 + 32, 12 from r16 + 48
+        float oc[7];
+        oc[0] = __shfl(yv[2], r16 + 16, 64);
+        oc[1] = __shfl(yv[3], r16 + 16, 64);
+        oc[2] = __shfl(yv[0], r16 + 32, 64);
+        oc[3] = __shfl(yv[1], r16 + 32, 64);
+        oc[4] = __shfl(yv[2], r16 + 32, 64);
+        oc[5] = __shfl(yv[3], r16 + 32, 64);
+        oc[6] = __shfl(yv[0], r16 + 48, 64);
+        if (listed && g == 0) {
+            float big = 0.f;
 #pragma unroll
-            for (int q = 0; q < 4; ++q)  // outputs 13..15: padding
-                dst[q] = __float2half(4 * g + q < MLP_OUT ? keep * yv[q] : 0.f);
+            for (int j = 0; j < 7; ++j) {
+                oc[j] *= keep;
+                big = fmaxf(big, fabsf(oc[j]));
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) qh[k].q[q] = __float2half(keep * yv[q]);
+            qh[k].q[6] = __float2half(big >= OCT_MIN ? 1.f : 0.f);
+            qh[k].q[7] = __float2half(0.f);
+            // w * O, the three dependent components from O_iik = 0
+            const float wd = keep > 0.f ? sqrtf(r.w2) : 0.f;
+            OctH e;
+            e.o[0] = wd * oc[0];  // xxx
+            e.o[1] = wd * oc[1];  // yyy
+            e.o[2] = wd * oc[2];  // zzz
+            e.o[3] = wd * oc[3];  // xxy
+            e.o[4] = wd * oc[4];  // xxz
+            e.o[5] = wd * oc[5];  // xyy
+            e.o[6] = wd * oc[6];  // xyz
+            e.o[7] = -(e.o[0] + e.o[5]);  // xzz
+            e.o[8] = -(e.o[3] + e.o[1]);  // yzz
+            e.o[9] = -(e.o[4] + e.o[2]);  // yyz
+            e.o[10] = 0.f;
+            e.o[11] = 0.f;
+            oh[k] = e;
+        }
+        if (listed && g == 1) {
+            qh[k].q[4] = __float2half(keep * yv[0]);
+            qh[k].q[5] = __float2half(keep * yv[1]);
         }
     }
 }
@@ -1231,7 +1280,9 @@ __device__ __forceinline__ float half_hi(uint32_t v) {
 // the records carry K = M w^p in `begin` (0 for leaves); an internal node is opened when the
 // geometric rule opens it or when the size of its first neglected term, M w^p / r^(p+2)
 // (r^2 = d^2 + eps^2), exceeds A: p = 2 for the plain walk (K > A r^4, the quadrupole is
-// neglected), p = 3 for WALK_QUAD (K / r > A r^4, the octupole is the first term it lacks).
+// neglected), p = 3 for WALK_QUAD (K / r > A r^4, the octupole is the first term it lacks) and
+// p = 4 for WALK_QUAD with `oct` (K / r^2 > A r^4: the network adds octupoles, the hexadecapole
+// is the first term it lacks).
 // REL buckets: a record whose `end` is non-zero packs the first body (27 bits) and the count
 // (5 bits) of a small internal node; lanes that open it add its bodies (`bodies`, the affecting
 // particles in tree order) directly and skip its subtree.
@@ -1245,6 +1296,8 @@ __device__ __forceinline__ float half_hi(uint32_t v) {
 // many short walks instead of a few long ones; every lane's result is the same either way (it
 // adds exactly the nodes of its own pre-order traversal, in that order).
 // dd: a capturable build's tree (`total` is then its capacity; the count is read from `info`).
+// oct (WALK_QUAD): the network has octupole outputs; every accepted internal node's octupole entry
+// is then fetched together with its quadrupole entry, so that the two loads overlap.
 template <int MODE, bool WIDE, bool REL>
 __global__ __launch_bounds__(256) void bh_walk_kernel(const NodeRec* __restrict__ nodes,
                                                       uint32_t total,
@@ -1254,12 +1307,14 @@ __global__ __launch_bounds__(256) void bh_walk_kernel(const NodeRec* __restrict_
                                                       float4* __restrict__ out,
                                                       BhInfo* __restrict__ info, WalkRows rows,
                                                       const QuadH* __restrict__ qh,
+                                                      const OctH* __restrict__ oh,
                                                       const float* __restrict__ est_in,
                                                       uint32_t est_shift,
                                                       float* __restrict__ est_out, float alpha,
                                                       const uint32_t* __restrict__ wave_order,
                                                       const float4* __restrict__ bodies,
-                                                      uint32_t lanes, int dd, int packed) {
+                                                      uint32_t lanes, int dd, int packed,
+                                                      int oct) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (packed) i = (i >> 6) * lanes + (threadIdx.x & 63);
     if (dd) {
@@ -1308,7 +1363,9 @@ __global__ __launch_bounds__(256) void bh_walk_kernel(const NodeRec* __restrict_
         const float r2_rel = REL ? add_rn(d2, eps2) : 0.f;
         const float inv_rel = REL_INV ? __builtin_amdgcn_rsqf(r2_rel) : 0.f;
         const float K = __uint_as_float(r.begin);
-        const float lhs = REL ? (REL_INV ? mul_rn(K, inv_rel) : K) : 0.f;
+        // WALK_QUAD: K / r (p = 3), or K / r^2 (p = 4, when the network adds octupoles)
+        const float lhs = REL ? (REL_INV ? mul_rn(oct ? mul_rn(K, inv_rel) : K, inv_rel) : K)
+                              : 0.f;
         const float rhs = REL ? mul_rn(mul_rn(A, r2_rel), r2_rel) : 0.f;
         const bool crit = mul_rn(theta2, d2) < r.w2 || (REL && rhs < lhs);
         bool open = active && nz && crit;
@@ -1357,39 +1414,49 @@ __global__ __launch_bounds__(256) void bh_walk_kernel(const NodeRec* __restrict_
             float yx, yy, yz;
             if (MODE == WALK_QUAD && __float_as_int(r.w2) >= 0) {  // internal (leaves: -inf)
                 // internal node: the monopole plus the field of Q = M w^2 y[0..5] (quad_field's
-                // formula, w2 = width^2) and of O = M w^3 y[6..12] (oct_field's), summed as
-                // d (s + s5 t - 7 s7 G |d|^-2) - s5 Q d + 3 s7 F in one compensated term
-                const char* e = reinterpret_cast<const char*>(qh) + ko;  // 32-byte entries
-                const uint4 h = *reinterpret_cast<const uint4*>(e);
-                const uint4 h2 = *reinterpret_cast<const uint4*>(e + 16);
+                // formula, w2 = width^2) and, when the node has one, of O = M w^3 y[6..12]
+                // (oct_field's), summed as d (s + s5 t - 7 s7 G |d|^-2) - s5 Q d + 3 s7 F in one
+                // compensated term
+                const uint4 h = *reinterpret_cast<const uint4*>(
+                    reinterpret_cast<const char*>(qh) + (ko >> 1));  // 16-byte entries
+                OctH oe;
+                if (oct)  // issued before h is used, so that both loads are in flight together
+                    oe = *reinterpret_cast<const OctH*>(reinterpret_cast<const char*>(oh) +
+                                                        (ko + (ko >> 1)));  // 48-byte entries
                 const float q[6] = {half_lo(h.x), half_hi(h.x), half_lo(h.y),
                                     half_hi(h.y), half_lo(h.z), half_hi(h.z)};
-                const float o[7] = {half_lo(h.w), half_hi(h.w), half_lo(h2.x), half_hi(h2.x),
-                                    half_lo(h2.y), half_hi(h2.y), half_lo(h2.z)};
                 // fma chains from +0 so that every half-precision product is one v_fma_mix
                 const float qx = fmaf(q[4], dz, fmaf(q[3], dy, fmaf(q[0], dx, 0.f)));
                 const float qy = fmaf(q[5], dz, fmaf(q[1], dy, fmaf(q[3], dx, 0.f)));
                 const float qz = fmaf(q[2], dz, fmaf(q[5], dy, fmaf(q[4], dx, 0.f)));
                 const float dqd = dx * qx + dy * qy + dz * qz;
-                float ob_x[7], ob_y[7], ob_z[7];
-                oct_basis(dx, dy, dz, ob_x, ob_y, ob_z);
-                float fx = 0.f, fy = 0.f, fz = 0.f;
-#pragma unroll
-                for (int j = 0; j < 7; ++j) {
-                    fx = fmaf(o[j], ob_x[j], fx);
-                    fy = fmaf(o[j], ob_y[j], fy);
-                    fz = fmaf(o[j], ob_z[j], fz);
-                }
-                const float gdd = dx * fx + dy * fy + dz * fz;
                 const float inv2 = inv * inv;
                 const float s5 = s * (r.w2 * inv2);  // M w^2 |d|^-5 from the monopole's M |d|^-3
-                // M w^3 |d|^-7 from M w^2 |d|^-5 (w = sqrt(w2))
-                const float s7 = s5 * (__builtin_amdgcn_sqrtf(r.w2) * inv2);
-                const float st = fmaf(-7.f * s7, gdd * inv2, fmaf(s5, 2.5f * dqd * inv2, s));
-                const float s7x3 = 3.f * s7;
-                yx = fmaf(s7x3, fx, fmaf(-s5, qx, fmaf(dx, st, -cx)));
-                yy = fmaf(s7x3, fy, fmaf(-s5, qy, fmaf(dy, st, -cy)));
-                yz = fmaf(s7x3, fz, fmaf(-s5, qz, fmaf(dz, st, -cz)));
+                float st = fmaf(s5, 2.5f * dqd * inv2, s);
+                float ex = -s5 * qx, ey = -s5 * qy, ez = -s5 * qz;
+                if (oct && (h.w & 0xffffu)) {  // the flag (q[6]): this node's octupole counts
+                    // F_i = O_ijk d_j d_k over the ten components of w O (oe.o)
+                    const float xx = dx * dx, yy = dy * dy, zz = dz * dz;
+                    const float xy = 2.f * dx * dy, xz = 2.f * dx * dz, yz = 2.f * dy * dz;
+                    const float* o = oe.o;
+                    const float fx = fmaf(o[0], xx, fmaf(o[5], yy, fmaf(o[7], zz,
+                                     fmaf(o[3], xy, fmaf(o[4], xz, o[6] * yz)))));
+                    const float fy = fmaf(o[3], xx, fmaf(o[1], yy, fmaf(o[8], zz,
+                                     fmaf(o[5], xy, fmaf(o[6], xz, o[9] * yz)))));
+                    const float fz = fmaf(o[4], xx, fmaf(o[9], yy, fmaf(o[2], zz,
+                                     fmaf(o[6], xy, fmaf(o[7], xz, o[8] * yz)))));
+                    const float gdd = dx * fx + dy * fy + dz * fz;
+                    // M w^3 |d|^-7 = M w^2 |d|^-5 * w |d|^-2, the w being in oe.o
+                    const float s7 = s5 * inv2;
+                    st = fmaf(-7.f * s7, gdd * inv2, st);
+                    const float s7x3 = 3.f * s7;
+                    ex = fmaf(s7x3, fx, ex);
+                    ey = fmaf(s7x3, fy, ey);
+                    ez = fmaf(s7x3, fz, ez);
+                }
+                yx = fmaf(dx, st, -cx) + ex;
+                yy = fmaf(dy, st, -cy) + ey;
+                yz = fmaf(dz, st, -cz) + ez;
             } else {
                 yx = fmaf(dx, s, -cx);
                 yy = fmaf(dy, s, -cy);
@@ -1768,13 +1835,14 @@ static int dd_resolve(pnb_ctx* c) {
 }
 
 const MlpWeights* mlp_weights(pnb_ctx* c);  // mlp.hip: nullptr until initialised
+bool mlp_octupole(pnb_ctx* c);               // mlp.hip: the network has octupole outputs
 
-// Walk records for the relative criterion: node k's record with begin = the bits of K = M w^2
-// (cubic: M w^3; f32, each operation rounded; 0 for leaves, which are never opened) and end =
+// Walk records for the relative criterion: node k's record with begin = the bits of K = M w^p,
+// p = 2, 3 or 4 (f32, each operation rounded; 0 for leaves, which are never opened) and end =
 // first body | count << 27 for an internal node of at most `bucket` (<= 31) bodies whose first
 // body is below 2^27, else 0; for the total + 1 records (the last is the sentinel the walk
 // prefetches).
-__global__ void bh_rel_rec_kernel(const NodeRec* __restrict__ rec, size_t total, int cubic,
+__global__ void bh_rel_rec_kernel(const NodeRec* __restrict__ rec, size_t total, int power,
                                   uint32_t bucket, NodeRec* __restrict__ rrec) {
     const size_t k = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (k > total) return;
@@ -1783,7 +1851,8 @@ __global__ void bh_rel_rec_kernel(const NodeRec* __restrict__ rec, size_t total,
     uint32_t packed = 0;
     if (k < total && __float_as_int(r.w2) >= 0) {
         K = mul_rn(r.mass, r.w2);
-        if (cubic) K = mul_rn(K, __fsqrt_rn(r.w2));
+        if (power == 3) K = mul_rn(K, __fsqrt_rn(r.w2));
+        if (power == 4) K = mul_rn(K, r.w2);
         const uint32_t count = r.end - r.begin;
         if (count <= bucket && r.begin < (1u << 27)) packed = r.begin | (count << 27);
     }
@@ -1850,15 +1919,16 @@ static void launch_walk(bool wide, unsigned grid, hipStream_t st, const NodeRec*
                         const QuadH* qh, const float* est_in = nullptr, uint32_t est_shift = 0,
                         float* est_out = nullptr, float alpha = 0.f,
                         const uint32_t* wave_order = nullptr, const float4* bodies = nullptr,
-                        uint32_t lanes = 64, int dd = 0, int packed = 0) {
+                        uint32_t lanes = 64, int dd = 0, int packed = 0,
+                        const OctH* oh = nullptr, int oct = 0) {
     if (wide)
         bh_walk_kernel<MODE, true, REL><<<grid, 256, 0, st>>>(
-            nr, total, pts, perm, n, theta2, eps2, out, info, rows, qh, est_in, est_shift, est_out,
-            alpha, wave_order, bodies, lanes, dd, packed);
+            nr, total, pts, perm, n, theta2, eps2, out, info, rows, qh, oh, est_in, est_shift,
+            est_out, alpha, wave_order, bodies, lanes, dd, packed, oct);
     else
         bh_walk_kernel<MODE, false, REL><<<grid, 256, 0, st>>>(
-            nr, total, pts, perm, n, theta2, eps2, out, info, rows, qh, est_in, est_shift, est_out,
-            alpha, wave_order, bodies, lanes, dd, packed);
+            nr, total, pts, perm, n, theta2, eps2, out, info, rows, qh, oh, est_in, est_shift,
+            est_out, alpha, wave_order, bodies, lanes, dd, packed, oct);
 }
 
 // Particles per wavefront of the plain and corrected walks: the largest power of two (at most
@@ -1918,13 +1988,15 @@ static int walk(pnb_ctx* c, const float4* pts, const uint32_t* perm, size_t n, f
         const bool wide = (uint64_t(total) + 1) * sizeof(NodeRec) >= (uint64_t(1) << 32) ||
                           (force && force[0] == '1');
         if (rel) {
-            const int cubic = quad ? 1 : 0;
-            if (!s->rrec_ok || s->rrec_cubic != cubic || s->rrec_bucket != s->rel_bucket) {
+            // the power of the first neglected term: the quadrupole (plain walk), the octupole
+            // (with the correction's quadrupole) or the hexadecapole (and its octupole)
+            const int power = quad ? (mlp_octupole(c) ? 4 : 3) : 2;
+            if (!s->rrec_ok || s->rrec_power != power || s->rrec_bucket != s->rel_bucket) {
                 bh_rel_rec_kernel<<<div_up(size_t(total) + 1, 256), 256, 0, c->stream>>>(
-                    nr, total, cubic, uint32_t(s->rel_bucket), s->rrec.as<NodeRec>());
+                    nr, total, power, uint32_t(s->rel_bucket), s->rrec.as<NodeRec>());
                 PNB_CHECK_LAUNCH(c);
                 s->rrec_ok = true;
-                s->rrec_cubic = cubic;
+                s->rrec_power = power;
                 s->rrec_bucket = s->rel_bucket;
             }
             const float te = s->rel_theta_est;
@@ -1984,7 +2056,8 @@ static int walk(pnb_ctx* c, const float4* pts, const uint32_t* perm, size_t n, f
                 launch_walk<WALK_QUAD, true>(wide, grid, c->stream, rr, total, pts, perm,
                                              uint32_t(n), theta2, eps2, out, info, WalkRows{},
                                              s->qh.as<QuadH>(), est, shift, nullptr, 0.f, order,
-                                             bodies);
+                                             bodies, 64, 0, 0, s->oh.as<OctH>(),
+                                             mlp_octupole(c) ? 1 : 0);
             else
                 launch_walk<WALK_PLAIN, true>(wide, grid, c->stream, rr, total, pts, perm,
                                               uint32_t(n), theta2, eps2, out, info, WalkRows{},
@@ -1999,7 +2072,8 @@ static int walk(pnb_ctx* c, const float4* pts, const uint32_t* perm, size_t n, f
                 launch_walk<WALK_QUAD, false>(wide, pgrid, c->stream, nr, total, pts, perm,
                                               uint32_t(n), theta2, eps2, out, info, WalkRows{},
                                               s->qh.as<QuadH>(), nullptr, 0, nullptr, 0.f, nullptr,
-                                              nullptr, lanes, dd ? 1 : 0, 1);
+                                              nullptr, lanes, dd ? 1 : 0, 1,
+                                              s->oh.as<OctH>(), mlp_octupole(c) ? 1 : 0);
             else
                 launch_walk<WALK_PLAIN, false>(wide, pgrid, c->stream, nr, total, pts, perm,
                                                uint32_t(n), theta2, eps2, out, info, WalkRows{},
@@ -2020,6 +2094,7 @@ static int node_network(pnb_ctx* c) {
     const size_t total = s->nodes;
     // one entry past the last node, as for the records (the walk prefetches k + 1 and skip)
     PNB_HIP(c, s->qh.ensure((size_t(total) + 1) * sizeof(QuadH)));
+    PNB_HIP(c, s->oh.ensure((size_t(total) + 1) * sizeof(OctH)));
     if (total == 0) return PNB_OK;
     if (s->internal == 0) return PNB_OK;
     return timed(c, PNB_PROF_MLP, [&]() -> int {
@@ -2028,7 +2103,7 @@ static int node_network(pnb_ctx* c) {
         mlp_node_kernel<<<grid, 256, 0, c->stream>>>(s->rec.as<NodeRec>(), s->quad.as<NodeQuad>(),
                                                      s->level.as<uint8_t>(), s->desc.as<NodeDesc>(),
                                                      s->ilist.as<uint32_t>(), uint32_t(s->internal),
-                                                     wg, s->qh.as<QuadH>());
+                                                     wg, s->qh.as<QuadH>(), s->oh.as<OctH>());
         PNB_CHECK_LAUNCH(c);
         return PNB_OK;
     });

@@ -31,6 +31,7 @@ constexpr int MLP_BWD_BLOCKS = 256;
 
 struct MlpState {
     bool ready = false;
+    bool octupole = false;  // the output layer may give non-zero octupole outputs
     int adam_t = 0;
     DevBuf params, m, v, grads, weights, partial, loss_hist;
     // training set
@@ -460,6 +461,10 @@ const MlpWeights* mlp_weights(pnb_ctx* c) {
     return (c->mlp && c->mlp->ready) ? c->mlp->weights.as<MlpWeights>() : nullptr;
 }
 
+// Whether the network may output a non-zero octupole (a non-zero weight or bias of outputs
+// 6..12): the corrected walk then fetches every accepted node's octupole with its quadrupole.
+bool mlp_octupole(pnb_ctx* c) { return c->mlp && c->mlp->ready && c->mlp->octupole; }
+
 // Appends `rows` rows and `waves` wavefronts (already on the device) to the training set.
 int mlp_append(pnb_ctx* c, const float* feats, const float4* geom, const unsigned long long* mask,
                const uint32_t* row_wave_local, const uint32_t* wave_rows_local, size_t rows,
@@ -635,6 +640,7 @@ int pnb_mlp_init(pnb_ctx* c, uint64_t seed) {
     PNB_HIP(c, hipStreamSynchronize(c->stream));
     s->adam_t = 0;
     s->ready = true;
+    s->octupole = false;  // zero output layer
     return PNB_OK;
 }
 
@@ -651,6 +657,11 @@ int pnb_mlp_set_params(pnb_ctx* c, const float* params, size_t n) {
     PNB_HIP(c, hipStreamSynchronize(c->stream));
     s->adam_t = 0;
     s->ready = true;
+    bool oct = false;
+    for (int k = 0; k < MLP_H; ++k)
+        for (int o = MLP_QOUT; o < MLP_OUT; ++o) oct |= params[MLP_OFF_W3 + k * MLP_OUT + o] != 0.f;
+    for (int o = MLP_QOUT; o < MLP_OUT; ++o) oct |= params[MLP_OFF_B3 + o] != 0.f;
+    s->octupole = oct;
     return PNB_OK;
 }
 
@@ -778,13 +789,15 @@ int pnb_mlp_train(pnb_ctx* c, size_t iterations, float lr, float* loss_history) 
         PNB_CHECK_LAUNCH(c);
         if (int rc = pack(c)) return rc;
     }
+    if (iterations) s->octupole = true;  // the output layer has been updated
     if (loss_history && iterations)
         PNB_HIP(c, hipMemcpyAsync(loss_history, s->loss_hist.ptr, iterations * 4, hipMemcpyDeviceToHost, c->stream));
     PNB_HIP(c, hipStreamSynchronize(c->stream));
     return PNB_OK;
 }
 
-// Standalone inference: y (rows x 8, 6 used) = MLP(feats (rows x 16, 11 used)); device pointers.
+// Standalone inference: y (rows x 16, 13 used) = MLP(feats (rows x 16)), plus feats[0..5] on
+// y[0..5]; device pointers.
 int pnb_mlp_forward_dev(pnb_ctx* c, const float* feats, size_t rows, float* y) {
     if (!c) return PNB_E_ARG;
     if (rows && (!feats || !y)) return fail(c, PNB_E_ARG, "null pointer with non-zero length");

@@ -10,7 +10,8 @@
    network replaces the starting one only if it also lowers the error on a separate validation
    snapshot.
 3. A held-out snapshot: plain Barnes-Hut at `--thetas`, the starting network (the quadrupole
-   term) and the trained network at `--nn-thetas`, each timed (tree build + walk [+ node network],
+   term), the trained network and, for reference, the network that passes each node's own
+   octupole through (learned.octupole_params: the analytic octupole term) at `--nn-thetas`, each timed (tree build + walk [+ node network],
    device time from HIP events) and checked against the FP64 direct sum on a seeded sample.
    Reports particles/s and error of each run and, by log-log interpolation of time against error
    on each curve, at `--target-rms` and at the error of each trained run: the trained network
@@ -171,7 +172,7 @@ def main():
     p_start = learned.get_params(res)
     val_start = val_error()
     res.profile(True)
-    fit = learned.fit(res, a.iters)
+    fit = learned.fit(res, a.iters, log=log)
     train_ms, _ = res.profile_read(_lib.PROF_MLP)
     res.profile(False)
     val_end = val_error()
@@ -187,6 +188,12 @@ def main():
 
     plain = [run(float(x), False) for x in a.thetas.split(",")]
     nn = [run(th, True) for th in nn_thetas]
+    trained_params = learned.get_params(res)
+    # the analytic reference: the network that passes each node's own octupole through
+    learned.set_params(res, learned.octupole_params())
+    octu = [run(th, True) for th in nn_thetas]
+    octu_rel = [run(1.0, True, al) for al in nn_rel_alphas]
+    learned.set_params(res, trained_params)
     for q in nn + start:
         t_plain = time_at_error(plain, q["rms_rel"])
         q["plain_ms_at_same_error"] = t_plain
@@ -199,14 +206,18 @@ def main():
         q["speedup_vs_quadrupole_at_same_error"] = t_q / q["ms"] if t_q else None
         same = next((r for r in start if abs(r["theta"] - q["theta"]) < 1e-9), None)
         q["error_vs_quadrupole_same_theta"] = q["rms_rel"] / same["rms_rel"] if same else None
+        same = next((r for r in octu if abs(r["theta"] - q["theta"]) < 1e-9), None)
+        q["error_vs_octupole_same_theta"] = q["rms_rel"] / same["rms_rel"] if same else None
     t_plain = time_at_error(plain, a.target_rms)
     t_nn = time_at_error(nn, a.target_rms)
     t_start = time_at_error(start, a.target_rms)
+    t_octu = time_at_error(octu, a.target_rms)
     plain_rel = [run(1.0, False, float(x)) for x in a.rel_alphas.split(",") if x]
     nn_rel = [run(1.0, True, al) for al in nn_rel_alphas]
     t_prel = time_at_error(plain_rel, a.target_rms) if plain_rel else None
     t_nrel = time_at_error(nn_rel, a.target_rms) if nn_rel else None
     t_srel = time_at_error(start_rel, a.target_rms) if start_rel else None
+    t_orel = time_at_error(octu_rel, a.target_rms) if octu_rel else None
 
     def ratio(x, y):
         return (x / y) if (x and y) else None
@@ -223,9 +234,11 @@ def main():
         "plain": plain,
         "nn_start": start,
         "nn": nn,
+        "octupole_reference": octu,
         "target_rms": a.target_rms,
         "plain_ms_at_target": t_plain,
         "quadrupole_ms_at_target": t_start,
+        "octupole_ms_at_target": t_octu,
         "nn_ms_at_target": t_nn,
         "speedup_at_target": ratio(t_plain, t_nn),
         "quadrupole_speedup_at_target": ratio(t_plain, t_start),
@@ -234,8 +247,10 @@ def main():
             "plain": plain_rel,
             "nn_start": start_rel,
             "nn": nn_rel,
+            "octupole_reference": octu_rel,
             "plain_ms_at_target": t_prel,
             "quadrupole_ms_at_target": t_srel,
+            "octupole_ms_at_target": t_orel,
             "nn_ms_at_target": t_nrel,
             "nn_speedup_vs_plain_relative": ratio(t_prel, t_nrel),
             "trained_vs_quadrupole_relative": ratio(t_srel, t_nrel),

@@ -83,6 +83,7 @@ def test_two_plummer_4m_corrected_walk(res, orc, two_plummer):
     nodes, order = pa.tree_export(res, p.shape[0])
     waves = _wave_index(p[order], d["lanes"])
     y, _ = M.forward(held, d["feats"][:, :M.IN].astype(np.float64), bf16=True)
+    y = learned.walk_outputs(y)  # the walk's rule for small octupoles
     pred = M.predict(y, d["row_wave"].astype(np.int64), d["mask"], d["geom"].astype(np.float64),
                      d["lanes"].astype(np.float64), len(waves))
     lanes_idx = (64 * waves[:, None] + np.arange(64)[None, :]).ravel()
