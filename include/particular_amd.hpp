@@ -142,6 +142,10 @@ class HipResources {
     void check(int rc) const {
         if (rc != PNB_OK) throw Error(rc, pnb_last_error(ctx_));
     }
+    // Barnes-Hut without host round trips for device-pointer calls (pnb_bh_set_capturable), and
+    // the status of the last such build (pnb_bh_check).
+    void set_bh_capturable(bool enable) { check(pnb_bh_set_capturable(ctx_, enable ? 1 : 0)); }
+    void bh_check() { check(pnb_bh_check(ctx_)); }
 
    private:
     pnb_ctx* ctx_ = nullptr;

@@ -37,6 +37,8 @@ extern "C" {
                                       flags: c_uint, out: *mut c_float) -> c_int;
     pub fn pnb_bh_set_relative(ctx: *mut PnbCtx, alpha: c_float, estimate_theta: c_float,
                                per_particle: c_int, bucket: c_int) -> c_int;
+    pub fn pnb_bh_set_capturable(ctx: *mut PnbCtx, enable: c_int) -> c_int;
+    pub fn pnb_bh_check(ctx: *mut PnbCtx) -> c_int;
 
     pub fn pnb_bf_accel_f32_2d(ctx: *mut PnbCtx, affected: *const c_float, n: usize,
                                affecting: *const c_float, m: usize, softening: c_float,
