@@ -109,6 +109,8 @@ def main():
                     help="the corrected walk with the relative criterion (empty: none)")
     ap.add_argument("--target-rms", type=float, default=1e-4)
     ap.add_argument("--seed", type=int, default=1808, help="hidden-layer initialisation")
+    ap.add_argument("--lr", type=float, default=0.0,
+                    help="learning rate (0: chosen by short probes, learned.fit)")
     ap.add_argument("--params-out", default="")
     a = ap.parse_args()
     import torch
@@ -172,7 +174,7 @@ def main():
     p_start = learned.get_params(res)
     val_start = val_error()
     res.profile(True)
-    fit = learned.fit(res, a.iters, log=log)
+    fit = learned.fit(res, a.iters, log=log, **({"lrs": (a.lr,)} if a.lr > 0 else {}))
     train_ms, _ = res.profile_read(_lib.PROF_MLP)
     res.profile(False)
     val_end = val_error()
