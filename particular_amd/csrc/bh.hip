@@ -1039,21 +1039,23 @@ __device__ __forceinline__ void node_features(const NodeRec& r, const NodeQuad& 
 }
 
 // The learned moments of a node, in two entries read with scalar loads when a lane accepts it.
-// QuadH (16 bytes): the quadrupole y[0..5] (units of M w^2) as halves and a flag, 1 when the
-// node's octupole (OctH) is worth adding: its largest coefficient is at least OCT_MIN, which
-// leaves out nodes whose octupole vanishes (two equal masses) or nearly does, about a third of
-// the accepted ones on Plummer spheres, at no measurable cost in accuracy. y is O(1) and the
+// QuadH (16 bytes): the quadrupole y[0..5] (units of M w^2) as halves. y is O(1) and the
 // correction it feeds is itself a small fraction of the force, so 11-bit mantissas cost nothing
 // measurable, while the walk's scalar loads take half of what floats would.
 // OctH (48 bytes): the octupole y[6..12] (units of M w^3) times the node's width w, as all ten
 // components of the traceless tensor {xxx, yyy, zzz, xxy, xxz, xyy, xyz, xzz, yzz, yyz} in FP32,
 // so that the walk multiplies each by a product of d with one FMA (a vector instruction reads one
-// scalar operand here) and needs no square root for w^3.
+// scalar operand here) and needs no square root for w^3; then a flag, non-zero
+// when the octupole is worth adding: its largest coefficient is at least OCT_MIN, which leaves
+// out nodes whose octupole vanishes (two equal masses) or nearly does, about a third of the
+// accepted ones on Plummer spheres, at no measurable cost in accuracy. The flag sits in the
+// octupole entry so that the walk's test of it waits for both loads at once.
 struct __align__(16) QuadH {
-    __half q[8];  // y[0..5], flag, 0
+    __half q[8];  // y[0..5], 0, 0
 };
 struct __align__(16) OctH {
-    float o[12];  // w * octupole, ten components, 0, 0
+    float o[10];  // w * octupole, ten components
+    uint32_t flag, pad;
 };
 constexpr float OCT_MIN = 3e-3f;
 
@@ -1122,7 +1124,7 @@ __global__ __launch_bounds__(256) void mlp_node_kernel(const NodeRec* __restrict
             }
 #pragma unroll
             for (int q = 0; q < 4; ++q) qh[k].q[q] = __float2half(keep * yv[q]);
-            qh[k].q[6] = __float2half(big >= OCT_MIN ? 1.f : 0.f);
+            qh[k].q[6] = __float2half(0.f);
             qh[k].q[7] = __float2half(0.f);
             // w * O, the three dependent components from O_iik = 0
             const float wd = keep > 0.f ? sqrtf(r.w2) : 0.f;
@@ -1137,8 +1139,8 @@ __global__ __launch_bounds__(256) void mlp_node_kernel(const NodeRec* __restrict
             e.o[7] = -(e.o[0] + e.o[5]);  // xzz
             e.o[8] = -(e.o[3] + e.o[1]);  // yzz
             e.o[9] = -(e.o[4] + e.o[2]);  // yyz
-            e.o[10] = 0.f;
-            e.o[11] = 0.f;
+            e.flag = big >= OCT_MIN ? 1u : 0u;
+            e.pad = 0u;
             oh[k] = e;
         }
         if (listed && g == 1) {
@@ -1167,6 +1169,7 @@ __global__ __launch_bounds__(256) void bh_features_kernel(const NodeRec* __restr
 constexpr int WALK_PLAIN = 0;  // plain Barnes-Hut
 constexpr int WALK_ROWS = 1;   // also list the internal nodes each wavefront accepted (rows)
 constexpr int WALK_QUAD = 2;   // add quad_field(q) of every accepted internal node
+constexpr int WALK_OCT = 3;    // WALK_QUAD with the network's octupole outputs (oct_field)
 
 // Per wavefront, the internal nodes that at least one of its lanes accepted, in walk order:
 // {node index, accepting-lane count, accepting-lane mask (low, high 32 bits)}.
@@ -1281,7 +1284,7 @@ __device__ __forceinline__ float half_hi(uint32_t v) {
 // geometric rule opens it or when the size of its first neglected term, M w^p / r^(p+2)
 // (r^2 = d^2 + eps^2), exceeds A: p = 2 for the plain walk (K > A r^4, the quadrupole is
 // neglected), p = 3 for WALK_QUAD (K / r > A r^4, the octupole is the first term it lacks) and
-// p = 4 for WALK_QUAD with `oct` (K / r^2 > A r^4: the network adds octupoles, the hexadecapole
+// p = 4 for WALK_OCT (K / r^2 > A r^4: the network adds octupoles, the hexadecapole
 // is the first term it lacks).
 // REL buckets: a record whose `end` is non-zero packs the first body (27 bits) and the count
 // (5 bits) of a small internal node; lanes that open it add its bodies (`bodies`, the affecting
@@ -1296,8 +1299,10 @@ __device__ __forceinline__ float half_hi(uint32_t v) {
 // many short walks instead of a few long ones; every lane's result is the same either way (it
 // adds exactly the nodes of its own pre-order traversal, in that order).
 // dd: a capturable build's tree (`total` is then its capacity; the count is read from `info`).
-// oct (WALK_QUAD): the network has octupole outputs; every accepted internal node's octupole entry
-// is then fetched together with its quadrupole entry, so that the two loads overlap.
+// WALK_OCT: the network has octupole outputs; every accepted internal node's octupole entry is
+// fetched together with its quadrupole entry, so that the two loads overlap. A mode of its own
+// rather than a run-time flag: behind a flag the compiler copied the ten octupole components
+// from SGPRs into VGPRs on every accepting step (ten VALU moves).
 template <int MODE, bool WIDE, bool REL>
 __global__ __launch_bounds__(256) void bh_walk_kernel(const NodeRec* __restrict__ nodes,
                                                       uint32_t total,
@@ -1313,8 +1318,9 @@ __global__ __launch_bounds__(256) void bh_walk_kernel(const NodeRec* __restrict_
                                                       float* __restrict__ est_out, float alpha,
                                                       const uint32_t* __restrict__ wave_order,
                                                       const float4* __restrict__ bodies,
-                                                      uint32_t lanes, int dd, int packed,
-                                                      int oct) {
+                                                      uint32_t lanes, int dd, int packed) {
+    constexpr bool QUAD = MODE == WALK_QUAD || MODE == WALK_OCT;
+    constexpr bool OCT = MODE == WALK_OCT;
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (packed) i = (i >> 6) * lanes + (threadIdx.x & 63);
     if (dd) {
@@ -1357,14 +1363,14 @@ __global__ __launch_bounds__(256) void bh_walk_kernel(const NodeRec* __restrict_
         const float dx = sub_rn(r.cx, p.x), dy = sub_rn(r.cy, p.y), dz = sub_rn(r.cz, p.z);
         const float d2 = norm2_ref(dx, dy, dz);
         const bool nz = d2 != 0.0f;
-        // REL with WALK_QUAD: the reciprocal distance is needed by every lane's criterion, not
+        // REL with QUAD: the reciprocal distance is needed by every lane's criterion, not
         // only by the accepting lanes' pair terms
-        constexpr bool REL_INV = REL && MODE == WALK_QUAD;
+        constexpr bool REL_INV = REL && QUAD;
         const float r2_rel = REL ? add_rn(d2, eps2) : 0.f;
         const float inv_rel = REL_INV ? __builtin_amdgcn_rsqf(r2_rel) : 0.f;
         const float K = __uint_as_float(r.begin);
-        // WALK_QUAD: K / r (p = 3), or K / r^2 (p = 4, when the network adds octupoles)
-        const float lhs = REL ? (REL_INV ? mul_rn(oct ? mul_rn(K, inv_rel) : K, inv_rel) : K)
+        // QUAD: K / r (p = 3), or K / r^2 (p = 4, when the network adds octupoles)
+        const float lhs = REL ? (REL_INV ? mul_rn(OCT ? mul_rn(K, inv_rel) : K, inv_rel) : K)
                               : 0.f;
         const float rhs = REL ? mul_rn(mul_rn(A, r2_rel), r2_rel) : 0.f;
         const bool crit = mul_rn(theta2, d2) < r.w2 || (REL && rhs < lhs);
@@ -1412,7 +1418,7 @@ __global__ __launch_bounds__(256) void bh_walk_kernel(const NodeRec* __restrict_
             const float inv = REL_INV ? inv_rel : __builtin_amdgcn_rsqf(r2);
             const float s = (r.mass * inv) * (inv * inv);
             float yx, yy, yz;
-            if (MODE == WALK_QUAD && __float_as_int(r.w2) >= 0) {  // internal (leaves: -inf)
+            if (QUAD && __float_as_int(r.w2) >= 0) {  // internal (leaves: -inf)
                 // internal node: the monopole plus the field of Q = M w^2 y[0..5] (quad_field's
                 // formula, w2 = width^2) and, when the node has one, of O = M w^3 y[6..12]
                 // (oct_field's), summed as d (s + s5 t - 7 s7 G |d|^-2) - s5 Q d + 3 s7 F in one
@@ -1420,9 +1426,16 @@ __global__ __launch_bounds__(256) void bh_walk_kernel(const NodeRec* __restrict_
                 const uint4 h = *reinterpret_cast<const uint4*>(
                     reinterpret_cast<const char*>(qh) + (ko >> 1));  // 16-byte entries
                 OctH oe;
-                if (oct)  // issued before h is used, so that both loads are in flight together
+                if constexpr (OCT) {
                     oe = *reinterpret_cast<const OctH*>(reinterpret_cast<const char*>(oh) +
                                                         (ko + (ko >> 1)));  // 48-byte entries
+                    // an empty asm that reads the whole entry here, before h is used: both
+                    // loads are then in flight together and one wait covers them (otherwise the
+                    // compiler fetches the flag first and the components behind its test)
+                    asm volatile("" ::"s"(oe.o[0]), "s"(oe.o[1]), "s"(oe.o[2]), "s"(oe.o[3]),
+                                 "s"(oe.o[4]), "s"(oe.o[5]), "s"(oe.o[6]), "s"(oe.o[7]),
+                                 "s"(oe.o[8]), "s"(oe.o[9]), "s"(oe.flag));
+                }
                 const float q[6] = {half_lo(h.x), half_hi(h.x), half_lo(h.y),
                                     half_hi(h.y), half_lo(h.z), half_hi(h.z)};
                 // fma chains from +0 so that every half-precision product is one v_fma_mix
@@ -1433,30 +1446,32 @@ __global__ __launch_bounds__(256) void bh_walk_kernel(const NodeRec* __restrict_
                 const float inv2 = inv * inv;
                 const float s5 = s * (r.w2 * inv2);  // M w^2 |d|^-5 from the monopole's M |d|^-3
                 float st = fmaf(s5, 2.5f * dqd * inv2, s);
-                float ex = -s5 * qx, ey = -s5 * qy, ez = -s5 * qz;
-                if (oct && (h.w & 0xffffu)) {  // the flag (q[6]): this node's octupole counts
+                const float ms5 = -s5;
+                // the vector terms chained onto the monopole's FMA, one FMA each
+                if (OCT && oe.flag) {  // this node's octupole counts
                     // F_i = O_ijk d_j d_k over the ten components of w O (oe.o)
-                    const float xx = dx * dx, yy = dy * dy, zz = dz * dz;
-                    const float xy = 2.f * dx * dy, xz = 2.f * dx * dz, yz = 2.f * dy * dz;
+                    const float pxx = dx * dx, pyy = dy * dy, pzz = dz * dz;
+                    const float pxy = 2.f * dx * dy, pxz = 2.f * dx * dz, pyz = 2.f * dy * dz;
                     const float* o = oe.o;
-                    const float fx = fmaf(o[0], xx, fmaf(o[5], yy, fmaf(o[7], zz,
-                                     fmaf(o[3], xy, fmaf(o[4], xz, o[6] * yz)))));
-                    const float fy = fmaf(o[3], xx, fmaf(o[1], yy, fmaf(o[8], zz,
-                                     fmaf(o[5], xy, fmaf(o[6], xz, o[9] * yz)))));
-                    const float fz = fmaf(o[4], xx, fmaf(o[9], yy, fmaf(o[2], zz,
-                                     fmaf(o[6], xy, fmaf(o[7], xz, o[8] * yz)))));
+                    const float fx = fmaf(o[0], pxx, fmaf(o[5], pyy, fmaf(o[7], pzz,
+                                     fmaf(o[3], pxy, fmaf(o[4], pxz, o[6] * pyz)))));
+                    const float fy = fmaf(o[3], pxx, fmaf(o[1], pyy, fmaf(o[8], pzz,
+                                     fmaf(o[5], pxy, fmaf(o[6], pxz, o[9] * pyz)))));
+                    const float fz = fmaf(o[4], pxx, fmaf(o[9], pyy, fmaf(o[2], pzz,
+                                     fmaf(o[6], pxy, fmaf(o[7], pxz, o[8] * pyz)))));
                     const float gdd = dx * fx + dy * fy + dz * fz;
                     // M w^3 |d|^-7 = M w^2 |d|^-5 * w |d|^-2, the w being in oe.o
                     const float s7 = s5 * inv2;
                     st = fmaf(-7.f * s7, gdd * inv2, st);
                     const float s7x3 = 3.f * s7;
-                    ex = fmaf(s7x3, fx, ex);
-                    ey = fmaf(s7x3, fy, ey);
-                    ez = fmaf(s7x3, fz, ez);
+                    yx = fmaf(s7x3, fx, fmaf(ms5, qx, fmaf(dx, st, -cx)));
+                    yy = fmaf(s7x3, fy, fmaf(ms5, qy, fmaf(dy, st, -cy)));
+                    yz = fmaf(s7x3, fz, fmaf(ms5, qz, fmaf(dz, st, -cz)));
+                } else {
+                    yx = fmaf(ms5, qx, fmaf(dx, st, -cx));
+                    yy = fmaf(ms5, qy, fmaf(dy, st, -cy));
+                    yz = fmaf(ms5, qz, fmaf(dz, st, -cz));
                 }
-                yx = fmaf(dx, st, -cx) + ex;
-                yy = fmaf(dy, st, -cy) + ey;
-                yz = fmaf(dz, st, -cz) + ez;
             } else {
                 yx = fmaf(dx, s, -cx);
                 yy = fmaf(dy, s, -cy);
@@ -1921,14 +1936,22 @@ static void launch_walk(bool wide, unsigned grid, hipStream_t st, const NodeRec*
                         const uint32_t* wave_order = nullptr, const float4* bodies = nullptr,
                         uint32_t lanes = 64, int dd = 0, int packed = 0,
                         const OctH* oh = nullptr, int oct = 0) {
+    if constexpr (MODE == WALK_QUAD) {
+        if (oct) {
+            launch_walk<WALK_OCT, REL>(wide, grid, st, nr, total, pts, perm, n, theta2, eps2, out,
+                                       info, rows, qh, est_in, est_shift, est_out, alpha,
+                                       wave_order, bodies, lanes, dd, packed, oh, 0);
+            return;
+        }
+    }
     if (wide)
         bh_walk_kernel<MODE, true, REL><<<grid, 256, 0, st>>>(
             nr, total, pts, perm, n, theta2, eps2, out, info, rows, qh, oh, est_in, est_shift,
-            est_out, alpha, wave_order, bodies, lanes, dd, packed, oct);
+            est_out, alpha, wave_order, bodies, lanes, dd, packed);
     else
         bh_walk_kernel<MODE, false, REL><<<grid, 256, 0, st>>>(
             nr, total, pts, perm, n, theta2, eps2, out, info, rows, qh, oh, est_in, est_shift,
-            est_out, alpha, wave_order, bodies, lanes, dd, packed, oct);
+            est_out, alpha, wave_order, bodies, lanes, dd, packed);
 }
 
 // Particles per wavefront of the plain and corrected walks: the largest power of two (at most
