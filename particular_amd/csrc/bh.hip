@@ -1039,23 +1039,26 @@ __device__ __forceinline__ void node_features(const NodeRec& r, const NodeQuad& 
 }
 
 // The learned moments of a node, in two entries read with scalar loads when a lane accepts it.
-// QuadH (16 bytes): the quadrupole y[0..5] (units of M w^2) as halves. y is O(1) and the
+// QuadH (16 bytes): the quadrupole y[0..5] (units of M w^2) as halves and a flag, 1 when the
+// node's octupole (OctH) is worth adding: its largest coefficient is at least OCT_MIN, which
+// leaves out nodes whose octupole vanishes (two equal masses) or nearly does, about a third of
+// the accepted ones on Plummer spheres, at no measurable cost in accuracy. y is O(1) and the
 // correction it feeds is itself a small fraction of the force, so 11-bit mantissas cost nothing
 // measurable, while the walk's scalar loads take half of what floats would.
-// OctH (48 bytes): the octupole y[6..12] (units of M w^3) times the node's width w, as all ten
-// components of the traceless tensor {xxx, yyy, zzz, xxy, xxz, xyy, xyz, xzz, yzz, yyz} in FP32,
-// so that the walk multiplies each by a product of d with one FMA (a vector instruction reads one
-// scalar operand here) and needs no square root for w^3; then a flag, non-zero
-// when the octupole is worth adding: its largest coefficient is at least OCT_MIN, which leaves
-// out nodes whose octupole vanishes (two equal masses) or nearly does, about a third of the
-// accepted ones on Plummer spheres, at no measurable cost in accuracy. The flag sits in the
-// octupole entry so that the walk's test of it waits for both loads at once.
+// OctH (64 bytes): the octupole y[6..12] (units of M w^3) times the node's width w, in FP32, laid
+// out for F_i = O_ijk d_j d_k written with the trace removed, O_izz = -(O_ixx + O_iyy):
+//   F_x = O_xxx a + O_xyy b + 2 O_xxy xy + 2 O_xxz xz + 2 O_xyz yz,
+//   F_y = O_xxy a + O_yyy b + 2 O_xyy xy + 2 O_xyz xz + 2 O_yyz yz,
+//   F_z = O_xxz a + O_yyz b + 2 O_xyz xy + 2 O_xzz xz + 2 O_yzz yz,  a = xx - zz, b = yy - zz,
+// o = {xxx, xyy, xxy, yyy, xxz, yyz, 2xxy, 2xxz, 2xyz, 2xyy, 2yyz, 2xzz, 2yzz}: one FMA per
+// coefficient and product (a vector instruction reads one scalar operand here), fifteen for F,
+// and no square root for w^3.
 struct __align__(16) QuadH {
-    __half q[8];  // y[0..5], 0, 0
+    __half q[8];  // y[0..5], flag, 0
 };
 struct __align__(16) OctH {
-    float o[10];  // w * octupole, ten components
-    uint32_t flag, pad;
+    float o[13];  // w * octupole, the layout above
+    float pad[3];
 };
 constexpr float OCT_MIN = 3e-3f;
 
@@ -1124,23 +1127,28 @@ __global__ __launch_bounds__(256) void mlp_node_kernel(const NodeRec* __restrict
             }
 #pragma unroll
             for (int q = 0; q < 4; ++q) qh[k].q[q] = __float2half(keep * yv[q]);
-            qh[k].q[6] = __float2half(0.f);
+            qh[k].q[6] = __float2half(big >= OCT_MIN ? 1.f : 0.f);
             qh[k].q[7] = __float2half(0.f);
-            // w * O, the three dependent components from O_iik = 0
+            // w * O in the walk's layout (OctH), the dependent components from O_iik = 0
             const float wd = keep > 0.f ? sqrtf(r.w2) : 0.f;
+            const float xxx = wd * oc[0], yyy = wd * oc[1], zzz = wd * oc[2], xxy = wd * oc[3],
+                        xxz = wd * oc[4], xyy = wd * oc[5], xyz = wd * oc[6];
+            const float xzz = -(xxx + xyy), yzz = -(xxy + yyy), yyz = -(xxz + zzz);
             OctH e;
-            e.o[0] = wd * oc[0];  // xxx
-            e.o[1] = wd * oc[1];  // yyy
-            e.o[2] = wd * oc[2];  // zzz
-            e.o[3] = wd * oc[3];  // xxy
-            e.o[4] = wd * oc[4];  // xxz
-            e.o[5] = wd * oc[5];  // xyy
-            e.o[6] = wd * oc[6];  // xyz
-            e.o[7] = -(e.o[0] + e.o[5]);  // xzz
-            e.o[8] = -(e.o[3] + e.o[1]);  // yzz
-            e.o[9] = -(e.o[4] + e.o[2]);  // yyz
-            e.flag = big >= OCT_MIN ? 1u : 0u;
-            e.pad = 0u;
+            e.o[0] = xxx;
+            e.o[1] = xyy;
+            e.o[2] = xxy;
+            e.o[3] = yyy;
+            e.o[4] = xxz;
+            e.o[5] = yyz;
+            e.o[6] = 2.f * xxy;
+            e.o[7] = 2.f * xxz;
+            e.o[8] = 2.f * xyz;
+            e.o[9] = 2.f * xyy;
+            e.o[10] = 2.f * yyz;
+            e.o[11] = 2.f * xzz;
+            e.o[12] = 2.f * yzz;
+            e.pad[0] = e.pad[1] = e.pad[2] = 0.f;
             oh[k] = e;
         }
         if (listed && g == 1) {
@@ -1262,6 +1270,20 @@ __device__ __forceinline__ void advance(uint64_t open, Off kn, Off ks, Off& k, N
     k = ok;
 }
 
+// The corrected walk's partial sums: the terms of PART_STEPS wavefront steps (at most one per
+// lane and step) summed in plain FP32, then added to the compensated sum.
+constexpr uint32_t PART_STEPS = 16;
+__device__ __forceinline__ void fold_partial(float& ax, float& ay, float& az, float& cx, float& cy,
+                                             float& cz, float& px, float& py, float& pz) {
+    float y = px - cx, t = ax + y;
+    cx = (t - ax) - y; ax = t;
+    y = py - cy; t = ay + y;
+    cy = (t - ay) - y; ay = t;
+    y = pz - cz; t = az + y;
+    cz = (t - az) - y; az = t;
+    px = py = pz = 0.f;
+}
+
 __device__ __forceinline__ float half_lo(uint32_t v) {
     return __half2float(__ushort_as_half(uint16_t(v)));
 }
@@ -1321,6 +1343,10 @@ __global__ __launch_bounds__(256) void bh_walk_kernel(const NodeRec* __restrict_
                                                       uint32_t lanes, int dd, int packed) {
     constexpr bool QUAD = MODE == WALK_QUAD || MODE == WALK_OCT;
     constexpr bool OCT = MODE == WALK_OCT;
+    // the geometric corrected walk is bound by its arithmetic: its terms go into plain partial
+    // sums (no per-term compensation); the relative one keeps the per-term form (with partial
+    // sums the compiler moved its cursor mask out of the scalar registers)
+    constexpr bool PARTIAL = QUAD && !REL;
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (packed) i = (i >> 6) * lanes + (threadIdx.x & 63);
     if (dd) {
@@ -1342,6 +1368,8 @@ __global__ __launch_bounds__(256) void bh_walk_kernel(const NodeRec* __restrict_
     const float4 p = pts[valid ? i : n - 1];
     const float A = REL ? est_in[(valid ? i : n - 1) >> est_shift] : 0.f;
     float ax = 0.f, ay = 0.f, az = 0.f, cx = 0.f, cy = 0.f, cz = 0.f;
+    float px = 0.f, py = 0.f, pz = 0.f;  // PARTIAL: the partial sums
+    uint32_t acc_wave = 0;               // QUAD: accepted terms of the wavefront
     using Off = std::conditional_t<WIDE, uint64_t, uint32_t>;
     Off resume = valid ? Off(0) : ~Off(0);
     uint32_t accepted = 0, steps = 0, nrows = 0;
@@ -1416,74 +1444,104 @@ __global__ __launch_bounds__(256) void bh_walk_kernel(const NodeRec* __restrict_
         if (acc) {
             const float r2 = REL ? r2_rel : add_rn(d2, eps2);
             const float inv = REL_INV ? inv_rel : __builtin_amdgcn_rsqf(r2);
-            const float s = (r.mass * inv) * (inv * inv);
-            float yx, yy, yz;
-            if (QUAD && __float_as_int(r.w2) >= 0) {  // internal (leaves: -inf)
-                // internal node: the monopole plus the field of Q = M w^2 y[0..5] (quad_field's
-                // formula, w2 = width^2) and, when the node has one, of O = M w^3 y[6..12]
-                // (oct_field's), summed as d (s + s5 t - 7 s7 G |d|^-2) - s5 Q d + 3 s7 F in one
-                // compensated term
-                const uint4 h = *reinterpret_cast<const uint4*>(
-                    reinterpret_cast<const char*>(qh) + (ko >> 1));  // 16-byte entries
-                OctH oe;
-                if constexpr (OCT) {
-                    oe = *reinterpret_cast<const OctH*>(reinterpret_cast<const char*>(oh) +
-                                                        (ko + (ko >> 1)));  // 48-byte entries
-                    // an empty asm that reads the whole entry here, before h is used: both
-                    // loads are then in flight together and one wait covers them (otherwise the
-                    // compiler fetches the flag first and the components behind its test)
-                    asm volatile("" ::"s"(oe.o[0]), "s"(oe.o[1]), "s"(oe.o[2]), "s"(oe.o[3]),
-                                 "s"(oe.o[4]), "s"(oe.o[5]), "s"(oe.o[6]), "s"(oe.o[7]),
-                                 "s"(oe.o[8]), "s"(oe.o[9]), "s"(oe.flag));
+            const float inv2 = inv * inv;
+            const float s = (r.mass * inv) * inv2;
+            if constexpr (QUAD) {
+                // the terms are chained onto v by FMAs: the vector terms first, in place, and the
+                // monopole's direction term, shared by leaves and internal nodes, last. v is the
+                // partial sum (PARTIAL) or the new term less the compensation, added at once.
+                float vx = PARTIAL ? px : -cx, vy = PARTIAL ? py : -cy, vz = PARTIAL ? pz : -cz;
+                float st = s;
+                if (__float_as_int(r.w2) >= 0) {  // internal (leaves: -inf)
+                    // internal node: the monopole plus the field of Q = M w^2 y[0..5]
+                    // (quad_field's formula, w2 = width^2) and, when the node has one, of
+                    // O = M w^3 y[6..12] (oct_field's): with u = |d|^-2 and s5 = M w^2 |d|^-5,
+                    // d (s + s5 u (5/2 dQd - 7 u G)) + s5 (3 u F - Q d), G = d.F, F = O d d
+                    const uint4 h = *reinterpret_cast<const uint4*>(
+                        reinterpret_cast<const char*>(qh) + (ko >> 1));  // 16-byte entries
+                    OctH oe;
+                    if constexpr (OCT) {
+                        oe = *reinterpret_cast<const OctH*>(reinterpret_cast<const char*>(oh) +
+                                                            (ko << 1));  // 64-byte entries
+                        // an empty asm that reads the whole entry here, before h is used: both
+                        // loads are then in flight together and one wait covers them (otherwise
+                        // the compiler fetches the components behind the flag's test)
+                        asm volatile("" ::"s"(oe.o[0]), "s"(oe.o[1]), "s"(oe.o[2]),
+                                     "s"(oe.o[3]), "s"(oe.o[4]), "s"(oe.o[5]), "s"(oe.o[6]),
+                                     "s"(oe.o[7]), "s"(oe.o[8]), "s"(oe.o[9]), "s"(oe.o[10]),
+                                     "s"(oe.o[11]), "s"(oe.o[12]));
+                    }
+                    const float q[6] = {half_lo(h.x), half_hi(h.x), half_lo(h.y),
+                                        half_hi(h.y), half_lo(h.z), half_hi(h.z)};
+                    // fma chains from +0 so that every half-precision product is one v_fma_mix
+                    const float qx = fmaf(q[4], dz, fmaf(q[3], dy, fmaf(q[0], dx, 0.f)));
+                    const float qy = fmaf(q[5], dz, fmaf(q[1], dy, fmaf(q[3], dx, 0.f)));
+                    const float qz = fmaf(q[2], dz, fmaf(q[5], dy, fmaf(q[4], dx, 0.f)));
+                    const float dqd = dx * qx + dy * qy + dz * qz;
+                    const float s5 = s * (r.w2 * inv2);  // M w^2 |d|^-5
+                    const float s5u = s5 * inv2;
+                    // the quadrupole, then the octupole added to the same values in place (an
+                    // if without an else: two exclusive branches cost register copies at their
+                    // join)
+                    st = fmaf(s5u, 2.5f * dqd, s);
+                    vx = fmaf(-s5, qx, vx);
+                    vy = fmaf(-s5, qy, vy);
+                    vz = fmaf(-s5, qz, vz);
+                    if (OCT && (h.w & 0xffffu)) {  // the flag: this node's octupole counts
+                        // F_i = O_ijk d_j d_k from the traceless form (OctH)
+                        const float* o = oe.o;
+                        const float ta = dx * dx - dz * dz, tb = dy * dy - dz * dz;
+                        const float pxy = dx * dy, pxz = dx * dz, pyz = dy * dz;
+                        const float fx = fmaf(o[0], ta, fmaf(o[1], tb, fmaf(o[6], pxy,
+                                         fmaf(o[7], pxz, o[8] * pyz))));
+                        const float fy = fmaf(o[2], ta, fmaf(o[3], tb, fmaf(o[9], pxy,
+                                         fmaf(o[8], pxz, o[10] * pyz))));
+                        const float fz = fmaf(o[4], ta, fmaf(o[5], tb, fmaf(o[8], pxy,
+                                         fmaf(o[11], pxz, o[12] * pyz))));
+                        const float gdd = dx * fx + dy * fy + dz * fz;
+                        // M w^3 |d|^-7 = s5 u w, the w being in oe.o
+                        st = fmaf(-7.f * s5u * inv2, gdd, st);
+                        const float s7x3 = 3.f * s5u;
+                        vx = fmaf(s7x3, fx, vx);
+                        vy = fmaf(s7x3, fy, vy);
+                        vz = fmaf(s7x3, fz, vz);
+                    }
                 }
-                const float q[6] = {half_lo(h.x), half_hi(h.x), half_lo(h.y),
-                                    half_hi(h.y), half_lo(h.z), half_hi(h.z)};
-                // fma chains from +0 so that every half-precision product is one v_fma_mix
-                const float qx = fmaf(q[4], dz, fmaf(q[3], dy, fmaf(q[0], dx, 0.f)));
-                const float qy = fmaf(q[5], dz, fmaf(q[1], dy, fmaf(q[3], dx, 0.f)));
-                const float qz = fmaf(q[2], dz, fmaf(q[5], dy, fmaf(q[4], dx, 0.f)));
-                const float dqd = dx * qx + dy * qy + dz * qz;
-                const float inv2 = inv * inv;
-                const float s5 = s * (r.w2 * inv2);  // M w^2 |d|^-5 from the monopole's M |d|^-3
-                float st = fmaf(s5, 2.5f * dqd * inv2, s);
-                const float ms5 = -s5;
-                // the vector terms chained onto the monopole's FMA, one FMA each
-                if (OCT && oe.flag) {  // this node's octupole counts
-                    // F_i = O_ijk d_j d_k over the ten components of w O (oe.o)
-                    const float pxx = dx * dx, pyy = dy * dy, pzz = dz * dz;
-                    const float pxy = 2.f * dx * dy, pxz = 2.f * dx * dz, pyz = 2.f * dy * dz;
-                    const float* o = oe.o;
-                    const float fx = fmaf(o[0], pxx, fmaf(o[5], pyy, fmaf(o[7], pzz,
-                                     fmaf(o[3], pxy, fmaf(o[4], pxz, o[6] * pyz)))));
-                    const float fy = fmaf(o[3], pxx, fmaf(o[1], pyy, fmaf(o[8], pzz,
-                                     fmaf(o[5], pxy, fmaf(o[6], pxz, o[9] * pyz)))));
-                    const float fz = fmaf(o[4], pxx, fmaf(o[9], pyy, fmaf(o[2], pzz,
-                                     fmaf(o[6], pxy, fmaf(o[7], pxz, o[8] * pyz)))));
-                    const float gdd = dx * fx + dy * fy + dz * fz;
-                    // M w^3 |d|^-7 = M w^2 |d|^-5 * w |d|^-2, the w being in oe.o
-                    const float s7 = s5 * inv2;
-                    st = fmaf(-7.f * s7, gdd * inv2, st);
-                    const float s7x3 = 3.f * s7;
-                    yx = fmaf(s7x3, fx, fmaf(ms5, qx, fmaf(dx, st, -cx)));
-                    yy = fmaf(s7x3, fy, fmaf(ms5, qy, fmaf(dy, st, -cy)));
-                    yz = fmaf(s7x3, fz, fmaf(ms5, qz, fmaf(dz, st, -cz)));
+                vx = fmaf(dx, st, vx);
+                vy = fmaf(dy, st, vy);
+                vz = fmaf(dz, st, vz);
+                if constexpr (PARTIAL) {
+                    px = vx;
+                    py = vy;
+                    pz = vz;
                 } else {
-                    yx = fmaf(ms5, qx, fmaf(dx, st, -cx));
-                    yy = fmaf(ms5, qy, fmaf(dy, st, -cy));
-                    yz = fmaf(ms5, qz, fmaf(dz, st, -cz));
+                    float t = ax + vx;
+                    cx = (t - ax) - vx; ax = t;
+                    t = ay + vy;
+                    cy = (t - ay) - vy; ay = t;
+                    t = az + vz;
+                    cz = (t - az) - vz; az = t;
                 }
             } else {
-                yx = fmaf(dx, s, -cx);
-                yy = fmaf(dy, s, -cy);
-                yz = fmaf(dz, s, -cz);
+                const float yx = fmaf(dx, s, -cx), yy = fmaf(dy, s, -cy), yz = fmaf(dz, s, -cz);
+                float t = ax + yx;
+                cx = (t - ax) - yx; ax = t;
+                t = ay + yy;
+                cy = (t - ay) - yy; ay = t;
+                t = az + yz;
+                cz = (t - az) - yz; az = t;
+                ++accepted;  // per lane (a wave-wide popcount costs scalar issue slots)
             }
-            float t = ax + yx;
-            cx = (t - ax) - yx; ax = t;
-            t = ay + yy;
-            cy = (t - ay) - yy; ay = t;
-            t = az + yz;
-            cz = (t - az) - yz; az = t;
-            ++accepted;  // per lane (a wave-wide popcount costs scalar issue slots the walk lacks)
+        }
+        if constexpr (QUAD) {
+            // the corrected walk's accepted terms, counted per wavefront from the accepting mask
+            // with scalar instructions; PARTIAL: the partial sums added with compensation every
+            // PART_STEPS wavefront steps (and once at the end)
+            acc_wave += uint32_t(__popcll(acc_mask));
+            if constexpr (PARTIAL) {
+                if ((steps & (PART_STEPS - 1)) == PART_STEPS - 1)
+                    fold_partial(ax, ay, az, cx, cy, cz, px, py, pz);
+            }
         }
         if (MODE == WALK_ROWS && __float_as_int(r.w2) >= 0) {  // internal: a row if accepted
             const unsigned long long m = acc_mask;
@@ -1502,6 +1560,7 @@ __global__ __launch_bounds__(256) void bh_walk_kernel(const NodeRec* __restrict_
         // skip > k for every node, and skip = k + 1 for a leaf, so the two candidates agree there
         advance<Off, REL>(open_mask, ko + Off(32), so, ko, r, r_next, r_skip);
     }
+    if constexpr (PARTIAL) fold_partial(ax, ay, az, cx, cy, cz, px, py, pz);
     if (MODE == WALK_ROWS && lane == 0 && wave_slot * 64 < n) {  // wavefronts holding particles
         rows.count[wave_slot] = nrows;
         atomicMax(&info->max_rows, nrows);
@@ -1514,6 +1573,7 @@ __global__ __launch_bounds__(256) void bh_walk_kernel(const NodeRec* __restrict_
     }
     unsigned long long acc64 = accepted;
     for (int off = 32; off > 0; off >>= 1) acc64 += __shfl_xor(acc64, off, 64);
+    acc64 += acc_wave;
     if (lane == 0) {
         atomicAdd(&info->steps, (unsigned long long)steps);
         atomicAdd(&info->accepted, acc64);
