@@ -663,10 +663,8 @@ __global__ __launch_bounds__(256) void bh_node_kernel(
             const double cnt_d = double(e - p);
             sums[k].m = make_double4(tot, tot * q.x, tot * q.y, tot * q.z);
             sums[k].p = make_double4(cnt_d * q.x, cnt_d * q.y, cnt_d * q.z, cnt_d);
-            if (mom) {  // one position: no spread about the centre
-                mom[k] = NodeMoments{};
+            if (mom)  // one position: no spread about the centre (its moments are never read)
                 quad[k] = NodeQuad{make_float4(0.f, 0.f, 0.f, 0.f), make_float4(0.f, 0.f, 0.f, 0.f)};
-            }
             rec[k].cx = q.x;
             rec[k].cy = q.y;
             rec[k].cz = q.z;
@@ -693,9 +691,11 @@ __device__ __forceinline__ void up_node(uint32_t k, NodeRec* __restrict__ rec,
     {
         const NodeRec r = rec[k];
         double4 a = make_double4(0, 0, 0, 0);
-        // with moments, in the same pass over the children: second moments about o (the first
-        // massive child's centre of mass) and first moments D = sum M_c (c_c - o), shifted to
-        // the centre of mass below
+        // with moments, in the same pass over the children: the second and third moments about
+        // o (the first massive child's centre of mass) by the parallel-axis rule, with
+        // u_c = c_c - o, S = sum T_c + M_c u_c u_c and U = sum U_c + sym(u_c, T_c) + M_c u_c u_c u_c,
+        // and the first moments D = sum M_c u_c; all shifted to the centre of mass below. A leaf
+        // child (skip = c + 1: one position) has no moments of its own and none is read.
         NodeMoments so;
         double ox = 0, oy = 0, oz = 0, dx_ = 0, dy_ = 0, dz_ = 0, mpos = 0;
         bool have_o = false;
@@ -703,26 +703,38 @@ __device__ __forceinline__ void up_node(uint32_t k, NodeRec* __restrict__ rec,
         while (c < r.skip) {
             const double4 s1 = sums[c].m;
             a.x += s1.x; a.y += s1.y; a.z += s1.z; a.w += s1.w;
+            const uint32_t nx = skipv[c];
             if (mom) {
                 if (s1.x > 0.0) {
-                    const NodeMoments tc = mom[c];
-                    const double cx = s1.y / s1.x, cy = s1.z / s1.x, cz = s1.w / s1.x;
+                    const NodeMoments tc = nx == c + 1 ? NodeMoments{} : mom[c];
+                    const double im = 1.0 / s1.x;
+                    const double cx = s1.y * im, cy = s1.z * im, cz = s1.w * im;
                     if (!have_o) {
                         ox = cx; oy = cy; oz = cz;
                         have_o = true;
                     }
                     const double ux = cx - ox, uy = cy - oy, uz = cz - oz;
-                    so.xx += tc.xx + s1.x * ux * ux;
-                    so.yy += tc.yy + s1.x * uy * uy;
-                    so.zz += tc.zz + s1.x * uz * uz;
-                    so.xy += tc.xy + s1.x * ux * uy;
-                    so.xz += tc.xz + s1.x * ux * uz;
-                    so.yz += tc.yz + s1.x * uy * uz;
-                    dx_ += s1.x * ux; dy_ += s1.x * uy; dz_ += s1.x * uz;
-                    mpos += s1.x;
+                    const double M = s1.x;
+                    so.xx += tc.xx + M * ux * ux;
+                    so.yy += tc.yy + M * uy * uy;
+                    so.zz += tc.zz + M * uz * uz;
+                    so.xy += tc.xy + M * ux * uy;
+                    so.xz += tc.xz + M * ux * uz;
+                    so.yz += tc.yz + M * uy * uz;
+                    so.u[0] += tc.u[0] + 3.0 * ux * tc.xx + M * ux * ux * ux;
+                    so.u[1] += tc.u[1] + 3.0 * uy * tc.yy + M * uy * uy * uy;
+                    so.u[2] += tc.u[2] + 3.0 * uz * tc.zz + M * uz * uz * uz;
+                    so.u[3] += tc.u[3] + 2.0 * ux * tc.xy + uy * tc.xx + M * ux * ux * uy;
+                    so.u[4] += tc.u[4] + 2.0 * ux * tc.xz + uz * tc.xx + M * ux * ux * uz;
+                    so.u[5] += tc.u[5] + ux * tc.yy + 2.0 * uy * tc.xy + M * ux * uy * uy;
+                    so.u[6] += tc.u[6] + 2.0 * uy * tc.yz + uz * tc.yy + M * uy * uy * uz;
+                    so.u[7] += tc.u[7] + ux * tc.zz + 2.0 * uz * tc.xz + M * ux * uz * uz;
+                    so.u[8] += tc.u[8] + uy * tc.zz + 2.0 * uz * tc.yz + M * uy * uz * uz;
+                    so.u[9] += tc.u[9] + ux * tc.yz + uy * tc.xz + uz * tc.xy + M * ux * uy * uz;
+                    dx_ += M * ux; dy_ += M * uy; dz_ += M * uz;
+                    mpos += M;
                 }
             }
-            const uint32_t nx = skipv[c];
             c = nx > c ? nx : c + 1;
         }
         sums[k].m = a;
@@ -782,30 +794,23 @@ __device__ __forceinline__ void up_node(uint32_t k, NodeRec* __restrict__ rec,
                 qd.b = make_float4(float(3.0 * t.xz * f), float(3.0 * t.yz * f), 0.f, 0.f);
             }
             if (a.x > 0.0 && width[k] > 0.0f) {
-                // third moments about the centre of mass p by the parallel-axis rule,
-                // U = sum over massive children of U_c + sym(u_c, T_c) + M_c u_c u_c u_c with
-                // u_c = c_c - p, then the normalised octupole from their traceless part
-                const double px = a.y / a.x, py = a.z / a.x, pz = a.w / a.x;
-                for (size_t q = k + 1; q < r.skip;) {
-                    const double4 s1 = sums[q].m;
-                    if (s1.x > 0.0) {
-                        const NodeMoments& mc = mom[q];
-                        const double ux = s1.y / s1.x - px, uy = s1.z / s1.x - py,
-                                     uz = s1.w / s1.x - pz;
-                        const double M = s1.x;
-                        t.u[0] += mc.u[0] + 3.0 * ux * mc.xx + M * ux * ux * ux;
-                        t.u[1] += mc.u[1] + 3.0 * uy * mc.yy + M * uy * uy * uy;
-                        t.u[2] += mc.u[2] + 3.0 * uz * mc.zz + M * uz * uz * uz;
-                        t.u[3] += mc.u[3] + 2.0 * ux * mc.xy + uy * mc.xx + M * ux * ux * uy;
-                        t.u[4] += mc.u[4] + 2.0 * ux * mc.xz + uz * mc.xx + M * ux * ux * uz;
-                        t.u[5] += mc.u[5] + ux * mc.yy + 2.0 * uy * mc.xy + M * ux * uy * uy;
-                        t.u[6] += mc.u[6] + 2.0 * uy * mc.yz + uz * mc.yy + M * uy * uy * uz;
-                        t.u[7] += mc.u[7] + ux * mc.zz + 2.0 * uz * mc.xz + M * ux * uz * uz;
-                        t.u[8] += mc.u[8] + uy * mc.zz + 2.0 * uz * mc.yz + M * uy * uz * uz;
-                        t.u[9] += mc.u[9] + ux * mc.yz + uy * mc.xz + uz * mc.xy + M * ux * uy * uz;
-                    }
-                    const uint32_t nx = skipv[q];
-                    q = nx > q ? nx : q + 1;
+                // third moments about the centre of mass p = o + v:
+                // U_ijk - (v_i S_jk + v_j S_ik + v_k S_ij) + (v_i v_j D_k + v_i v_k D_j + v_j v_k D_i)
+                //   - M v_i v_j v_k, then the normalised octupole from their traceless part
+                const double v[3] = {a.y / a.x - ox, a.z / a.x - oy, a.w / a.x - oz};
+                const double S[3][3] = {{so.xx, so.xy, so.xz}, {so.xy, so.yy, so.yz},
+                                        {so.xz, so.yz, so.zz}};
+                const double D[3] = {dx_, dy_, dz_};
+                // index triples of {xxx, yyy, zzz, xxy, xxz, xyy, yyz, xzz, yzz, xyz}
+                constexpr int I3[10][3] = {{0, 0, 0}, {1, 1, 1}, {2, 2, 2}, {0, 0, 1}, {0, 0, 2},
+                                           {0, 1, 1}, {1, 1, 2}, {0, 2, 2}, {1, 2, 2}, {0, 1, 2}};
+#pragma unroll
+                for (int q = 0; q < 10; ++q) {
+                    const int i0 = I3[q][0], i1 = I3[q][1], i2 = I3[q][2];
+                    t.u[q] = so.u[q] - (v[i0] * S[i1][i2] + v[i1] * S[i0][i2] + v[i2] * S[i0][i1]) +
+                             (v[i0] * v[i1] * D[i2] + v[i0] * v[i2] * D[i1] +
+                              v[i1] * v[i2] * D[i0]) -
+                             mpos * v[i0] * v[i1] * v[i2];
                 }
                 // traceless part: U~_ijk = U_ijk - (d_ij V_k + d_ik V_j + d_jk V_i) / 5, V_k = U_llk
                 const double* u = t.u;
