@@ -1323,8 +1323,9 @@ __device__ __forceinline__ float half_hi(uint32_t v) {
 // packed (not with REL): the wavefront's particles are consecutive, `lanes` of them from
 // (launch wavefront index) * lanes, instead of 64 from (launch wavefront index) * 64. Narrow
 // wavefronts walk smaller unions of their particles' traversals, so that a small problem runs as
-// many short walks instead of a few long ones; every lane's result is the same either way (it
-// adds exactly the nodes of its own pre-order traversal, in that order).
+// many short walks instead of a few long ones; every lane adds exactly the nodes of its own
+// pre-order traversal, in that order, either way, so the plain walk's results are bit-identical;
+// the corrected walk's partial sums are grouped by wavefront steps, so its last bits can differ.
 // dd: a capturable build's tree (`total` is then its capacity; the count is read from `info`).
 // WALK_OCT: the network has octupole outputs; every accepted internal node's octupole entry is
 // fetched together with its quadrupole entry, so that the two loads overlap. A mode of its own

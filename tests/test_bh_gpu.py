@@ -246,8 +246,9 @@ def test_sort_size_classes_with_ties(res, orc, m):
 def test_wavefront_width_changes_no_result(res, monkeypatch, n):
     """Each lane adds exactly the nodes of its own traversal, in pre-order, whatever particles
     share its wavefront: walks with 1 to 64 particles per wavefront (PNB_WALK_LANES; small
-    problems take narrow ones by default) give bit-identical results, plain and corrected, for one
-    set and for two."""
+    problems take narrow ones by default) give bit-identical plain results, for one set and for
+    two. The corrected walk adds the same terms, but in partial sums folded every 16 wavefront
+    steps, whose grouping follows the wavefront's steps: its results agree to rounding."""
     from particular_amd import learned
 
     s = workloads.direct_sum_case(n, seed=62)
@@ -261,8 +262,13 @@ def test_wavefront_width_changes_no_result(res, monkeypatch, n):
                        for th, m, st in ((0.5, False, s.particles), (0.5, True, s.particles),
                                          (0.7, False, pa.Between(other, s.particles)))]
     for lanes in ("16", "3", "1"):
-        for a, b in zip(runs["64"], runs[lanes]):
-            np.testing.assert_array_equal(a, b, err_msg=f"{lanes} lanes")
+        for k, (a, b) in enumerate(zip(runs["64"], runs[lanes])):
+            if k == 1:  # corrected: the partial sums' grouping
+                row = np.linalg.norm(a - b, axis=1) / np.linalg.norm(b, axis=1)
+                print(f"{lanes} lanes: corrected walk, largest row difference {row.max():.2e}")
+                assert metrics.rms_rel(a[:, :3], b[:, :3]) < 1e-6 and row.max() < 1e-5
+            else:
+                np.testing.assert_array_equal(a, b, err_msg=f"{lanes} lanes")
     monkeypatch.delenv("PNB_WALK_LANES")
     default = pa.BarnesHut(res, 0.5, pa.Acceleration.checked()).compute(s.particles)
     np.testing.assert_array_equal(default, runs["64"][0])
@@ -355,10 +361,12 @@ def test_relative_criterion_matches_oracle(res, orc, alpha, per_particle):
 
 
 def test_relative_criterion_limits_are_geometric_walks(res):
-    """Without buckets, alpha = 1e30 never opens by the relative rule: bit-identical to the
-    geometric walk at the same theta (plain and corrected); alpha = 0 opens every massive internal
-    node: bit-identical to theta = 0 (where the corrected walk, accepting only leaves, equals the
-    plain one). With buckets, alpha = 0 sums the same pair terms as theta = 0 in another order."""
+    """Without buckets, alpha = 1e30 never opens by the relative rule: the geometric walk at the
+    same theta, bit-identical for the plain walk, to rounding for the corrected one (its geometric
+    form sums in partial sums, its relative form term by term); alpha = 0 opens every massive
+    internal node: bit-identical to theta = 0 (where the corrected walk, accepting only leaves,
+    equals the plain one). With buckets, alpha = 0 sums the same pair terms as theta = 0 in
+    another order."""
     from particular_amd import learned
 
     p = workloads.direct_sum_case(30000, seed=43).particles
@@ -368,7 +376,12 @@ def test_relative_criterion_limits_are_geometric_walks(res):
     every = pa.BarnesHut(res, 0.0, chk).compute(p)
     for mlp in (False, True):
         geo = pa.BarnesHut(res, 0.6, chk, mlp=mlp).compute(p)
-        np.testing.assert_array_equal(rel_bh(res, 0.6, 1e30, mlp=mlp, bucket=0).compute(p), geo)
+        big = rel_bh(res, 0.6, 1e30, mlp=mlp, bucket=0).compute(p)
+        if mlp:
+            row = np.linalg.norm(big - geo, axis=1) / np.linalg.norm(geo, axis=1)
+            assert metrics.rms_rel(big, geo) < 1e-6 and row.max() < 1e-5, row.max()
+        else:
+            np.testing.assert_array_equal(big, geo)
         np.testing.assert_array_equal(rel_bh(res, 0.6, 0.0, mlp=mlp, bucket=0).compute(p), every)
         np.testing.assert_allclose(rel_bh(res, 0.6, 0.0, mlp=mlp).compute(p), every, rtol=1e-5,
                                    atol=1e-7)
