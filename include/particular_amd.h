@@ -203,9 +203,12 @@ int pnb_reordered_bh_accel_f32_dev(pnb_ctx *ctx, const float *particles, size_t 
  * context's stream (pnb_set_stream) and replayed with new particle data. A later tree that
  * needs more nodes or levels than were sized is flagged on the device: its accelerations are NaN
  * and pnb_bh_check then returns PNB_E_STATE (call again outside a capture to size anew). The other
- * flags (PNB_BH_MLP, PNB_BH_REF_COM, PNB_BH_RELATIVE) are refused in this mode; the Reordered
- * entry points run in it but read the massive count on the host, so they are refused inside a
- * capture. Changing the mode forgets the sizes. */
+ * learned correction (PNB_BH_MLP) runs in this mode too: the node moments and the network pass
+ * over the internal nodes use the device's counts, and the sizing call's flag is part of the
+ * sizes (a call with the other flag sizes anew, so inside a capture it is refused). The other
+ * flags (PNB_BH_REF_COM, PNB_BH_RELATIVE) are refused in this mode; the Reordered entry points
+ * run in it but read the massive count on the host, so they are refused inside a capture.
+ * Changing the mode forgets the sizes. */
 int pnb_bh_set_capturable(pnb_ctx *ctx, int enable);
 /* Waits for the context's stream and reports the status of the last capturable build: PNB_OK,
  * PNB_E_STATE (did not fit its sizes), PNB_E_DEPTH or PNB_E_ARG (non-finite positions). */

@@ -191,7 +191,8 @@ class HipResources:
 
     def set_bh_capturable(self, enable: bool = True) -> None:
         """Barnes-Hut without host round trips (pnb_bh_set_capturable): after one ordinary call
-        at given sizes, device-pointer calls can be captured in a graph and replayed."""
+        at given sizes (and with the same mlp flag), device-pointer calls, plain or with the
+        learned correction, can be captured in a graph and replayed."""
         self.check(_lib.lib().pnb_bh_set_capturable(self.handle, int(enable)))
 
     def bh_check(self) -> None:

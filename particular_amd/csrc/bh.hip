@@ -146,7 +146,7 @@ struct BhState {
     DevBuf info, bbox_part, hist, bits, afg_tmp;
     // capturable builds (pnb_bh_set_capturable): sized once by an ordinary build, then no host
     // round trip; dd_last: the last build's node count and status are only on the device
-    bool dd_mode = false, dd_last = false;
+    bool dd_mode = false, dd_last = false, dd_mlp = false;
     size_t dd_m = 0, dd_n = 0, dd_cap = 0;
     int dd_levels = 0;
     std::vector<uint32_t> last_lvl;  // internal nodes per level of the last ordinary build
@@ -885,17 +885,22 @@ __global__ void bh_dd_check_kernel(const uint32_t* __restrict__ S, size_t m, uin
 
 // The bottom-up sums of one level, its count and place in the level-grouped list read from the
 // device (the host launches every level up to the sized depth; levels the tree lacks are empty).
+// With `mom` (the learned correction), also the node moments, as in the ordinary build.
 __global__ __launch_bounds__(256) void bh_up_level_dd_kernel(NodeRec* __restrict__ rec,
                                                              NodeSums* __restrict__ sums,
                                                              const uint32_t* __restrict__ ilist,
                                                              const BhInfo* __restrict__ info,
                                                              int level,
                                                              const uint32_t* __restrict__ skipv,
-                                                             const float4* __restrict__ sp) {
+                                                             const float4* __restrict__ sp,
+                                                             NodeMoments* __restrict__ mom,
+                                                             NodeQuad* __restrict__ quad,
+                                                             NodeDesc* __restrict__ desc,
+                                                             const float* __restrict__ width) {
     if (info->flags) return;
     const uint32_t cnt = info->lvl_count[level], off = info->lvl_off[level];
     for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < cnt; t += gridDim.x * blockDim.x)
-        up_node(ilist[off + t], rec, sums, nullptr, nullptr, nullptr, nullptr, skipv, sp);
+        up_node(ilist[off + t], rec, sums, mom, quad, desc, width, skipv, sp);
 }
 
 // A run of levels hi, hi - 1, ..., lo in one workgroup, a barrier between levels (levels the tree
@@ -906,12 +911,16 @@ __global__ __launch_bounds__(1024) void bh_up_levels_dd_kernel(NodeRec* __restri
                                                                const BhInfo* __restrict__ info,
                                                                int hi, int lo,
                                                                const uint32_t* __restrict__ skipv,
-                                                               const float4* __restrict__ sp) {
+                                                               const float4* __restrict__ sp,
+                                                               NodeMoments* __restrict__ mom,
+                                                               NodeQuad* __restrict__ quad,
+                                                               NodeDesc* __restrict__ desc,
+                                                               const float* __restrict__ width) {
     if (info->flags) return;  // uniform: every thread returns, or none
     for (int L = hi; L >= lo; --L) {
         const uint32_t cnt = info->lvl_count[L], off = info->lvl_off[L];
         for (uint32_t t = threadIdx.x; t < cnt; t += blockDim.x)
-            up_node(ilist[off + t], rec, sums, nullptr, nullptr, nullptr, nullptr, skipv, sp);
+            up_node(ilist[off + t], rec, sums, mom, quad, desc, width, skipv, sp);
         __syncthreads();
     }
 }
@@ -1069,7 +1078,8 @@ constexpr float OCT_MIN = 3e-3f;
 
 // qh[k] = MLP(features of node k) for the internal nodes listed in inodes (zero for massless
 // ones; the walk reads qh only for internal nodes): 16 nodes per MFMA tile, one tile per
-// wavefront and iteration, weights staged in LDS.
+// wavefront and iteration, weights staged in LDS. dd_info (a capturable build): the count is the
+// build's internal-node total, read on the device, and a build that did not fit writes nothing.
 __global__ __launch_bounds__(256) void mlp_node_kernel(const NodeRec* __restrict__ rec,
                                                        const NodeQuad* __restrict__ quad,
                                                        const uint8_t* __restrict__ level,
@@ -1078,7 +1088,12 @@ __global__ __launch_bounds__(256) void mlp_node_kernel(const NodeRec* __restrict
                                                        uint32_t count,
                                                        const MlpWeights* __restrict__ wg,
                                                        QuadH* __restrict__ qh,
-                                                       OctH* __restrict__ oh) {
+                                                       OctH* __restrict__ oh,
+                                                       const BhInfo* __restrict__ dd_info) {
+    if (dd_info) {
+        if (dd_info->flags) return;  // uniform
+        count = dd_info->lvl_off[LVL_SLOTS - 1] + dd_info->lvl_count[LVL_SLOTS - 1];
+    }
     __shared__ MlpForward w;
     {
         const uint32_t* src = reinterpret_cast<const uint32_t*>(wg);
@@ -1775,7 +1790,7 @@ static bool stream_capturing(pnb_ctx* c) {
 // Capacities of a capturable build from an ordinary build of the same size (its node count and
 // depth known on the host): a quarter more nodes and 8 more levels, so that later trees of
 // similar inputs fit.
-static int dd_size(pnb_ctx* c, size_t m) {
+static int dd_size(pnb_ctx* c, size_t m, bool mlp) {
     BhState* s = c->bh;
     const size_t cap = s->nodes + s->nodes / 4 + 1024;
     if (cap + 1 >= (size_t(1) << 27))
@@ -1786,6 +1801,14 @@ static int dd_size(pnb_ctx* c, size_t m) {
     PNB_HIP(c, s->width.ensure(cap * sizeof(float)));
     PNB_HIP(c, s->skipv.ensure(cap * sizeof(uint32_t)));
     PNB_HIP(c, s->ilist.ensure(cap * sizeof(uint32_t)));
+    if (mlp) {  // the learned correction's node data and walk entries
+        PNB_HIP(c, s->mom.ensure(cap * sizeof(NodeMoments)));
+        PNB_HIP(c, s->quad.ensure(cap * sizeof(NodeQuad)));
+        PNB_HIP(c, s->desc.ensure(cap * sizeof(NodeDesc)));
+        PNB_HIP(c, s->qh.ensure((cap + 1) * sizeof(QuadH)));
+        PNB_HIP(c, s->oh.ensure((cap + 1) * sizeof(OctH)));
+    }
+    s->dd_mlp = mlp;
     s->dd_cap = cap;
     s->dd_levels = std::min(std::max(s->max_level, 0) + 8, MAX_LEVEL + 1);
     s->dd_plan = s->last_lvl;
@@ -1796,7 +1819,7 @@ static int dd_size(pnb_ctx* c, size_t m) {
 // The octree of a capturable build: the ordinary build's kernels with no host round trip (every
 // digit of the sort, the node count checked and used on the device, the bottom-up sums launched
 // for every level up to the sized depth). Buffers were sized by dd_size for these m.
-static int build_tree_dd(pnb_ctx* c, const float4* afg, size_t m) {
+static int build_tree_dd(pnb_ctx* c, const float4* afg, size_t m, bool moments) {
     BhState* s = c->bh;
     s->built = false;
     s->has_moments = false;
@@ -1853,9 +1876,13 @@ static int build_tree_dd(pnb_ctx* c, const float4* afg, size_t m) {
         PNB_CHECK_LAUNCH(c);
         NodeRec* rec = s->rec.as<NodeRec>();
         uint32_t* skipv = s->skipv.as<uint32_t>();
+        NodeMoments* mom = moments ? s->mom.as<NodeMoments>() : nullptr;
+        NodeQuad* quad = moments ? s->quad.as<NodeQuad>() : nullptr;
+        NodeDesc* desc = moments ? s->desc.as<NodeDesc>() : nullptr;
+        const float* width = s->width.as<float>();
         bh_node_kernel<<<g, 256, 0, c->stream>>>(sk, sp, lcp, s->ip.as<int>(), cnt, m, info, rec,
                                                  s->sums.as<NodeSums>(), s->level.as<uint8_t>(),
-                                                 s->width.as<float>(), nullptr, nullptr, skipv);
+                                                 s->width.as<float>(), mom, quad, skipv);
         PNB_CHECK_LAUNCH(c);
         uint32_t* ilist = s->ilist.as<uint32_t>();
         bh_level_list_kernel<<<lg, 256, 0, c->stream>>>(lcp, s->ip.as<int>(), cnt, m, lvl_cur,
@@ -1874,7 +1901,8 @@ static int build_tree_dd(pnb_ctx* c, const float4* afg, size_t m) {
                 const uint32_t n_l = planned(L);
                 const unsigned ug = std::min<unsigned>(div_up(n_l + n_l / 4, 256u), max_grid);
                 bh_up_level_dd_kernel<<<ug, 256, 0, c->stream>>>(rec, s->sums.as<NodeSums>(),
-                                                                 ilist, info, L, skipv, sp);
+                                                                 ilist, info, L, skipv, sp, mom,
+                                                                 quad, desc, width);
                 PNB_CHECK_LAUNCH(c);
                 --L;
                 continue;
@@ -1882,9 +1910,11 @@ static int build_tree_dd(pnb_ctx* c, const float4* afg, size_t m) {
             const int hi = L;
             while (L >= 0 && planned(L) <= UP_RUN_NODES / 2) --L;
             bh_up_levels_dd_kernel<<<1, 1024, 0, c->stream>>>(rec, s->sums.as<NodeSums>(), ilist,
-                                                              info, hi, L + 1, skipv, sp);
+                                                              info, hi, L + 1, skipv, sp, mom,
+                                                              quad, desc, width);
             PNB_CHECK_LAUNCH(c);
         }
+        s->has_moments = moments;
         s->built = true;
         return PNB_OK;
     });
@@ -2180,19 +2210,22 @@ static int node_network(pnb_ctx* c) {
     const MlpWeights* wg = mlp_weights(c);
     if (!wg) return fail(c, PNB_E_STATE, "PNB_BH_MLP needs an initialised MLP (pnb_mlp_init)");
     if (!s->has_moments) return fail(c, PNB_E_STATE, "octree built without node moments");
-    const size_t total = s->nodes;
-    // one entry past the last node, as for the records (the walk prefetches k + 1 and skip)
+    const size_t total = s->nodes;  // a capturable build: its capacity
+    // one entry past the last node, as for the records (the walk prefetches k + 1 and skip); a
+    // capturable build's entries were sized with its buffers, so nothing grows here
     PNB_HIP(c, s->qh.ensure((size_t(total) + 1) * sizeof(QuadH)));
     PNB_HIP(c, s->oh.ensure((size_t(total) + 1) * sizeof(OctH)));
     if (total == 0) return PNB_OK;
-    if (s->internal == 0) return PNB_OK;
+    const bool dd = s->dd_last;  // the internal-node count is on the device
+    if (!dd && s->internal == 0) return PNB_OK;
     return timed(c, PNB_PROF_MLP, [&]() -> int {
-        const size_t tiles = div_up(s->internal, 16);
+        const size_t tiles = div_up(dd ? total : s->internal, 16);
         const unsigned grid = unsigned(std::min<size_t>(div_up(tiles, 4), 16 * size_t(std::max(c->cu_count, 1))));
         mlp_node_kernel<<<grid, 256, 0, c->stream>>>(s->rec.as<NodeRec>(), s->quad.as<NodeQuad>(),
                                                      s->level.as<uint8_t>(), s->desc.as<NodeDesc>(),
                                                      s->ilist.as<uint32_t>(), uint32_t(s->internal),
-                                                     wg, s->qh.as<QuadH>(), s->oh.as<OctH>());
+                                                     wg, s->qh.as<QuadH>(), s->oh.as<OctH>(),
+                                                     dd ? s->info.as<BhInfo>() : nullptr);
         PNB_CHECK_LAUNCH(c);
         return PNB_OK;
     });
@@ -2210,16 +2243,18 @@ static int accel_dev(pnb_ctx* c, const float4* affected, size_t n, const float4*
     if (!c->bh) c->bh = new BhState();
     bool dd = false;
     if (c->bh->dd_mode) {
-        if (mlp || ref_com || rel)
-            return fail(c, PNB_E_ARG, "capturable Barnes-Hut takes none of PNB_BH_MLP, "
-                                      "PNB_BH_REF_COM and PNB_BH_RELATIVE");
+        if (ref_com || rel)
+            return fail(c, PNB_E_ARG, "capturable Barnes-Hut takes neither PNB_BH_REF_COM nor "
+                                      "PNB_BH_RELATIVE");
         BhState* s = c->bh;
-        const bool sized = s->dd_cap > 0 && s->dd_m == m && ((affected == afg && n == m) || s->dd_n == n);
+        const bool sized = s->dd_cap > 0 && s->dd_m == m && s->dd_mlp == mlp &&
+                           ((affected == afg && n == m) || s->dd_n == n);
         if (sized) {
             dd = true;
         } else if (stream_capturing(c)) {
             return fail(c, PNB_E_STATE, "capturable Barnes-Hut: call once outside the capture "
-                                        "with these sizes first (it sizes the tree's buffers)");
+                                        "with these sizes and flags first (it sizes the tree's "
+                                        "buffers)");
         }
     }
     // inside an outside capture: no events on the stream
@@ -2231,17 +2266,17 @@ static int accel_dev(pnb_ctx* c, const float4* affected, size_t n, const float4*
         ~Restore() { c->capturing = v; }
     } restore{c, was_capturing};
     if (dd) {
-        if (int rc = build_tree_dd(c, afg, m)) return rc;
+        if (int rc = build_tree_dd(c, afg, m, mlp)) return rc;
     } else {
         if (int rc = build_tree(c, afg, m, ref_com, mlp)) return rc;
         if (c->bh->dd_mode) {
             // The sizing call of a capturable build: capacities from this tree, then the tree
             // again in the sized buffers (growing a buffer discards its contents, so the tree
             // just built may be gone), built the capturable way.
-            if (int rc = dd_size(c, m)) return rc;
+            if (int rc = dd_size(c, m, mlp)) return rc;
             c->bh->dd_n = n;
             dd = true;
-            if (int rc = build_tree_dd(c, afg, m)) return rc;
+            if (int rc = build_tree_dd(c, afg, m, mlp)) return rc;
         }
     }
     BhState* s = c->bh;

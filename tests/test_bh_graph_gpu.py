@@ -1,8 +1,8 @@
 """Capturable Barnes-Hut (pnb_bh_set_capturable): once sized by an ordinary call, the build and
 walk make no host round trip, so they can be captured in a hipGraph. The tree, its node data and
-the accelerations are those of the ordinary build, bit for bit; graph replays with new positions
-equal eager calls on the same positions; a tree that outgrows the sizes gives NaN and an error
-from pnb_bh_check (never a hang)."""
+the accelerations (plain and with the learned correction) are those of the ordinary build, bit
+for bit; graph replays with new positions equal eager calls on the same positions; a tree that
+outgrows the sizes gives NaN and an error from pnb_bh_check (never a hang)."""
 import numpy as np
 import pytest
 
@@ -133,6 +133,9 @@ def test_refused_flags(cres):
     with pytest.raises(pa.PnbError) as e:
         pa.BarnesHut(cres, 0.5, pa.Acceleration.checked(), reference_com=True).compute(dev)
     assert e.value.status == _lib.PNB_E_ARG
+    with pytest.raises(pa.PnbError) as e:
+        pa.BarnesHut(cres, 1.0, pa.Acceleration.checked(), relative=1e-4).compute(dev)
+    assert e.value.status == _lib.PNB_E_ARG
     cres.set_bh_capturable(False)
     _bh(cres, dev)  # the ordinary mode again
 
@@ -159,5 +162,60 @@ def test_reordered(cres):
     with torch.cuda.graph(g):
         with pytest.raises(pa.PnbError) as e:
             alg.compute(pa.Reordered(dev))
+    cres.reset_stream()
+    assert e.value.status == _lib.PNB_E_STATE
+
+
+@pytest.mark.parametrize("network", ["quadrupole", "octupole"])
+def test_corrected_walk(cres, network):
+    """The learned correction in the capturable mode: the node moments summed on the sizing
+    build's level plan, the network over the internal nodes counted on the device, the corrected
+    walk; eager calls equal the ordinary mode's bit for bit, graph replays with new positions
+    equal eager calls, and a call whose flags differ from the sizing call's is refused inside a
+    capture."""
+    import torch
+
+    from particular_amd import learned
+
+    learned.init(cres, 0)
+    learned.set_params(cres, learned.quadrupole_params() if network == "quadrupole"
+                       else learned.octupole_params())
+    n = 20_000
+    s = workloads.two_plummer_case(n, seed=81)
+    dev = torch.from_numpy(s.particles.copy()).cuda()
+    both = pa.Between(dev, dev)
+    alg = pa.BarnesHut(cres, 0.5, pa.Acceleration.checked(), mlp=True)
+    ref = alg.compute(both).cpu().numpy()
+    plain = _bh(cres, dev).cpu().numpy()
+    assert np.abs(ref - plain).max() > 0  # the correction is applied
+    cres.set_bh_capturable(True)
+    np.testing.assert_array_equal(alg.compute(both).cpu().numpy(), ref)  # sizes
+    np.testing.assert_array_equal(alg.compute(both).cpu().numpy(), ref)  # no round trip
+    cres.bh_check()
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        static_out = alg.compute(both)
+    cres.reset_stream()
+    rng = np.random.default_rng(2)
+    for step in range(3):
+        p = s.particles.copy()
+        p[:, :3] += rng.normal(scale=2e-3, size=(n, 3)).astype(np.float32)
+        dev.copy_(torch.from_numpy(p))
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        got = static_out.cpu().numpy()
+        cres.bh_check()
+        eager = alg.compute(both).cpu().numpy()
+        cres.bh_check()
+        np.testing.assert_array_equal(got, eager)
+        assert np.all(np.isfinite(got))
+        print(f"replay {step}: bit-identical to the eager call", flush=True)
+    # the plain walk was not sized with these buffers: refused inside a capture
+    g2 = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g2):
+        with pytest.raises(pa.PnbError) as e:
+            _bh(cres, dev)
     cres.reset_stream()
     assert e.value.status == _lib.PNB_E_STATE
