@@ -1052,28 +1052,33 @@ __device__ __forceinline__ void node_features(const NodeRec& r, const NodeQuad& 
     for (int i = 0; i < 7; ++i) f[9 + i] = OCT_FEATURE_SCALE * dsc.oct[i];
 }
 
-// The learned moments of a node, in two entries read with scalar loads when a lane accepts it.
-// QuadH (16 bytes): the quadrupole y[0..5] (units of M w^2) as halves and a flag, 1 when the
-// node's octupole (OctH) is worth adding: its largest coefficient is at least OCT_MIN, which
-// leaves out nodes whose octupole vanishes (two equal masses) or nearly does, about a third of
-// the accepted ones on Plummer spheres, at no measurable cost in accuracy. y is O(1) and the
+// The learned moments of a node, read with scalar loads when a lane accepts it: a QuadH entry
+// when the network has only quadrupole outputs, an OctH entry (which holds the quadrupole too)
+// when it has octupole outputs.
+// QuadH (16 bytes): the quadrupole y[0..5] (units of M w^2) as halves. y is O(1) and the
 // correction it feeds is itself a small fraction of the force, so 11-bit mantissas cost nothing
 // measurable, while the walk's scalar loads take half of what floats would.
-// OctH (64 bytes): the octupole y[6..12] (units of M w^3) times the node's width w, in FP32, laid
-// out for F_i = O_ijk d_j d_k written with the trace removed, O_izz = -(O_ixx + O_iyy):
-//   F_x = O_xxx a + O_xyy b + 2 O_xxy xy + 2 O_xxz xz + 2 O_xyz yz,
-//   F_y = O_xxy a + O_yyy b + 2 O_xyy xy + 2 O_xyz xz + 2 O_yyz yz,
+// OctH (64 bytes, one scalar load): the octupole y[6..12] (units of M w^3) times the node's width
+// w, in FP32, laid out for F_i = O_ijk d_j d_k written with the trace removed,
+// O_izz = -(O_ixx + O_iyy):
+//   F_x = O_xxx a + O_xyy b + O_xxy (2xy) + 2 O_xxz xz + 2 O_xyz yz,
+//   F_y = O_xxy a + O_yyy b + O_xyy (2xy) + 2 O_xyz xz + 2 O_yyz yz,
 //   F_z = O_xxz a + O_yyz b + 2 O_xyz xy + 2 O_xzz xz + 2 O_yzz yz,  a = xx - zz, b = yy - zz,
-// o = {xxx, xyy, xxy, yyy, xxz, yyz, 2xxy, 2xxz, 2xyz, 2xyy, 2yyz, 2xzz, 2yzz}: one FMA per
-// coefficient and product (a vector instruction reads one scalar operand here), fifteen for F,
-// and no square root for w^3.
+// o = {xxx, xyy, xxy, yyy, xxz, yyz, 2xxz, 2xyz, 2yyz, 2xzz, 2yzz}: one FMA per coefficient and
+// product (a vector instruction reads one scalar operand here), fifteen for F, and no square
+// root for w^3; then the quadrupole as in QuadH and a flag, non-zero when the octupole is worth
+// adding: its largest coefficient is at least OCT_MIN, which leaves out nodes whose octupole
+// vanishes (two equal masses) or nearly does, about a third of the accepted ones on Plummer
+// spheres, at no measurable cost in accuracy.
 struct __align__(16) QuadH {
-    __half q[8];  // y[0..5], flag, 0
+    __half q[8];  // y[0..5], 0, 0
 };
-struct __align__(16) OctH {
-    float o[13];  // w * octupole, the layout above
-    float pad[3];
+struct __align__(64) OctH {
+    float o[11];  // w * octupole, the layout above
+    __half q[6];  // y[0..5]
+    uint32_t flag, pad;
 };
+static_assert(sizeof(OctH) == 64, "octupole entry is 64 bytes");
 constexpr float OCT_MIN = 3e-3f;
 
 // qh[k] = MLP(features of node k) for the internal nodes listed in inodes (zero for massless
@@ -1138,6 +1143,7 @@ __global__ __launch_bounds__(256) void mlp_node_kernel(const NodeRec* __restrict
         oc[4] = __shfl(yv[2], r16 + 32, 64);
         oc[5] = __shfl(yv[3], r16 + 32, 64);
         oc[6] = __shfl(yv[0], r16 + 48, 64);
+        const float q4 = __shfl(yv[0], r16 + 16, 64), q5 = __shfl(yv[1], r16 + 16, 64);
         if (listed && g == 0) {
             float big = 0.f;
 #pragma unroll
@@ -1147,7 +1153,7 @@ __global__ __launch_bounds__(256) void mlp_node_kernel(const NodeRec* __restrict
             }
 #pragma unroll
             for (int q = 0; q < 4; ++q) qh[k].q[q] = __float2half(keep * yv[q]);
-            qh[k].q[6] = __float2half(big >= OCT_MIN ? 1.f : 0.f);
+            qh[k].q[6] = __float2half(0.f);
             qh[k].q[7] = __float2half(0.f);
             // w * O in the walk's layout (OctH), the dependent components from O_iik = 0
             const float wd = keep > 0.f ? sqrtf(r.w2) : 0.f;
@@ -1161,14 +1167,17 @@ __global__ __launch_bounds__(256) void mlp_node_kernel(const NodeRec* __restrict
             e.o[3] = yyy;
             e.o[4] = xxz;
             e.o[5] = yyz;
-            e.o[6] = 2.f * xxy;
-            e.o[7] = 2.f * xxz;
-            e.o[8] = 2.f * xyz;
-            e.o[9] = 2.f * xyy;
-            e.o[10] = 2.f * yyz;
-            e.o[11] = 2.f * xzz;
-            e.o[12] = 2.f * yzz;
-            e.pad[0] = e.pad[1] = e.pad[2] = 0.f;
+            e.o[6] = 2.f * xxz;
+            e.o[7] = 2.f * xyz;
+            e.o[8] = 2.f * yyz;
+            e.o[9] = 2.f * xzz;
+            e.o[10] = 2.f * yzz;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) e.q[q] = __float2half(keep * yv[q]);
+            e.q[4] = __float2half(keep * q4);
+            e.q[5] = __float2half(keep * q5);
+            e.flag = big >= OCT_MIN ? 1u : 0u;
+            e.pad = 0u;
             oh[k] = e;
         }
         if (listed && g == 1) {
@@ -1478,22 +1487,31 @@ __global__ __launch_bounds__(256) void bh_walk_kernel(const NodeRec* __restrict_
                     // (quad_field's formula, w2 = width^2) and, when the node has one, of
                     // O = M w^3 y[6..12] (oct_field's): with u = |d|^-2 and s5 = M w^2 |d|^-5,
                     // d (s + s5 u (5/2 dQd - 7 u G)) + s5 (3 u F - Q d), G = d.F, F = O d d
-                    const uint4 h = *reinterpret_cast<const uint4*>(
-                        reinterpret_cast<const char*>(qh) + (ko >> 1));  // 16-byte entries
+                    // the quadrupole's halves: QuadH, or (OCT) inside the node's one OctH entry
+                    uint32_t h0, h1, h2;
                     OctH oe;
                     if constexpr (OCT) {
                         oe = *reinterpret_cast<const OctH*>(reinterpret_cast<const char*>(oh) +
                                                             (ko << 1));  // 64-byte entries
-                        // an empty asm that reads the whole entry here, before h is used: both
-                        // loads are then in flight together and one wait covers them (otherwise
-                        // the compiler fetches the components behind the flag's test)
+                        const uint32_t* hq = reinterpret_cast<const uint32_t*>(oe.q);
+                        h0 = hq[0];
+                        h1 = hq[1];
+                        h2 = hq[2];
+                        // an empty asm that reads the whole entry here: one load and one wait
+                        // (otherwise the compiler fetches the octupole behind the flag's test)
                         asm volatile("" ::"s"(oe.o[0]), "s"(oe.o[1]), "s"(oe.o[2]),
                                      "s"(oe.o[3]), "s"(oe.o[4]), "s"(oe.o[5]), "s"(oe.o[6]),
                                      "s"(oe.o[7]), "s"(oe.o[8]), "s"(oe.o[9]), "s"(oe.o[10]),
-                                     "s"(oe.o[11]), "s"(oe.o[12]));
+                                     "s"(h0), "s"(h1), "s"(h2), "s"(oe.flag));
+                    } else {
+                        const uint4 h = *reinterpret_cast<const uint4*>(
+                            reinterpret_cast<const char*>(qh) + (ko >> 1));  // 16-byte entries
+                        h0 = h.x;
+                        h1 = h.y;
+                        h2 = h.z;
                     }
-                    const float q[6] = {half_lo(h.x), half_hi(h.x), half_lo(h.y),
-                                        half_hi(h.y), half_lo(h.z), half_hi(h.z)};
+                    const float q[6] = {half_lo(h0), half_hi(h0), half_lo(h1),
+                                        half_hi(h1), half_lo(h2), half_hi(h2)};
                     // fma chains from +0 so that every half-precision product is one v_fma_mix
                     const float qx = fmaf(q[4], dz, fmaf(q[3], dy, fmaf(q[0], dx, 0.f)));
                     const float qy = fmaf(q[5], dz, fmaf(q[1], dy, fmaf(q[3], dx, 0.f)));
@@ -1508,17 +1526,18 @@ __global__ __launch_bounds__(256) void bh_walk_kernel(const NodeRec* __restrict_
                     vx = fmaf(-s5, qx, vx);
                     vy = fmaf(-s5, qy, vy);
                     vz = fmaf(-s5, qz, vz);
-                    if (OCT && (h.w & 0xffffu)) {  // the flag: this node's octupole counts
+                    if (OCT && oe.flag) {  // this node's octupole counts
                         // F_i = O_ijk d_j d_k from the traceless form (OctH)
                         const float* o = oe.o;
                         const float ta = dx * dx - dz * dz, tb = dy * dy - dz * dz;
                         const float pxy = dx * dy, pxz = dx * dz, pyz = dy * dz;
-                        const float fx = fmaf(o[0], ta, fmaf(o[1], tb, fmaf(o[6], pxy,
+                        const float p2xy = pxy + pxy;
+                        const float fx = fmaf(o[0], ta, fmaf(o[1], tb, fmaf(o[2], p2xy,
+                                         fmaf(o[6], pxz, o[7] * pyz))));
+                        const float fy = fmaf(o[2], ta, fmaf(o[3], tb, fmaf(o[1], p2xy,
                                          fmaf(o[7], pxz, o[8] * pyz))));
-                        const float fy = fmaf(o[2], ta, fmaf(o[3], tb, fmaf(o[9], pxy,
-                                         fmaf(o[8], pxz, o[10] * pyz))));
-                        const float fz = fmaf(o[4], ta, fmaf(o[5], tb, fmaf(o[8], pxy,
-                                         fmaf(o[11], pxz, o[12] * pyz))));
+                        const float fz = fmaf(o[4], ta, fmaf(o[5], tb, fmaf(o[7], pxy,
+                                         fmaf(o[9], pxz, o[10] * pyz))));
                         const float gdd = dx * fx + dy * fy + dz * fz;
                         // M w^3 |d|^-7 = s5 u w, the w being in oe.o
                         st = fmaf(-7.f * s5u * inv2, gdd, st);
