@@ -25,6 +25,7 @@ from __future__ import annotations
 
 import argparse
 import ctypes
+import gc
 import json
 import os
 import sys
@@ -540,8 +541,15 @@ def bench_small_bh(args):
             call = lambda: alg.compute(both)
             if mode == "graph":
                 g = torch.cuda.CUDAGraph()
-                with torch.cuda.graph(g):
-                    alg.compute(both)
+                # no garbage collection while capturing: a dead cycle freed then would release
+                # device memory on the capturing stream
+                gc.collect()
+                gc.disable()
+                try:
+                    with torch.cuda.graph(g):
+                        alg.compute(both)
+                finally:
+                    gc.enable()
                 res.reset_stream()
                 call = g.replay
             times = []

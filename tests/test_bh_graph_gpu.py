@@ -3,6 +3,9 @@ walk make no host round trip, so they can be captured in a hipGraph. The tree, i
 the accelerations (plain and with the learned correction) are those of the ordinary build, bit
 for bit; graph replays with new positions equal eager calls on the same positions; a tree that
 outgrows the sizes gives NaN and an error from pnb_bh_check (never a hang)."""
+import contextlib
+import gc
+
 import numpy as np
 import pytest
 
@@ -18,6 +21,22 @@ def cres():
     r = pa.HipResources(0)
     yield r
     r.close()
+
+
+@contextlib.contextmanager
+def capture(g):
+    """torch.cuda.graph(g) with no garbage collection during the capture: a dead reference cycle
+    collected then (an earlier test's graph and its memory pool, held by a traceback) would free
+    device memory while the stream is capturing, which aborts the process."""
+    import torch
+
+    gc.collect()
+    gc.disable()
+    try:
+        with torch.cuda.graph(g):
+            yield
+    finally:
+        gc.enable()
 
 
 def _bh(res, dev, theta=0.5):
@@ -77,7 +96,7 @@ def test_graph_replays_with_new_positions(cres):
     cres.bh_check()
     torch.cuda.synchronize()
     g = torch.cuda.CUDAGraph()
-    with torch.cuda.graph(g):
+    with capture(g):
         static_out = alg.compute(both)
     cres.reset_stream()
     rng = np.random.default_rng(1)
@@ -159,7 +178,7 @@ def test_reordered(cres):
     np.testing.assert_array_equal(got, ref)
     torch.cuda.synchronize()
     g = torch.cuda.CUDAGraph()
-    with torch.cuda.graph(g):
+    with capture(g):
         with pytest.raises(pa.PnbError) as e:
             alg.compute(pa.Reordered(dev))
     cres.reset_stream()
@@ -194,7 +213,7 @@ def test_corrected_walk(cres, network):
     cres.bh_check()
     torch.cuda.synchronize()
     g = torch.cuda.CUDAGraph()
-    with torch.cuda.graph(g):
+    with capture(g):
         static_out = alg.compute(both)
     cres.reset_stream()
     rng = np.random.default_rng(2)
@@ -214,7 +233,7 @@ def test_corrected_walk(cres, network):
         print(f"replay {step}: bit-identical to the eager call", flush=True)
     # the plain walk was not sized with these buffers: refused inside a capture
     g2 = torch.cuda.CUDAGraph()
-    with torch.cuda.graph(g2):
+    with capture(g2):
         with pytest.raises(pa.PnbError) as e:
             _bh(cres, dev)
     cres.reset_stream()
