@@ -327,7 +327,8 @@ int pnb_batch_info(pnb_ctx *ctx, size_t *particles, size_t *items,
  * seven components). Every lane that accepts the node adds, with d = centre of mass - lane,
  * a = -Q d |d|^-5 + 2.5 (d.Q.d) d |d|^-7 + 3 F |d|^-7 - 7 (d.F) d |d|^-9 with F_i = O_ijk d_j d_k;
  * with y[6..12] = f9..15 / 32 the last two terms are the octupole term of the node's multipole
- * expansion. The network runs once per node after the octree build. Training data are generated
+ * expansion; they are left out for a node whose largest y[6..12] coefficient is below 3e-3. The
+ * network runs once per node after the octree build. Training data are generated
  * by the method itself: per-lane targets are brute force minus Barnes-Hut. */
 
 int pnb_mlp_num_params(void);
