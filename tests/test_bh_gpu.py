@@ -204,20 +204,26 @@ def test_stats_count_accepted_terms(res, orc):
 
 def test_wide_record_offsets_match_narrow(res, monkeypatch):
     """Trees of 2^27 nodes or more are walked with 64-bit record offsets; PNB_WIDE_WALK=1 takes
-    that form for a small tree, which must give bit-identical results (plain and corrected)."""
+    that form for a small tree, which must give bit-identical results (plain and corrected, with
+    quadrupole outputs and with octupole outputs)."""
     from particular_amd import learned
 
     s = workloads.direct_sum_case(30000, seed=61)
     learned.init(res, 0)
-    learned.set_params(res, learned.quadrupole_params())
     runs = {}
     for wide in ("0", "1"):
         monkeypatch.setenv("PNB_WIDE_WALK", wide)
-        runs[wide] = [pa.BarnesHut(res, th, pa.Acceleration.checked(), mlp=m).compute(s.particles)
-                      for th, m in ((0.5, False), (0.8, False), (0.5, True))]
+        runs[wide] = [pa.BarnesHut(res, th, pa.Acceleration.checked()).compute(s.particles)
+                      for th in (0.5, 0.8)]
+        # the corrected walk with quadrupole outputs only, then with octupole outputs
+        for params in (learned.quadrupole_params(), learned.octupole_params()):
+            learned.set_params(res, params)
+            runs[wide].append(pa.BarnesHut(res, 0.5, pa.Acceleration.checked(), mlp=True)
+                              .compute(s.particles))
     for a, b in zip(runs["0"], runs["1"]):
         np.testing.assert_array_equal(a, b)
     assert np.abs(runs["0"][2] - runs["0"][0]).max() > 0  # the correction did something
+    assert np.abs(runs["0"][3] - runs["0"][2]).max() > 0  # and the octupole too
 
 
 @pytest.mark.parametrize("m", [4096, 4097, 9000, 16384, 16385])
