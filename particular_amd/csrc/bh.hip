@@ -1322,12 +1322,13 @@ __device__ __forceinline__ float half_hi(uint32_t v) {
 
 
 // One wavefront walks the pre-order node records with one cursor (k, in SGPRs); lane decisions
-// are the reference's (see the file comment). The walk is bound by VALU issue (about 34 VALU,
-// 15 SALU and 2 scalar loads per step for the plain walk), so the step keeps everything
-// wave-uniform in SGPRs: the records of both possible next nodes (k + 1, skip) are requested at
-// the top of the step and picked with scalar selects at its end, and WALK_QUAD fetches the
-// learned moment of node k only when some lane accepts it as an internal node (prefetching it for
-// both candidates costs more scalar-cache misses than it hides).
+// are the reference's (see the file comment). The walk is bound by its instruction stream as a
+// whole (about 33 VALU, 15 SALU and 2 scalar loads per step for the plain walk, and the latency
+// of those loads), so the step keeps everything wave-uniform in SGPRs: the records of both
+// possible next nodes (k + 1, skip) are requested at the top of the step and picked with scalar
+// selects at its end, and the corrected walks fetch the learned entry of node k only when some
+// lane accepts it as an internal node (prefetching it for both candidates costs more
+// scalar-cache misses than it hides).
 // WIDE: 64-bit byte offsets, for trees whose records span 4 GiB or more (2^27 nodes).
 // REL: the relative criterion. Each lane holds A = alpha |a~| (est_in[i >> est_shift], from an
 // estimate pass per particle, est_shift = 0, or per wavefront, est_shift = 6) and
