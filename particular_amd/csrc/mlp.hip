@@ -244,31 +244,30 @@ __global__ __launch_bounds__(256) void mlp_rowgrad_kernel(size_t rows,
 
 // ---- backward ------------------------------------------------------------------------------------
 
-struct BwdTile {
-    MlpTile f;
-    uint16_t xt[MLP_KIN][16];   // x transposed [k][row]
-    uint16_t h1t[MLP_H][16];    // h1 transposed
-    uint16_t h2t[MLP_H][16];    // h2 transposed
-    uint16_t gyr[16][MLP_NOUT]; // g_y row-major
-    uint16_t gyt[MLP_NOUT][16]; // g_y transposed
-    uint16_t g2r[16][MLP_H];    // g_z2 row-major
-    uint16_t g2t[MLP_H][16];    // g_z2 transposed
-    uint16_t g1t[MLP_H][16];    // g_z1 transposed
-};
-
+// Per wavefront and 16-row tile, every product in registers: each activation and gradient the
+// backward pass needs is produced in two layouts by computing its product in both orientations
+// (MFMA time is cheap here, LDS transposes were not). With lane l = 16 g + r (r = l % 16):
+//   chain layout: D of a transposed product Z^T = W^T H^T, lane holds {hidden 4g+i, row r}, which
+//     is also the A fragment of H (m = row r, k = hidden 4g+i) and the B fragment of H^T for the
+//     next transposed product;
+//   grad layout: D of Z = H W, lane holds {row 4g+i, hidden r}: the A fragment of H^T (m = hidden,
+//     k = rows) and the B fragment of H (k = rows, n = hidden) of the weight-gradient products,
+//     whose reduction runs over the tile's rows.
+// Forward: Z1, Z2 in both layouts (biases: b[hidden]); backward: G_h2 in both (from g_y, read
+// from memory in both layouts), G_h1 in the grad layout (from G_z2 in the chain layout);
+// gradients: gW3 += H2^T g_y, gW2 += H1^T G_z2, gW1 += X^T G_z1, bias gradients summed from the
+// grad layouts (rows 4g..4g+3 per lane, then across the four row groups).
 __global__ __launch_bounds__(256) void mlp_bwd_kernel(const float* __restrict__ feats,
                                                       const float* __restrict__ gyin, size_t rows,
                                                       const MlpWeights* __restrict__ wg,
                                                       float* __restrict__ partial) {
     __shared__ MlpWeights w;
-    __shared__ BwdTile tiles[MLP_WAVES_PER_BLOCK];
     load_weights(w, wg);
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int r16 = lane & 15, g = lane >> 4;
-    BwdTile& t = tiles[wv];
     const f32x4_t zero = {0.f, 0.f, 0.f, 0.f};
     f32x4_t gw1[4], gw2[4][4], gw3[4];
-    float gb1[4] = {0, 0, 0, 0}, gb2[4] = {0, 0, 0, 0}, gb3[MLP_OUT] = {};
+    float gb1[4] = {0, 0, 0, 0}, gb2[4] = {0, 0, 0, 0}, gb3 = 0.f;
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
         gw1[a] = zero;
@@ -278,69 +277,139 @@ __global__ __launch_bounds__(256) void mlp_bwd_kernel(const float* __restrict__ 
     }
     const size_t gwave = size_t(blockIdx.x) * MLP_WAVES_PER_BLOCK + wv;
     const size_t nw = size_t(gridDim.x) * MLP_WAVES_PER_BLOCK;
-    for (size_t tile = gwave; tile * 16 < rows; tile += nw) {
-        const size_t row0 = tile * 16;
-        load_x16(t.f, feats, row0, rows, t.xt);
-        if (lane < 16) {
-            const size_t r = row0 + lane;
-            const bool valid = r < rows;
-#pragma unroll
-            for (int n = 0; n < MLP_NOUT; ++n) {
-                const float gv = (valid && n < MLP_OUT) ? gyin[MLP_YS * r + n] : 0.f;
-                if (n < MLP_OUT) gb3[n] += gv;
-                const uint16_t b = f2bf(gv);
-                t.gyr[lane][n] = b;
-                t.gyt[n][lane] = b;
-            }
+    // A tile's inputs, read one tile ahead (one wavefront per SIMD here: the loads' latency is
+    // hidden behind the previous tile's products): x and g_y of row r, columns 4g..4g+3 (chain
+    // layout / A fragment of the row-major matrix) and of rows 4g..4g+3, column r (grad layout).
+    float4 nxv, ngv;
+    float nxs[4], ngs[4];
+    const auto fetch = [&](size_t tile) {
+        const size_t row0 = tile * 16, r = row0 + r16;
+        nxv = make_float4(0.f, 0.f, 0.f, 0.f);
+        ngv = nxv;
+        if (r < rows) {
+            nxv = *reinterpret_cast<const float4*>(feats + r * MLP_FEAT + 4 * g);
+            ngv = *reinterpret_cast<const float4*>(gyin + r * MLP_YS + 4 * g);
         }
-        f32x4_t z1[4], z2[4];
-        mlp_forward16(w, t.f, nullptr, z1, z2, t.h1t, t.h2t);  // syncs the tiles written above
-        // gW3 += h2^T g_y
 #pragma unroll
-        for (int mt = 0; mt < 4; ++mt)
-            gw3[mt] = mfma16(frag(&t.h2t[16 * mt + r16][4 * g]), frag(&t.gyt[r16][4 * g]), gw3[mt]);
-        // g_z2 = (g_y W3^T) * relu'(z2)
-        const bf16x4_t agy = frag(&t.gyr[r16][4 * g]);
+        for (int i = 0; i < 4; ++i) {
+            const size_t rr = row0 + 4 * g + i;
+            const bool ok = rr < rows;
+            nxs[i] = (ok && r16 < MLP_IN) ? feats[rr * MLP_FEAT + r16] : 0.f;
+            ngs[i] = (ok && r16 < MLP_OUT) ? gyin[rr * MLP_YS + r16] : 0.f;
+        }
+    };
+    fetch(gwave);
+    for (size_t tile = gwave; tile * 16 < rows; tile += nw) {
+        const float4 xv = nxv, gv = ngv;
+        float xs[4], gs[4];
 #pragma unroll
-        for (int nt = 0; nt < 4; ++nt) {
-            const f32x4_t gh = mfma16(agy, frag(&w.w3[16 * nt + r16][4 * g]), zero);
+        for (int i = 0; i < 4; ++i) {
+            xs[i] = nxs[i];
+            gs[i] = ngs[i];
+        }
+        fetch(tile + nw);  // past the last tile: zeros, nothing read
+        bf16x4_t bx, gyc;
+        {
+            const float xa[4] = {xv.x, xv.y, xv.z, xv.w}, ga[4] = {gv.x, gv.y, gv.z, gv.w};
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const float gz = z2[nt][i] > 0.f ? gh[i] : 0.f;
-                gb2[nt] += gz;
-                const uint16_t b = f2bf(gz);
-                t.g2r[4 * g + i][16 * nt + r16] = b;
-                t.g2t[16 * nt + r16][4 * g + i] = b;
+                const int col = 4 * g + i;
+                bx[i] = short(f2bf(col < MLP_IN ? xa[i] : 0.f));
+                gyc[i] = short(f2bf(col < MLP_OUT ? ga[i] : 0.f));
             }
         }
-        wave_lds_sync();
-        // gW2 += h1^T g_z2
+        // forward: H1 in the chain layout; the signs of Z2 (chain) and Z1, Z2 (grad) as bit masks
+        // (bit 4 t + i), H1 and H2 in the grad layout
+        bf16x4_t h1c[4], h1g[4], h2g[4];
+        uint32_t pos2c = 0, pos1g = 0, pos2g = 0;
 #pragma unroll
-        for (int mt = 0; mt < 4; ++mt)
+        for (int mt = 0; mt < 4; ++mt) {
+            f32x4_t z = mfma16(frag(&w.w1t[16 * mt + r16][4 * g]), bx, zero);
 #pragma unroll
-            for (int nt = 0; nt < 4; ++nt)
-                gw2[mt][nt] = mfma16(frag(&t.h1t[16 * mt + r16][4 * g]), frag(&t.g2t[16 * nt + r16][4 * g]),
-                                     gw2[mt][nt]);
-        // g_z1 = (g_z2 W2^T) * relu'(z1)
+            for (int i = 0; i < 4; ++i) z[i] += w.b1[16 * mt + 4 * g + i];
+            h1c[mt] = pack_relu(z);
+        }
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt) {
+            f32x4_t z = zero;
+#pragma unroll
+            for (int ks = 0; ks < 4; ++ks)
+                z = mfma16(frag(&w.w2t[16 * mt + r16][16 * ks + 4 * g]), h1c[ks], z);
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                pos2c |= (z[i] + w.b2[16 * mt + 4 * g + i] > 0.f ? 1u : 0u) << (4 * mt + i);
+        }
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt) {
+            f32x4_t z = mfma16(bx, frag(&w.w1t[16 * nt + r16][4 * g]), zero);
+            const float b = w.b1[16 * nt + r16];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                z[i] += b;
+                pos1g |= (z[i] > 0.f ? 1u : 0u) << (4 * nt + i);
+            }
+            h1g[nt] = pack_relu(z);
+        }
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt) {
+            f32x4_t z = zero;
+#pragma unroll
+            for (int ks = 0; ks < 4; ++ks)
+                z = mfma16(h1c[ks], frag(&w.w2t[16 * nt + r16][16 * ks + 4 * g]), z);
+            const float b = w.b2[16 * nt + r16];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                z[i] += b;
+                pos2g |= (z[i] > 0.f ? 1u : 0u) << (4 * nt + i);
+            }
+            h2g[nt] = pack_relu(z);
+        }
+        // g_z2 = (g_y W3^T) * relu'(z2): chain layout (the A operand of G_h1) and grad layout
+        bf16x4_t g2c[4], g2g[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const bf16x4_t w3f = frag(&w.w3[16 * t + r16][4 * g]);
+            const f32x4_t ghc = mfma16(w3f, gyc, zero);
+            const f32x4_t ghg = mfma16(gyc, w3f, zero);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                g2c[t][i] = short(f2bf((pos2c >> (4 * t + i)) & 1u ? ghc[i] : 0.f));
+                const float gz = (pos2g >> (4 * t + i)) & 1u ? ghg[i] : 0.f;
+                gb2[t] += gz;
+                g2g[t][i] = short(f2bf(gz));
+            }
+        }
+        // g_z1 = (g_z2 W2^T) * relu'(z1), grad layout
+        bf16x4_t g1g[4];
 #pragma unroll
         for (int nt = 0; nt < 4; ++nt) {
             f32x4_t gh = zero;
 #pragma unroll
             for (int ks = 0; ks < 4; ++ks)
-                gh = mfma16(frag(&t.g2r[r16][16 * ks + 4 * g]), frag(&w.w2[16 * nt + r16][16 * ks + 4 * g]), gh);
+                gh = mfma16(g2c[ks], frag(&w.w2[16 * nt + r16][16 * ks + 4 * g]), gh);
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const float gz = z1[nt][i] > 0.f ? gh[i] : 0.f;
+                const float gz = (pos1g >> (4 * nt + i)) & 1u ? gh[i] : 0.f;
                 gb1[nt] += gz;
-                t.g1t[16 * nt + r16][4 * g + i] = f2bf(gz);
+                g1g[nt][i] = short(f2bf(gz));
             }
         }
-        wave_lds_sync();
-        // gW1 += x^T g_z1
+        // weight gradients: gW3 += H2^T g_y, gW2 += H1^T g_z2, gW1 += X^T g_z1, with x and g_y
+        // of rows 4g..4g+3, column r
+        bf16x4_t xg, gyg;
 #pragma unroll
-        for (int nt = 0; nt < 4; ++nt)
-            gw1[nt] = mfma16(frag(&t.xt[r16][4 * g]), frag(&t.g1t[16 * nt + r16][4 * g]), gw1[nt]);
-        wave_lds_sync();
+        for (int i = 0; i < 4; ++i) {
+            xg[i] = short(f2bf(xs[i]));
+            gyg[i] = short(f2bf(gs[i]));
+            gb3 += gs[i];
+        }
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt) {
+            gw3[mt] = mfma16(h2g[mt], gyg, gw3[mt]);
+            gw1[mt] = mfma16(xg, g1g[mt], gw1[mt]);
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt) gw2[mt][nt] = mfma16(h1g[mt], g2g[nt], gw2[mt][nt]);
+        }
     }
     float* out = partial + gwave * MLP_NPARAM;
 #pragma unroll
@@ -362,6 +431,7 @@ __global__ __launch_bounds__(256) void mlp_bwd_kernel(const float* __restrict__ 
 #pragma unroll
         for (int i = 0; i < 4; ++i)
             if (r16 < MLP_OUT) out[MLP_OFF_W3 + (16 * mt + 4 * g + i) * MLP_OUT + r16] = gw3[mt][i];
+    // bias gradients: each lane's sums over its rows 4g..4g+3, then over the four row groups
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt) {
         float a = gb1[nt], b = gb2[nt];
@@ -374,12 +444,9 @@ __global__ __launch_bounds__(256) void mlp_bwd_kernel(const float* __restrict__ 
             out[MLP_OFF_B2 + 16 * nt + r16] = b;
         }
     }
-#pragma unroll
-    for (int n = 0; n < MLP_OUT; ++n) {
-        float a = gb3[n];
-        for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
-        if (lane == 0) out[MLP_OFF_B3 + n] = a;
-    }
+    gb3 += __shfl_xor(gb3, 16, 64);
+    gb3 += __shfl_xor(gb3, 32, 64);
+    if (g == 0 && r16 < MLP_OUT) out[MLP_OFF_B3 + r16] = gb3;
 }
 
 // grads[k] = sum over the partials in a fixed order: 8 groups of 32 threads per workgroup, group
