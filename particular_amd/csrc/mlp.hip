@@ -119,23 +119,33 @@ __global__ __launch_bounds__(256) void mlp_fwd_kernel(const float* __restrict__ 
     const int r16 = lane & 15, g = lane >> 4;
     const size_t gw = size_t(blockIdx.x) * MLP_WAVES_PER_BLOCK + wv;
     const size_t nw = size_t(gridDim.x) * MLP_WAVES_PER_BLOCK;
+    // features 4g..4g+3 of row r16 of a tile, read one tile ahead (16-byte loads)
+    const auto fetch = [&](size_t tile) {
+        const size_t r = tile * 16 + r16;
+        return r < rows ? *reinterpret_cast<const float4*>(feats + r * MLP_FEAT + 4 * g)
+                        : make_float4(0.f, 0.f, 0.f, 0.f);
+    };
+    float4 nxv = fetch(gw);
     for (size_t tile = gw; tile * 16 < rows; tile += nw) {
         const size_t r = tile * 16 + r16;
+        const float4 xv = nxv;
+        nxv = fetch(tile + nw);  // past the last tile: zeros, nothing read
+        const float xa[4] = {xv.x, xv.y, xv.z, xv.w};
         bf16x4_t bx;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int k = 4 * g + i;
-            bx[i] = short((r < rows && k < MLP_IN) ? f2bf(feats[r * MLP_FEAT + k]) : uint16_t(0));
-        }
+        for (int i = 0; i < 4; ++i) bx[i] = short(f2bf(4 * g + i < MLP_IN ? xa[i] : 0.f));
         const f32x4_t yv = mlp_forward16_regs(w, bx);
         if (g < MLP_YS / 4 && r < rows) {  // lanes of group g hold outputs 4 g .. 4 g + 3
             // plus the residual on 0..5: the row's normalised quadrupole, features 0..5 in FP32
+            // (features 4g..4g+3 of this row are the lane's own)
+            float o4[4];
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const int o = 4 * g + i;
-                const float res = o < MLP_QOUT ? feats[r * MLP_FEAT + o] : 0.f;
-                y[r * MLP_YS + o] = o < MLP_OUT ? yv[i] + res : 0.f;
+                o4[i] = o < MLP_OUT ? yv[i] + (o < MLP_QOUT ? xa[i] : 0.f) : 0.f;
             }
+            *reinterpret_cast<float4*>(y + r * MLP_YS + 4 * g) =
+                make_float4(o4[0], o4[1], o4[2], o4[3]);
         }
     }
 }
