@@ -833,6 +833,7 @@ __device__ __forceinline__ void up_node(uint32_t k, NodeRec* __restrict__ rec,
     }
 }
 
+template <bool MOM>
 __global__ __launch_bounds__(256) void bh_up_kernel(NodeRec* __restrict__ rec,
                                                     NodeSums* __restrict__ sums,
                                                     const uint32_t* __restrict__ nodes_at_level,
@@ -844,7 +845,7 @@ __global__ __launch_bounds__(256) void bh_up_kernel(NodeRec* __restrict__ rec,
                                                     const float4* __restrict__ sp) {
     for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < count;
          t += gridDim.x * blockDim.x)
-        up_node(nodes_at_level[t], rec, sums, mom, quad, desc, width, skipv, sp);
+        up_node(nodes_at_level[t], rec, sums, MOM ? mom : nullptr, quad, desc, width, skipv, sp);
 }
 
 // Consecutive small levels (deepest first) in one workgroup: a barrier between levels makes one
@@ -857,6 +858,7 @@ struct UpRun {
     int levels;
 };
 
+template <bool MOM>
 __global__ __launch_bounds__(1024) void bh_up_run_kernel(NodeRec* __restrict__ rec,
                                                          NodeSums* __restrict__ sums,
                                                          const uint32_t* __restrict__ ilist,
@@ -868,7 +870,8 @@ __global__ __launch_bounds__(1024) void bh_up_run_kernel(NodeRec* __restrict__ r
                                                          const float4* __restrict__ sp) {
     for (int j = 0; j < run.levels; ++j) {
         for (uint32_t t = threadIdx.x; t < run.cnt[j]; t += blockDim.x)
-            up_node(ilist[run.off[j] + t], rec, sums, mom, quad, desc, width, skipv, sp);
+            up_node(ilist[run.off[j] + t], rec, sums, MOM ? mom : nullptr, quad, desc, width, skipv,
+                    sp);
         __syncthreads();
     }
 }
@@ -886,6 +889,7 @@ __global__ void bh_dd_check_kernel(const uint32_t* __restrict__ S, size_t m, uin
 // The bottom-up sums of one level, its count and place in the level-grouped list read from the
 // device (the host launches every level up to the sized depth; levels the tree lacks are empty).
 // With `mom` (the learned correction), also the node moments, as in the ordinary build.
+template <bool MOM>
 __global__ __launch_bounds__(256) void bh_up_level_dd_kernel(NodeRec* __restrict__ rec,
                                                              NodeSums* __restrict__ sums,
                                                              const uint32_t* __restrict__ ilist,
@@ -900,11 +904,12 @@ __global__ __launch_bounds__(256) void bh_up_level_dd_kernel(NodeRec* __restrict
     if (info->flags) return;
     const uint32_t cnt = info->lvl_count[level], off = info->lvl_off[level];
     for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < cnt; t += gridDim.x * blockDim.x)
-        up_node(ilist[off + t], rec, sums, mom, quad, desc, width, skipv, sp);
+        up_node(ilist[off + t], rec, sums, MOM ? mom : nullptr, quad, desc, width, skipv, sp);
 }
 
 // A run of levels hi, hi - 1, ..., lo in one workgroup, a barrier between levels (levels the tree
 // lacks are empty).
+template <bool MOM>
 __global__ __launch_bounds__(1024) void bh_up_levels_dd_kernel(NodeRec* __restrict__ rec,
                                                                NodeSums* __restrict__ sums,
                                                                const uint32_t* __restrict__ ilist,
@@ -920,7 +925,8 @@ __global__ __launch_bounds__(1024) void bh_up_levels_dd_kernel(NodeRec* __restri
     for (int L = hi; L >= lo; --L) {
         const uint32_t cnt = info->lvl_count[L], off = info->lvl_off[L];
         for (uint32_t t = threadIdx.x; t < cnt; t += blockDim.x)
-            up_node(ilist[off + t], rec, sums, mom, quad, desc, width, skipv, sp);
+            up_node(ilist[off + t], rec, sums, MOM ? mom : nullptr, quad, desc, width, skipv,
+                    sp);
         __syncthreads();
     }
 }
@@ -1741,7 +1747,8 @@ static int build_tree(pnb_ctx* c, const float4* afg, size_t m, bool ref_com = fa
         for (int L = levels - 1; L >= 0;) {
             if (h.lvl_count[L] > UP_RUN_NODES) {
                 const uint32_t n_l = h.lvl_count[L];
-                bh_up_kernel<<<div_up(n_l, 256), 256, 0, c->stream>>>(
+                (mom ? bh_up_kernel<true> : bh_up_kernel<false>)<<<div_up(n_l, 256), 256, 0,
+                                                                   c->stream>>>(
                     rec, s->sums.as<NodeSums>(), ilist + off[L], n_l, mom, quad,
                     desc, s->width.as<float>(), skipv, s->sp.as<float4>());
                 PNB_CHECK_LAUNCH(c);
@@ -1754,9 +1761,9 @@ static int build_tree(pnb_ctx* c, const float4* afg, size_t m, bool ref_com = fa
                 run.cnt[run.levels] = h.lvl_count[L];
                 ++run.levels;
             }
-            bh_up_run_kernel<<<1, 1024, 0, c->stream>>>(rec, s->sums.as<NodeSums>(), ilist, run,
-                                                        mom, quad, desc, s->width.as<float>(),
-                                                        skipv, s->sp.as<float4>());
+            (mom ? bh_up_run_kernel<true> : bh_up_run_kernel<false>)<<<1, 1024, 0, c->stream>>>(
+                rec, s->sums.as<NodeSums>(), ilist, run, mom, quad, desc, s->width.as<float>(),
+                skipv, s->sp.as<float4>());
             PNB_CHECK_LAUNCH(c);
         }
         s->has_moments = moments;
@@ -1920,18 +1927,18 @@ static int build_tree_dd(pnb_ctx* c, const float4* afg, size_t m, bool moments) 
             if (planned(L) > UP_RUN_NODES / 2) {
                 const uint32_t n_l = planned(L);
                 const unsigned ug = std::min<unsigned>(div_up(n_l + n_l / 4, 256u), max_grid);
-                bh_up_level_dd_kernel<<<ug, 256, 0, c->stream>>>(rec, s->sums.as<NodeSums>(),
-                                                                 ilist, info, L, skipv, sp, mom,
-                                                                 quad, desc, width);
+                (mom ? bh_up_level_dd_kernel<true> : bh_up_level_dd_kernel<false>)<<<
+                    ug, 256, 0, c->stream>>>(rec, s->sums.as<NodeSums>(), ilist, info, L, skipv,
+                                             sp, mom, quad, desc, width);
                 PNB_CHECK_LAUNCH(c);
                 --L;
                 continue;
             }
             const int hi = L;
             while (L >= 0 && planned(L) <= UP_RUN_NODES / 2) --L;
-            bh_up_levels_dd_kernel<<<1, 1024, 0, c->stream>>>(rec, s->sums.as<NodeSums>(), ilist,
-                                                              info, hi, L + 1, skipv, sp, mom,
-                                                              quad, desc, width);
+            (mom ? bh_up_levels_dd_kernel<true> : bh_up_levels_dd_kernel<false>)<<<
+                1, 1024, 0, c->stream>>>(rec, s->sums.as<NodeSums>(), ilist, info, hi, L + 1, skipv,
+                                         sp, mom, quad, desc, width);
             PNB_CHECK_LAUNCH(c);
         }
         s->has_moments = moments;
