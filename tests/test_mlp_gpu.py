@@ -339,6 +339,26 @@ def test_untrained_network_is_the_quadrupole_term(res):
     assert np.abs(nn0 - plain).max() > 0
 
 
+@pytest.mark.parametrize("n", [3000, 120000])
+def test_moments_build_keeps_the_tree(res, n):
+    """A build with node moments (the corrected walk's) and a plain build of the same bodies give
+    the same nodes bit for bit: centres of mass, masses (f32 and FP64), widths, skips, ranges and
+    levels. The bottom-up pass has an instance with and one without the moments; 3,000 bodies
+    keep every level in the single-workgroup runs of small levels, 120,000 use the per-level
+    launches too. The bodies include massless ones (zero-mass nodes take the mean position)."""
+    p = workloads.direct_sum_case(n, seed=51).particles.copy()
+    p[::5, 3] = 0.0  # every fifth body massless
+    learned.init(res, 5)
+    pa.BarnesHut(res, 0.6, pa.Acceleration.checked(), mlp=True).compute(p)
+    with_moments, _ = pa.tree_export(res, n)
+    pa.BarnesHut(res, 0.6, pa.Acceleration.checked()).compute(p)
+    plain, _ = pa.tree_export(res, n)
+    assert with_moments.shape == plain.shape and plain.shape[0] > n // 2
+    assert (plain["mass"] == 0).any() and (plain["mass"] > 0).any()
+    for f in ("com", "mass", "mass64", "width", "skip", "begin", "count", "level", "internal"):
+        np.testing.assert_array_equal(with_moments[f], plain[f], err_msg=f)
+
+
 def test_mlp_flag_needs_initialised_network():
     r = pa.HipResources(0)
     try:
